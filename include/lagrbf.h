@@ -1,0 +1,117 @@
+/*
+ * lagrbf.h -- C ABI of the MI355X-native LagRBF hot path (liblagrbf.so).
+ *
+ * Drop-in for the symbols the reference's Python layer binds through ctypes
+ * (python_api/lagrbfpy/interface/register_cfunc.py:44-195).  Types follow the
+ * reference's 32-bit-index PETSc build: PetscInt = int32_t, PetscReal = double,
+ * PetscInt64 = int64_t, PetscBool is passed as a C bool (ctypes.c_bool).
+ * Every function returns 0 on success or a PETSc-style error code (62 wrong
+ * argument, 63 out of range, 73 wrong state, 76 device/library failure,
+ * 85 null argument); the message is printed to stderr and kept for
+ * lagrbf_last_error().
+ *
+ * Configuration does not cross the ABI in the reference: lagrbf_create() reads
+ * the PETSc options database.  Here the same option names (without PETSc) are
+ * given through lagrbf_set_option(), a file (lagrbf_options_insert_file) or the
+ * LAGRBF_OPTIONS environment variable ("-name value -name value ...").
+ *
+ * All computation runs on the current HIP device.  Host arrays passed in are
+ * copied; host output arrays are caller-allocated and fully written.
+ */
+#ifndef LAGRBF_H
+#define LAGRBF_H
+
+#include <stdbool.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------- reference ABI ---------------- */
+
+/* Replaces lagrbf_create, src/interface/interface_ctx.c:93-104.
+ * Builds the PU grid, patches and both swarms from the options database.
+ * Returns NULL on error (bad options, no HIP device). */
+void *lagrbf_create(void);
+
+/* Replaces lagrbf_destroy, src/interface/interface_ctx.c:118-123. */
+void lagrbf_destroy(void *ctx);
+
+/* Replaces lagrbf_fill_swarm, src/interface/interface_utils.c:403-430.
+ * is_data: fill the data (node) swarm, else the interpolation (particle) swarm.
+ * coords [npoints*dim] AoS, values [npoints*n_interpolation_fields] or NULL,
+ * pidx [npoints] (1-based unique ids) or NULL (ids 1..npoints assigned).
+ * Points outside the PU box [O,L] are rejected with error 62. */
+int lagrbf_fill_swarm(void *ctx, bool is_data, int32_t npoints, double *coords,
+                      double *values, int64_t *pidx);
+
+/* Replaces lagrbf_interpolation, src/interface/interface_interp.c:210-221:
+ * lazy patch set-up + solve (partition.c:417-478) then PU evaluation of "fields". */
+int lagrbf_interpolation(void *ctx);
+
+/* Replaces lagrbf_output, src/interface/interface_utils.c:362-384.
+ * Writes rows by pid-1 (AssembleArray_Allreduce semantics); rows of removed
+ * particles stay zero.  fieldname: "fields", "velocity" or "weights". */
+int lagrbf_output(void *ctx, bool is_data, int32_t npoints, double *coords, double *fields,
+                  int64_t *pidx, const char *fieldname);
+
+/* Replaces lagrbf_avect_analytical (sic), src/interface/interface_adv.c:75-88:
+ * ForwardAnalyticalAdvection (time_loop.c:176-221) with the registered callback. */
+int lagrbf_avect_analytical(void *ctx, double time, double time_next);
+
+/* Replaces lagrbf_advect_interpolate, src/interface/interface_adv.c:90-106:
+ * InternalForwardAdvectionLoop (time_loop.c:39-100), CFL-driven dt; at exit the
+ * coefficients are reset and the data swarm is cleared, as in the reference. */
+int lagrbf_advect_interpolate(void *ctx, double time, double time_next);
+
+/* Replaces lagrbf_advection_function, src/interface/interface_adv.c:35-53. */
+int lagrbf_advection_function(void *ctx, int (*velocity_fn)(double *, double *, void *), void *fn_ctx);
+
+/* Replaces lagrbf_advection_function_batch, src/interface/interface_adv.c:55-73. */
+int lagrbf_advection_function_batch(void *ctx,
+                                    int (*velocity_fn_batch)(int32_t, int32_t, double *, double *, void *),
+                                    void *fn_ctx);
+
+/* Replaces lagrbf_reset_coefficients, src/interface/interface_utils.c:447-458
+ * (LagRBF_PartitionsReset, partition.c:494-524). */
+int lagrbf_reset_coefficients(void *ctx);
+
+/* Replaces lagrbf_interpolate_driver, src/interface/interface_interp.c:167-194:
+ * one-shot interpolation with options read from options_file (may be NULL). */
+void lagrbf_interpolate_driver(int32_t ndata_local, double *data_coords, double *data_values,
+                               int32_t npoints_local, double *points_coords, double *points_values,
+                               int32_t npoints_global, char *options_file);
+
+/* ---------------- extensions (no reference counterpart) ---------------- */
+
+/* Options database replacing PETSc.Options (context.py:119-201 writes these names). */
+int lagrbf_set_option(const char *name, const char *value);
+int lagrbf_clear_options(void);
+int lagrbf_options_insert_file(const char *path);
+
+/* Fixed-step form of the internal advection loop: nsteps steps of size dt with
+ * the per-step sequence of time_loop.c:80-90, no CFL, no end-of-call reset.
+ * Used for fixed-dt workloads and the throughput benchmark. */
+int lagrbf_advect_steps(void *ctx, int32_t nsteps, double dt);
+
+/* Runtime counters: "removed" (particles that left the domain), "fallback"
+ * (patches refactored in FP64 after FP32 refinement stalled), "refine_iters",
+ * "patches_setup", "dt" (current advection dt), "eval_ms", "setup_ms",
+ * "n_particles", "n_nodes", "max_patch", "eps_w", "eps_r". */
+int lagrbf_get_info(void *ctx, const char *key, double *value);
+
+/* Introspection for tests: copies device state into `out` (capacity `cap`
+ * elements) and returns the element count (negative error code on failure).
+ * what: "cellid_points", "cellid_data", "cell_start", "cell_nodes", "alive"
+ * (int32), "patch_members" (int32, patch `index`), "patch_coeffs" (double,
+ * row-major [n][nf]), "patch_flags" (int32 setup, solved, n). */
+int64_t lagrbf_debug_get(void *ctx, const char *what, int32_t index, void *out, int64_t cap);
+
+/* Copy the last error message (NUL-terminated, truncated to len). */
+int lagrbf_last_error(char *buf, int32_t len);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,13 @@
+"""lagrbf_amd: MI355X-native PU-RBF interpolation and Lagrangian advection.
+
+Python mirror of lagrbfpy's public classes over liblagrbf.so (HIP, gfx950).
+The library runs on the GPU only; importing works without it, but creating a
+context requires the built liblagrbf.so and a HIP device.
+"""
+from .utils import PointContainer, FunctionData, FunctionPointer, halton_sequence, structured_points, halton_points
+from .context import LagRBFCtx
+from .interpolation import LagRBFInterpolator
+from .advection import Advection, AdvectionAnalytical, AdvectionInterpolation
+
+__all__ = ["PointContainer", "FunctionData", "FunctionPointer", "halton_sequence", "structured_points", "halton_points",
+           "LagRBFCtx", "LagRBFInterpolator", "Advection", "AdvectionAnalytical", "AdvectionInterpolation"]

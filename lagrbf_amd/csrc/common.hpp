@@ -1,0 +1,106 @@
+// Device-side building blocks shared by the gfx950 kernels of liblagrbf.so.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <cstddef>
+
+namespace lrb {
+
+// KernelType, include/kernels/kernels_rbf.h:34-42
+enum KernelType { K_LINEAR = 0, K_WC0 = 1, K_WC2 = 2, K_BUMP = 3, K_MQ = 4, K_IMQ = 5, K_GAUSS = 6 };
+
+// Uniform PU grid: n nodes per axis on [O, L] (L is the max corner), nc = n-1 cells.
+struct Grid {
+  int dim, nen;
+  int n[3], nc[3];
+  double O[3], L[3], h[3];
+  double eps_w, eps_r;
+  int P, C;
+};
+
+// phi(r, eps) of kernels_rbf.c:31-91 for a kernel fixed at compile time.
+template <int K, typename T>
+__device__ __forceinline__ T rbf(T r, T eps) {
+  if constexpr (K == K_LINEAR) {
+    return eps * r;
+  } else if constexpr (K == K_WC0) {
+    T u = T(1) - eps * r;
+    return u < T(0) ? T(0) : u * u;
+  } else if constexpr (K == K_WC2) {
+    T u = T(1) - eps * r;
+    T u2 = u * u;
+    return u < T(0) ? T(0) : (u2 * u2) * (T(5) - T(4) * u);
+  } else if constexpr (K == K_BUMP) {
+    T u = T(1) - eps * r;
+    return u < T(0) ? T(0) : exp(T(-1) / (u * (T(2) - u)));
+  } else if constexpr (K == K_MQ) {
+    T u = eps * r;
+    return sqrt(T(1) + u * u);
+  } else if constexpr (K == K_IMQ) {
+    T u = eps * r;
+    return T(1) / sqrt(T(1) + u * u);
+  } else {
+    T u = eps * r;
+    return exp(-u * u);
+  }
+}
+
+// Runtime-dispatched variant for the cold paths (assembly fallbacks, tests).
+template <typename T>
+__device__ __forceinline__ T rbf_rt(int k, T r, T eps) {
+  switch (k) {
+    case K_LINEAR: return rbf<K_LINEAR, T>(r, eps);
+    case K_WC0: return rbf<K_WC0, T>(r, eps);
+    case K_WC2: return rbf<K_WC2, T>(r, eps);
+    case K_BUMP: return rbf<K_BUMP, T>(r, eps);
+    case K_MQ: return rbf<K_MQ, T>(r, eps);
+    case K_IMQ: return rbf<K_IMQ, T>(r, eps);
+    default: return rbf<K_GAUSS, T>(r, eps);
+  }
+}
+
+// Particle -> cell, lexicographic cell index (i fastest) of the uniform Q1 grid,
+// as DMLocatePoints on the PU DMDA (swarm_utils.c:995-1030).  A coordinate with
+// O <= x <= L is inside, x == L falls in the last cell, anything else (incl. NaN)
+// returns -1.  IEEE subtract, correctly rounded divide and floor: host and device
+// agree bit for bit.
+__device__ __forceinline__ int locate_axis(double x, double O, double L, double h, int nc) {
+  if (!(x >= O && x <= L)) return -1;
+  double t = (x - O) / h;   // no multiply => nothing to contract
+  int i = (int)floor(t);
+  if (i >= nc) i = nc - 1;
+  if (i < 0) i = 0;
+  return i;
+}
+
+__device__ __forceinline__ int locate(const Grid& g, double x, double y, double z) {
+  int i = locate_axis(x, g.O[0], g.L[0], g.h[0], g.nc[0]);
+  int j = locate_axis(y, g.O[1], g.L[1], g.h[1], g.nc[1]);
+  if (i < 0 || j < 0) return -1;
+  if (g.dim == 3) {
+    int k = locate_axis(z, g.O[2], g.L[2], g.h[2], g.nc[2]);
+    if (k < 0) return -1;
+    return i + g.nc[0] * (j + g.nc[1] * k);
+  }
+  return i + g.nc[0] * j;
+}
+
+// Q1 node k of a cell (DMDAGetElements order: counter-clockwise bottom face, then top).
+__device__ __forceinline__ int corner_node(const Grid& g, int cell, int k) {
+  int i = cell % g.nc[0];
+  int j = (cell / g.nc[0]) % g.nc[1];
+  int kk = g.dim == 3 ? cell / (g.nc[0] * g.nc[1]) : 0;
+  int di = ((k & 3) == 1 || (k & 3) == 2) ? 1 : 0;
+  int dj = ((k & 3) >= 2) ? 1 : 0;
+  int dk = k >> 2;
+  return (i + di) + g.n[0] * ((j + dj) + g.n[1] * (kk + dk));
+}
+
+// Offset of Q1 corner k: (di, dj, dk)
+__host__ __device__ __forceinline__ void corner_offset(int k, int& di, int& dj, int& dk) {
+  di = ((k & 3) == 1 || (k & 3) == 2) ? 1 : 0;
+  dj = ((k & 3) >= 2) ? 1 : 0;
+  dk = k >> 2;
+}
+
+}  // namespace lrb

@@ -1,0 +1,940 @@
+// Host orchestration behind the lagrbf_* C ABI: options, device state, lazy patch
+// set-up, PU evaluation and the RK stage loop.  All state lives in HBM; the host
+// only keeps per-patch bookkeeping (P entries) and the swarm point ids.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cfloat>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/lagrbf.h"
+#include "kernels.hpp"
+#include "options.hpp"
+
+namespace lrb {
+
+// PETSc error codes returned for the same conditions as the reference
+enum { E_MEM = 55, E_SUP = 56, E_SIZ = 60, E_WRONG = 62, E_RANGE = 63, E_STATE = 73, E_LIB = 76, E_NULL = 85 };
+
+static thread_local std::string g_last_error;
+
+static int set_err(int code, const char* fn, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  char full[1200];
+  snprintf(full, sizeof(full), "ERROR: %s() failed with error code %d: %s", fn, code, buf);
+  g_last_error = full;
+  fprintf(stderr, "%s\n", full);
+  return code;
+}
+
+#define HIPCHK(call)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess) return set_err(E_LIB, __func__, "%s: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+#define CHK(call)              \
+  do {                         \
+    int r_ = (call);           \
+    if (r_) return r_;         \
+  } while (0)
+
+// Device buffer with grow-only capacity.
+template <typename T>
+struct DBuf {
+  T* p = nullptr;
+  size_t cap = 0;
+  ~DBuf() { release(); }
+  void release() { if (p) (void)hipFree(p); p = nullptr; cap = 0; }
+  hipError_t reserve(size_t n) {
+    if (n <= cap && p) return hipSuccess;
+    release();
+    size_t m = n ? n : 1;
+    hipError_t e = hipMalloc(&p, m * sizeof(T));
+    if (e == hipSuccess) cap = m;
+    return e;
+  }
+  // grow keeping the first `keep` elements
+  hipError_t grow(size_t n, size_t keep, hipStream_t s) {
+    if (n <= cap && p) return hipSuccess;
+    T* q = nullptr;
+    hipError_t e = hipMalloc(&q, (n ? n : 1) * sizeof(T));
+    if (e != hipSuccess) return e;
+    if (p && keep) {
+      e = hipMemcpyAsync(q, p, keep * sizeof(T), hipMemcpyDeviceToDevice, s);
+      if (e != hipSuccess) { (void)hipFree(q); return e; }
+      e = hipStreamSynchronize(s);
+      if (e != hipSuccess) { (void)hipFree(q); return e; }
+    }
+    release();
+    p = q;
+    cap = n ? n : 1;
+    return hipSuccess;
+  }
+};
+
+struct Ctx {
+  Grid g{};
+  int ktype = K_LINEAR, solve_method = 0, nf = 0;
+  double rmin = 0.0, rmax = DBL_MAX, cfl_fac = 0.4;
+  bool advection = false;
+  int scheme = 3, order = 0;
+  double adv_dt = 1e-10;
+  double dopri_fac = 0.9, dopri_fac_min = 1e-2, dopri_fac_max = 1e2, dopri_atol = 1e-3, dopri_rtol = 1e-3;
+  double dopri_err = 0.0;
+  std::string velocity_name = "velocity";
+  hipStream_t s = nullptr;
+  int device = 0;
+
+  DBuf<double> pux, puy, puz;
+  // data swarm
+  int N = 0;
+  bool data_filled = false;
+  DBuf<double> dx, dval;
+  DBuf<int32_t> dcell, cell_start, cell_nodes, scratch, pcount;
+  std::vector<int32_t> h_pcount;
+  std::vector<int64_t> h_dpid;
+  // particle swarm
+  int Np = 0;
+  int64_t ld = 0;
+  bool points_filled = false;
+  DBuf<double> px, w, fields, vel, x0, vadv[7];
+  DBuf<int32_t> pcell, alive;
+  std::vector<int64_t> h_ppid;
+  DBuf<unsigned long long> counters;   // [0] removed, [1] reduction bits, [2] count
+  // patches
+  std::vector<uint8_t> h_setup, h_solved, h_active;
+  std::vector<int32_t> h_n;
+  std::vector<int64_t> h_mem_off, h_fac_off;
+  size_t mem_used = 0, fac_used = 0;
+  int setup_N = -1;
+  DBuf<int32_t> p_n, mem_pool, piv_pool, list, info;
+  DBuf<uint8_t> p_active, touched;
+  DBuf<int64_t> p_mem_off, p_fac_off;
+  DBuf<double> fac_pool, coef_pool;
+  // analytical velocity
+  int (*vfn)(double*, double*, void*) = nullptr;
+  int (*vfn_batch)(int32_t, int32_t, double*, double*, void*) = nullptr;
+  void* vuser = nullptr;
+  bool analytical = false;
+  // stats
+  double removed = 0, patches_setup = 0, setup_ms = 0, eval_ms = 0;
+  int max_patch = 0;
+};
+
+static const char* kVelocity = "velocity";
+
+// ---------------------------------------------------------------------------
+static int ctx_from_options(Ctx& c, const OptView& o) {
+  const char* fn = "lagrbf_create";
+  bool bad = false;
+  // PUSetFromOptions, partition_of_unity.c:68-111
+  int dim = 0;
+  if (!o.get_int("pu_mesh_dim", dim, bad)) return set_err(E_WRONG, fn, "Option -pu_mesh_dim not provided");
+  if (dim < 2 || dim > 3) return set_err(E_RANGE, fn, "Spatial dimension must be 2 or 3. Found: %d", dim);
+  std::vector<int> n;
+  if (!o.get_int_array("pu_mesh_n", n, bad)) return set_err(E_WRONG, fn, "Option -pu_mesh_n not provided");
+  if ((int)n.size() != dim) return set_err(E_SIZ, fn, "Option -pu_mesh_n. Expected %d entries, found %d", dim, (int)n.size());
+  std::vector<double> O(dim, 0.0), L(dim, 1.0), t;
+  if (o.get_real_array("pu_mesh_O", t, bad)) { if ((int)t.size() != dim) return set_err(E_SIZ, fn, "Option -pu_mesh_O. Expected %d entries", dim); O = t; }
+  if (o.get_real_array("pu_mesh_L", t, bad)) { if ((int)t.size() != dim) return set_err(E_SIZ, fn, "Option -pu_mesh_L. Expected %d entries", dim); L = t; }
+  double overlap = 1.0, shape = 1.0;
+  o.get_real("pu_mesh_overlap", overlap, bad);
+  o.get_real("pu_mesh_shape_factor", shape, bad);
+  if (!(shape > 0.0)) return set_err(E_RANGE, fn, "Shape factor must be positive. Found: %g", shape);
+  if (bad) return set_err(E_WRONG, fn, "malformed PU option value");
+  Grid& g = c.g;
+  g.dim = dim; g.nen = 1 << dim; g.P = 1; g.C = 1;
+  double box = 0.0;
+  for (int d = 0; d < 3; d++) {
+    g.n[d] = d < dim ? n[d] : 1;
+    g.nc[d] = d < dim ? n[d] - 1 : 1;
+    g.O[d] = d < dim ? O[d] : 0.0;
+    g.L[d] = d < dim ? L[d] : 0.0;
+    if (d < dim) {
+      if (n[d] < 2) return set_err(E_RANGE, fn, "-pu_mesh_n must be >= 2 per axis");
+      if (!(L[d] > O[d])) return set_err(E_RANGE, fn, "-pu_mesh_L must exceed -pu_mesh_O");
+      g.h[d] = (L[d] - O[d]) / (double)(n[d] - 1);
+      g.P *= n[d];
+      g.C *= n[d] - 1;
+      box = std::max(box, L[d] - O[d]);
+    } else {
+      g.h[d] = 1.0;
+    }
+  }
+  // PartitionOfUnityComputeSupport / ComputeShapeFactor, partition_of_unity.c:322-355
+  double ov = overlap * box;
+  double radius = ov / std::pow((double)g.P, 1.0 / dim);
+  g.eps_w = 1.0 / radius;
+  g.eps_r = shape * g.eps_w;
+  // LagRBFSetFromOptions, lagrangian_rbf.c:135-195
+  o.get_int("n_interpolation_fields", c.nf, bad);
+  o.get_int("kernel_type", c.ktype, bad);
+  o.get_int("solve_method", c.solve_method, bad);
+  o.get_real("rmin", c.rmin, bad);
+  o.get_real("rmax", c.rmax, bad);
+  o.get_real("cfl_fac", c.cfl_fac, bad);
+  o.get_bool("advection", c.advection, bad);
+  if (bad) return set_err(E_WRONG, fn, "malformed option value");
+  if (c.ktype < 0 || c.ktype > 6) return set_err(E_RANGE, fn, "Unknown kernel type %d", c.ktype);
+  if (c.solve_method < 0 || c.solve_method > 2) return set_err(E_RANGE, fn, "Unknown method type: %d", c.solve_method);
+  if (c.nf < 0) return set_err(E_RANGE, fn, "-n_interpolation_fields must be >= 0");
+  if (c.advection) {
+    c.scheme = 3;
+    o.get_int("advection_scheme", c.scheme, bad);
+    if (c.scheme < 0 || c.scheme > 3) return set_err(E_WRONG, fn, "Advection scheme %d Create function not found", c.scheme);
+    static const int ord[4] = {1, 2, 4, 7};
+    c.order = ord[c.scheme];
+    o.get_string("velocity_name", c.velocity_name);
+    o.get_real("dt", c.adv_dt, bad);
+    o.get_real("dopri_dt_fac", c.dopri_fac, bad);
+    o.get_real("dopri_dt_fac_min", c.dopri_fac_min, bad);
+    o.get_real("dopri_dt_fac_max", c.dopri_fac_max, bad);
+    o.get_real("dopri_atol", c.dopri_atol, bad);
+    o.get_real("dopri_rtol", c.dopri_rtol, bad);
+    if (bad) return set_err(E_WRONG, fn, "malformed advection option value");
+  }
+  o.get_int("lagrbf_device", c.device, bad);
+  return 0;
+}
+
+static int ctx_init_device(Ctx& c) {
+  const char* fn = "lagrbf_create";
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0)
+    return set_err(E_LIB, fn, "no HIP device available: liblagrbf runs on the GPU only");
+  if (c.device < 0 || c.device >= ndev) return set_err(E_RANGE, fn, "lagrbf_device %d out of range", c.device);
+  HIPCHK(hipSetDevice(c.device));
+  HIPCHK(hipStreamCreateWithFlags(&c.s, hipStreamNonBlocking));
+  // PU node coordinates per axis, x_i = O + i*h (DMDASetUniformCoordinates)
+  std::vector<double> ax[3];
+  for (int d = 0; d < 3; d++) {
+    ax[d].resize(c.g.n[d]);
+    for (int i = 0; i < c.g.n[d]; i++) ax[d][i] = c.g.O[d] + i * c.g.h[d];
+  }
+  HIPCHK(c.pux.reserve(c.g.n[0])); HIPCHK(c.puy.reserve(c.g.n[1])); HIPCHK(c.puz.reserve(c.g.n[2]));
+  HIPCHK(hipMemcpy(c.pux.p, ax[0].data(), sizeof(double) * c.g.n[0], hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c.puy.p, ax[1].data(), sizeof(double) * c.g.n[1], hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(c.puz.p, ax[2].data(), sizeof(double) * c.g.n[2], hipMemcpyHostToDevice));
+  const int P = c.g.P;
+  c.h_setup.assign(P, 0); c.h_solved.assign(P, 0); c.h_active.assign(P, 0);
+  c.h_n.assign(P, 0); c.h_mem_off.assign(P, 0); c.h_fac_off.assign(P, 0); c.h_pcount.assign(P, 0);
+  HIPCHK(c.p_n.reserve(P)); HIPCHK(c.p_active.reserve(P)); HIPCHK(c.p_mem_off.reserve(P));
+  HIPCHK(c.p_fac_off.reserve(P)); HIPCHK(c.touched.reserve(P)); HIPCHK(c.list.reserve(P)); HIPCHK(c.info.reserve(P));
+  HIPCHK(c.pcount.reserve(P));
+  HIPCHK(hipMemset(c.p_n.p, 0, sizeof(int32_t) * P));
+  HIPCHK(hipMemset(c.p_active.p, 0, P));
+  HIPCHK(hipMemset(c.p_mem_off.p, 0, sizeof(int64_t) * P));
+  HIPCHK(hipMemset(c.p_fac_off.p, 0, sizeof(int64_t) * P));
+  HIPCHK(c.cell_start.reserve(c.g.C + 1));
+  HIPCHK(hipMemset(c.cell_start.p, 0, sizeof(int32_t) * (c.g.C + 1)));
+  HIPCHK(c.counters.reserve(4));
+  HIPCHK(hipMemset(c.counters.p, 0, sizeof(unsigned long long) * 4));
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+static int check_bbox(const Ctx& c, int n, const double* x, const char* fn) {
+  const int dim = c.g.dim;
+  for (int p = 0; p < n; p++)
+    for (int d = 0; d < dim; d++) {
+      double v = x[(size_t)p * dim + d];
+      // DMSwarmAssignRankFromDMDA, swarm_utils.c:862-870 (NaN rejected too)
+      if (!(v >= c.g.O[d] && v <= c.g.L[d]))
+        return set_err(E_WRONG, fn, "Point %d, with coordinates (%g,%g) does not belong to any DMDA local bounding box.",
+                       p, x[(size_t)p * dim], x[(size_t)p * dim + 1]);
+    }
+  return 0;
+}
+
+static int fill_data(Ctx& c, int n, const double* coords, const double* values, const int64_t* pidx) {
+  const char* fn = "FillDataSwarm";
+  const int dim = c.g.dim, nf = c.nf;
+  CHK(check_bbox(c, n, coords, fn));
+  if (c.setup_N >= 0 && c.solve_method != 2 && n != c.setup_N && c.mem_used > 0)
+    return set_err(E_SIZ, fn, "solve_method %d keeps patch factors across data fills; node count changed from %d to %d "
+                   "(call with solve_method 2 or destroy the context)", c.solve_method, c.setup_N, n);
+  c.N = n;
+  HIPCHK(c.dx.reserve((size_t)n * dim));
+  HIPCHK(c.dval.reserve((size_t)n * std::max(nf, 1)));
+  HIPCHK(c.dcell.reserve(n));
+  HIPCHK(c.cell_nodes.reserve(n));
+  const int C = c.g.C;
+  HIPCHK(c.scratch.reserve((size_t)C + 1 + (size_t)std::max(n, 1) + (size_t)(C + 1) / 1024 + 64));
+  if (n > 0) {
+    HIPCHK(hipMemcpyAsync(c.dx.p, coords, sizeof(double) * (size_t)n * dim, hipMemcpyHostToDevice, c.s));
+    if (values && nf > 0) HIPCHK(hipMemcpyAsync(c.dval.p, values, sizeof(double) * (size_t)n * nf, hipMemcpyHostToDevice, c.s));
+    else if (nf > 0) HIPCHK(hipMemsetAsync(c.dval.p, 0, sizeof(double) * (size_t)n * nf, c.s));
+  }
+  c.h_dpid.resize(n);
+  for (int i = 0; i < n; i++) c.h_dpid[i] = pidx ? pidx[i] : (int64_t)i + 1;
+  // DMSwarmPICSetCellId + CellInfoSetUp: node -> cell, per-cell node lists
+  launch_locate_aos(c.g, c.dx.p, n, c.dcell.p, c.s);
+  launch_cell_lists(c.dcell.p, n, C, c.cell_start.p, c.cell_nodes.p, c.scratch.p, c.s);
+  launch_patch_count(c.g, c.cell_start.p, c.pcount.p, c.s);
+  HIPCHK(hipMemcpyAsync(c.h_pcount.data(), c.pcount.p, sizeof(int32_t) * c.g.P, hipMemcpyDeviceToHost, c.s));
+  HIPCHK(hipStreamSynchronize(c.s));
+  c.data_filled = true;
+  return 0;
+}
+
+static int alloc_particles(Ctx& c) {
+  const int dim = c.g.dim;
+  const size_t ld = (size_t)c.ld;
+  HIPCHK(c.px.reserve(ld * dim));
+  HIPCHK(c.w.reserve(ld * c.g.nen));
+  HIPCHK(c.fields.reserve(ld * std::max(c.nf, 1)));
+  HIPCHK(c.pcell.reserve(ld));
+  HIPCHK(c.alive.reserve(ld));
+  HIPCHK(c.vel.reserve(ld * dim));
+  if (c.advection) {
+    HIPCHK(c.x0.reserve(ld * dim));
+    for (int s = 0; s < c.order; s++) HIPCHK(c.vadv[s].reserve(ld * dim));
+  }
+  return 0;
+}
+
+__global__ void k_fill_i32(int32_t* a, int n, int32_t v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+static int fill_points(Ctx& c, int n, const double* coords, const double* values, const int64_t* pidx) {
+  const char* fn = "FillInterpSwarm";
+  const int dim = c.g.dim, nf = c.nf;
+  CHK(check_bbox(c, n, coords, fn));
+  c.Np = n;
+  c.ld = ((int64_t)n + 63) / 64 * 64;
+  if (c.ld == 0) c.ld = 64;
+  CHK(alloc_particles(c));
+  DBuf<double> tmp;
+  HIPCHK(tmp.reserve((size_t)std::max(n, 1) * std::max(dim, std::max(nf, 1))));
+  if (n > 0) {
+    HIPCHK(hipMemcpyAsync(tmp.p, coords, sizeof(double) * (size_t)n * dim, hipMemcpyHostToDevice, c.s));
+    launch_aos_to_soa(tmp.p, n, dim, c.px.p, c.ld, c.s);
+    HIPCHK(hipMemsetAsync(c.fields.p, 0, sizeof(double) * c.ld * std::max(nf, 1), c.s));
+    if (values && nf > 0) {
+      HIPCHK(hipMemcpyAsync(tmp.p, values, sizeof(double) * (size_t)n * nf, hipMemcpyHostToDevice, c.s));
+      launch_aos_to_soa(tmp.p, n, nf, c.fields.p, c.ld, c.s);
+    }
+    HIPCHK(hipMemsetAsync(c.w.p, 0, sizeof(double) * c.ld * c.g.nen, c.s));
+    HIPCHK(hipMemsetAsync(c.vel.p, 0, sizeof(double) * c.ld * dim, c.s));
+    hipLaunchKernelGGL(k_fill_i32, dim3((n + 255) / 256), dim3(256), 0, c.s, c.alive.p, n, 1);
+  }
+  c.h_ppid.resize(n);
+  for (int i = 0; i < n; i++) c.h_ppid[i] = pidx ? pidx[i] : (int64_t)i + 1;
+  // DMSwarmPICSetCellId + SwarmInterpSetWeights (interface_utils.c:118-138)
+  launch_locate_particles(c.g, c.px.p, c.ld, n, c.pcell.p, c.alive.p, c.counters.p, c.s);
+  launch_weights(c.g, c.pux.p, c.puy.p, c.puz.p, c.px.p, c.ld, n, c.pcell.p, c.alive.p, c.w.p, c.s);
+  HIPCHK(hipStreamSynchronize(c.s));
+  c.points_filled = true;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// LagRBF_PartitionsSetUp (partition.c:417-478): touch the corner patches of every
+// particle's cell, set up (gather + assemble + factor) the new ones, solve the
+// unsolved ones.  Batched: one launch per phase over the patch list.
+static int partitions_setup(Ctx& c) {
+  const char* fn = "LagRBF_PartitionsSetUp";
+  const int P = c.g.P;
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0)); HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, c.s));
+  HIPCHK(hipMemsetAsync(c.touched.p, 0, P, c.s));
+  launch_touch(c.g, c.pcell.p, c.alive.p, c.Np, c.touched.p, c.s);
+  std::vector<uint8_t> t(P);
+  HIPCHK(hipMemcpyAsync(t.data(), c.touched.p, P, hipMemcpyDeviceToHost, c.s));
+  HIPCHK(hipStreamSynchronize(c.s));
+  std::vector<int32_t> newl, solvel;
+  size_t need_mem = c.mem_used, need_fac = c.fac_used;
+  for (int p = 0; p < P; p++) {
+    if (!t[p]) continue;
+    if (!c.h_setup[p]) {
+      const int n = c.data_filled ? c.h_pcount[p] : 0;
+      c.h_n[p] = n;
+      c.h_mem_off[p] = (int64_t)need_mem;
+      c.h_fac_off[p] = (int64_t)need_fac;
+      need_mem += n;
+      need_fac += (size_t)n * n;
+      newl.push_back(p);
+      c.max_patch = std::max(c.max_patch, n);
+    }
+  }
+  if (!newl.empty()) {
+    HIPCHK(c.mem_pool.grow(need_mem + 1, c.mem_used, c.s));
+    HIPCHK(c.piv_pool.grow(need_mem + 1, c.mem_used, c.s));
+    HIPCHK(c.coef_pool.grow((need_mem + 1) * std::max(c.nf, 1), c.mem_used * std::max(c.nf, 1), c.s));
+    HIPCHK(c.fac_pool.grow(need_fac + 1, c.fac_used, c.s));
+    c.mem_used = need_mem;
+    c.fac_used = need_fac;
+    HIPCHK(hipMemcpyAsync(c.p_n.p, c.h_n.data(), sizeof(int32_t) * P, hipMemcpyHostToDevice, c.s));
+    HIPCHK(hipMemcpyAsync(c.p_mem_off.p, c.h_mem_off.data(), sizeof(int64_t) * P, hipMemcpyHostToDevice, c.s));
+    HIPCHK(hipMemcpyAsync(c.p_fac_off.p, c.h_fac_off.data(), sizeof(int64_t) * P, hipMemcpyHostToDevice, c.s));
+    HIPCHK(hipMemcpyAsync(c.list.p, newl.data(), sizeof(int32_t) * newl.size(), hipMemcpyHostToDevice, c.s));
+    launch_patch_gather(c.g, c.list.p, (int)newl.size(), c.cell_start.p, c.cell_nodes.p, c.p_mem_off.p, c.mem_pool.p, c.s);
+    launch_factor_bk(c.g, c.ktype, c.list.p, (int)newl.size(), c.p_n.p, c.p_mem_off.p, c.p_fac_off.p, c.mem_pool.p,
+                     c.dx.p, c.fac_pool.p, c.piv_pool.p, c.info.p, c.s);
+    for (int p : newl) c.h_setup[p] = 1;
+    c.patches_setup += (double)newl.size();
+    c.setup_N = c.N;
+    HIPCHK(hipStreamSynchronize(c.s));   // list buffer is reused below
+  }
+  for (int p = 0; p < P; p++)
+    if (t[p] && c.h_setup[p] && !c.h_solved[p] && c.h_n[p] > 0) solvel.push_back(p);
+  if (!solvel.empty()) {
+    HIPCHK(hipMemcpyAsync(c.list.p, solvel.data(), sizeof(int32_t) * solvel.size(), hipMemcpyHostToDevice, c.s));
+    launch_solve_bk(c.list.p, (int)solvel.size(), c.p_n.p, c.p_mem_off.p, c.p_fac_off.p, c.mem_pool.p, c.fac_pool.p,
+                    c.piv_pool.p, c.dval.p, c.nf, c.coef_pool.p, c.s);
+  }
+  for (int p : solvel) c.h_solved[p] = 1;
+  for (int p = 0; p < P; p++) if (t[p] && c.h_setup[p] && c.h_n[p] == 0) c.h_solved[p] = 1;
+  bool changed = false;
+  for (int p = 0; p < P; p++) {
+    uint8_t a = (c.h_setup[p] && c.h_n[p] > 0) ? 1 : 0;
+    if (a != c.h_active[p]) { c.h_active[p] = a; changed = true; }
+  }
+  if (changed) HIPCHK(hipMemcpyAsync(c.p_active.p, c.h_active.data(), P, hipMemcpyHostToDevice, c.s));
+  HIPCHK(hipEventRecord(e1, c.s));
+  HIPCHK(hipStreamSynchronize(c.s));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  c.setup_ms += ms;
+  HIPCHK(hipEventDestroy(e0)); HIPCHK(hipEventDestroy(e1));
+  (void)fn;
+  return 0;
+}
+
+// LagRBF_PartitionsReset, partition.c:494-524
+static int partitions_reset(Ctx& c) {
+  const int P = c.g.P;
+  if (c.solve_method == 2) {
+    std::fill(c.h_setup.begin(), c.h_setup.end(), 0);
+    std::fill(c.h_solved.begin(), c.h_solved.end(), 0);
+    std::fill(c.h_n.begin(), c.h_n.end(), 0);
+    std::fill(c.h_active.begin(), c.h_active.end(), 0);
+    c.mem_used = 0; c.fac_used = 0; c.setup_N = -1;
+    HIPCHK(hipMemsetAsync(c.p_active.p, 0, P, c.s));
+    HIPCHK(hipMemsetAsync(c.p_n.p, 0, sizeof(int32_t) * P, c.s));
+  } else {
+    std::fill(c.h_solved.begin(), c.h_solved.end(), 0);
+  }
+  return 0;
+}
+
+// RBFInterpolation(ctx, swi, swd, name), interpolation.c:178-272
+static int rbf_interpolation(Ctx& c, double* out, int nfields) {
+  if (nfields <= 0 || c.Np == 0) return 0;
+  PatchView pv{c.p_n.p, c.p_active.p, c.p_mem_off.p, c.mem_pool.p, c.coef_pool.p, c.nf};
+  if (!c.mem_pool.p) {
+    // no patch was ever set up: every patch is inactive, nothing to evaluate but the
+    // kernel still needs valid (unused) pool pointers
+    HIPCHK(c.mem_pool.reserve(1)); HIPCHK(c.coef_pool.reserve(1));
+    pv.members = c.mem_pool.p; pv.coef = c.coef_pool.p;
+  }
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0)); HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, c.s));
+  launch_eval_ref(c.g, c.ktype, c.px.p, c.ld, c.Np, c.pcell.p, c.alive.p, c.w.p, pv, c.dx.p ? c.dx.p : c.pux.p,
+                  c.cell_start.p, nfields, out, c.s);
+  HIPCHK(hipEventRecord(e1, c.s));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  c.eval_ms += ms;
+  HIPCHK(hipEventDestroy(e0)); HIPCHK(hipEventDestroy(e1));
+  return 0;
+}
+
+static int interpolation(Ctx& c) {
+  const char* fn = "Interface_Interpolation";
+  if (!c.points_filled) return set_err(E_NULL, fn, "Interpolation swarm (ctx->swi) is empty: call lagrbf_fill_swarm first");
+  CHK(partitions_setup(c));
+  CHK(rbf_interpolation(c, c.fields.p, c.nf));
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Advection
+static int eval_callback(Ctx& c) {
+  const char* fn = "AdvectionEvaluateVelocityFunction";
+  const int dim = c.g.dim, Np = c.Np;
+  if (!c.vfn && !c.vfn_batch) return set_err(E_NULL, fn, "AdvectionCtx->velocity_fn is NULL, please provide a valid function");
+  std::vector<double> xs((size_t)dim * c.ld), vs((size_t)dim * c.ld, 0.0);
+  std::vector<int32_t> al(Np);
+  if (Np == 0) return 0;
+  HIPCHK(hipMemcpyAsync(xs.data(), c.px.p, sizeof(double) * dim * c.ld, hipMemcpyDeviceToHost, c.s));
+  HIPCHK(hipMemcpyAsync(vs.data(), c.vel.p, sizeof(double) * dim * c.ld, hipMemcpyDeviceToHost, c.s));
+  HIPCHK(hipMemcpyAsync(al.data(), c.alive.p, sizeof(int32_t) * Np, hipMemcpyDeviceToHost, c.s));
+  HIPCHK(hipStreamSynchronize(c.s));
+  std::vector<double> X, V;
+  std::vector<int> idx;
+  for (int i = 0; i < Np; i++) if (al[i]) idx.push_back(i);
+  const int m = (int)idx.size();
+  X.resize((size_t)m * dim + 1); V.resize((size_t)m * dim + 1, 0.0);
+  for (int q = 0; q < m; q++) for (int d = 0; d < dim; d++) X[(size_t)q * dim + d] = xs[(size_t)d * c.ld + idx[q]];
+  int err = 0;
+  if (c.vfn_batch) err = c.vfn_batch(m, dim, X.data(), V.data(), c.vuser);
+  else for (int q = 0; q < m && !err; q++) err = c.vfn(&X[(size_t)q * dim], &V[(size_t)q * dim], c.vuser);
+  if (err) return set_err(err, fn, "velocity callback returned %d", err);
+  for (int q = 0; q < m; q++) for (int d = 0; d < dim; d++) vs[(size_t)d * c.ld + idx[q]] = V[(size_t)q * dim + d];
+  HIPCHK(hipMemcpyAsync(c.vel.p, vs.data(), sizeof(double) * dim * c.ld, hipMemcpyHostToDevice, c.s));
+  HIPCHK(hipStreamSynchronize(c.s));
+  return 0;
+}
+
+// AdvectionEvaluateVelocity, advection.c:391-414
+static int eval_velocity(Ctx& c) {
+  if (c.analytical) return eval_callback(c);
+  return rbf_interpolation(c, c.vel.p, c.g.dim);
+}
+
+// AdvectionInitializeTimestepVelocity, advection.c:353-373
+static int init_step_velocity(Ctx& c) {
+  if (c.analytical) return eval_callback(c);
+  launch_copy_rows(c.fields.p, c.ld, c.vel.p, c.ld, c.g.dim, c.Np, c.alive.p, c.s);
+  return 0;
+}
+
+static int cfl_dt(Ctx& c, double& dt) {
+  unsigned long long init = (unsigned long long)0x7fefffffffffffffull;   // DBL_MAX bits
+  HIPCHK(hipMemcpyAsync(c.counters.p + 1, &init, sizeof(init), hipMemcpyHostToDevice, c.s));
+  launch_cfl(c.vel.p, c.ld, c.Np, c.g.dim, c.alive.p, c.cfl_fac * (1.0 / c.g.eps_w), c.counters.p + 1, c.s);
+  unsigned long long bits = 0;
+  HIPCHK(hipMemcpyAsync(&bits, c.counters.p + 1, sizeof(bits), hipMemcpyDeviceToHost, c.s));
+  HIPCHK(hipStreamSynchronize(c.s));
+  std::memcpy(&dt, &bits, sizeof(dt));
+  return 0;
+}
+
+static StageArgs stage_args(int nv, const double* const* v, const double* a, double scale, int project, int proj_in_loop,
+                            int write_vel, const Ctx& c) {
+  StageArgs s{};
+  s.nv = nv;
+  for (int j = 0; j < nv; j++) { s.v[j] = v[j]; s.a[j] = a[j]; }
+  s.scale = scale; s.project = project; s.proj_in_loop = proj_in_loop; s.write_vel = write_vel;
+  s.rmin = c.rmin; s.rmax = c.rmax;
+  return s;
+}
+
+static void stage(Ctx& c, const double* x0, const StageArgs& a) {
+  launch_stage(c.g, c.pux.p, c.puy.p, c.puz.p, c.px.p, x0, c.vel.p, c.ld, c.Np, c.pcell.p, c.alive.p, c.counters.p,
+               c.w.p, a, c.s);
+}
+
+static int advect_swarm(Ctx& c, double dt) {
+  const int dim = c.g.dim;
+  const size_t bytes = sizeof(double) * dim * c.ld;
+  switch (c.scheme) {
+    case 0: {  // runge_kutta_1.c:37-75
+      const double* v[1] = {c.vel.p};
+      const double a[1] = {1.0};
+      stage(c, c.px.p, stage_args(1, v, a, dt, 0, 0, 0, c));
+      return 0;
+    }
+    case 1: {  // runge_kutta_2.c:38-91
+      launch_save(c.px.p, c.x0.p, c.vel.p, nullptr, c.ld, c.Np, dim, c.alive.p, c.s);
+      const double* v[1] = {c.vel.p};
+      const double a[1] = {1.0};
+      stage(c, c.x0.p, stage_args(1, v, a, 0.5 * dt, 0, 0, 0, c));
+      CHK(eval_velocity(c));
+      stage(c, c.x0.p, stage_args(1, v, a, dt, 0, 0, 0, c));
+      return 0;
+    }
+    case 2: {  // runge_kutta_4.c:45-156
+      static const double cc[3] = {0.5, 0.5, 1.0};
+      static const double b[4] = {1.0, 2.0, 2.0, 1.0};
+      launch_save(c.px.p, c.x0.p, c.vel.p, c.vadv[0].p, c.ld, c.Np, dim, c.alive.p, c.s);
+      for (int s = 0; s < 3; s++) {
+        const double* v[1] = {c.vadv[s].p};
+        const double a[1] = {1.0};
+        stage(c, c.x0.p, stage_args(1, v, a, cc[s] * dt, 1, 0, 1, c));
+        CHK(eval_velocity(c));
+        HIPCHK(hipMemcpyAsync(c.vadv[s + 1].p, c.vel.p, bytes, hipMemcpyDeviceToDevice, c.s));
+      }
+      const double* v[4] = {c.vadv[0].p, c.vadv[1].p, c.vadv[2].p, c.vadv[3].p};
+      stage(c, c.x0.p, stage_args(4, v, b, dt / 6.0, 1, 1, 0, c));
+      return 0;
+    }
+    default: {  // dormand_prince.c:216-310
+      static const double A[6][6] = {
+          {1.0 / 5.0, 0, 0, 0, 0, 0},
+          {3.0 / 40.0, 9.0 / 40.0, 0, 0, 0, 0},
+          {44.0 / 45.0, -56.0 / 15.0, 32.0 / 9.0, 0, 0, 0},
+          {19372.0 / 6561.0, -25360.0 / 2187.0, 64448.0 / 6561.0, -212.0 / 729.0, 0, 0},
+          {9017.0 / 3168.0, -355.0 / 33.0, 46732.0 / 5247.0, 49.0 / 176.0, -5103.0 / 18656.0, 0},
+          {35.0 / 384.0, 0, 500.0 / 1113.0, 125.0 / 192.0, -2187.0 / 6784.0, 11.0 / 84.0}};
+      const double dtd = c.adv_dt;
+      launch_save(c.px.p, c.x0.p, c.vel.p, c.vadv[0].p, c.ld, c.Np, dim, c.alive.p, c.s);
+      for (int s = 0; s < 6; s++) {
+        const double* v[7];
+        for (int j = 0; j <= s; j++) v[j] = c.vadv[j].p;
+        stage(c, c.x0.p, stage_args(s + 1, v, A[s], dtd, 1, 0, 1, c));
+        CHK(eval_velocity(c));
+        HIPCHK(hipMemcpyAsync(c.vadv[s + 1].p, c.vel.p, bytes, hipMemcpyDeviceToDevice, c.s));
+      }
+      const double* vv[7];
+      for (int j = 0; j < 7; j++) vv[j] = c.vadv[j].p;
+      unsigned long long zero[2] = {0ull, 0ull};
+      HIPCHK(hipMemcpyAsync(c.counters.p + 1, zero, sizeof(zero), hipMemcpyHostToDevice, c.s));
+      launch_dopri_err(c.px.p, c.x0.p, vv, c.ld, c.Np, dim, c.alive.p, dtd, c.dopri_atol, c.dopri_rtol, c.counters.p + 1,
+                       reinterpret_cast<unsigned int*>(c.counters.p + 2), c.s);
+      unsigned long long res[2];
+      HIPCHK(hipMemcpyAsync(res, c.counters.p + 1, sizeof(res), hipMemcpyDeviceToHost, c.s));
+      HIPCHK(hipStreamSynchronize(c.s));
+      double max_err;
+      if ((unsigned int)res[1] == 0) max_err = -DBL_MAX;
+      else std::memcpy(&max_err, &res[0], sizeof(double));
+      c.dopri_err = max_err;
+      // AdvectionComputeAdaptiveTimeStep_DOPRI, dormand_prince.c:179-214 (PetscMin/PetscMax macros)
+      const double fac_max = max_err <= 1.0 ? c.dopri_fac_max : 1.0;
+      const double g = c.dopri_fac * std::pow(1.0 / max_err, 1.0 / 5.0);
+      const double inner = (c.dopri_fac_min < g) ? g : c.dopri_fac_min;
+      const double f = (fac_max < inner) ? fac_max : inner;
+      c.adv_dt = dtd * f;
+      return 0;
+    }
+  }
+}
+
+static int update_removed(Ctx& c) {
+  unsigned long long r = 0;
+  HIPCHK(hipMemcpyAsync(&r, c.counters.p, sizeof(r), hipMemcpyDeviceToHost, c.s));
+  HIPCHK(hipStreamSynchronize(c.s));
+  c.removed = (double)r;
+  return 0;
+}
+
+// InternalForwardAdvectionLoop, time_loop.c:39-100
+static int advect_interpolate(Ctx& c, double t, double t_next) {
+  const char* fn = "InternalForwardAdvectionLoop";
+  if (!c.advection) return set_err(E_NULL, fn, "Advection context is NULL (set -advection)");
+  if (!c.points_filled) return set_err(E_NULL, fn, "Interpolation swarm (ctx->swi) is NULL");
+  if (!c.analytical && c.nf < c.g.dim) return set_err(E_SIZ, fn, "n_interpolation_fields (%d) < dim (%d)", c.nf, c.g.dim);
+  CHK(init_step_velocity(c));
+  while (t < t_next) {
+    double dt, dtc;
+    CHK(cfl_dt(c, dtc));
+    if (c.scheme == 3) dt = (c.adv_dt < dtc) ? c.adv_dt : dtc;
+    else dt = dtc;
+    if (t + dt > t_next) dt = t_next - t;
+    c.adv_dt = dt;
+    CHK(advect_swarm(c, dt));
+    if (!c.analytical) {
+      CHK(partitions_setup(c));
+      CHK(rbf_interpolation(c, c.fields.p, c.nf));
+    }
+    CHK(init_step_velocity(c));
+    t += dt;
+  }
+  CHK(update_removed(c));
+  if (!c.analytical) {
+    CHK(partitions_reset(c));
+    // CellInfoReset + DMSwarmSetLocalSizes(swarm_data,0,-1), time_loop.c:94-98
+    c.N = 0;
+    c.data_filled = false;
+    HIPCHK(hipMemsetAsync(c.cell_start.p, 0, sizeof(int32_t) * (c.g.C + 1), c.s));
+    HIPCHK(hipStreamSynchronize(c.s));
+  }
+  return 0;
+}
+
+// ForwardAnalyticalAdvection, time_loop.c:176-221
+static int advect_analytical(Ctx& c, double t, double t_next) {
+  const char* fn = "ForwardAnalyticalAdvection";
+  if (!c.advection) return set_err(E_NULL, fn, "Advection context is NULL (set -advection)");
+  if (!c.points_filled) return set_err(E_NULL, fn, "Interpolation swarm (ctx->swi) is NULL");
+  if (!c.vfn && !c.vfn_batch) return set_err(E_NULL, fn, "AdvectionCtx->velocity_fn is NULL, please provide a valid function");
+  c.analytical = true;
+  CHK(init_step_velocity(c));
+  while (t < t_next) {
+    double dt, dtc;
+    CHK(cfl_dt(c, dtc));
+    if (c.scheme == 3) dt = (c.adv_dt < dtc) ? c.adv_dt : dtc;
+    else dt = dtc;
+    if (t + dt > t_next) dt = t_next - t;
+    c.adv_dt = dt;
+    CHK(advect_swarm(c, dt));
+    CHK(init_step_velocity(c));
+    t += dt;
+  }
+  return update_removed(c);
+}
+
+static int advect_steps(Ctx& c, int nsteps, double dt) {
+  const char* fn = "lagrbf_advect_steps";
+  if (!c.advection) return set_err(E_NULL, fn, "Advection context is NULL (set -advection)");
+  if (!c.points_filled) return set_err(E_NULL, fn, "Interpolation swarm (ctx->swi) is NULL");
+  if (!c.analytical && c.nf < c.g.dim) return set_err(E_SIZ, fn, "n_interpolation_fields (%d) < dim (%d)", c.nf, c.g.dim);
+  if (nsteps < 0) return set_err(E_RANGE, fn, "nsteps must be >= 0");
+  CHK(init_step_velocity(c));
+  for (int s = 0; s < nsteps; s++) {
+    c.adv_dt = dt;
+    CHK(advect_swarm(c, dt));
+    if (!c.analytical) {
+      CHK(partitions_setup(c));
+      CHK(rbf_interpolation(c, c.fields.p, c.nf));
+    }
+    CHK(init_step_velocity(c));
+  }
+  return update_removed(c);
+}
+
+// lagrbf_output: rows by pid - 1 (AssembleArray_Allreduce, interface_utils.c:188-263)
+static int output(Ctx& c, bool is_data, int n, double* coords, double* fields, int64_t* pidx, const char* fieldname) {
+  const char* fn = "Interface_Output_Allreduce";
+  const int dim = c.g.dim;
+  HIPCHK(hipStreamSynchronize(c.s));
+  if (is_data) {
+    const int N = c.N;
+    std::vector<double> xs((size_t)N * dim), vs((size_t)N * std::max(c.nf, 1));
+    if (N > 0 && coords) HIPCHK(hipMemcpy(xs.data(), c.dx.p, sizeof(double) * N * dim, hipMemcpyDeviceToHost));
+    if (N > 0 && fields && c.nf > 0) HIPCHK(hipMemcpy(vs.data(), c.dval.p, sizeof(double) * N * c.nf, hipMemcpyDeviceToHost));
+    if (fields && fieldname && strcmp(fieldname, "fields")) return set_err(E_WRONG, fn, "data swarm has no field '%s'", fieldname);
+    if (coords) memset(coords, 0, sizeof(double) * (size_t)n * dim);
+    if (fields) memset(fields, 0, sizeof(double) * (size_t)n * c.nf);
+    if (pidx) memset(pidx, 0, sizeof(int64_t) * (size_t)n);
+    for (int p = 0; p < N; p++) {
+      int64_t g = c.h_dpid[p] - 1;
+      if (g < 0 || g >= n) return set_err(E_RANGE, fn, "pid %lld out of range [0, %d)", (long long)g, n);
+      if (coords) for (int d = 0; d < dim; d++) coords[g * dim + d] = xs[(size_t)p * dim + d];
+      if (fields) for (int f = 0; f < c.nf; f++) fields[g * c.nf + f] = vs[(size_t)p * c.nf + f];
+      if (pidx) pidx[g] = c.h_dpid[p];
+    }
+    return 0;
+  }
+  const int Np = c.Np;
+  const double* src = nullptr;
+  int nfl = 0;
+  if (fields) {
+    if (!fieldname || !strcmp(fieldname, "fields")) { src = c.fields.p; nfl = c.nf; }
+    else if (!strcmp(fieldname, kVelocity) || (c.advection && c.velocity_name == fieldname)) { src = c.vel.p; nfl = dim; }
+    else if (!strcmp(fieldname, "weights")) { src = c.w.p; nfl = c.g.nen; }
+    else return set_err(E_WRONG, fn, "unknown field '%s'", fieldname);
+  }
+  std::vector<double> xs((size_t)dim * c.ld), fs((size_t)std::max(nfl, 1) * c.ld);
+  std::vector<int32_t> al(std::max(Np, 1));
+  if (Np > 0) {
+    if (coords) HIPCHK(hipMemcpy(xs.data(), c.px.p, sizeof(double) * dim * c.ld, hipMemcpyDeviceToHost));
+    if (fields && nfl > 0) HIPCHK(hipMemcpy(fs.data(), src, sizeof(double) * nfl * c.ld, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(al.data(), c.alive.p, sizeof(int32_t) * Np, hipMemcpyDeviceToHost));
+  }
+  if (coords) memset(coords, 0, sizeof(double) * (size_t)n * dim);
+  if (fields) memset(fields, 0, sizeof(double) * (size_t)n * nfl);
+  if (pidx) memset(pidx, 0, sizeof(int64_t) * (size_t)n);
+  for (int p = 0; p < Np; p++) {
+    if (!al[p]) continue;
+    int64_t g = c.h_ppid[p] - 1;
+    if (g < 0 || g >= n) return set_err(E_RANGE, fn, "pid %lld out of range [0, %d)", (long long)g, n);
+    if (coords) for (int d = 0; d < dim; d++) coords[g * dim + d] = xs[(size_t)d * c.ld + p];
+    if (fields) for (int f = 0; f < nfl; f++) fields[g * nfl + f] = fs[(size_t)f * c.ld + p];
+    if (pidx) pidx[g] = c.h_ppid[p];
+  }
+  return 0;
+}
+
+static Ctx* create_from_view(const OptView& o) {
+  Ctx* c = new Ctx();
+  if (ctx_from_options(*c, o) || ctx_init_device(*c)) {
+    if (c->s) (void)hipStreamDestroy(c->s);
+    delete c;
+    return nullptr;
+  }
+  return c;
+}
+
+static OptView current_options() {
+  OptView v;
+  Options env;
+  if (const char* e = std::getenv("LAGRBF_OPTIONS")) env.insert_string(e);
+  v.db = env.snapshot();
+  for (auto& kv : global_options().snapshot()) v.db[kv.first] = kv.second;
+  return v;
+}
+
+}  // namespace lrb
+
+using namespace lrb;
+
+#define CTX_OR_NULL(fnname)                                                       \
+  Ctx* c = static_cast<Ctx*>(ctx);                                                \
+  if (!c) return set_err(E_NULL, fnname, "LagRBFCtx is NULL");                    \
+  if (hipSetDevice(c->device) != hipSuccess) return set_err(E_LIB, fnname, "hipSetDevice failed")
+
+extern "C" {
+
+void* lagrbf_create(void) {
+  OptView o = current_options();
+  return create_from_view(o);
+}
+
+void lagrbf_destroy(void* ctx) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  (void)hipStreamSynchronize(c->s);
+  (void)hipStreamDestroy(c->s);
+  delete c;
+}
+
+int lagrbf_fill_swarm(void* ctx, bool is_data, int32_t npoints, double* coords, double* values, int64_t* pidx) {
+  CTX_OR_NULL(is_data ? "FillDataSwarm" : "FillInterpSwarm");
+  if (npoints < 0) return set_err(E_RANGE, "lagrbf_fill_swarm", "npoints must be >= 0");
+  if (npoints > 0 && !coords) return set_err(E_NULL, "lagrbf_fill_swarm", "coords is NULL");
+  return is_data ? fill_data(*c, npoints, coords, values, pidx) : fill_points(*c, npoints, coords, values, pidx);
+}
+
+int lagrbf_interpolation(void* ctx) {
+  CTX_OR_NULL("Interface_Interpolation");
+  return interpolation(*c);
+}
+
+int lagrbf_output(void* ctx, bool is_data, int32_t npoints, double* coords, double* fields, int64_t* pidx,
+                  const char* fieldname) {
+  CTX_OR_NULL("Interface_Output_Allreduce");
+  if (npoints < 0) return set_err(E_RANGE, "lagrbf_output", "npoints must be >= 0");
+  return output(*c, is_data, npoints, coords, fields, pidx, fieldname);
+}
+
+int lagrbf_avect_analytical(void* ctx, double time, double time_next) {
+  CTX_OR_NULL("ForwardAnalyticalAdvection");
+  return advect_analytical(*c, time, time_next);
+}
+
+int lagrbf_advect_interpolate(void* ctx, double time, double time_next) {
+  CTX_OR_NULL("InternalForwardAdvectionLoop");
+  return advect_interpolate(*c, time, time_next);
+}
+
+int lagrbf_advection_function(void* ctx, int (*velocity_fn)(double*, double*, void*), void* fn_ctx) {
+  CTX_OR_NULL("AdvectionSetVelocityFunction");
+  if (!c->advection) return set_err(E_NULL, "AdvectionSetVelocityFunction", "Advection context is NULL");
+  c->vfn = velocity_fn; c->vuser = fn_ctx; c->analytical = true;
+  return 0;
+}
+
+int lagrbf_advection_function_batch(void* ctx, int (*fnb)(int32_t, int32_t, double*, double*, void*), void* fn_ctx) {
+  CTX_OR_NULL("AdvectionSetVelocityFunctionBatch");
+  if (!c->advection) return set_err(E_NULL, "AdvectionSetVelocityFunctionBatch", "Advection context is NULL");
+  c->vfn_batch = fnb; c->vuser = fn_ctx; c->analytical = true;
+  return 0;
+}
+
+int lagrbf_reset_coefficients(void* ctx) {
+  CTX_OR_NULL("LagRBF_PartitionsReset");
+  return partitions_reset(*c);
+}
+
+void lagrbf_interpolate_driver(int32_t ndata, double* dcoords, double* dvalues, int32_t npts, double* pcoords,
+                               double* pvalues, int32_t nglobal, char* options_file) {
+  OptView o = current_options();
+  if (options_file && options_file[0]) {
+    Options f;
+    int e = f.insert_file(options_file);
+    if (e) { set_err(e, "lagrbf_interpolate", "cannot read options file %s", options_file); return; }
+    for (auto& kv : f.snapshot()) o.db[kv.first] = kv.second;
+  }
+  Ctx* c = create_from_view(o);
+  if (!c) return;
+  int e = fill_data(*c, ndata, dcoords, dvalues, nullptr);
+  if (!e) e = fill_points(*c, npts, pcoords, nullptr, nullptr);
+  if (!e) e = interpolation(*c);
+  if (!e) e = output(*c, false, nglobal, nullptr, pvalues, nullptr, "fields");
+  lagrbf_destroy(c);
+}
+
+int lagrbf_set_option(const char* name, const char* value) {
+  if (!name) return set_err(E_NULL, "lagrbf_set_option", "name is NULL");
+  global_options().set(name, value ? value : "");
+  return 0;
+}
+
+int lagrbf_clear_options(void) {
+  global_options().clear();
+  return 0;
+}
+
+int lagrbf_options_insert_file(const char* path) {
+  if (!path) return set_err(E_NULL, "lagrbf_options_insert_file", "path is NULL");
+  int e = global_options().insert_file(path);
+  if (e) return set_err(e, "lagrbf_options_insert_file", "cannot read %s", path);
+  return 0;
+}
+
+int lagrbf_advect_steps(void* ctx, int32_t nsteps, double dt) {
+  CTX_OR_NULL("lagrbf_advect_steps");
+  return advect_steps(*c, nsteps, dt);
+}
+
+int lagrbf_get_info(void* ctx, const char* key, double* value) {
+  CTX_OR_NULL("lagrbf_get_info");
+  if (!key || !value) return set_err(E_NULL, "lagrbf_get_info", "key/value is NULL");
+  std::string k(key);
+  if (k == "removed") { int e = update_removed(*c); if (e) return e; *value = c->removed; }
+  else if (k == "fallback") *value = 0.0;
+  else if (k == "refine_iters") *value = 0.0;
+  else if (k == "patches_setup") *value = c->patches_setup;
+  else if (k == "dt") *value = c->adv_dt;
+  else if (k == "dopri_err") *value = c->dopri_err;
+  else if (k == "eval_ms") *value = c->eval_ms;
+  else if (k == "setup_ms") *value = c->setup_ms;
+  else if (k == "n_particles") *value = c->Np;
+  else if (k == "n_nodes") *value = c->N;
+  else if (k == "max_patch") *value = c->max_patch;
+  else if (k == "eps_w") *value = c->g.eps_w;
+  else if (k == "eps_r") *value = c->g.eps_r;
+  else return set_err(E_WRONG, "lagrbf_get_info", "unknown key '%s'", key);
+  return 0;
+}
+
+int64_t lagrbf_debug_get(void* ctx, const char* what, int32_t index, void* out, int64_t cap) {
+  Ctx* c = static_cast<Ctx*>(ctx);
+  if (!c || !what) return -set_err(E_NULL, "lagrbf_debug_get", "NULL argument");
+  if (hipSetDevice(c->device) != hipSuccess || hipStreamSynchronize(c->s) != hipSuccess) return -E_LIB;
+  std::string w(what);
+  const void* src = nullptr;
+  int64_t n = 0, esz = 4;
+  if (w == "cellid_points") { src = c->pcell.p; n = c->Np; }
+  else if (w == "cellid_data") { src = c->dcell.p; n = c->N; }
+  else if (w == "cell_start") { src = c->cell_start.p; n = c->g.C + 1; }
+  else if (w == "cell_nodes") { src = c->cell_nodes.p; n = c->N; }
+  else if (w == "alive") { src = c->alive.p; n = c->Np; }
+  else if (w == "patch_members" || w == "patch_coeffs") {
+    if (index < 0 || index >= c->g.P) return -set_err(E_RANGE, "lagrbf_debug_get", "patch %d out of range", index);
+    if (!c->h_setup[index]) return 0;
+    const int64_t off = c->h_mem_off[index];
+    n = c->h_n[index];
+    if (w == "patch_members") src = c->mem_pool.p + off;
+    else { src = c->coef_pool.p + off * c->nf; n *= c->nf; esz = 8; }
+  } else if (w == "patch_flags") {
+    if (index < 0 || index >= c->g.P) return -set_err(E_RANGE, "lagrbf_debug_get", "patch %d out of range", index);
+    if (out && cap >= 3) {
+      int32_t* o = static_cast<int32_t*>(out);
+      o[0] = c->h_setup[index]; o[1] = c->h_solved[index]; o[2] = c->h_n[index];
+    }
+    return 3;
+  } else return -set_err(E_WRONG, "lagrbf_debug_get", "unknown item '%s'", what);
+  if (out && n > 0 && src) {
+    if (cap < n) return -set_err(E_SIZ, "lagrbf_debug_get", "buffer too small (%lld < %lld)", (long long)cap, (long long)n);
+    if (hipMemcpy(out, src, (size_t)(n * esz), hipMemcpyDeviceToHost) != hipSuccess) return -E_LIB;
+  }
+  return n;
+}
+
+int lagrbf_last_error(char* buf, int32_t len) {
+  if (!buf || len <= 0) return E_NULL;
+  snprintf(buf, (size_t)len, "%s", g_last_error.c_str());
+  return 0;
+}
+
+}  // extern "C"

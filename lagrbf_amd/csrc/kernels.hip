@@ -1,0 +1,757 @@
+// gfx950 kernels of the PU-RBF interpolation + Lagrangian advection path.
+//
+// Layout: particles are SoA ([d][Np], leading dimension ld), data nodes keep the
+// caller's AoS [N][dim] order, patch state lives in pools indexed by per-patch
+// offsets.  Wave size is 64 (CDNA); blocks are multiples of 64.
+#include "kernels.hpp"
+
+#include <cfloat>
+#include <cmath>
+
+namespace lrb {
+
+static constexpr int WAVE = 64;
+
+// --------------------------------------------------------------------------
+// block reductions
+// --------------------------------------------------------------------------
+struct ArgMax { double v; int i; };
+
+__device__ __forceinline__ ArgMax better(ArgMax a, ArgMax b) {
+  // larger |value| wins; ties keep the smaller index (LAPACK idamax: first max)
+  if (b.v > a.v || (b.v == a.v && b.i < a.i)) return b;
+  return a;
+}
+
+template <int NT>
+__device__ ArgMax block_argmax(ArgMax x, ArgMax* red) {
+  for (int o = WAVE / 2; o > 0; o >>= 1) {
+    ArgMax y; y.v = __shfl_xor(x.v, o); y.i = __shfl_xor(x.i, o);
+    x = better(x, y);
+  }
+  const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+  if (lane == 0) red[wid] = x;
+  __syncthreads();
+  ArgMax r = red[0];
+  for (int w = 1; w < NT / WAVE; w++) r = better(r, red[w]);
+  __syncthreads();
+  return r;
+}
+
+template <int NT>
+__device__ double block_sum(double x, double* red) {
+  for (int o = WAVE / 2; o > 0; o >>= 1) x += __shfl_xor(x, o);
+  const int lane = threadIdx.x & (WAVE - 1), wid = threadIdx.x / WAVE;
+  if (lane == 0) red[wid] = x;
+  __syncthreads();
+  double r = red[0];
+  for (int w = 1; w < NT / WAVE; w++) r += red[w];
+  __syncthreads();
+  return r;
+}
+
+// --------------------------------------------------------------------------
+// binning: particle/node -> cell, per-cell node lists, patch membership
+// --------------------------------------------------------------------------
+__global__ void k_locate_aos(Grid g, const double* __restrict__ x, int n, int32_t* __restrict__ cell) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double* p = x + (size_t)i * g.dim;
+  cell[i] = locate(g, p[0], p[1], g.dim == 3 ? p[2] : 0.0);
+}
+
+__global__ void k_locate_particles(Grid g, const double* __restrict__ px, int64_t ld, int Np,
+                                   int32_t* __restrict__ cell, int32_t* __restrict__ alive,
+                                   unsigned long long* removed) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= Np || !alive[i]) return;
+  int c = locate(g, px[i], px[ld + i], g.dim == 3 ? px[2 * ld + i] : 0.0);
+  cell[i] = c;
+  if (c < 0) { alive[i] = 0; atomicAdd(removed, 1ull); }
+}
+
+__global__ void k_hist(const int32_t* __restrict__ cell, int n, int32_t* __restrict__ counts) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n && cell[i] >= 0) atomicAdd(&counts[cell[i]], 1);
+}
+
+// Exclusive scan of n ints in place: per-block scan + block totals.
+static constexpr int SCAN_NT = 256, SCAN_PER = 4, SCAN_CHUNK = SCAN_NT * SCAN_PER;
+
+__global__ __launch_bounds__(SCAN_NT) void k_scan_blocks(int32_t* a, int n, int32_t* sums) {
+  __shared__ int32_t s[SCAN_NT];
+  const int base = blockIdx.x * SCAN_CHUNK + threadIdx.x * SCAN_PER;
+  int32_t v[SCAN_PER], run = 0;
+  for (int k = 0; k < SCAN_PER; k++) { v[k] = (base + k < n) ? a[base + k] : 0; run += v[k]; }
+  s[threadIdx.x] = run;
+  __syncthreads();
+  for (int o = 1; o < SCAN_NT; o <<= 1) {
+    int32_t t = threadIdx.x >= o ? s[threadIdx.x - o] : 0;
+    __syncthreads();
+    s[threadIdx.x] += t;
+    __syncthreads();
+  }
+  int32_t ex = s[threadIdx.x] - run;
+  for (int k = 0; k < SCAN_PER; k++) { if (base + k < n) a[base + k] = ex; ex += v[k]; }
+  if (threadIdx.x == SCAN_NT - 1) sums[blockIdx.x] = s[SCAN_NT - 1];
+}
+
+__global__ void k_scan_add(int32_t* a, int n, const int32_t* sums) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] += sums[i / SCAN_CHUNK];
+}
+
+static void exclusive_scan(int32_t* a, int n, int32_t* tmp, hipStream_t s) {
+  int nb = (n + SCAN_CHUNK - 1) / SCAN_CHUNK;
+  hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(SCAN_NT), 0, s, a, n, tmp);
+  if (nb > 1) {
+    exclusive_scan(tmp, nb, tmp + nb, s);
+    hipLaunchKernelGGL(k_scan_add, dim3((n + 255) / 256), dim3(256), 0, s, a, n, tmp);
+  }
+}
+
+__global__ void k_scatter(const int32_t* __restrict__ cell, int n, int32_t* __restrict__ cursor,
+                          int32_t* __restrict__ nodes) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  int c = cell[i];
+  if (c >= 0) nodes[atomicAdd(&cursor[c], 1)] = i;
+}
+
+// Restore ascending node index inside each cell (the atomic scatter is unordered):
+// one wave per cell, every element's rank = number of smaller elements.
+__global__ __launch_bounds__(256) void k_segsort(const int32_t* __restrict__ cell_start, int C,
+                                                 const int32_t* __restrict__ unsorted, int32_t* __restrict__ sorted) {
+  const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
+  if (wid >= C) return;
+  const int b = cell_start[wid], e = cell_start[wid + 1], m = e - b;
+  for (int t = lane; t < m; t += WAVE) {
+    int v = unsorted[b + t], r = 0;
+    for (int u = 0; u < m; u++) r += unsorted[b + u] < v;
+    sorted[b + r] = v;
+  }
+}
+
+void launch_cell_lists(const int32_t* cell, int n, int C, int32_t* cell_start, int32_t* cell_nodes,
+                       int32_t* scratch, hipStream_t s) {
+  // scratch layout: [cursor C+1][unsorted n][scan tmp]
+  int32_t* cursor = scratch;
+  int32_t* unsorted = scratch + (C + 1);
+  int32_t* tmp = unsorted + (n > 0 ? n : 1);
+  hipMemsetAsync(cell_start, 0, sizeof(int32_t) * (C + 1), s);
+  if (n > 0) hipLaunchKernelGGL(k_hist, dim3((n + 255) / 256), dim3(256), 0, s, cell, n, cell_start);
+  exclusive_scan(cell_start, C + 1, tmp, s);
+  hipMemcpyAsync(cursor, cell_start, sizeof(int32_t) * (C + 1), hipMemcpyDeviceToDevice, s);
+  if (n > 0) {
+    hipLaunchKernelGGL(k_scatter, dim3((n + 255) / 256), dim3(256), 0, s, cell, n, cursor, unsorted);
+    hipLaunchKernelGGL(k_segsort, dim3((C + 3) / 4), dim3(256), 0, s, cell_start, C, unsorted, cell_nodes);
+  }
+}
+
+// Visit the (up to 2^d) cells touching PU node p in ascending cell index
+// (partition.c:54-87 builds the inverse map by ascending element index).
+template <typename F>
+__device__ __forceinline__ void for_patch_cells(const Grid& g, int p, F&& f) {
+  int a = p % g.n[0], b = (p / g.n[0]) % g.n[1], c = g.dim == 3 ? p / (g.n[0] * g.n[1]) : 0;
+  int k0 = g.dim == 3 ? c - 1 : 0, k1 = g.dim == 3 ? c : 0;
+  for (int k = k0; k <= k1; k++) {
+    if (g.dim == 3 && (k < 0 || k >= g.nc[2])) continue;
+    for (int j = b - 1; j <= b; j++) {
+      if (j < 0 || j >= g.nc[1]) continue;
+      for (int i = a - 1; i <= a; i++) {
+        if (i < 0 || i >= g.nc[0]) continue;
+        f(i + g.nc[0] * (j + g.nc[1] * k));
+      }
+    }
+  }
+}
+
+__global__ void k_patch_count(Grid g, const int32_t* __restrict__ cs, int32_t* __restrict__ pcount) {
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= g.P) return;
+  int cnt = 0;
+  for_patch_cells(g, p, [&](int c) { cnt += cs[c + 1] - cs[c]; });
+  pcount[p] = cnt;
+}
+
+__global__ void k_patch_gather(Grid g, const int32_t* __restrict__ list, int nlist, const int32_t* __restrict__ cs,
+                               const int32_t* __restrict__ cn, const int64_t* __restrict__ mem_off,
+                               int32_t* __restrict__ mem) {
+  const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
+  if (wid >= nlist) return;
+  const int p = list[wid];
+  int32_t* out = mem + mem_off[p];
+  int o = 0;
+  for_patch_cells(g, p, [&](int c) {
+    const int b = cs[c], m = cs[c + 1] - b;
+    for (int t = lane; t < m; t += WAVE) out[o + t] = cn[b + t];
+    o += m;
+  });
+}
+
+__global__ void k_touch(Grid g, const int32_t* __restrict__ cell, const int32_t* __restrict__ alive, int Np,
+                        uint8_t* __restrict__ touched) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= Np || !alive[i]) return;
+  const int c = cell[i];
+  for (int k = 0; k < g.nen; k++) touched[corner_node(g, c, k)] = 1;
+}
+
+void launch_locate_aos(const Grid& g, const double* x, int n, int32_t* cell, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_locate_aos, dim3((n + 255) / 256), dim3(256), 0, s, g, x, n, cell);
+}
+void launch_locate_particles(const Grid& g, double* px, int64_t ld, int Np, int32_t* cell, int32_t* alive,
+                             unsigned long long* removed, hipStream_t s) {
+  if (Np > 0) hipLaunchKernelGGL(k_locate_particles, dim3((Np + 255) / 256), dim3(256), 0, s, g, px, ld, Np, cell, alive, removed);
+}
+void launch_patch_count(const Grid& g, const int32_t* cs, int32_t* pcount, hipStream_t s) {
+  hipLaunchKernelGGL(k_patch_count, dim3((g.P + 255) / 256), dim3(256), 0, s, g, cs, pcount);
+}
+void launch_patch_gather(const Grid& g, const int32_t* list, int nlist, const int32_t* cs, const int32_t* cn,
+                         const int64_t* mem_off, int32_t* mem, hipStream_t s) {
+  if (nlist > 0) hipLaunchKernelGGL(k_patch_gather, dim3((nlist + 3) / 4), dim3(256), 0, s, g, list, nlist, cs, cn, mem_off, mem);
+}
+void launch_touch(const Grid& g, const int32_t* cell, const int32_t* alive, int Np, uint8_t* touched, hipStream_t s) {
+  if (Np > 0) hipLaunchKernelGGL(k_touch, dim3((Np + 255) / 256), dim3(256), 0, s, g, cell, alive, Np, touched);
+}
+
+// --------------------------------------------------------------------------
+// Patch assembly + Bunch-Kaufman LDL^T (dsytf2 'L'), one workgroup per patch,
+// matrix column-major in the patch's factor slot.  Restates the factorisation
+// PETSc's SeqDense Cholesky performs through dsytrf (partition_factor.c:37-55).
+// --------------------------------------------------------------------------
+static constexpr int FNT = 256;
+
+template <int KT>
+__global__ __launch_bounds__(FNT) void k_factor_bk(Grid g, const int32_t* __restrict__ list, const int32_t* __restrict__ pn,
+                                                   const int64_t* __restrict__ mem_off, const int64_t* __restrict__ fac_off,
+                                                   const int32_t* __restrict__ mem_pool, const double* __restrict__ xd,
+                                                   double* __restrict__ fac, int32_t* __restrict__ piv_pool,
+                                                   int32_t* __restrict__ info_out) {
+  __shared__ ArgMax red[FNT / WAVE];
+  __shared__ double redd[FNT / WAVE];
+  __shared__ int s_kp, s_kstep;
+  const int p = list[blockIdx.x];
+  const int n = pn[p];
+  if (n == 0) return;
+  double* A = fac + fac_off[p];
+  const int32_t* mem = mem_pool + mem_off[p];
+  int32_t* piv = piv_pool + mem_off[p];
+  const int tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE;
+  const int dim = g.dim;
+  const double eps = g.eps_r;
+  // assemble the lower triangle: A_ij = phi(|x_i - x_j|, eps_r)  (partition_solve.c:47-82)
+  for (int j = wid; j < n; j += FNT / WAVE) {
+    const double* xj = xd + (size_t)mem[j] * dim;
+    const double xj0 = xj[0], xj1 = xj[1], xj2 = dim == 3 ? xj[2] : 0.0;
+    for (int i = j + lane; i < n; i += WAVE) {
+      const double* xi = xd + (size_t)mem[i] * dim;
+      double d0 = xi[0] - xj0, d1 = xi[1] - xj1, d2 = dim == 3 ? xi[2] - xj2 : 0.0;
+      double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
+      A[(size_t)j * n + i] = rbf<KT, double>(r, eps);
+    }
+  }
+  __syncthreads();
+  const double alpha = (1.0 + sqrt(17.0)) / 8.0;
+  int info = 0;
+  int k = 0;
+  while (k < n) {
+    // colmax over A(k+1:n, k)
+    ArgMax am; am.v = -1.0; am.i = n;
+    for (int i = k + 1 + tid; i < n; i += FNT) { ArgMax c; c.v = fabs(A[(size_t)k * n + i]); c.i = i; am = better(am, c); }
+    am = block_argmax<FNT>(am, red);
+    const double absakk = fabs(A[(size_t)k * n + k]);
+    const double colmax = (k < n - 1) ? am.v : 0.0;
+    const int imax = (k < n - 1) ? am.i : k;
+    int kp = k, kstep = 1;
+    bool zero = (fmax(absakk, colmax) == 0.0) || isnan(absakk);
+    if (!zero && !(absakk >= alpha * colmax)) {
+      // rowmax: off-diagonal entries of row/column imax in the trailing matrix
+      double rm = 0.0;
+      for (int j = k + tid; j < imax; j += FNT) rm = fmax(rm, fabs(A[(size_t)j * n + imax]));
+      for (int j = imax + 1 + tid; j < n; j += FNT) rm = fmax(rm, fabs(A[(size_t)imax * n + j]));
+      for (int o = WAVE / 2; o > 0; o >>= 1) rm = fmax(rm, __shfl_xor(rm, o));
+      if (lane == 0) redd[wid] = rm;
+      __syncthreads();
+      double rowmax = redd[0];
+      for (int w = 1; w < FNT / WAVE; w++) rowmax = fmax(rowmax, redd[w]);
+      __syncthreads();
+      if (absakk >= alpha * colmax * (colmax / rowmax)) kp = k;
+      else if (fabs(A[(size_t)imax * n + imax]) >= alpha * rowmax) kp = imax;
+      else { kp = imax; kstep = 2; }
+    }
+    if (zero) { if (info == 0) info = k + 1; }
+    if (!zero) {
+      const int kk = k + kstep - 1;
+      if (kp != kk) {
+        for (int i = kp + 1 + tid; i < n; i += FNT) {
+          double t = A[(size_t)kk * n + i]; A[(size_t)kk * n + i] = A[(size_t)kp * n + i]; A[(size_t)kp * n + i] = t;
+        }
+        for (int j = kk + 1 + tid; j < kp; j += FNT) {
+          double t = A[(size_t)kk * n + j]; A[(size_t)kk * n + j] = A[(size_t)j * n + kp]; A[(size_t)j * n + kp] = t;
+        }
+        if (tid == 0) {
+          double t = A[(size_t)kk * n + kk]; A[(size_t)kk * n + kk] = A[(size_t)kp * n + kp]; A[(size_t)kp * n + kp] = t;
+          if (kstep == 2) { t = A[(size_t)k * n + k + 1]; A[(size_t)k * n + k + 1] = A[(size_t)k * n + kp]; A[(size_t)k * n + kp] = t; }
+        }
+        __syncthreads();
+      }
+      if (kstep == 1) {
+        if (k < n - 1) {
+          const double d11 = 1.0 / A[(size_t)k * n + k];
+          const double* ck = A + (size_t)k * n;
+          for (int j = k + 1 + wid; j < n; j += FNT / WAVE) {
+            const double t = -d11 * ck[j];
+            double* cj = A + (size_t)j * n;
+            for (int i = j + lane; i < n; i += WAVE) cj[i] += ck[i] * t;
+          }
+          __syncthreads();
+          for (int i = k + 1 + tid; i < n; i += FNT) A[(size_t)k * n + i] *= d11;
+        }
+      } else if (k < n - 2) {
+        const double* c0 = A + (size_t)k * n;
+        const double* c1 = A + (size_t)(k + 1) * n;
+        double d21 = c0[k + 1];
+        const double d11 = c1[k + 1] / d21;
+        const double d22 = c0[k] / d21;
+        const double t = 1.0 / (d11 * d22 - 1.0);
+        d21 = t / d21;
+        for (int j = k + 2 + wid; j < n; j += FNT / WAVE) {
+          const double wk = d21 * (d11 * c0[j] - c1[j]);
+          const double wkp1 = d21 * (d22 * c1[j] - c0[j]);
+          double* cj = A + (size_t)j * n;
+          for (int i = j + lane; i < n; i += WAVE) cj[i] = cj[i] - c0[i] * wk - c1[i] * wkp1;
+        }
+        __syncthreads();
+        for (int j = k + 2 + tid; j < n; j += FNT) {
+          const double a0 = c0[j], a1 = c1[j];
+          A[(size_t)k * n + j] = d21 * (d11 * a0 - a1);
+          A[(size_t)(k + 1) * n + j] = d21 * (d22 * a1 - a0);
+        }
+      }
+    }
+    if (tid == 0) {
+      if (kstep == 1) piv[k] = kp + 1;
+      else { piv[k] = -(kp + 1); piv[k + 1] = -(kp + 1); }
+    }
+    __syncthreads();
+    k += kstep;
+  }
+  if (tid == 0 && info_out) info_out[p] = info;
+}
+
+void launch_factor_bk(const Grid& g, int ktype, const int32_t* list, int nlist, const int32_t* pn,
+                      const int64_t* mem_off, const int64_t* fac_off, const int32_t* mem_pool, const double* xd,
+                      double* fac, int32_t* piv, int32_t* info, hipStream_t s) {
+  if (nlist <= 0) return;
+#define LRB_FBK(K) hipLaunchKernelGGL(k_factor_bk<K>, dim3(nlist), dim3(FNT), 0, s, g, list, pn, mem_off, fac_off, mem_pool, xd, fac, piv, info)
+  switch (ktype) {
+    case K_LINEAR: LRB_FBK(K_LINEAR); break;
+    case K_WC0: LRB_FBK(K_WC0); break;
+    case K_WC2: LRB_FBK(K_WC2); break;
+    case K_BUMP: LRB_FBK(K_BUMP); break;
+    case K_MQ: LRB_FBK(K_MQ); break;
+    case K_IMQ: LRB_FBK(K_IMQ); break;
+    default: LRB_FBK(K_GAUSS); break;
+  }
+#undef LRB_FBK
+}
+
+// dsytrs 'L' (petsc_dense.c:66) for nf right-hand sides gathered from the data
+// values (PartitionMatSetRHS, partition_factor.c:57-90).  X row-major [n][nf].
+__global__ __launch_bounds__(FNT) void k_solve_bk(const int32_t* __restrict__ list, const int32_t* __restrict__ pn,
+                                                  const int64_t* __restrict__ mem_off, const int64_t* __restrict__ fac_off,
+                                                  const int32_t* __restrict__ mem_pool, const double* __restrict__ fac,
+                                                  const int32_t* __restrict__ piv_pool, const double* __restrict__ values,
+                                                  int nf, double* __restrict__ coef) {
+  __shared__ double redd[FNT / WAVE];
+  const int p = list[blockIdx.x];
+  const int n = pn[p];
+  if (n == 0 || nf == 0) return;
+  const double* A = fac + fac_off[p];
+  const int32_t* mem = mem_pool + mem_off[p];
+  const int32_t* piv = piv_pool + mem_off[p];
+  double* B = coef + mem_off[p] * nf;
+  const int tid = threadIdx.x;
+  for (int t = tid; t < n * nf; t += FNT) { int i = t / nf, f = t % nf; B[t] = values[(size_t)mem[i] * nf + f]; }
+  __syncthreads();
+  // forward: L D y = b
+  int k = 0;
+  while (k < n) {
+    if (piv[k] > 0) {
+      const int kp = piv[k] - 1;
+      if (kp != k && tid < nf) { double t = B[k * nf + tid]; B[k * nf + tid] = B[kp * nf + tid]; B[kp * nf + tid] = t; }
+      __syncthreads();
+      const double* ck = A + (size_t)k * n;
+      for (int f = 0; f < nf; f++) {
+        const double bk = B[k * nf + f];
+        for (int i = k + 1 + tid; i < n; i += FNT) B[i * nf + f] -= ck[i] * bk;
+      }
+      __syncthreads();
+      if (tid < nf) B[k * nf + tid] *= 1.0 / ck[k];
+      __syncthreads();
+      k += 1;
+    } else {
+      const int kp = -piv[k] - 1;
+      if (kp != k + 1 && tid < nf) { double t = B[(k + 1) * nf + tid]; B[(k + 1) * nf + tid] = B[kp * nf + tid]; B[kp * nf + tid] = t; }
+      __syncthreads();
+      const double* c0 = A + (size_t)k * n;
+      const double* c1 = A + (size_t)(k + 1) * n;
+      for (int f = 0; f < nf; f++) {
+        const double b0 = B[k * nf + f], b1 = B[(k + 1) * nf + f];
+        for (int i = k + 2 + tid; i < n; i += FNT) { double v = B[i * nf + f] - c0[i] * b0; B[i * nf + f] = v - c1[i] * b1; }
+      }
+      __syncthreads();
+      if (tid < nf) {
+        const double akm1k = c0[k + 1], akm1 = c0[k] / akm1k, ak = c1[k + 1] / akm1k, denom = akm1 * ak - 1.0;
+        const double bkm1 = B[k * nf + tid] / akm1k, bk = B[(k + 1) * nf + tid] / akm1k;
+        B[k * nf + tid] = (ak * bkm1 - bk) / denom;
+        B[(k + 1) * nf + tid] = (akm1 * bk - bkm1) / denom;
+      }
+      __syncthreads();
+      k += 2;
+    }
+  }
+  // backward: L^T x = y
+  k = n - 1;
+  while (k >= 0) {
+    if (piv[k] > 0) {
+      const double* ck = A + (size_t)k * n;
+      for (int f = 0; f < nf; f++) {
+        double s = 0.0;
+        for (int i = k + 1 + tid; i < n; i += FNT) s += B[i * nf + f] * ck[i];
+        s = block_sum<FNT>(s, redd);
+        if (tid == 0) B[k * nf + f] -= s;
+      }
+      __syncthreads();
+      const int kp = piv[k] - 1;
+      if (kp != k && tid < nf) { double t = B[k * nf + tid]; B[k * nf + tid] = B[kp * nf + tid]; B[kp * nf + tid] = t; }
+      __syncthreads();
+      k -= 1;
+    } else {
+      const double* ck = A + (size_t)k * n;
+      const double* ckm1 = A + (size_t)(k - 1) * n;
+      for (int f = 0; f < nf; f++) {
+        double s = 0.0, s1 = 0.0;
+        for (int i = k + 1 + tid; i < n; i += FNT) { s += B[i * nf + f] * ck[i]; s1 += B[i * nf + f] * ckm1[i]; }
+        s = block_sum<FNT>(s, redd);
+        s1 = block_sum<FNT>(s1, redd);
+        if (tid == 0) { B[k * nf + f] -= s; B[(k - 1) * nf + f] -= s1; }
+      }
+      __syncthreads();
+      const int kp = -piv[k] - 1;
+      if (kp != k && tid < nf) { double t = B[k * nf + tid]; B[k * nf + tid] = B[kp * nf + tid]; B[kp * nf + tid] = t; }
+      __syncthreads();
+      k -= 2;
+    }
+  }
+}
+
+void launch_solve_bk(const int32_t* list, int nlist, const int32_t* pn, const int64_t* mem_off, const int64_t* fac_off,
+                     const int32_t* mem_pool, const double* fac, const int32_t* piv, const double* values, int nf,
+                     double* coef, hipStream_t s) {
+  if (nlist > 0) hipLaunchKernelGGL(k_solve_bk, dim3(nlist), dim3(FNT), 0, s, list, pn, mem_off, fac_off, mem_pool, fac, piv, values, nf, coef);
+}
+
+// --------------------------------------------------------------------------
+// Shepard weights (swarm_interp.c:283-380): Wendland-C2 weight of the cell's
+// 2^d corner nodes normalised by the sum over every PU node; nodes outside the
+// support contribute exact zeros, so the sum runs over the support window in
+// ascending node order.
+// --------------------------------------------------------------------------
+__device__ __forceinline__ void weights_point(const Grid& g, const double* __restrict__ pux, const double* __restrict__ puy,
+                                              const double* __restrict__ puz, double x, double y, double z, int cell,
+                                              double* w) {
+  const int ci = cell % g.nc[0], cj = (cell / g.nc[0]) % g.nc[1], ck = g.dim == 3 ? cell / (g.nc[0] * g.nc[1]) : 0;
+  const double R = 1.0 / g.eps_w;
+  int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+  const double xs[3] = {x, y, z};
+  for (int d = 0; d < g.dim; d++) {
+    double a = floor((xs[d] - g.O[d] - R) / g.h[d]) - 1.0;
+    double b = ceil((xs[d] - g.O[d] + R) / g.h[d]) + 1.0;
+    lo[d] = a < 0.0 ? 0 : (a > g.n[d] - 1 ? g.n[d] - 1 : (int)a);
+    hi[d] = b < 0.0 ? 0 : (b > g.n[d] - 1 ? g.n[d] - 1 : (int)b);
+  }
+  double sum = 0.0;
+  for (int k = 0; k < g.nen; k++) w[k] = 0.0;
+  for (int kk = lo[2]; kk <= hi[2]; kk++) {
+    const double dz = g.dim == 3 ? z - puz[kk] : 0.0;
+    const int ek = kk - ck;
+    for (int j = lo[1]; j <= hi[1]; j++) {
+      const double dy = y - puy[j];
+      const int ej = j - cj;
+      for (int i = lo[0]; i <= hi[0]; i++) {
+        const double dx = x - pux[i];
+        double r = sqrt(dx * dx + dy * dy + dz * dz);
+        double wt = rbf<K_WC2, double>(r, g.eps_w);
+        sum += wt;
+        const int ei = i - ci;
+        if ((unsigned)ei <= 1u && (unsigned)ej <= 1u && (unsigned)ek <= 1u) {
+          const int q = (ej ? (ei ? 2 : 3) : (ei ? 1 : 0)) + 4 * ek;
+          w[q] = wt;
+        }
+      }
+    }
+  }
+  for (int k = 0; k < g.nen; k++) w[k] = sum > 0.0 ? w[k] / sum : 0.0;
+}
+
+__global__ void k_weights(Grid g, const double* __restrict__ pux, const double* __restrict__ puy, const double* __restrict__ puz,
+                          const double* __restrict__ px, int64_t ld, int Np, const int32_t* __restrict__ cell,
+                          const int32_t* __restrict__ alive, double* __restrict__ w) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= Np || !alive[i]) return;
+  double wl[8];
+  weights_point(g, pux, puy, puz, px[i], px[ld + i], g.dim == 3 ? px[2 * ld + i] : 0.0, cell[i], wl);
+  for (int k = 0; k < g.nen; k++) w[(size_t)k * ld + i] = wl[k];
+}
+
+void launch_weights(const Grid& g, const double* pux, const double* puy, const double* puz, const double* px, int64_t ld,
+                    int Np, const int32_t* cell, const int32_t* alive, double* w, hipStream_t s) {
+  if (Np > 0) hipLaunchKernelGGL(k_weights, dim3((Np + 255) / 256), dim3(256), 0, s, g, pux, puy, puz, px, ld, Np, cell, alive, w);
+}
+
+// --------------------------------------------------------------------------
+// PU evaluation in the reference's summation order (interpolation.c:39-154):
+// per corner patch, interpolant_f = sum_j phi(|x - x_j|) X_j,f over the patch's
+// members, then values_f += interpolant_f * w_k.  Particles whose 3^d cell
+// neighbourhood holds no node keep their previous values.
+// --------------------------------------------------------------------------
+template <int KT, int NF>
+__global__ __launch_bounds__(256) void k_eval_ref(Grid g, const double* __restrict__ px, int64_t ld, int Np,
+                                                  const int32_t* __restrict__ cell, const int32_t* __restrict__ alive,
+                                                  const double* __restrict__ w, PatchView pv,
+                                                  const double* __restrict__ xd, const int32_t* __restrict__ cs,
+                                                  int nfields, int f0, double* __restrict__ out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= Np || !alive[i]) return;
+  const int c = cell[i];
+  const int ci = c % g.nc[0], cj = (c / g.nc[0]) % g.nc[1], ck = g.dim == 3 ? c / (g.nc[0] * g.nc[1]) : 0;
+  int nnb = 0;
+  for (int dk = (g.dim == 3 ? -1 : 0); dk <= (g.dim == 3 ? 1 : 0); dk++)
+    for (int dj = -1; dj <= 1; dj++)
+      for (int di = -1; di <= 1; di++) {
+        const int a = ci + di, b = cj + dj, e = ck + dk;
+        if (a < 0 || a >= g.nc[0] || b < 0 || b >= g.nc[1]) continue;
+        if (g.dim == 3 && (e < 0 || e >= g.nc[2])) continue;
+        const int q = a + g.nc[0] * (b + g.nc[1] * e);
+        nnb += cs[q + 1] - cs[q];
+      }
+  if (nnb == 0) return;
+  const double x = px[i], y = px[ld + i], z = g.dim == 3 ? px[2 * ld + i] : 0.0;
+  const int dim = g.dim;
+  const int nfc = min(NF, nfields - f0);
+  double val[NF];
+  for (int f = 0; f < NF; f++) val[f] = 0.0;
+  for (int k = 0; k < g.nen; k++) {
+    const int p = corner_node(g, c, k);
+    if (!pv.active[p]) continue;
+    const int n = pv.n[p];
+    const int64_t off = pv.mem_off[p];
+    const int32_t* mem = pv.members + off;
+    const double* X = pv.coef + off * pv.nf_total + f0;
+    double it[NF];
+    for (int f = 0; f < NF; f++) it[f] = 0.0;
+    for (int nd = 0; nd < n; nd++) {
+      const double* xj = xd + (size_t)mem[nd] * dim;
+      const double d0 = x - xj[0], d1 = y - xj[1], d2 = dim == 3 ? z - xj[2] : 0.0;
+      const double phi = rbf<KT, double>(sqrt(d0 * d0 + d1 * d1 + d2 * d2), g.eps_r);
+      const double* Xr = X + (size_t)nd * pv.nf_total;
+#pragma unroll
+      for (int f = 0; f < NF; f++) if (f < nfc) it[f] += phi * Xr[f];
+    }
+    const double wk = w[(size_t)k * ld + i];
+#pragma unroll
+    for (int f = 0; f < NF; f++) val[f] += it[f] * wk;
+  }
+#pragma unroll
+  for (int f = 0; f < NF; f++) if (f < nfc) out[(size_t)(f0 + f) * ld + i] = val[f];
+}
+
+template <int KT>
+static void eval_ref_k(const Grid& g, const double* px, int64_t ld, int Np, const int32_t* cell, const int32_t* alive,
+                       const double* w, const PatchView& pv, const double* xd, const int32_t* cs, int nfields,
+                       double* out, hipStream_t s) {
+  const dim3 grid((Np + 255) / 256), blk(256);
+  for (int f0 = 0; f0 < nfields; f0 += 4) {
+    const int r = nfields - f0;
+    if (r == 1) hipLaunchKernelGGL((k_eval_ref<KT, 1>), grid, blk, 0, s, g, px, ld, Np, cell, alive, w, pv, xd, cs, nfields, f0, out);
+    else if (r == 2) hipLaunchKernelGGL((k_eval_ref<KT, 2>), grid, blk, 0, s, g, px, ld, Np, cell, alive, w, pv, xd, cs, nfields, f0, out);
+    else if (r == 3) hipLaunchKernelGGL((k_eval_ref<KT, 3>), grid, blk, 0, s, g, px, ld, Np, cell, alive, w, pv, xd, cs, nfields, f0, out);
+    else hipLaunchKernelGGL((k_eval_ref<KT, 4>), grid, blk, 0, s, g, px, ld, Np, cell, alive, w, pv, xd, cs, nfields, f0, out);
+  }
+}
+
+void launch_eval_ref(const Grid& g, int ktype, const double* px, int64_t ld, int Np, const int32_t* cell,
+                     const int32_t* alive, const double* w, const PatchView& pv, const double* xd,
+                     const int32_t* cs, int nfields, double* out, hipStream_t s) {
+  if (Np <= 0 || nfields <= 0) return;
+  switch (ktype) {
+    case K_LINEAR: eval_ref_k<K_LINEAR>(g, px, ld, Np, cell, alive, w, pv, xd, cs, nfields, out, s); break;
+    case K_WC0: eval_ref_k<K_WC0>(g, px, ld, Np, cell, alive, w, pv, xd, cs, nfields, out, s); break;
+    case K_WC2: eval_ref_k<K_WC2>(g, px, ld, Np, cell, alive, w, pv, xd, cs, nfields, out, s); break;
+    case K_BUMP: eval_ref_k<K_BUMP>(g, px, ld, Np, cell, alive, w, pv, xd, cs, nfields, out, s); break;
+    case K_MQ: eval_ref_k<K_MQ>(g, px, ld, Np, cell, alive, w, pv, xd, cs, nfields, out, s); break;
+    case K_IMQ: eval_ref_k<K_IMQ>(g, px, ld, Np, cell, alive, w, pv, xd, cs, nfields, out, s); break;
+    default: eval_ref_k<K_GAUSS>(g, px, ld, Np, cell, alive, w, pv, xd, cs, nfields, out, s); break;
+  }
+}
+
+// --------------------------------------------------------------------------
+// RK stage loop (runge_kutta_{1,2,4}.c, dormand_prince.c): x = x0 + s*sum_j a_j v_j,
+// optional sphere projection (advection.c:472-491), re-location (removal of
+// points that left the domain, DMSwarmMigrate) and Shepard weights, fused.
+// The update is evaluated without FMA contraction, in the reference's order.
+// --------------------------------------------------------------------------
+__device__ __forceinline__ void project_point(double* x, int dim, double rmin, double rmax) {
+#pragma clang fp contract(off)
+  double pos = 0.0, nrm[3];
+  for (int d = 0; d < dim; d++) pos += x[d] * x[d];
+  if (pos < rmax * rmax && pos > rmin * rmin) return;
+  pos = sqrt(pos);
+  for (int d = 0; d < dim; d++) nrm[d] = x[d] / pos;
+  if (pos > rmax) for (int d = 0; d < dim; d++) x[d] = rmax * nrm[d];
+  else for (int d = 0; d < dim; d++) x[d] = rmin * nrm[d];
+}
+
+__global__ void k_stage(Grid g, const double* __restrict__ pux, const double* __restrict__ puy, const double* __restrict__ puz,
+                        double* px, const double* x0, double* __restrict__ vel, int64_t ld, int Np,
+                        int32_t* __restrict__ cell, int32_t* __restrict__ alive, unsigned long long* removed,
+                        double* __restrict__ w, StageArgs a) {
+#pragma clang fp contract(off)
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= Np || !alive[i]) return;
+  const int dim = g.dim;
+  double x[3] = {0.0, 0.0, 0.0};
+  for (int d = 0; d < dim; d++) x[d] = px[d * ld + i];
+  for (int d = 0; d < dim; d++) {
+    double v = 0.0;
+    for (int j = 0; j < a.nv; j++) v += a.a[j] * a.v[j][d * ld + i];
+    if (a.write_vel) vel[d * ld + i] = v;
+    x[d] = x0[d * ld + i] + a.scale * v;
+    if (a.proj_in_loop) project_point(x, dim, a.rmin, a.rmax);
+  }
+  if (a.project && !a.proj_in_loop) project_point(x, dim, a.rmin, a.rmax);
+  for (int d = 0; d < dim; d++) px[d * ld + i] = x[d];
+  const int c = locate(g, x[0], x[1], x[2]);
+  cell[i] = c;
+  if (c < 0) { alive[i] = 0; atomicAdd(removed, 1ull); return; }
+  double wl[8];
+  weights_point(g, pux, puy, puz, x[0], x[1], x[2], c, wl);
+  for (int k = 0; k < g.nen; k++) w[(size_t)k * ld + i] = wl[k];
+}
+
+void launch_stage(const Grid& g, const double* pux, const double* puy, const double* puz, double* px, const double* x0,
+                  double* vel, int64_t ld, int Np, int32_t* cell, int32_t* alive, unsigned long long* removed, double* w,
+                  const StageArgs& a, hipStream_t s) {
+  if (Np > 0) hipLaunchKernelGGL(k_stage, dim3((Np + 255) / 256), dim3(256), 0, s, g, pux, puy, puz, px, x0, vel, ld, Np, cell, alive, removed, w, a);
+}
+
+__global__ void k_save(double* __restrict__ px, double* __restrict__ x0, const double* __restrict__ vel, double* __restrict__ v0,
+                       int64_t ld, int Np, int dim, const int32_t* __restrict__ alive) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= Np || !alive[i]) return;
+  for (int d = 0; d < dim; d++) {
+    x0[d * ld + i] = px[d * ld + i];
+    if (v0) v0[d * ld + i] = vel[d * ld + i];
+  }
+}
+
+void launch_save(double* px, double* x0, const double* vel, double* v0, int64_t ld, int Np, int dim, const int32_t* alive,
+                 hipStream_t s) {
+  if (Np > 0) hipLaunchKernelGGL(k_save, dim3((Np + 255) / 256), dim3(256), 0, s, px, x0, vel, v0, ld, Np, dim, alive);
+}
+
+__global__ void k_copy_rows(const double* __restrict__ src, int64_t lds, double* __restrict__ dst, int64_t ldd, int nrows,
+                            int Np, const int32_t* __restrict__ alive) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= Np || (alive && !alive[i])) return;
+  for (int r = 0; r < nrows; r++) dst[r * ldd + i] = src[r * lds + i];
+}
+
+void launch_copy_rows(const double* src, int64_t lds, double* dst, int64_t ldd, int nrows, int Np, const int32_t* alive,
+                      hipStream_t s) {
+  if (Np > 0) hipLaunchKernelGGL(k_copy_rows, dim3((Np + 255) / 256), dim3(256), 0, s, src, lds, dst, ldd, nrows, Np, alive);
+}
+
+// CFL dt = min_p cfl * (1/eps_w) / (|v_p| + 1e-16), advection.c:430-470.
+// Positive doubles order like their bit patterns, so atomicMin on the bits works.
+__global__ __launch_bounds__(256) void k_cfl(const double* __restrict__ vel, int64_t ld, int Np, int dim,
+                                             const int32_t* __restrict__ alive, double c, unsigned long long* out) {
+#pragma clang fp contract(off)
+  __shared__ double red[256 / WAVE];
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double dt = DBL_MAX;
+  if (i < Np && alive[i]) {
+    double nv = 0.0;
+    for (int d = 0; d < dim; d++) nv += vel[d * ld + i] * vel[d * ld + i];
+    nv = sqrt(nv);
+    dt = c / (nv + 1e-16);
+  }
+  for (int o = WAVE / 2; o > 0; o >>= 1) { double y = __shfl_xor(dt, o); dt = dt < y ? dt : y; }
+  if ((threadIdx.x & (WAVE - 1)) == 0) red[threadIdx.x / WAVE] = dt;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = red[0];
+    for (int k = 1; k < 256 / WAVE; k++) m = m < red[k] ? m : red[k];
+    atomicMin(out, (unsigned long long)__double_as_longlong(m));
+  }
+}
+
+void launch_cfl(const double* vel, int64_t ld, int Np, int dim, const int32_t* alive, double c, unsigned long long* out,
+                hipStream_t s) {
+  if (Np > 0) hipLaunchKernelGGL(k_cfl, dim3((Np + 255) / 256), dim3(256), 0, s, vel, ld, Np, dim, alive, c, out);
+}
+
+// DOPRI error estimate, dormand_prince.c:37-102 (max over particles).
+struct VPtrs { const double* v[7]; };
+__global__ __launch_bounds__(256) void k_dopri_err(const double* __restrict__ px, const double* __restrict__ x0, VPtrs vp,
+                                                   int64_t ld, int Np, int dim, const int32_t* __restrict__ alive, double dt,
+                                                   double atol, double rtol, unsigned long long* maxb, unsigned int* count) {
+#pragma clang fp contract(off)
+  const double Bs[7] = {5179.0 / 57600.0, 0, 7571.0 / 16695.0, 393.0 / 640.0, -92097.0 / 339200.0, 187.0 / 2100.0, 1.0 / 40.0};
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= Np || !alive[i]) return;
+  double E = 0.0;
+  for (int d = 0; d < dim; d++) {
+    double vl = 0.0;
+    for (int j = 0; j < 7; j++) vl += Bs[j] * vp.v[j][d * ld + i];
+    const double a0 = fabs(x0[d * ld + i]), a1 = fabs(px[d * ld + i]);
+    const double mv = a0 < a1 ? a1 : a0;
+    const double tol = atol + rtol * mv;
+    const double xl = x0[d * ld + i] + dt * vl;
+    const double e = px[d * ld + i] - xl;
+    E += (e * e) / (tol * tol);
+  }
+  E = sqrt(1.0 / dim * E);
+  if (E >= 0.0) atomicMax(maxb, (unsigned long long)__double_as_longlong(E));
+  atomicAdd(count, 1u);
+}
+
+void launch_dopri_err(const double* px, const double* x0, const double* const* vadv, int64_t ld, int Np, int dim,
+                      const int32_t* alive, double dt, double atol, double rtol, unsigned long long* maxb,
+                      unsigned int* count, hipStream_t s) {
+  VPtrs vp;
+  for (int j = 0; j < 7; j++) vp.v[j] = vadv[j];
+  if (Np > 0) hipLaunchKernelGGL(k_dopri_err, dim3((Np + 255) / 256), dim3(256), 0, s, px, x0, vp, ld, Np, dim, alive, dt, atol, rtol, maxb, count);
+}
+
+// --------------------------------------------------------------------------
+__global__ void k_aos_to_soa(const double* __restrict__ aos, int n, int bs, double* __restrict__ soa, int64_t ld) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int b = 0; b < bs; b++) soa[b * ld + i] = aos[(size_t)i * bs + b];
+}
+__global__ void k_soa_to_aos(const double* __restrict__ soa, int64_t ld, int n, int bs, double* __restrict__ aos) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  for (int b = 0; b < bs; b++) aos[(size_t)i * bs + b] = soa[b * ld + i];
+}
+void launch_aos_to_soa(const double* aos, int n, int bs, double* soa, int64_t ld, hipStream_t s) {
+  if (n > 0 && bs > 0) hipLaunchKernelGGL(k_aos_to_soa, dim3((n + 255) / 256), dim3(256), 0, s, aos, n, bs, soa, ld);
+}
+void launch_soa_to_aos(const double* soa, int64_t ld, int n, int bs, double* aos, hipStream_t s) {
+  if (n > 0 && bs > 0) hipLaunchKernelGGL(k_soa_to_aos, dim3((n + 255) / 256), dim3(256), 0, s, soa, ld, n, bs, aos);
+}
+
+}  // namespace lrb

@@ -1,0 +1,75 @@
+// Host-side launchers of the gfx950 kernels (defined in kernels.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include "common.hpp"
+
+namespace lrb {
+
+// Combination x = x0 + scale * sum_j a_j v_j used by every RK stage.
+struct StageArgs {
+  const double* v[7];
+  double a[7];
+  int nv;
+  double scale;
+  int project;        // 1: sphere projection after the update
+  int proj_in_loop;   // 1: projection inside the per-axis loop (RK4 combine)
+  int write_vel;      // 1: velocity field receives the stage combination
+  double rmin, rmax;
+};
+
+// Read-only patch tables used by evaluation.
+struct PatchView {
+  const int32_t* n;        // [P] patch size (frozen at set-up)
+  const uint8_t* active;   // [P] setup && n > 0
+  const int64_t* mem_off;  // [P] offset into member / coefficient pools
+  const int32_t* members;  // member pool
+  const double* coef;      // coefficient pool, row-major [n][nf_total]
+  int nf_total;
+};
+
+// ---- binning ----
+void launch_locate_aos(const Grid& g, const double* x, int n, int32_t* cell, hipStream_t s);
+void launch_locate_particles(const Grid& g, double* px, int64_t ld, int Np, int32_t* cell, int32_t* alive,
+                             unsigned long long* removed, hipStream_t s);
+void launch_cell_lists(const int32_t* cell, int n, int C, int32_t* cell_start, int32_t* cell_nodes,
+                       int32_t* scratch /* >= C+1 + scan blocks */, hipStream_t s);
+void launch_patch_count(const Grid& g, const int32_t* cell_start, int32_t* pcount, hipStream_t s);
+void launch_patch_gather(const Grid& g, const int32_t* list, int nlist, const int32_t* cell_start,
+                         const int32_t* cell_nodes, const int64_t* mem_off, int32_t* mem_pool, hipStream_t s);
+void launch_touch(const Grid& g, const int32_t* cell, const int32_t* alive, int Np, uint8_t* touched, hipStream_t s);
+
+// ---- patch factor / solve ----
+void launch_factor_bk(const Grid& g, int ktype, const int32_t* list, int nlist, const int32_t* pn,
+                      const int64_t* mem_off, const int64_t* fac_off, const int32_t* mem_pool,
+                      const double* xdata, double* fac, int32_t* piv, int32_t* info, hipStream_t s);
+void launch_solve_bk(const int32_t* list, int nlist, const int32_t* pn, const int64_t* mem_off,
+                     const int64_t* fac_off, const int32_t* mem_pool, const double* fac, const int32_t* piv,
+                     const double* values, int nf, double* coef, hipStream_t s);
+
+// ---- weights / evaluation ----
+void launch_weights(const Grid& g, const double* pux, const double* puy, const double* puz, const double* px,
+                    int64_t ld, int Np, const int32_t* cell, const int32_t* alive, double* w, hipStream_t s);
+void launch_eval_ref(const Grid& g, int ktype, const double* px, int64_t ld, int Np, const int32_t* cell,
+                     const int32_t* alive, const double* w, const PatchView& pv, const double* xdata,
+                     const int32_t* cell_start, int nfields, double* out, hipStream_t s);
+
+// ---- RK stage loop ----
+void launch_stage(const Grid& g, const double* pux, const double* puy, const double* puz, double* px,
+                  const double* x0, double* vel, int64_t ld, int Np, int32_t* cell, int32_t* alive,
+                  unsigned long long* removed, double* w, const StageArgs& a, hipStream_t s);
+void launch_save(double* px, double* x0, const double* vel, double* v0, int64_t ld, int Np, int dim,
+                 const int32_t* alive, hipStream_t s);
+void launch_copy_rows(const double* src, int64_t lds, double* dst, int64_t ldd, int nrows, int Np,
+                      const int32_t* alive, hipStream_t s);
+void launch_cfl(const double* vel, int64_t ld, int Np, int dim, const int32_t* alive, double c,
+                unsigned long long* out_bits, hipStream_t s);
+void launch_dopri_err(const double* px, const double* x0, const double* const* vadv, int64_t ld, int Np,
+                      int dim, const int32_t* alive, double dt, double atol, double rtol,
+                      unsigned long long* max_bits, unsigned int* count, hipStream_t s);
+
+// ---- layout conversion ----
+void launch_aos_to_soa(const double* aos, int n, int bs, double* soa, int64_t ld, hipStream_t s);
+void launch_soa_to_aos(const double* soa, int64_t ld, int n, int bs, double* aos, hipStream_t s);
+
+}  // namespace lrb

@@ -400,8 +400,8 @@ void orc_destroy(orc_ctx *ctx)
   int p;
   if (!ctx) return;
   for (p = 0; p < ctx->P; p++) patch_clear(&ctx->patch[p]);
-  free(ctx->patch); free(ctx->pu); free(ctx->corner); free(ctx->cell_start);
   free_particles(ctx); free_data(ctx);
+  free(ctx->patch); free(ctx->pu); free(ctx->corner); free(ctx->cell_start);
   free(ctx);
 }
 
