@@ -59,6 +59,19 @@ __device__ __forceinline__ T rbf_rt(int k, T r, T eps) {
   }
 }
 
+// sqrt for the hot loops: v_rsq_f64 (about 23 correct bits) refined by two
+// Newton steps on sqrt (r <- r + (x - r^2) / (2 sqrt x)), within 1 ulp for
+// normal x >= 0; the IEEE-rounded libm sequence costs twice the instructions.
+__device__ __forceinline__ double sqrt_fast(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  const double hy = 0.5 * y;
+  double r = x * y;
+  r = fma(hy, fma(-r, r, x), r);
+  r = fma(hy, fma(-r, r, x), r);
+  return x > 0.0 ? r : 0.0;
+}
+__device__ __forceinline__ float sqrt_fast(float x) { return __builtin_amdgcn_sqrtf(x); }
+
 // Particle -> cell, lexicographic cell index (i fastest) of the uniform Q1 grid,
 // as DMLocatePoints on the PU DMDA (swarm_utils.c:995-1030).  A coordinate with
 // O <= x <= L is inside, x == L falls in the last cell, anything else (incl. NaN)

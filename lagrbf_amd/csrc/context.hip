@@ -103,6 +103,10 @@ struct Ctx {
   DBuf<double> dx, dval;
   DBuf<int32_t> dcell, cell_start, cell_nodes, scratch, pcount;
   std::vector<int32_t> h_pcount;
+  DBuf<double> xs_sorted, zeros;           // cell-sorted node coordinates (SoA), zero coefficients
+  std::vector<int32_t> h_cell_start_prev, h_cell_nodes_prev;   // cell lists the frozen patches were built from
+  bool memb_current = true;                // every set-up patch's members == current cell lists
+  int eval_mode = 0;                       // 0 auto (cell-ordered when possible), 1 cell-ordered, 2 reference order
   std::vector<int64_t> h_dpid;
   // particle swarm
   int Np = 0;
@@ -110,6 +114,8 @@ struct Ctx {
   bool points_filled = false;
   DBuf<double> px, w, fields, vel, x0, vadv[7];
   DBuf<int32_t> pcell, alive;
+  DBuf<int32_t> perm, pstart, pscratch;    // particles grouped by cell
+  bool sorted_valid = false;
   std::vector<int64_t> h_ppid;
   DBuf<unsigned long long> counters;   // [0] removed, [1] reduction bits, [2] count
   // patches
@@ -122,17 +128,27 @@ struct Ctx {
   DBuf<uint8_t> p_active, touched;
   DBuf<int64_t> p_mem_off, p_fac_off;
   DBuf<double> fac_pool, coef_pool;
+  DBuf<float> fac32_pool;
+  size_t fac32_used = 0;
+  std::vector<uint8_t> h_method;           // per patch: M_BK64, M_CHOL64, M_CHOL32
+  DBuf<int32_t> pstatus, piters;
+  int precision = 0;                       // 0 FP64, 1 FP32 factor + FP64 refinement
+  int factor_mode = 0;                     // 0 auto (Cholesky for PD kernels), 1 Bunch-Kaufman always
+  int refine_max_iter = 10;
+  double refine_tol = 1e-14;
+  double fallback = 0, refine_iters_max = 0, refine_iters_sum = 0, refined_patches = 0;
   // analytical velocity
   int (*vfn)(double*, double*, void*) = nullptr;
   int (*vfn_batch)(int32_t, int32_t, double*, double*, void*) = nullptr;
   void* vuser = nullptr;
   bool analytical = false;
   // stats
-  double removed = 0, patches_setup = 0, setup_ms = 0, eval_ms = 0;
+  double removed = 0, patches_setup = 0, setup_ms = 0, eval_ms = 0, eval_count = 0;
   int max_patch = 0;
 };
 
 static const char* kVelocity = "velocity";
+enum { M_BK64 = 0, M_CHOL64 = 1, M_CHOL32 = 2 };
 
 // ---------------------------------------------------------------------------
 static int ctx_from_options(Ctx& c, const OptView& o) {
@@ -205,6 +221,28 @@ static int ctx_from_options(Ctx& c, const OptView& o) {
     if (bad) return set_err(E_WRONG, fn, "malformed advection option value");
   }
   o.get_int("lagrbf_device", c.device, bad);
+  std::string pr;
+  if (o.get_string("lagrbf_precision", pr)) {
+    if (pr == "fp64") c.precision = 0;
+    else if (pr == "mixed") c.precision = 1;
+    else return set_err(E_WRONG, fn, "-lagrbf_precision must be fp64 or mixed (found %s)", pr.c_str());
+  }
+  std::string fm;
+  if (o.get_string("lagrbf_factor", fm)) {
+    if (fm == "auto") c.factor_mode = 0;
+    else if (fm == "bk") c.factor_mode = 1;
+    else return set_err(E_WRONG, fn, "-lagrbf_factor must be auto or bk (found %s)", fm.c_str());
+  }
+  o.get_int("lagrbf_refine_max_iter", c.refine_max_iter, bad);
+  o.get_real("lagrbf_refine_tol", c.refine_tol, bad);
+  if (bad) return set_err(E_WRONG, fn, "malformed lagrbf_refine_* option");
+  std::string ev;
+  if (o.get_string("lagrbf_eval", ev)) {
+    if (ev == "auto") c.eval_mode = 0;
+    else if (ev == "cell") c.eval_mode = 1;
+    else if (ev == "ref") c.eval_mode = 2;
+    else return set_err(E_WRONG, fn, "-lagrbf_eval must be auto, cell or ref (found %s)", ev.c_str());
+  }
   return 0;
 }
 
@@ -232,6 +270,9 @@ static int ctx_init_device(Ctx& c) {
   HIPCHK(c.p_n.reserve(P)); HIPCHK(c.p_active.reserve(P)); HIPCHK(c.p_mem_off.reserve(P));
   HIPCHK(c.p_fac_off.reserve(P)); HIPCHK(c.touched.reserve(P)); HIPCHK(c.list.reserve(P)); HIPCHK(c.info.reserve(P));
   HIPCHK(c.pcount.reserve(P));
+  c.h_method.assign(P, 0);
+  HIPCHK(c.pstatus.reserve(P)); HIPCHK(c.piters.reserve(P));
+  HIPCHK(c.list.reserve((size_t)3 * P));
   HIPCHK(hipMemset(c.p_n.p, 0, sizeof(int32_t) * P));
   HIPCHK(hipMemset(c.p_active.p, 0, P));
   HIPCHK(hipMemset(c.p_mem_off.p, 0, sizeof(int64_t) * P));
@@ -283,7 +324,22 @@ static int fill_data(Ctx& c, int n, const double* coords, const double* values, 
   launch_cell_lists(c.dcell.p, n, C, c.cell_start.p, c.cell_nodes.p, c.scratch.p, c.s);
   launch_patch_count(c.g, c.cell_start.p, c.pcount.p, c.s);
   HIPCHK(hipMemcpyAsync(c.h_pcount.data(), c.pcount.p, sizeof(int32_t) * c.g.P, hipMemcpyDeviceToHost, c.s));
+  // node coordinates in cell order (SoA) for the cell-ordered evaluation
+  HIPCHK(c.xs_sorted.reserve((size_t)std::max(n, 1) * dim));
+  launch_sorted_coords(c.dx.p, n, dim, c.cell_nodes.p, c.xs_sorted.p, c.s);
   HIPCHK(hipStreamSynchronize(c.s));
+  int maxp = 1;
+  for (int p = 0; p < c.g.P; p++) maxp = std::max(maxp, c.h_pcount[p]);
+  HIPCHK(c.zeros.reserve((size_t)maxp * std::max(nf, 1)));
+  HIPCHK(hipMemset(c.zeros.p, 0, sizeof(double) * (size_t)maxp * std::max(nf, 1)));
+  // patches kept from an earlier fill (solve_method 0/1) were built from the cell
+  // lists of that fill: the cell-ordered evaluation needs them unchanged
+  if (c.mem_used > 0 && c.memb_current) {
+    std::vector<int32_t> cs(C + 1), cn(n);
+    HIPCHK(hipMemcpy(cs.data(), c.cell_start.p, sizeof(int32_t) * (C + 1), hipMemcpyDeviceToHost));
+    if (n > 0) HIPCHK(hipMemcpy(cn.data(), c.cell_nodes.p, sizeof(int32_t) * n, hipMemcpyDeviceToHost));
+    if (cs != c.h_cell_start_prev || cn != c.h_cell_nodes_prev) c.memb_current = false;
+  }
   c.data_filled = true;
   return 0;
 }
@@ -296,6 +352,10 @@ static int alloc_particles(Ctx& c) {
   HIPCHK(c.fields.reserve(ld * std::max(c.nf, 1)));
   HIPCHK(c.pcell.reserve(ld));
   HIPCHK(c.alive.reserve(ld));
+  HIPCHK(c.perm.reserve(ld));
+  HIPCHK(c.pstart.reserve((size_t)c.g.C + 1));
+  HIPCHK(c.pscratch.reserve((size_t)c.g.C + 1 + (size_t)(c.g.C + 1) / 1024 + 64));
+  c.sorted_valid = false;
   HIPCHK(c.vel.reserve(ld * dim));
   if (c.advection) {
     HIPCHK(c.x0.reserve(ld * dim));
@@ -335,6 +395,7 @@ static int fill_points(Ctx& c, int n, const double* coords, const double* values
   for (int i = 0; i < n; i++) c.h_ppid[i] = pidx ? pidx[i] : (int64_t)i + 1;
   // DMSwarmPICSetCellId + SwarmInterpSetWeights (interface_utils.c:118-138)
   launch_locate_particles(c.g, c.px.p, c.ld, n, c.pcell.p, c.alive.p, c.counters.p, c.s);
+  c.sorted_valid = false;
   launch_weights(c.g, c.pux.p, c.puy.p, c.puz.p, c.px.p, c.ld, n, c.pcell.p, c.alive.p, c.w.p, c.s);
   HIPCHK(hipStreamSynchronize(c.s));
   c.points_filled = true;
@@ -342,6 +403,98 @@ static int fill_points(Ctx& c, int n, const double* coords, const double* values
 }
 
 // ---------------------------------------------------------------------------
+// Factor slots: n*n elements in the FP64 pool (Bunch-Kaufman, FP64 Cholesky) or
+// the FP32 pool (FP32 Cholesky of the mixed path).
+static int alloc_factor_slots(Ctx& c, const std::vector<int32_t>& l, int m) {
+  if (l.empty()) return 0;
+  size_t& used = m == M_CHOL32 ? c.fac32_used : c.fac_used;
+  size_t need = used;
+  for (int p : l) { c.h_fac_off[p] = (int64_t)need; need += (size_t)c.h_n[p] * c.h_n[p]; }
+  if (m == M_CHOL32) HIPCHK(c.fac32_pool.grow(need + 1, used, c.s));
+  else HIPCHK(c.fac_pool.grow(need + 1, used, c.s));
+  used = need;
+  return 0;
+}
+
+static int upload_list(Ctx& c, const std::vector<int32_t>& l, int seg) {
+  HIPCHK(hipMemcpyAsync(c.list.p + (size_t)seg * c.g.P, l.data(), sizeof(int32_t) * l.size(), hipMemcpyHostToDevice, c.s));
+  return 0;
+}
+
+// Factor the new patches of each method; FP32/FP64 Cholesky breakdowns are
+// refactored with Bunch-Kaufman in FP64 (counted in "fallback").
+static int factor_lists(Ctx& c, std::vector<int32_t>* fl) {
+  const int P = c.g.P;
+  for (int m = 0; m < 3; m++) if (!fl[m].empty()) CHK(upload_list(c, fl[m], m));
+  if (!fl[M_BK64].empty())
+    launch_factor_bk(c.g, c.ktype, c.list.p, (int)fl[M_BK64].size(), c.p_n.p, c.p_mem_off.p, c.p_fac_off.p, c.mem_pool.p,
+                     c.dx.p, c.fac_pool.p, c.piv_pool.p, c.info.p, c.s);
+  for (int m = M_CHOL64; m <= M_CHOL32; m++)
+    if (!fl[m].empty())
+      launch_factor_chol(c.g, c.ktype, m == M_CHOL32, c.list.p + (size_t)m * P, (int)fl[m].size(), c.p_n.p, c.p_mem_off.p,
+                         c.p_fac_off.p, c.mem_pool.p, c.dx.p, m == M_CHOL32 ? (void*)c.fac32_pool.p : (void*)c.fac_pool.p,
+                         c.pstatus.p, c.s);
+  if (fl[M_CHOL64].empty() && fl[M_CHOL32].empty()) return 0;
+  std::vector<int32_t> st(P);
+  HIPCHK(hipMemcpyAsync(st.data(), c.pstatus.p, sizeof(int32_t) * P, hipMemcpyDeviceToHost, c.s));
+  HIPCHK(hipStreamSynchronize(c.s));
+  std::vector<int32_t> redo;
+  for (int m = M_CHOL64; m <= M_CHOL32; m++)
+    for (int p : fl[m]) if (st[p] != 0) { redo.push_back(p); c.h_method[p] = M_BK64; }
+  if (redo.empty()) return 0;
+  c.fallback += (double)redo.size();
+  CHK(alloc_factor_slots(c, redo, M_BK64));
+  HIPCHK(hipMemcpyAsync(c.p_fac_off.p, c.h_fac_off.data(), sizeof(int64_t) * P, hipMemcpyHostToDevice, c.s));
+  CHK(upload_list(c, redo, 0));
+  launch_factor_bk(c.g, c.ktype, c.list.p, (int)redo.size(), c.p_n.p, c.p_mem_off.p, c.p_fac_off.p, c.mem_pool.p,
+                   c.dx.p, c.fac_pool.p, c.piv_pool.p, c.info.p, c.s);
+  HIPCHK(hipStreamSynchronize(c.s));
+  return 0;
+}
+
+// Solve the coefficients of the listed patches with each patch's factor; an FP32
+// factor whose refinement does not reach the tolerance is replaced by an FP64
+// Bunch-Kaufman factor and solved again (counted in "fallback").
+static int solve_list(Ctx& c, const std::vector<int32_t>& sl) {
+  const int P = c.g.P;
+  std::vector<int32_t> l[3];
+  for (int p : sl) l[c.h_method[p]].push_back(p);
+  for (int m = 0; m < 3; m++) if (!l[m].empty()) CHK(upload_list(c, l[m], m));
+  if (!l[M_BK64].empty())
+    launch_solve_bk(c.list.p, (int)l[M_BK64].size(), c.p_n.p, c.p_mem_off.p, c.p_fac_off.p, c.mem_pool.p, c.fac_pool.p,
+                    c.piv_pool.p, c.dval.p, c.nf, c.coef_pool.p, c.s);
+  if (c.nf > 8 && (!l[M_CHOL64].empty() || !l[M_CHOL32].empty()))
+    return set_err(E_SUP, "PartitionSolve", "Cholesky solve handles up to 8 interpolated fields (found %d)", c.nf);
+  for (int m = M_CHOL64; m <= M_CHOL32; m++)
+    if (!l[m].empty())
+      launch_solve_chol(c.g, c.ktype, m == M_CHOL32, c.list.p + (size_t)m * P, (int)l[m].size(), c.p_n.p, c.p_mem_off.p,
+                        c.p_fac_off.p, c.mem_pool.p, c.dx.p, m == M_CHOL32 ? (const void*)c.fac32_pool.p : (const void*)c.fac_pool.p,
+                        c.dval.p, c.nf, c.coef_pool.p, c.refine_max_iter, c.refine_tol, c.pstatus.p, c.piters.p, c.s);
+  if (l[M_CHOL32].empty()) return 0;
+  std::vector<int32_t> st(P), its(P);
+  HIPCHK(hipMemcpyAsync(st.data(), c.pstatus.p, sizeof(int32_t) * P, hipMemcpyDeviceToHost, c.s));
+  HIPCHK(hipMemcpyAsync(its.data(), c.piters.p, sizeof(int32_t) * P, hipMemcpyDeviceToHost, c.s));
+  HIPCHK(hipStreamSynchronize(c.s));
+  std::vector<int32_t> redo;
+  for (int p : l[M_CHOL32]) {
+    c.refine_iters_max = std::max(c.refine_iters_max, (double)its[p]);
+    c.refine_iters_sum += its[p];
+    c.refined_patches += 1;
+    if (st[p] != 0) { redo.push_back(p); c.h_method[p] = M_BK64; }
+  }
+  if (redo.empty()) return 0;
+  c.fallback += (double)redo.size();
+  CHK(alloc_factor_slots(c, redo, M_BK64));
+  HIPCHK(hipMemcpyAsync(c.p_fac_off.p, c.h_fac_off.data(), sizeof(int64_t) * P, hipMemcpyHostToDevice, c.s));
+  CHK(upload_list(c, redo, 0));
+  launch_factor_bk(c.g, c.ktype, c.list.p, (int)redo.size(), c.p_n.p, c.p_mem_off.p, c.p_fac_off.p, c.mem_pool.p,
+                   c.dx.p, c.fac_pool.p, c.piv_pool.p, c.info.p, c.s);
+  launch_solve_bk(c.list.p, (int)redo.size(), c.p_n.p, c.p_mem_off.p, c.p_fac_off.p, c.mem_pool.p, c.fac_pool.p,
+                  c.piv_pool.p, c.dval.p, c.nf, c.coef_pool.p, c.s);
+  HIPCHK(hipStreamSynchronize(c.s));
+  return 0;
+}
+
 // LagRBF_PartitionsSetUp (partition.c:417-478): touch the corner patches of every
 // particle's cell, set up (gather + assemble + factor) the new ones, solve the
 // unsolved ones.  Batched: one launch per phase over the patch list.
@@ -357,46 +510,50 @@ static int partitions_setup(Ctx& c) {
   HIPCHK(hipMemcpyAsync(t.data(), c.touched.p, P, hipMemcpyDeviceToHost, c.s));
   HIPCHK(hipStreamSynchronize(c.s));
   std::vector<int32_t> newl, solvel;
-  size_t need_mem = c.mem_used, need_fac = c.fac_used;
+  std::vector<int32_t> fl[3];   // new patches per factor method
+  size_t need_mem = c.mem_used;
   for (int p = 0; p < P; p++) {
     if (!t[p]) continue;
     if (!c.h_setup[p]) {
       const int n = c.data_filled ? c.h_pcount[p] : 0;
       c.h_n[p] = n;
       c.h_mem_off[p] = (int64_t)need_mem;
-      c.h_fac_off[p] = (int64_t)need_fac;
       need_mem += n;
-      need_fac += (size_t)n * n;
       newl.push_back(p);
       c.max_patch = std::max(c.max_patch, n);
+      int m = M_BK64;
+      if (c.factor_mode != 1 && chol_kernel_supported(c.ktype) && n <= chol_max_n()) m = c.precision ? M_CHOL32 : M_CHOL64;
+      c.h_method[p] = (uint8_t)m;
+      if (n > 0) fl[m].push_back(p);
     }
   }
   if (!newl.empty()) {
     HIPCHK(c.mem_pool.grow(need_mem + 1, c.mem_used, c.s));
     HIPCHK(c.piv_pool.grow(need_mem + 1, c.mem_used, c.s));
     HIPCHK(c.coef_pool.grow((need_mem + 1) * std::max(c.nf, 1), c.mem_used * std::max(c.nf, 1), c.s));
-    HIPCHK(c.fac_pool.grow(need_fac + 1, c.fac_used, c.s));
     c.mem_used = need_mem;
-    c.fac_used = need_fac;
+    for (int m = 0; m < 3; m++) CHK(alloc_factor_slots(c, fl[m], m));
     HIPCHK(hipMemcpyAsync(c.p_n.p, c.h_n.data(), sizeof(int32_t) * P, hipMemcpyHostToDevice, c.s));
     HIPCHK(hipMemcpyAsync(c.p_mem_off.p, c.h_mem_off.data(), sizeof(int64_t) * P, hipMemcpyHostToDevice, c.s));
     HIPCHK(hipMemcpyAsync(c.p_fac_off.p, c.h_fac_off.data(), sizeof(int64_t) * P, hipMemcpyHostToDevice, c.s));
     HIPCHK(hipMemcpyAsync(c.list.p, newl.data(), sizeof(int32_t) * newl.size(), hipMemcpyHostToDevice, c.s));
     launch_patch_gather(c.g, c.list.p, (int)newl.size(), c.cell_start.p, c.cell_nodes.p, c.p_mem_off.p, c.mem_pool.p, c.s);
-    launch_factor_bk(c.g, c.ktype, c.list.p, (int)newl.size(), c.p_n.p, c.p_mem_off.p, c.p_fac_off.p, c.mem_pool.p,
-                     c.dx.p, c.fac_pool.p, c.piv_pool.p, c.info.p, c.s);
+    CHK(factor_lists(c, fl));
     for (int p : newl) c.h_setup[p] = 1;
+    if (c.h_cell_start_prev.empty() && c.data_filled) {
+      // first patches of this generation: remember the cell lists they were built from
+      c.h_cell_start_prev.resize(c.g.C + 1);
+      c.h_cell_nodes_prev.resize(c.N);
+      HIPCHK(hipMemcpyAsync(c.h_cell_start_prev.data(), c.cell_start.p, sizeof(int32_t) * (c.g.C + 1), hipMemcpyDeviceToHost, c.s));
+      if (c.N > 0) HIPCHK(hipMemcpyAsync(c.h_cell_nodes_prev.data(), c.cell_nodes.p, sizeof(int32_t) * c.N, hipMemcpyDeviceToHost, c.s));
+    }
     c.patches_setup += (double)newl.size();
     c.setup_N = c.N;
     HIPCHK(hipStreamSynchronize(c.s));   // list buffer is reused below
   }
   for (int p = 0; p < P; p++)
     if (t[p] && c.h_setup[p] && !c.h_solved[p] && c.h_n[p] > 0) solvel.push_back(p);
-  if (!solvel.empty()) {
-    HIPCHK(hipMemcpyAsync(c.list.p, solvel.data(), sizeof(int32_t) * solvel.size(), hipMemcpyHostToDevice, c.s));
-    launch_solve_bk(c.list.p, (int)solvel.size(), c.p_n.p, c.p_mem_off.p, c.p_fac_off.p, c.mem_pool.p, c.fac_pool.p,
-                    c.piv_pool.p, c.dval.p, c.nf, c.coef_pool.p, c.s);
-  }
+  if (!solvel.empty()) CHK(solve_list(c, solvel));
   for (int p : solvel) c.h_solved[p] = 1;
   for (int p = 0; p < P; p++) if (t[p] && c.h_setup[p] && c.h_n[p] == 0) c.h_solved[p] = 1;
   bool changed = false;
@@ -423,7 +580,9 @@ static int partitions_reset(Ctx& c) {
     std::fill(c.h_solved.begin(), c.h_solved.end(), 0);
     std::fill(c.h_n.begin(), c.h_n.end(), 0);
     std::fill(c.h_active.begin(), c.h_active.end(), 0);
-    c.mem_used = 0; c.fac_used = 0; c.setup_N = -1;
+    c.mem_used = 0; c.fac_used = 0; c.fac32_used = 0; c.setup_N = -1;
+    c.h_cell_start_prev.clear(); c.h_cell_nodes_prev.clear();
+    c.memb_current = true;
     HIPCHK(hipMemsetAsync(c.p_active.p, 0, P, c.s));
     HIPCHK(hipMemsetAsync(c.p_n.p, 0, sizeof(int32_t) * P, c.s));
   } else {
@@ -442,16 +601,32 @@ static int rbf_interpolation(Ctx& c, double* out, int nfields) {
     HIPCHK(c.mem_pool.reserve(1)); HIPCHK(c.coef_pool.reserve(1));
     pv.members = c.mem_pool.p; pv.coef = c.coef_pool.p;
   }
+  const bool cell_order = c.eval_mode == 1 || (c.eval_mode == 0 && c.memb_current);
+  if (c.eval_mode == 1 && !c.memb_current)
+    return set_err(E_STATE, "RBFInterpolation", "-lagrbf_eval cell needs patches built from the current data");
+  if (cell_order && !c.sorted_valid) {
+    launch_sort_particles(c.pcell.p, c.alive.p, c.Np, c.g.C, c.pstart.p, c.perm.p, c.pscratch.p, c.s);
+    c.sorted_valid = true;
+  }
+  if (!c.zeros.p) { HIPCHK(c.zeros.reserve(1)); HIPCHK(hipMemset(c.zeros.p, 0, sizeof(double))); }
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0)); HIPCHK(hipEventCreate(&e1));
   HIPCHK(hipEventRecord(e0, c.s));
-  launch_eval_ref(c.g, c.ktype, c.px.p, c.ld, c.Np, c.pcell.p, c.alive.p, c.w.p, pv, c.dx.p ? c.dx.p : c.pux.p,
-                  c.cell_start.p, nfields, out, c.s);
+  if (cell_order) {
+    const int per_cell = (c.Np + c.g.C - 1) / std::max(c.g.C, 1);
+    const int block = std::min(256, std::max(64, (per_cell + 63) / 64 * 64));
+    launch_eval_cell(c.g, c.ktype, c.px.p, c.ld, c.perm.p, c.pstart.p, c.w.p, pv, c.zeros.p,
+                     c.xs_sorted.p ? c.xs_sorted.p : c.pux.p, c.N, c.cell_start.p, nfields, out, block, c.s);
+  } else {
+    launch_eval_ref(c.g, c.ktype, c.px.p, c.ld, c.Np, c.pcell.p, c.alive.p, c.w.p, pv, c.dx.p ? c.dx.p : c.pux.p,
+                    c.cell_start.p, nfields, out, c.s);
+  }
   HIPCHK(hipEventRecord(e1, c.s));
   HIPCHK(hipEventSynchronize(e1));
   float ms = 0.f;
   HIPCHK(hipEventElapsedTime(&ms, e0, e1));
   c.eval_ms += ms;
+  c.eval_count += 1;
   HIPCHK(hipEventDestroy(e0)); HIPCHK(hipEventDestroy(e1));
   return 0;
 }
@@ -528,6 +703,7 @@ static StageArgs stage_args(int nv, const double* const* v, const double* a, dou
 }
 
 static void stage(Ctx& c, const double* x0, const StageArgs& a) {
+  c.sorted_valid = false;
   launch_stage(c.g, c.pux.p, c.puy.p, c.puz.p, c.px.p, x0, c.vel.p, c.ld, c.Np, c.pcell.p, c.alive.p, c.counters.p,
                c.w.p, a, c.s);
 }
@@ -881,12 +1057,15 @@ int lagrbf_get_info(void* ctx, const char* key, double* value) {
   if (!key || !value) return set_err(E_NULL, "lagrbf_get_info", "key/value is NULL");
   std::string k(key);
   if (k == "removed") { int e = update_removed(*c); if (e) return e; *value = c->removed; }
-  else if (k == "fallback") *value = 0.0;
-  else if (k == "refine_iters") *value = 0.0;
+  else if (k == "fallback") *value = c->fallback;
+  else if (k == "refine_iters") *value = c->refine_iters_max;
+  else if (k == "refine_iters_mean") *value = c->refined_patches > 0 ? c->refine_iters_sum / c->refined_patches : 0.0;
+  else if (k == "precision") *value = c->precision;
   else if (k == "patches_setup") *value = c->patches_setup;
   else if (k == "dt") *value = c->adv_dt;
   else if (k == "dopri_err") *value = c->dopri_err;
   else if (k == "eval_ms") *value = c->eval_ms;
+  else if (k == "eval_count") *value = c->eval_count;
   else if (k == "setup_ms") *value = c->setup_ms;
   else if (k == "n_particles") *value = c->Np;
   else if (k == "n_nodes") *value = c->N;

@@ -47,12 +47,32 @@ void launch_solve_bk(const int32_t* list, int nlist, const int32_t* pn, const in
                      const int64_t* fac_off, const int32_t* mem_pool, const double* fac, const int32_t* piv,
                      const double* values, int nf, double* coef, hipStream_t s);
 
+// blocked Cholesky (positive definite kernels), FP64 or FP32 factor; the FP32
+// solve refines FP64 coefficients against the FP64 matrix (patch_chol.hip)
+bool chol_kernel_supported(int ktype);
+int chol_max_n();
+void launch_factor_chol(const Grid& g, int ktype, bool fp32, const int32_t* list, int nlist, const int32_t* pn,
+                        const int64_t* mem_off, const int64_t* fac_off, const int32_t* mem_pool, const double* xd,
+                        void* fac, int32_t* status, hipStream_t s);
+void launch_solve_chol(const Grid& g, int ktype, bool fp32, const int32_t* list, int nlist, const int32_t* pn,
+                       const int64_t* mem_off, const int64_t* fac_off, const int32_t* mem_pool, const double* xd,
+                       const void* fac, const double* values, int nf, double* coef, int max_iter, double tol,
+                       int32_t* status, int32_t* iters, hipStream_t s);
+
 // ---- weights / evaluation ----
 void launch_weights(const Grid& g, const double* pux, const double* puy, const double* puz, const double* px,
                     int64_t ld, int Np, const int32_t* cell, const int32_t* alive, double* w, hipStream_t s);
 void launch_eval_ref(const Grid& g, int ktype, const double* px, int64_t ld, int Np, const int32_t* cell,
                      const int32_t* alive, const double* w, const PatchView& pv, const double* xdata,
                      const int32_t* cell_start, int nfields, double* out, hipStream_t s);
+
+// cell-ordered evaluation: particles grouped by cell, nodes in cell-sorted SoA
+void launch_sort_particles(const int32_t* cell, const int32_t* alive, int Np, int C, int32_t* pstart, int32_t* perm,
+                           int32_t* scratch /* >= C+1 + scan blocks */, hipStream_t s);
+void launch_sorted_coords(const double* x, int n, int dim, const int32_t* cell_nodes, double* xs, hipStream_t s);
+void launch_eval_cell(const Grid& g, int ktype, const double* px, int64_t ld, const int32_t* perm, const int32_t* pstart,
+                      const double* w, const PatchView& pv, const double* zeros, const double* xs, int N,
+                      const int32_t* cell_start, int nfields, double* out, int block, hipStream_t s);
 
 // ---- RK stage loop ----
 void launch_stage(const Grid& g, const double* pux, const double* puy, const double* puz, double* px,

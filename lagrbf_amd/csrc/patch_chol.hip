@@ -1,0 +1,300 @@
+// Batched Cholesky factorisation and solve of the patch RBF systems for the
+// positive definite kernels (Wendland C0/C2, Gaussian, inverse multiquadric).
+//
+// One workgroup per patch.  Left-looking, blocked by NB columns: the current
+// panel A[J:n, J:J+NB] is assembled on the fly (phi of member distances, never
+// stored in HBM), updated with the already factored columns L[J:n, 0:J] read
+// from the patch's factor slot (L2/MALL resident), factored in LDS and written
+// back.  T = double gives the FP64 path, T = float the FP32 factor of the
+// mixed path, whose solve refines the FP64 coefficients against the FP64
+// matrix recomputed on the fly (iterative refinement, residual in FP64).
+//
+// Coordinates are shifted to the patch's PU node in FP64 before any rounding to
+// T, so the FP32 matrix carries relative, not absolute, rounding.
+#include "kernels.hpp"
+
+#include <cmath>
+
+namespace lrb {
+
+static constexpr int CW = 64;          // wavefront
+static constexpr int CNT = 256;        // threads per patch
+static constexpr int CNB = 32;         // panel width
+static constexpr int CTT = 64;         // k-chunk of the panel update
+static constexpr int CNMAX = 512;      // largest patch handled here
+
+// loop over the nf <= 8 data columns with the register arrays kept in VGPRs
+#define FOR_F _Pragma("unroll") for (int f = 0; f < 8; f++) if (f < nf)
+
+__device__ __forceinline__ double patch_centre(const Grid& g, int p, int d) {
+  int idx = d == 0 ? p % g.n[0] : (d == 1 ? (p / g.n[0]) % g.n[1] : p / (g.n[0] * g.n[1]));
+  return g.O[d] + idx * g.h[d];
+}
+
+template <typename T, int KT>
+__global__ __launch_bounds__(CNT) void k_factor_chol(Grid g, const int32_t* __restrict__ list,
+                                                     const int32_t* __restrict__ pn, const int64_t* __restrict__ mem_off,
+                                                     const int64_t* __restrict__ fac_off,
+                                                     const int32_t* __restrict__ mem_pool,
+                                                     const double* __restrict__ xd, T* __restrict__ fac,
+                                                     int32_t* __restrict__ status) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int p = list[blockIdx.x];
+  const int n = pn[p];
+  if (n == 0) { if (threadIdx.x == 0) status[p] = 0; return; }
+  T* xr = reinterpret_cast<T*>(smem);                 // [3][CNMAX] member coords relative to the patch node
+  T* P = xr + 3 * CNMAX;                              // panel [rows][CNB] column-major, ld = rows
+  T* Lr = P + (size_t)CNMAX * CNB;                    // [CNB][CTT] rows J..J+w of L, columns t0..t0+CTT
+  __shared__ int s_fail;
+  const int tid = threadIdx.x;
+  const int dim = g.dim;
+  const int32_t* mem = mem_pool + mem_off[p];
+  T* L = fac + fac_off[p];
+  if (tid == 0) s_fail = 0;
+  for (int i = tid; i < n; i += CNT) {
+    const double* x = xd + (size_t)mem[i] * dim;
+    for (int d = 0; d < 3; d++) xr[d * CNMAX + i] = d < dim ? (T)(x[d] - patch_centre(g, p, d)) : T(0);
+  }
+  __syncthreads();
+  const T eps = (T)g.eps_r;
+  for (int J = 0; J < n; J += CNB) {
+    const int w = min(CNB, n - J), rows = n - J;
+    for (int r0 = 0; r0 < rows; r0 += CNT) {
+      const int r = r0 + tid;
+      const bool live = r < rows;
+      const int gi = J + (live ? r : 0);
+      T acc[CNB];
+      const T xi0 = xr[gi], xi1 = xr[CNMAX + gi], xi2 = xr[2 * CNMAX + gi];
+#pragma unroll
+      for (int c = 0; c < CNB; c++) {
+        const int gj = J + (c < w ? c : 0);
+        const T d0 = xi0 - xr[gj], d1 = xi1 - xr[CNMAX + gj], d2 = xi2 - xr[2 * CNMAX + gj];
+        acc[c] = rbf<KT, T>(sqrt_fast(d0 * d0 + d1 * d1 + d2 * d2), eps);
+      }
+      for (int t0 = 0; t0 < J; t0 += CTT) {
+        const int tw = min(CTT, J - t0);
+        __syncthreads();
+        for (int e = tid; e < CNB * CTT; e += CNT) {
+          const int c = e / CTT, tt = e % CTT;
+          Lr[e] = (c < w && tt < tw) ? L[(size_t)(t0 + tt) * n + J + c] : T(0);
+        }
+        __syncthreads();
+        if (live) {
+          for (int tt = 0; tt < tw; tt++) {
+            const T lit = L[(size_t)(t0 + tt) * n + gi];
+#pragma unroll
+            for (int c = 0; c < CNB; c++) acc[c] -= lit * Lr[c * CTT + tt];
+          }
+        }
+      }
+      if (live)
+#pragma unroll
+        for (int c = 0; c < CNB; c++) if (c < w) P[(size_t)c * rows + r] = acc[c];
+    }
+    __syncthreads();
+    // unblocked right-looking factorisation of the panel (rows J..n-1, w columns)
+    for (int c = 0; c < w; c++) {
+      const T pcc = P[(size_t)c * rows + c];
+      if (!(pcc > T(0))) { if (tid == 0) s_fail = J + c + 1; }
+      const T dd = sqrt(pcc > T(0) ? pcc : T(1));
+      const T inv = T(1) / dd;
+      for (int r = c + 1 + tid; r < rows; r += CNT) P[(size_t)c * rows + r] *= inv;
+      __syncthreads();
+      if (tid == 0) P[(size_t)c * rows + c] = dd;
+      for (int r = c + 1 + tid; r < rows; r += CNT) {
+        const T lrc = P[(size_t)c * rows + r];
+        const int cmax = min(w - 1, r);
+        for (int c2 = c + 1; c2 <= cmax; c2++) P[(size_t)c2 * rows + r] -= lrc * P[(size_t)c * rows + c2];
+      }
+      __syncthreads();
+    }
+    for (int c = 0; c < w; c++)
+      for (int r = c + tid; r < rows; r += CNT) L[(size_t)(J + c) * n + J + r] = P[(size_t)c * rows + r];
+    __syncthreads();
+  }
+  if (tid == 0) status[p] = s_fail;
+}
+
+// Solve L L^T X = B for the nf data columns of a patch, B gathered from the node
+// values (PartitionMatSetRHS, partition_factor.c:57-90).  With a float factor the
+// FP64 solution is refined: r = B - A X with A recomputed in FP64, X += solve(r),
+// until the correction is below tol * |X| (or max_iter); status 1 = not converged.
+template <typename TL, int KT>
+__global__ __launch_bounds__(CNT) void k_solve_chol(Grid g, const int32_t* __restrict__ list, const int32_t* __restrict__ pn,
+                                                    const int64_t* __restrict__ mem_off, const int64_t* __restrict__ fac_off,
+                                                    const int32_t* __restrict__ mem_pool, const double* __restrict__ xd,
+                                                    const TL* __restrict__ fac, const double* __restrict__ values, int nf,
+                                                    double* __restrict__ coef, int max_iter, double tol,
+                                                    int32_t* __restrict__ status, int32_t* __restrict__ iters) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  double* xr = reinterpret_cast<double*>(smem);      // [3][CNMAX] relative coords (FP64)
+  double* b = xr + 3 * CNMAX;                        // [nf][n] right-hand side / residual
+  double* y = b + (size_t)8 * CNMAX;                 // [nf][n] work vector
+  __shared__ double red[CNT / CW][8];
+  const int p = list[blockIdx.x];
+  const int n = pn[p];
+  const int tid = threadIdx.x, lane = tid & (CW - 1), wid = tid / CW;
+  if (n == 0 || nf == 0) return;
+  const int dim = g.dim;
+  const int32_t* mem = mem_pool + mem_off[p];
+  const TL* L = fac + fac_off[p];
+  double* X = coef + mem_off[p] * nf;                // row-major [n][nf]
+  for (int i = tid; i < n; i += CNT) {
+    const double* x = xd + (size_t)mem[i] * dim;
+    for (int d = 0; d < 3; d++) xr[d * CNMAX + i] = d < dim ? x[d] - patch_centre(g, p, d) : 0.0;
+    for (int f = 0; f < nf; f++) { const double v = values[(size_t)mem[i] * nf + f]; b[f * CNMAX + i] = v; X[(size_t)i * nf + f] = 0.0; }
+  }
+  __syncthreads();
+  const bool refine = sizeof(TL) == sizeof(float);
+  int it = 0;
+  int converged = 0;
+  for (; it <= (refine ? max_iter : 0); it++) {
+    if (it > 0) {
+      // residual b - A X in FP64, A recomputed from the member coordinates
+      for (int i = tid; i < n; i += CNT) {
+        double s[8];
+        FOR_F s[f] = values[(size_t)mem[i] * nf + f];
+        const double xi0 = xr[i], xi1 = xr[CNMAX + i], xi2 = xr[2 * CNMAX + i];
+        for (int j = 0; j < n; j++) {
+          const double d0 = xi0 - xr[j], d1 = xi1 - xr[CNMAX + j], d2 = xi2 - xr[2 * CNMAX + j];
+          const double a = rbf<KT, double>(sqrt_fast(d0 * d0 + d1 * d1 + d2 * d2), g.eps_r);
+          FOR_F s[f] -= a * X[(size_t)j * nf + f];
+        }
+        FOR_F b[f * CNMAX + i] = s[f];
+      }
+      __syncthreads();
+    }
+    for (int f = 0; f < nf; f++)
+      for (int i = tid; i < n; i += CNT) y[f * CNMAX + i] = b[f * CNMAX + i];
+    __syncthreads();
+    // forward: L z = b (column oriented)
+    for (int k = 0; k < n; k++) {
+      const double lkk = (double)L[(size_t)k * n + k];
+      double zk[8];
+      FOR_F zk[f] = y[f * CNMAX + k] / lkk;
+      for (int i = k + 1 + tid; i < n; i += CNT) {
+        const double lik = (double)L[(size_t)k * n + i];
+        FOR_F y[f * CNMAX + i] -= lik * zk[f];
+      }
+      __syncthreads();
+      if (tid == 0) FOR_F y[f * CNMAX + k] = zk[f];
+    }
+    __syncthreads();
+    // backward: L^T d = z (dot products down each column)
+    for (int k = n - 1; k >= 0; k--) {
+      double s[8];
+      FOR_F s[f] = 0.0;
+      for (int i = k + 1 + tid; i < n; i += CNT) {
+        const double lik = (double)L[(size_t)k * n + i];
+        FOR_F s[f] += lik * y[f * CNMAX + i];
+      }
+      FOR_F {
+        double v = s[f];
+        for (int o = CW / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
+        if (lane == 0) red[wid][f] = v;
+      }
+      __syncthreads();
+      if (tid < nf) {
+        double t = red[0][tid];
+        for (int w2 = 1; w2 < CNT / CW; w2++) t += red[w2][tid];
+        y[tid * CNMAX + k] = (y[tid * CNMAX + k] - t) / (double)L[(size_t)k * n + k];
+      }
+      __syncthreads();
+    }
+    // X += d; convergence test on max |d| / max |X|
+    double dmax = 0.0, xmax = 0.0;
+    for (int i = tid; i < n; i += CNT)
+      for (int f = 0; f < nf; f++) {
+        const double d = y[f * CNMAX + i];
+        const double xn = X[(size_t)i * nf + f] + d;
+        X[(size_t)i * nf + f] = xn;
+        dmax = fmax(dmax, fabs(d));
+        xmax = fmax(xmax, fabs(xn));
+      }
+    for (int o = CW / 2; o > 0; o >>= 1) { dmax = fmax(dmax, __shfl_xor(dmax, o)); xmax = fmax(xmax, __shfl_xor(xmax, o)); }
+    if (lane == 0) { red[wid][0] = dmax; red[wid][1] = xmax; }
+    __syncthreads();
+    dmax = red[0][0]; xmax = red[0][1];
+    for (int w2 = 1; w2 < CNT / CW; w2++) { dmax = fmax(dmax, red[w2][0]); xmax = fmax(xmax, red[w2][1]); }
+    __syncthreads();
+    if (!refine) { converged = 1; break; }
+    if (it > 0 && dmax <= tol * xmax) { converged = 1; break; }
+  }
+  if (tid == 0) {
+    if (status) status[p] = converged ? 0 : 1;
+    if (iters) iters[p] = it;
+  }
+}
+
+static size_t factor_smem(size_t esz) { return esz * (3 * CNMAX + (size_t)CNMAX * CNB + CNB * CTT); }
+static size_t solve_smem() { return sizeof(double) * (3 * CNMAX + 16 * (size_t)CNMAX); }
+
+int chol_max_n() { return CNMAX; }
+
+template <typename T, int KT>
+static void factor_chol_k(const Grid& g, const int32_t* list, int nlist, const int32_t* pn, const int64_t* mem_off,
+                          const int64_t* fac_off, const int32_t* mem_pool, const double* xd, T* fac, int32_t* status,
+                          hipStream_t s) {
+  static bool attr = false;
+  const size_t sm = factor_smem(sizeof(T));
+  if (!attr) { (void)hipFuncSetAttribute((const void*)k_factor_chol<T, KT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm); attr = true; }
+  hipLaunchKernelGGL((k_factor_chol<T, KT>), dim3(nlist), dim3(CNT), sm, s, g, list, pn, mem_off, fac_off, mem_pool, xd, fac, status);
+}
+
+template <typename TL, int KT>
+static void solve_chol_k(const Grid& g, const int32_t* list, int nlist, const int32_t* pn, const int64_t* mem_off,
+                         const int64_t* fac_off, const int32_t* mem_pool, const double* xd, const TL* fac,
+                         const double* values, int nf, double* coef, int max_iter, double tol, int32_t* status,
+                         int32_t* iters, hipStream_t s) {
+  static bool attr = false;
+  const size_t sm = solve_smem();
+  if (!attr) { (void)hipFuncSetAttribute((const void*)k_solve_chol<TL, KT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm); attr = true; }
+  hipLaunchKernelGGL((k_solve_chol<TL, KT>), dim3(nlist), dim3(CNT), sm, s, g, list, pn, mem_off, fac_off, mem_pool, xd, fac,
+                     values, nf, coef, max_iter, tol, status, iters);
+}
+
+bool chol_kernel_supported(int ktype) { return ktype == K_WC0 || ktype == K_WC2 || ktype == K_GAUSS || ktype == K_IMQ; }
+
+template <typename T>
+static void factor_chol_dispatch(const Grid& g, int ktype, const int32_t* list, int nlist, const int32_t* pn,
+                                 const int64_t* mem_off, const int64_t* fac_off, const int32_t* mem_pool, const double* xd,
+                                 T* fac, int32_t* status, hipStream_t s) {
+  switch (ktype) {
+    case K_WC0: factor_chol_k<T, K_WC0>(g, list, nlist, pn, mem_off, fac_off, mem_pool, xd, fac, status, s); break;
+    case K_WC2: factor_chol_k<T, K_WC2>(g, list, nlist, pn, mem_off, fac_off, mem_pool, xd, fac, status, s); break;
+    case K_IMQ: factor_chol_k<T, K_IMQ>(g, list, nlist, pn, mem_off, fac_off, mem_pool, xd, fac, status, s); break;
+    default: factor_chol_k<T, K_GAUSS>(g, list, nlist, pn, mem_off, fac_off, mem_pool, xd, fac, status, s); break;
+  }
+}
+
+void launch_factor_chol(const Grid& g, int ktype, bool fp32, const int32_t* list, int nlist, const int32_t* pn,
+                        const int64_t* mem_off, const int64_t* fac_off, const int32_t* mem_pool, const double* xd,
+                        void* fac, int32_t* status, hipStream_t s) {
+  if (nlist <= 0) return;
+  if (fp32) factor_chol_dispatch<float>(g, ktype, list, nlist, pn, mem_off, fac_off, mem_pool, xd, (float*)fac, status, s);
+  else factor_chol_dispatch<double>(g, ktype, list, nlist, pn, mem_off, fac_off, mem_pool, xd, (double*)fac, status, s);
+}
+
+template <typename TL>
+static void solve_chol_dispatch(const Grid& g, int ktype, const int32_t* list, int nlist, const int32_t* pn,
+                                const int64_t* mem_off, const int64_t* fac_off, const int32_t* mem_pool, const double* xd,
+                                const TL* fac, const double* values, int nf, double* coef, int max_iter, double tol,
+                                int32_t* status, int32_t* iters, hipStream_t s) {
+  switch (ktype) {
+    case K_WC0: solve_chol_k<TL, K_WC0>(g, list, nlist, pn, mem_off, fac_off, mem_pool, xd, fac, values, nf, coef, max_iter, tol, status, iters, s); break;
+    case K_WC2: solve_chol_k<TL, K_WC2>(g, list, nlist, pn, mem_off, fac_off, mem_pool, xd, fac, values, nf, coef, max_iter, tol, status, iters, s); break;
+    case K_IMQ: solve_chol_k<TL, K_IMQ>(g, list, nlist, pn, mem_off, fac_off, mem_pool, xd, fac, values, nf, coef, max_iter, tol, status, iters, s); break;
+    default: solve_chol_k<TL, K_GAUSS>(g, list, nlist, pn, mem_off, fac_off, mem_pool, xd, fac, values, nf, coef, max_iter, tol, status, iters, s); break;
+  }
+}
+
+void launch_solve_chol(const Grid& g, int ktype, bool fp32, const int32_t* list, int nlist, const int32_t* pn,
+                       const int64_t* mem_off, const int64_t* fac_off, const int32_t* mem_pool, const double* xd,
+                       const void* fac, const double* values, int nf, double* coef, int max_iter, double tol,
+                       int32_t* status, int32_t* iters, hipStream_t s) {
+  if (nlist <= 0) return;
+  if (fp32) solve_chol_dispatch<float>(g, ktype, list, nlist, pn, mem_off, fac_off, mem_pool, xd, (const float*)fac, values, nf, coef, max_iter, tol, status, iters, s);
+  else solve_chol_dispatch<double>(g, ktype, list, nlist, pn, mem_off, fac_off, mem_pool, xd, (const double*)fac, values, nf, coef, max_iter, tol, status, iters, s);
+}
+
+}  // namespace lrb

@@ -80,6 +80,24 @@ def test_interp_3d(solve_method):
     assert rel_err(res[0][0], res[1][0]) < TOL_VAL
 
 
+@pytest.mark.parametrize("dim,kernel,mode", [(3, 2, "cell"), (3, 2, "ref"), (2, 2, "cell"), (2, 0, "cell"),
+                                             (3, 6, "cell"), (2, 1, "ref")])
+def test_eval_kernels_vs_oracle(dim, kernel, mode):
+    """Both evaluation kernels (cell-ordered and per-patch reference order) against the oracle."""
+    if dim == 3:
+        opts, X, V, P = cases.fourier_3d(nlat=24, Np=6000, n=6, kernel=kernel)
+    else:
+        opts, X, V, P = cases.rotation_2d(N=3000, Np=6000, n=11, kernel=kernel, advection=False)
+    if kernel == 6:
+        opts["pu_mesh_shape_factor"] = 10.0
+    opts["lagrbf_eval"] = mode
+    print(f"case: eval dim={dim} kernel={kernel} mode={mode}", flush=True)
+    g, o, res = interp_both(opts, X, V, P)
+    err = rel_err(res[0][0], res[1][0])
+    print(f"max rel err {err:.3e}")
+    assert err < (TOL_VAL if kernel != 0 else 1e-11)
+
+
 def test_binning_bit_exact():
     opts, X, V, P = cases.fourier_3d(nlat=16, Np=3000, n=5)
     print("case: binning 3D", flush=True)
