@@ -104,6 +104,7 @@ struct Ctx {
   DBuf<int32_t> dcell, cell_start, cell_nodes, scratch, pcount;
   std::vector<int32_t> h_pcount;
   DBuf<double> xs_sorted, zeros;           // cell-sorted node coordinates (SoA), zero coefficients
+  DBuf<float> xs32, zeros32, coef32_pool;  // mixed path: cell-relative FP32 coordinates, FP32 coefficients
   std::vector<int32_t> h_cell_start_prev, h_cell_nodes_prev;   // cell lists the frozen patches were built from
   bool memb_current = true;                // every set-up patch's members == current cell lists
   int eval_mode = 0;                       // 0 auto (cell-ordered when possible), 1 cell-ordered, 2 reference order
@@ -332,6 +333,13 @@ static int fill_data(Ctx& c, int n, const double* coords, const double* values, 
   for (int p = 0; p < c.g.P; p++) maxp = std::max(maxp, c.h_pcount[p]);
   HIPCHK(c.zeros.reserve((size_t)maxp * std::max(nf, 1)));
   HIPCHK(hipMemset(c.zeros.p, 0, sizeof(double) * (size_t)maxp * std::max(nf, 1)));
+  if (c.precision) {
+    HIPCHK(c.xs32.reserve((size_t)std::max(n, 1) * dim));
+    launch_sorted_coords32(c.g, c.dx.p, n, c.cell_nodes.p, c.dcell.p, c.xs32.p, c.s);
+    HIPCHK(c.zeros32.reserve((size_t)maxp * std::max(nf, 1)));
+    HIPCHK(hipMemset(c.zeros32.p, 0, sizeof(float) * (size_t)maxp * std::max(nf, 1)));
+    HIPCHK(hipStreamSynchronize(c.s));
+  }
   // patches kept from an earlier fill (solve_method 0/1) were built from the cell
   // lists of that fill: the cell-ordered evaluation needs them unchanged
   if (c.mem_used > 0 && c.memb_current) {
@@ -531,6 +539,7 @@ static int partitions_setup(Ctx& c) {
     HIPCHK(c.mem_pool.grow(need_mem + 1, c.mem_used, c.s));
     HIPCHK(c.piv_pool.grow(need_mem + 1, c.mem_used, c.s));
     HIPCHK(c.coef_pool.grow((need_mem + 1) * std::max(c.nf, 1), c.mem_used * std::max(c.nf, 1), c.s));
+    if (c.precision) HIPCHK(c.coef32_pool.grow((need_mem + 1) * std::max(c.nf, 1), c.mem_used * std::max(c.nf, 1), c.s));
     c.mem_used = need_mem;
     for (int m = 0; m < 3; m++) CHK(alloc_factor_slots(c, fl[m], m));
     HIPCHK(hipMemcpyAsync(c.p_n.p, c.h_n.data(), sizeof(int32_t) * P, hipMemcpyHostToDevice, c.s));
@@ -553,7 +562,15 @@ static int partitions_setup(Ctx& c) {
   }
   for (int p = 0; p < P; p++)
     if (t[p] && c.h_setup[p] && !c.h_solved[p] && c.h_n[p] > 0) solvel.push_back(p);
-  if (!solvel.empty()) CHK(solve_list(c, solvel));
+  if (!solvel.empty()) {
+    CHK(solve_list(c, solvel));
+    if (c.precision) {
+      // FP32 copy of the refined coefficients for the FP32 evaluation
+      CHK(upload_list(c, solvel, 0));
+      launch_coef_to_f32(c.list.p, (int)solvel.size(), c.p_n.p, c.p_mem_off.p, c.nf, c.coef_pool.p, c.coef32_pool.p, c.s);
+      HIPCHK(hipStreamSynchronize(c.s));
+    }
+  }
   for (int p : solvel) c.h_solved[p] = 1;
   for (int p = 0; p < P; p++) if (t[p] && c.h_setup[p] && c.h_n[p] == 0) c.h_solved[p] = 1;
   bool changed = false;
@@ -615,8 +632,14 @@ static int rbf_interpolation(Ctx& c, double* out, int nfields) {
   if (cell_order) {
     const int per_cell = (c.Np + c.g.C - 1) / std::max(c.g.C, 1);
     const int block = std::min(256, std::max(64, (per_cell + 63) / 64 * 64));
-    launch_eval_cell(c.g, c.ktype, c.px.p, c.ld, c.perm.p, c.pstart.p, c.w.p, pv, c.zeros.p,
-                     c.xs_sorted.p ? c.xs_sorted.p : c.pux.p, c.N, c.cell_start.p, nfields, out, block, c.s);
+    const bool f32 = c.precision && eval_f32_supported(c.ktype) && c.xs32.p && c.coef32_pool.p;
+    if (f32)
+      launch_eval_cell(c.g, c.ktype, true, c.px.p, c.ld, c.perm.p, c.pstart.p, c.w.p, c.p_active.p, c.p_mem_off.p,
+                       c.coef32_pool.p, c.nf, c.zeros32.p, c.xs32.p, c.N, c.cell_start.p, nfields, out, block, c.s);
+    else
+      launch_eval_cell(c.g, c.ktype, false, c.px.p, c.ld, c.perm.p, c.pstart.p, c.w.p, c.p_active.p, c.p_mem_off.p,
+                       c.coef_pool.p, c.nf, c.zeros.p, c.xs_sorted.p ? c.xs_sorted.p : c.pux.p, c.N, c.cell_start.p,
+                       nfields, out, block, c.s);
   } else {
     launch_eval_ref(c.g, c.ktype, c.px.p, c.ld, c.Np, c.pcell.p, c.alive.p, c.w.p, pv, c.dx.p ? c.dx.p : c.pux.p,
                     c.cell_start.p, nfields, out, c.s);

@@ -641,171 +641,6 @@ void launch_sorted_coords(const double* x, int n, int dim, const int32_t* cn, do
   if (n > 0) hipLaunchKernelGGL(k_sorted_coords, dim3((n + 255) / 256), dim3(256), 0, s, x, n, dim, cn, xs);
 }
 
-template <int DIM, int KT, int NF>
-__global__ __launch_bounds__(256) void k_eval_cell(Grid g, const double* __restrict__ px, int64_t ld,
-                                                   const int32_t* __restrict__ perm, const int32_t* __restrict__ pstart,
-                                                   const double* __restrict__ w, PatchView pv,
-                                                   const double* __restrict__ zeros, const double* __restrict__ xs,
-                                                   int N, const int32_t* __restrict__ cs, int nfields, int f0,
-                                                   double* __restrict__ out) {
-  constexpr int NEN = 1 << DIM;
-  __shared__ int s_off[NEN][NEN];   // [corner][cell slot of that corner's patch] member offset
-  __shared__ const double* s_X[NEN];
-  // XCD-aware cell order: consecutive cells land on the same XCD (shared L2)
-  const int C = g.C;
-  const int nx = gridDim.x;
-  const int per = (nx + 7) / 8;
-  const int b = blockIdx.x;
-  const int c = (b % 8) * per + b / 8;
-  if (c >= C) return;
-  const int pb = pstart[c], pe = pstart[c + 1];
-  if (pb == pe) return;
-  const int ci = c % g.nc[0], cj = (c / g.nc[0]) % g.nc[1], ck = DIM == 3 ? c / (g.nc[0] * g.nc[1]) : 0;
-  const int tid = threadIdx.x;
-  if (tid < NEN * NEN) {
-    // corner k, slot q = (a,b,cc) of its patch's cells (a fastest); patch node = cell + o_k
-    const int k = tid / NEN, q = tid % NEN;
-    int oi, oj, ok;
-    corner_offset(k, oi, oj, ok);
-    const int ni = ci + oi, nj = cj + oj, nk = ck + ok;   // patch node (i,j,k)
-    int off = 0;
-    for (int qq = 0; qq < q; qq++) {
-      const int a = qq & 1, bb = (qq >> 1) & 1, cc = qq >> 2;
-      // in 2D the patch cells all sit in layer 0 (q < 4, cc == 0)
-      const int ii = ni - 1 + a, jj = nj - 1 + bb, kk = DIM == 3 ? nk - 1 + cc : 0;
-      bool valid = ii >= 0 && ii < g.nc[0] && jj >= 0 && jj < g.nc[1];
-      if (DIM == 3) valid = valid && kk >= 0 && kk < g.nc[2];
-      if (valid) { const int cc2 = ii + g.nc[0] * (jj + g.nc[1] * kk); off += cs[cc2 + 1] - cs[cc2]; }
-    }
-    s_off[k][q] = off;
-    if (q == 0) {
-      const int p = ni + g.n[0] * (nj + g.n[1] * nk);
-      s_X[k] = pv.active[p] ? pv.coef + pv.mem_off[p] * pv.nf_total + f0 : zeros;
-    }
-  }
-  __syncthreads();
-  // 3^d neighbourhood population: empty => the particles keep their values (interpolation.c:89)
-  int nnb = 0;
-#pragma unroll
-  for (int dk = (DIM == 3 ? -1 : 0); dk <= (DIM == 3 ? 1 : 0); dk++)
-#pragma unroll
-    for (int dj = -1; dj <= 1; dj++)
-#pragma unroll
-      for (int di = -1; di <= 1; di++) {
-        const int a = ci + di, bb = cj + dj, e = ck + dk;
-        if (a < 0 || a >= g.nc[0] || bb < 0 || bb >= g.nc[1]) continue;
-        if (DIM == 3 && (e < 0 || e >= g.nc[2])) continue;
-        const int q = a + g.nc[0] * (bb + g.nc[1] * e);
-        nnb += cs[q + 1] - cs[q];
-      }
-  if (nnb == 0) return;
-  const double eps = g.eps_r;
-  const int nft = pv.nf_total;
-  for (int s = pb + tid; s - tid < pe; s += blockDim.x) {
-    const bool live = s < pe;
-    const int i = live ? perm[s] : perm[pb];
-    const double x = px[i], y = px[ld + i], z = DIM == 3 ? px[2 * ld + i] : 0.0;
-    double acc[NEN][NF];
-#pragma unroll
-    for (int k = 0; k < NEN; k++)
-#pragma unroll
-      for (int f = 0; f < NF; f++) acc[k][f] = 0.0;
-#pragma unroll
-    for (int dk = (DIM == 3 ? -1 : 0); dk <= (DIM == 3 ? 1 : 0); dk++)
-#pragma unroll
-      for (int dj = -1; dj <= 1; dj++)
-#pragma unroll
-        for (int di = -1; di <= 1; di++) {
-          const int a = ci + di, bb = cj + dj, e = ck + dk;
-          if (a < 0 || a >= g.nc[0] || bb < 0 || bb >= g.nc[1]) continue;
-          if (DIM == 3 && (e < 0 || e >= g.nc[2])) continue;
-          const int q = a + g.nc[0] * (bb + g.nc[1] * e);
-          const int nb = cs[q], ne = cs[q + 1];
-          // corner patches of cell c that contain cell q, and q's slot in each
-          const double* Xp[NEN];
-#pragma unroll
-          for (int k = 0; k < NEN; k++) {
-            int oi, oj, ok;
-            corner_offset(k, oi, oj, ok);
-            const int sa = di - oi + 1, sb = dj - oj + 1, sc = dk - ok + 1;
-            const bool has = sa >= 0 && sa <= 1 && sb >= 0 && sb <= 1 && (DIM == 2 || (sc >= 0 && sc <= 1));
-            Xp[k] = has ? s_X[k] + (size_t)(s_off[k][sa + 2 * sb + 4 * (DIM == 3 ? sc : 0)]) * nft : nullptr;
-          }
-          for (int j = nb; j < ne; j++) {
-            const double d0 = x - xs[j], d1 = y - xs[(size_t)N + j];
-            const double d2 = DIM == 3 ? z - xs[2 * (size_t)N + j] : 0.0;
-            const double phi = rbf<KT, double>(sqrt(d0 * d0 + d1 * d1 + d2 * d2), eps);
-            const int r = j - nb;
-#pragma unroll
-            for (int k = 0; k < NEN; k++) {
-              int oi, oj, ok;
-              corner_offset(k, oi, oj, ok);
-              const int sa = di - oi + 1, sb = dj - oj + 1, sc = dk - ok + 1;
-              const bool has = sa >= 0 && sa <= 1 && sb >= 0 && sb <= 1 && (DIM == 2 || (sc >= 0 && sc <= 1));
-              if (has) {
-                const double* Xr = Xp[k] + (size_t)r * nft;
-#pragma unroll
-                for (int f = 0; f < NF; f++) acc[k][f] += phi * Xr[f];
-              }
-            }
-          }
-        }
-    if (!live) continue;
-    double val[NF];
-#pragma unroll
-    for (int f = 0; f < NF; f++) val[f] = 0.0;
-#pragma unroll
-    for (int k = 0; k < NEN; k++) {
-      const double wk = w[(size_t)k * ld + i];
-#pragma unroll
-      for (int f = 0; f < NF; f++) val[f] += acc[k][f] * wk;
-    }
-    const int nfc = min(NF, nfields - f0);
-#pragma unroll
-    for (int f = 0; f < NF; f++) if (f < nfc) out[(size_t)(f0 + f) * ld + i] = val[f];
-  }
-}
-
-template <int DIM, int KT>
-static void eval_cell_k(const Grid& g, const double* px, int64_t ld, const int32_t* perm, const int32_t* pstart,
-                        const double* w, const PatchView& pv, const double* zeros, const double* xs, int N,
-                        const int32_t* cs, int nfields, double* out, int block, hipStream_t s) {
-  const int per = (g.C + 7) / 8;
-  const dim3 grid(per * 8), blk(block);
-  for (int f0 = 0; f0 < nfields; f0 += 4) {
-    const int r = nfields - f0;
-#define LRB_EC(NF) hipLaunchKernelGGL((k_eval_cell<DIM, KT, NF>), grid, blk, 0, s, g, px, ld, perm, pstart, w, pv, zeros, xs, N, cs, nfields, f0, out)
-    if (r == 1) LRB_EC(1);
-    else if (r == 2) LRB_EC(2);
-    else if (r == 3) LRB_EC(3);
-    else LRB_EC(4);
-#undef LRB_EC
-  }
-}
-
-template <int DIM>
-static void eval_cell_dim(const Grid& g, int ktype, const double* px, int64_t ld, const int32_t* perm,
-                          const int32_t* pstart, const double* w, const PatchView& pv, const double* zeros,
-                          const double* xs, int N, const int32_t* cs, int nfields, double* out, int block,
-                          hipStream_t s) {
-  switch (ktype) {
-    case K_LINEAR: eval_cell_k<DIM, K_LINEAR>(g, px, ld, perm, pstart, w, pv, zeros, xs, N, cs, nfields, out, block, s); break;
-    case K_WC0: eval_cell_k<DIM, K_WC0>(g, px, ld, perm, pstart, w, pv, zeros, xs, N, cs, nfields, out, block, s); break;
-    case K_WC2: eval_cell_k<DIM, K_WC2>(g, px, ld, perm, pstart, w, pv, zeros, xs, N, cs, nfields, out, block, s); break;
-    case K_BUMP: eval_cell_k<DIM, K_BUMP>(g, px, ld, perm, pstart, w, pv, zeros, xs, N, cs, nfields, out, block, s); break;
-    case K_MQ: eval_cell_k<DIM, K_MQ>(g, px, ld, perm, pstart, w, pv, zeros, xs, N, cs, nfields, out, block, s); break;
-    case K_IMQ: eval_cell_k<DIM, K_IMQ>(g, px, ld, perm, pstart, w, pv, zeros, xs, N, cs, nfields, out, block, s); break;
-    default: eval_cell_k<DIM, K_GAUSS>(g, px, ld, perm, pstart, w, pv, zeros, xs, N, cs, nfields, out, block, s); break;
-  }
-}
-
-void launch_eval_cell(const Grid& g, int ktype, const double* px, int64_t ld, const int32_t* perm, const int32_t* pstart,
-                      const double* w, const PatchView& pv, const double* zeros, const double* xs, int N,
-                      const int32_t* cs, int nfields, double* out, int block, hipStream_t s) {
-  if (nfields <= 0 || g.C <= 0) return;
-  if (g.dim == 3) eval_cell_dim<3>(g, ktype, px, ld, perm, pstart, w, pv, zeros, xs, N, cs, nfields, out, block, s);
-  else eval_cell_dim<2>(g, ktype, px, ld, perm, pstart, w, pv, zeros, xs, N, cs, nfields, out, block, s);
-}
 
 // --------------------------------------------------------------------------
 // RK stage loop (runge_kutta_{1,2,4}.c, dormand_prince.c): x = x0 + s*sum_j a_j v_j,

@@ -70,9 +70,16 @@ void launch_eval_ref(const Grid& g, int ktype, const double* px, int64_t ld, int
 void launch_sort_particles(const int32_t* cell, const int32_t* alive, int Np, int C, int32_t* pstart, int32_t* perm,
                            int32_t* scratch /* >= C+1 + scan blocks */, hipStream_t s);
 void launch_sorted_coords(const double* x, int n, int dim, const int32_t* cell_nodes, double* xs, hipStream_t s);
-void launch_eval_cell(const Grid& g, int ktype, const double* px, int64_t ld, const int32_t* perm, const int32_t* pstart,
-                      const double* w, const PatchView& pv, const double* zeros, const double* xs, int N,
-                      const int32_t* cell_start, int nfields, double* out, int block, hipStream_t s);
+// (eval_cell.hip) fp32: coef/zeros/xs are float (cell-relative coordinates)
+bool eval_f32_supported(int ktype);
+void launch_eval_cell(const Grid& g, int ktype, bool fp32, const double* px, int64_t ld, const int32_t* perm,
+                      const int32_t* pstart, const double* w, const uint8_t* active, const int64_t* mem_off,
+                      const void* coef, int nft, const void* zeros, const void* xs, int N, const int32_t* cell_start,
+                      int nfields, double* out, int block, hipStream_t s);
+void launch_sorted_coords32(const Grid& g, const double* x, int n, const int32_t* cell_nodes, const int32_t* cell,
+                            float* xs, hipStream_t s);
+void launch_coef_to_f32(const int32_t* list, int nlist, const int32_t* pn, const int64_t* mem_off, int nf,
+                        const double* coef, float* coef32, hipStream_t s);
 
 // ---- RK stage loop ----
 void launch_stage(const Grid& g, const double* pux, const double* puy, const double* puz, double* px,

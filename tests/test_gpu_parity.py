@@ -98,6 +98,54 @@ def test_eval_kernels_vs_oracle(dim, kernel, mode):
     assert err < (TOL_VAL if kernel != 0 else 1e-11)
 
 
+def _coef_errors(g, o, P_nodes):
+    errs = []
+    for p in range(P_nodes):
+        xo = o.patch_coeffs(p)
+        if xo.size == 0:
+            continue
+        xg = g.debug_get("patch_coeffs", p, dtype=np.float64).reshape(xo.shape)
+        errs.append(np.max(np.abs(xg - xo)) / np.max(np.abs(xo)))
+    return np.array(errs)
+
+
+@pytest.mark.parametrize("precision,factor,tol", [("fp64", "auto", 1e-12), ("fp64", "bk", 1e-12), ("mixed", "auto", 1e-10)])
+def test_patch_coefficients_vs_oracle(precision, factor, tol):
+    """Per-patch coefficients: FP64 Cholesky and FP64 Bunch-Kaufman within 1e-12,
+    FP32 factor + FP64 refinement within 1e-10 of the oracle's FP64 dsytrf solve."""
+    opts, X, V, P = cases.fourier_3d(nlat=24, Np=4000, n=6)
+    opts["lagrbf_precision"] = precision
+    opts["lagrbf_factor"] = factor
+    g, o = pair(opts)
+    for c in (g, o):
+        c.fill_data(X, V); c.fill_points(P); c.interpolate_c()
+    errs = _coef_errors(g, o, 6 ** 3)
+    print(f"{precision}/{factor}: {errs.size} patches, max rel coefficient err {errs.max():.3e}, "
+          f"fallback {g.info('fallback'):.0f}, refine iters max {g.info('refine_iters'):.0f}")
+    assert errs.max() < tol
+    if precision == "mixed":
+        vg = np.zeros((P.shape[0], 3)); vo = np.zeros_like(vg)
+        g.output(fvals=vg); o.output(fvals=vo)
+        err = rel_err(vg, vo)
+        print(f"mixed FP32 evaluation: max rel value err {err:.3e}")
+        assert err < 1e-5      # FP32 evaluation (values), stated tolerance
+
+
+def test_mixed_rk4_positions():
+    """Mixed path through 10 RK4 steps: positions vs the FP64 oracle (FP32 evaluation
+    error accumulates as dt * 1e-7 per step)."""
+    opts, X, V, P = cases.fourier_3d(nlat=24, Np=3000, n=6)
+    opts["lagrbf_precision"] = "mixed"
+    g, o = pair(opts)
+    out = []
+    for c in (g, o):
+        c.fill_data(X, V); c.fill_points(P); c.interpolate_c(); c.advect_steps_c(10, 1e-3)
+        xs = np.zeros_like(P); c.output(coords=xs); out.append(xs)
+    err = rel_err(out[0], out[1])
+    print(f"mixed RK4 10 steps: max rel position err {err:.3e}")
+    assert err < 1e-8
+
+
 def test_binning_bit_exact():
     opts, X, V, P = cases.fourier_3d(nlat=16, Np=3000, n=5)
     print("case: binning 3D", flush=True)
