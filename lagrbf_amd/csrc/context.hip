@@ -136,7 +136,7 @@ struct Ctx {
   int precision = 0;                       // 0 FP64, 1 FP32 factor + FP64 refinement
   int factor_mode = 0;                     // 0 auto (Cholesky for PD kernels), 1 Bunch-Kaufman always
   int refine_max_iter = 10;
-  double refine_tol = 1e-14;
+  double refine_tol = 1e-12;
   double fallback = 0, refine_iters_max = 0, refine_iters_sum = 0, refined_patches = 0;
   // analytical velocity
   int (*vfn)(double*, double*, void*) = nullptr;

@@ -38,14 +38,22 @@ __host__ __device__ constexpr int corner_slot(int nbi, int k) {
 template <int KT>
 __host__ __device__ constexpr bool compact_support() { return KT == K_WC0 || KT == K_WC2 || KT == K_BUMP; }
 
+__device__ __forceinline__ int uniform_int(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
 template <int DIM, int KT, int NF, typename T, int NBI>
 __device__ __forceinline__ void accum_cell(T xp, T yp, T zp, const T* __restrict__ xs, int N, int nb, int ne, T sx,
                                            T sy, T sz, const T* const (&Xk)[1 << DIM], int nft, T (&acc)[1 << DIM][NF],
                                            T eps, T rcut2) {
   constexpr int NEN = 1 << DIM;
   for (int j = nb; j < ne; j++) {
-    const T d0 = xp - (xs[j] + sx), d1 = yp - (xs[(size_t)N + j] + sy);
-    const T d2 = DIM == 3 ? zp - (xs[2 * (size_t)N + j] + sz) : T(0);
+    T d0, d1, d2;
+    if constexpr (sizeof(T) == 4) {
+      d0 = xp - (xs[j] + sx); d1 = yp - (xs[(size_t)N + j] + sy);
+      d2 = DIM == 3 ? zp - (xs[2 * (size_t)N + j] + sz) : T(0);
+    } else {
+      d0 = xp - xs[j]; d1 = yp - xs[(size_t)N + j];
+      d2 = DIM == 3 ? zp - xs[2 * (size_t)N + j] : T(0);
+    }
     const T r2 = d0 * d0 + d1 * d1 + d2 * d2;
     if constexpr (compact_support<KT>()) {
       // every lane beyond the support: phi = 0 exactly, nothing to add
@@ -55,12 +63,10 @@ __device__ __forceinline__ void accum_cell(T xp, T yp, T zp, const T* __restrict
     const int r = j - nb;
 #pragma unroll
     for (int k = 0; k < NEN; k++) {
-      if constexpr (true) {
-        if (corner_slot<DIM>(NBI, k) >= 0) {
-          const T* Xr = Xk[k] + (size_t)r * nft;
+      if (corner_slot<DIM>(NBI, k) >= 0) {
+        const T* Xr = Xk[k] + (size_t)r * nft;
 #pragma unroll
-          for (int f = 0; f < NF; f++) acc[k][f] += phi * Xr[f];
-        }
+        for (int f = 0; f < NF; f++) acc[k][f] += phi * Xr[f];
       }
     }
   }
@@ -77,7 +83,7 @@ __global__ __launch_bounds__(256) void k_eval_cell(Grid g, const double* __restr
   constexpr int NEN = 1 << DIM;
   constexpr int NNB = DIM == 3 ? 27 : 9;
   __shared__ int s_off[NEN][NEN];
-  __shared__ const T* s_X[NEN];
+  __shared__ long long s_base[NEN];  // element offset of corner k's coefficient rows, -1: inactive patch
   // XCD-aware order: block b runs on XCD b % 8, which gets a contiguous range of cells
   const int C = g.C;
   const int per = (gridDim.x + 7) / 8;
@@ -105,7 +111,7 @@ __global__ __launch_bounds__(256) void k_eval_cell(Grid g, const double* __restr
     s_off[k][q] = off;
     if (q == 0) {
       const int p = ni + g.n[0] * (nj + g.n[1] * nk);
-      s_X[k] = active[p] ? coef + mem_off[p] * nft + f0 : zeros;
+      s_base[k] = active[p] ? (long long)mem_off[p] * nft + f0 : -1ll;
     }
   }
   __syncthreads();
@@ -125,6 +131,15 @@ __global__ __launch_bounds__(256) void k_eval_cell(Grid g, const double* __restr
   const double ox = sizeof(T) == 4 ? g.O[0] + ci * g.h[0] : 0.0;
   const double oy = sizeof(T) == 4 ? g.O[1] + cj * g.h[1] : 0.0;
   const double oz = sizeof(T) == 4 && DIM == 3 ? g.O[2] + ck * g.h[2] : 0.0;
+  // corner bases as wave-uniform (SGPR) values: the coefficient rows are then
+  // read through the scalar cache, like the node coordinates
+  long long base[NEN];
+#pragma unroll
+  for (int k = 0; k < NEN; k++) {
+    const long long v = s_base[k];
+    const int lo = uniform_int((int)(v & 0xffffffffll)), hi = uniform_int((int)(v >> 32));
+    base[k] = (long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+  }
   for (int s = pb + tid; s - tid < pe; s += blockDim.x) {
     const bool live = s < pe;
     const int i = live ? perm[s] : perm[pb];
@@ -152,16 +167,15 @@ __global__ __launch_bounds__(256) void k_eval_cell(Grid g, const double* __restr
         corner_offset(k, oi, oj, ok);
         const int sa = di - oi + 1, sb = dj - oj + 1, sc = dk - ok + 1;
         const bool has = sa >= 0 && sa <= 1 && sb >= 0 && sb <= 1 && (DIM == 2 || (sc >= 0 && sc <= 1));
-        Xk[k] = has ? s_X[k] + (size_t)s_off[k][sa + 2 * sb + (DIM == 3 ? 4 * sc : 0)] * nft : zeros;
+        const int slot = has ? uniform_int(s_off[k][sa + 2 * sb + (DIM == 3 ? 4 * sc : 0)]) : 0;
+        Xk[k] = (has && base[k] >= 0) ? coef + base[k] + (long long)slot * nft : zeros;
       }
       switch (nbi) {
 #define LRB_CASE(V) case V: accum_cell<DIM, KT, NF, T, V>(xp, yp, zp, xs, N, nb, ne, sx, sy, sz, Xk, nft, acc, eps, rcut2); break;
         LRB_CASE(0) LRB_CASE(1) LRB_CASE(2) LRB_CASE(3) LRB_CASE(4) LRB_CASE(5) LRB_CASE(6) LRB_CASE(7) LRB_CASE(8)
-#if 1
         LRB_CASE(9) LRB_CASE(10) LRB_CASE(11) LRB_CASE(12) LRB_CASE(13) LRB_CASE(14) LRB_CASE(15) LRB_CASE(16)
         LRB_CASE(17) LRB_CASE(18) LRB_CASE(19) LRB_CASE(20) LRB_CASE(21) LRB_CASE(22) LRB_CASE(23) LRB_CASE(24)
         LRB_CASE(25) LRB_CASE(26)
-#endif
 #undef LRB_CASE
         default: break;
       }

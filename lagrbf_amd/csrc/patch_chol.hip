@@ -148,6 +148,7 @@ __global__ __launch_bounds__(CNT) void k_solve_chol(Grid g, const int32_t* __res
   const bool refine = sizeof(TL) == sizeof(float);
   int it = 0;
   int converged = 0;
+  double dprev = 1e300;
   for (; it <= (refine ? max_iter : 0); it++) {
     if (it > 0) {
       // residual b - A X in FP64, A recomputed from the member coordinates
@@ -218,7 +219,11 @@ __global__ __launch_bounds__(CNT) void k_solve_chol(Grid g, const int32_t* __res
     for (int w2 = 1; w2 < CNT / CW; w2++) { dmax = fmax(dmax, red[w2][0]); xmax = fmax(xmax, red[w2][1]); }
     __syncthreads();
     if (!refine) { converged = 1; break; }
+    // done when the correction is below tol * |X|, or when it has stopped shrinking
+    // (FP64 rounding floor of the residual, ~cond(A) * 1e-16) at or below 100 tol * |X|
     if (it > 0 && dmax <= tol * xmax) { converged = 1; break; }
+    if (it > 1 && dmax > 0.5 * dprev && dmax <= 100.0 * tol * xmax) { converged = 1; break; }
+    dprev = dmax;
   }
   if (tid == 0) {
     if (status) status[p] = converged ? 0 : 1;
