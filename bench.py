@@ -5,11 +5,20 @@ of a seeded permutation of a 102^3 lattice jittered by +-0.3 spacing) sampling a
 divergence-free 16-mode Fourier velocity; PU grid 32^3 nodes on [0,1]^3, Wendland
 C2 kernel, overlap 1, shape factor 1; 4,194,304 particles uniform in the box;
 RK4 with dt = 1e-3.  A step is one RK4 step of the whole swarm (four PU
-evaluations, weights, re-location, lazy patch set-up check).
+evaluations, Shepard weights, re-location, lazy patch set-up check).
 
-The timed region holds exactly --steps steps with the factored patches and the
-particles resident in HBM; set-up (binning, assembly, factorisation, first
-interpolation) and --warmup steps run before it.
+Both precision paths run on the same inputs, one after the other: FP32 factor +
+FP64 refinement with FP32 evaluation ("mixed", the headline `value`) and plain
+FP64.  Each timed region holds exactly --steps steps with the factored patches
+and the particles resident in HBM; set-up (binning, assembly, factorisation,
+first interpolation) and --warmup steps run before it.  The line also carries
+the mixed path's error against the FP64 path (patch coefficients, positions).
+
+Roofline of the dominant kernel (PU evaluation): algorithmic FLOPs per launch
+(per particle n_nb * 16 + sum_k 2 nf n_pk + 2^d 2 nf) over the kernel's average
+duration, timed with HIP events on the library's stream.  HBM traffic per
+launch comes from rocprofv3 FETCH_SIZE / WRITE_SIZE runs stored beside this
+file in profiles/pu_evaluation_c2_traffic.json (null when absent).
 """
 import argparse
 import json
@@ -22,8 +31,10 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK = {"f64": 78.6e12, "f32": 157.3e12}      # MI355X dense vector FP64 / FP32, FLOP/s
-HBM_PEAK = 8.0e12
+PEAK = {"fp64": 78.6e12, "mixed": 157.3e12}     # MI355X dense FP64 / FP32, FLOP/s
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pu_evaluation_c2_traffic.json")
+WORKLOAD = ("3D plain-vs-mixed: 2^20 jittered-lattice nodes, 16-mode Fourier field, 32^3 PU, Wendland C2, "
+            "4,194,304 particles, RK4 dt=1e-3")
 
 
 def log(*a):
@@ -51,69 +62,8 @@ def build_workload(Np, nlat=102, N=1 << 20, n=32):
     return opts, X, V, P
 
 
-def cpu_baseline(opts, X, V, P, steps, dt, threads):
-    """The oracle (C restatement, OpenMP) on a bounded sample: particles of the
-    workload that start inside [0.25,0.5]^3, same nodes, same PU; steady-state RK4
-    steps after set-up."""
-    from oracle.oracle import OracleCtx
-    sel = np.all((P >= 0.25) & (P <= 0.5), axis=1)
-    Ps = P[sel][:65536]
-    o = OracleCtx(opts)
-    o.fill_data(X, V)
-    o.fill_points(Ps)
-    o.interpolate_c()
-    t0 = time.perf_counter()
-    o.advect_steps_c(steps, dt)
-    t = time.perf_counter() - t0
-    return {"value": Ps.shape[0] * steps / t, "unit": "particle-RK4-steps/s", "cores": threads, "kind": "port",
-            "sample": f"{Ps.shape[0]} particles of the workload inside [0.25,0.5]^3, {steps} RK4 steps after set-up "
-                      f"(same 2^20 nodes and 32^3 PU), {t:.2f} s"}
-
-
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--particles", type=int, default=4194304)
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"])
-    ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=2)
-    args = ap.parse_args()
-    if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
-        sys.exit(2)
-    rank = int(os.environ.get("RANK", "0"))
-
-    import lagrbf_amd as L
-    dt = 1e-3
-    t0 = time.perf_counter()
-    opts, X, V, P = build_workload(args.particles)
-    log(f"workload built: {X.shape[0]} nodes, {P.shape[0]} particles ({time.perf_counter() - t0:.1f}s)")
-    ctx = L.LagRBFCtx(petsc_options=dict(opts), precision=args.precision)
-    ctx.fill_data(X, V)
-    ctx.fill_points(P)
-    ts = time.perf_counter()
-    ctx.interpolate_c()
-    t_setup = time.perf_counter() - ts
-    log(f"set-up + first interpolation: {t_setup:.2f}s, max patch {ctx.info('max_patch'):.0f}")
-    ctx.advect_steps_c(args.warmup, dt)
-    log(f"warmup {args.warmup} steps done")
-    ev0, n_ev0 = ctx.info("eval_ms"), ctx.info("eval_count")
-    t1 = time.perf_counter()
-    ctx.advect_steps_c(args.steps, dt)
-    t2 = time.perf_counter()
-    ev1, n_ev1 = ctx.info("eval_ms"), ctx.info("eval_count")
-    wall = t2 - t1
-    removed = ctx.info("removed")
-    log(f"timed {args.steps} steps: {wall:.3f}s, removed {removed:.0f}")
-    value = P.shape[0] * args.steps / wall
-
-    # roofline of the dominant kernel (PU evaluation), algorithmic FLOPs per launch
-    cs = ctx.debug_get("cell_start")
-    alive = ctx.debug_get("alive")
-    cellid = ctx.debug_get("cellid_points")
-    nc = opts["pu_mesh_n"][0] - 1
+def patch_sizes(cs, nc):
+    """Patch sizes (members) from the per-cell node counts: patch p holds the <= 8 cells around node p."""
     cnt = np.diff(cs).reshape(nc, nc, nc)
     n = nc + 1
     pc = np.zeros((n, n, n), dtype=np.int64)
@@ -122,32 +72,172 @@ def main():
         for dj in (0, 1):
             for di in (0, 1):
                 pc += padc[dk:dk + n, dj:dj + n, di:di + n]
-    flops = eval_flops(cs, nc, pc.ravel(), cellid, alive)
+    return pc.ravel()
+
+
+def load_traffic(precision):
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)[precision]
+        return float(t["fetch_bytes"]) + float(t["write_bytes"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def copy_calibration(mib):
+    """A plain copy of known size (reads and writes mib MiB) for scaling rocprofv3's
+    FETCH_SIZE / WRITE_SIZE: torch.neg over float64, launched three times."""
+    import torch
+    n = mib * (1 << 20) // 8
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        torch.neg(a, out=b)
+    torch.cuda.synchronize()
+    log(f"copy calibration: 3 x torch.neg over {n * 8} bytes")
+    del a, b
+
+
+def run_path(L, precision, opts, X, V, P, steps, warmup, dt, sample):
+    """Set up one context, run warmup + timed steps; returns the result record,
+    a sample of patch coefficients and the final positions (rows by pid)."""
+    ctx = L.LagRBFCtx(petsc_options=dict(opts), precision=precision)
+    ctx.fill_data(X, V)
+    ctx.fill_points(P)
+    ts = time.perf_counter()
+    ctx.interpolate_c()
+    t_setup = time.perf_counter() - ts
+    log(f"[{precision}] set-up + first interpolation: {t_setup:.2f}s, max patch {ctx.info('max_patch'):.0f}, "
+        f"fallback {ctx.info('fallback'):.0f}, refine iters max {ctx.info('refine_iters'):.0f}")
+    ctx.advect_steps_c(warmup, dt)
+    ev0, n_ev0 = ctx.info("eval_ms"), ctx.info("eval_count")
+    t1 = time.perf_counter()
+    ctx.advect_steps_c(steps, dt)
+    t2 = time.perf_counter()
+    ev1, n_ev1 = ctx.info("eval_ms"), ctx.info("eval_count")
+    wall = t2 - t1
+    removed = ctx.info("removed")
+    log(f"[{precision}] timed {steps} steps: {wall:.3f}s ({wall / max(steps, 1) * 1e3:.2f} ms/step), removed {removed:.0f}")
+
+    cs = ctx.debug_get("cell_start")
+    alive = ctx.debug_get("alive")
+    cellid = ctx.debug_get("cellid_points")
+    nc = opts["pu_mesh_n"][0] - 1
+    flops = eval_flops(cs, nc, patch_sizes(cs, nc), cellid, alive)
     n_launch = max(n_ev1 - n_ev0, 1)
     avg_ms = (ev1 - ev0) / n_launch
     achieved = flops / (avg_ms * 1e-3)
-    peak = PEAK["f64" if args.precision == "fp64" else "f32"]
+    traffic = load_traffic(precision)
+    coefs = {}
+    for p in sample:
+        c = ctx.debug_get("patch_coeffs", int(p), dtype=np.float64)
+        if c.size:
+            coefs[int(p)] = c
+    pos = np.zeros_like(P)
+    ctx.output(coords=pos)
+    rec = {
+        "value": P.shape[0] * steps / wall, "ms_per_step": wall / max(steps, 1) * 1e3,
+        "end_to_end": P.shape[0] * steps / (wall + t_setup), "setup_s": t_setup, "removed_particles": removed,
+        "fallback_patches": ctx.info("fallback"), "refine_iters_max": ctx.info("refine_iters"),
+        "roofline": {"bound": "valu", "kernel": "pu_evaluation (k_eval_cell)",
+                     "achieved": achieved / 1e12, "peak": PEAK[precision] / 1e12, "unit": "TFLOP/s",
+                     "frac": achieved / PEAK[precision], "traffic": traffic,
+                     "flops_per_launch": flops, "avg_launch_ms": avg_ms, "launches_timed": int(n_launch)},
+    }
+    ctx.destroy()
+    return rec, coefs, pos
+
+
+def cpu_baseline(opts, X, V, P, steps, dt, nsample, threads):
+    """The oracle (C restatement of the reference, OpenMP) on a bounded sample:
+    the first nsample particles of the workload inside [0.25,0.5]^3, same nodes,
+    same PU, steady-state RK4 steps after set-up.  Reference-faithful mode
+    (Shepard sums over every PU node, as swarm_interp.c:283-396), and the
+    cutoff-equivalent mode (stencil of PU nodes inside the weight support)."""
+    from oracle.oracle import OracleCtx
+    sel = np.all((P >= 0.25) & (P <= 0.5), axis=1)
+    Ps = P[sel][:nsample]
+    rates = {}
+    for mode, extra in (("faithful", {"oracle_weights_brute": True}), ("cutoff", {})):
+        o = OracleCtx(dict(opts, **extra))
+        o.fill_data(X, V)
+        o.fill_points(Ps)
+        o.interpolate_c()
+        t0 = time.perf_counter()
+        o.advect_steps_c(steps, dt)
+        rates[mode] = (Ps.shape[0] * steps / (time.perf_counter() - t0), time.perf_counter() - t0)
+        o.destroy()
+    v, t = rates["faithful"]
+    return {"value": v, "unit": "particle-RK4-steps/s", "cores": threads, "kind": "port",
+            "sample": f"{Ps.shape[0]} particles of the workload inside [0.25,0.5]^3, {steps} RK4 steps after set-up "
+                      f"(same 2^20 nodes and 32^3 PU), brute-force Shepard sums as the reference, {t:.1f} s",
+            "cutoff_value": rates["cutoff"][0]}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--particles", type=int, default=4194304)
+    ap.add_argument("--precision", default="both", choices=["both", "fp64", "mixed"])
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-particles", type=int, default=32768)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="launch a plain copy of MIB MiB first (scale for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only (one GPU's share of the swarm); "
+                                   "--gpus must be 1"}), flush=True)
+        sys.exit(2)
+    rank = int(os.environ.get("RANK", "0"))
+
+    import lagrbf_amd as L
+    if args.copy_calibration > 0:
+        copy_calibration(args.copy_calibration)
+    dt = 1e-3
+    t0 = time.perf_counter()
+    opts, X, V, P = build_workload(args.particles)
+    log(f"workload built: {X.shape[0]} nodes, {P.shape[0]} particles ({time.perf_counter() - t0:.1f}s)")
+    n_pu = int(np.prod(opts["pu_mesh_n"]))
+    sample = np.linspace(0, n_pu - 1, 512).astype(np.int64)
+    paths = ["mixed", "fp64"] if args.precision == "both" else [args.precision]
+    res, coefs, pos = {}, {}, {}
+    for prec in paths:
+        res[prec], coefs[prec], pos[prec] = run_path(L, prec, opts, X, V, P, args.steps, args.warmup, dt, sample)
+    main_path = paths[0]
+    r = res[main_path]
+    cmp = None
+    if len(paths) == 2:
+        common = [p for p in coefs["mixed"] if p in coefs["fp64"]]
+        cerr = max((np.max(np.abs(coefs["mixed"][p] - coefs["fp64"][p])) / max(np.max(np.abs(coefs["fp64"][p])), 1e-300)
+                    for p in common), default=float("nan"))
+        live = np.any(pos["fp64"] != 0, axis=1) & np.any(pos["mixed"] != 0, axis=1)
+        perr = float(np.max(np.abs(pos["mixed"][live] - pos["fp64"][live])) / np.max(np.abs(pos["fp64"][live])))
+        cmp = {"speedup_mixed_over_fp64": res["mixed"]["value"] / res["fp64"]["value"],
+               "coef_rel_err_max": float(cerr), "patches_compared": len(common),
+               "position_rel_err_max": perr, "position_steps": args.warmup + args.steps}
+        log(f"mixed vs fp64: speedup {cmp['speedup_mixed_over_fp64']:.3f}, coefficient rel err {cerr:.3e} "
+            f"({len(common)} patches), position rel err {perr:.3e}")
     cpu = None
     if rank == 0 and not args.no_cpu:
-        import multiprocessing
-        thr = int(os.environ.get("OMP_NUM_THREADS", multiprocessing.cpu_count()))
-        log("cpu baseline ...")
-        cpu = cpu_baseline(opts, X, V, P, args.cpu_steps, dt, thr)
-        log(f"cpu baseline: {cpu['value']:.3e} {cpu['unit']} ({cpu['sample']})")
+        thr = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        log(f"cpu baseline ({thr} threads) ...")
+        cpu = cpu_baseline(opts, X, V, P, args.cpu_steps, dt, args.cpu_particles, thr)
+        log(f"cpu baseline: {cpu['value']:.3e} {cpu['unit']} ({cpu['sample']}); cutoff mode {cpu['cutoff_value']:.3e}")
     out = {
         "metric": "particle-RK4 steps/s, FP32+refine vs FP64 @1e-10 rel err; serve p99 ms",
-        "value": value, "unit": "particle-RK4-steps/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-        "ms_per_step": wall / args.steps * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f32+f64-refine", "data": "synthetic",
-        "config": {"workload": "3D plain-vs-mixed: 2^20 jittered-lattice nodes, 16-mode Fourier field, 32^3 PU, "
-                               "Wendland C2, 4,194,304 particles, RK4 dt=1e-3",
-                   "nodes": int(X.shape[0]), "particles": int(P.shape[0]), "pu_n": 32, "kernel": "wendland_c2",
-                   "precision": args.precision},
-        "roofline": {"bound": "valu", "kernel": "pu_evaluation", "achieved": achieved / 1e12, "peak": peak / 1e12,
-                     "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
-                     "flops_per_launch": flops, "avg_launch_ms": avg_ms},
+        "value": r["value"], "unit": "particle-RK4-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32+f64" if main_path == "mixed" else "f64", "data": "synthetic",
+        "config": {"workload": WORKLOAD, "nodes": int(X.shape[0]), "particles": int(P.shape[0]), "pu_n": 32,
+                   "kernel": "wendland_c2", "precision": main_path},
+        "roofline": r["roofline"],
         "cpu_baseline": cpu,
-        "setup_s": t_setup, "removed_particles": removed,
+        "setup_s": r["setup_s"], "end_to_end": r["end_to_end"], "removed_particles": r["removed_particles"],
+        "paths": {p: {k: v for k, v in res[p].items()} for p in paths},
+        "mixed_vs_fp64": cmp,
     }
     if rank == 0:
         print(json.dumps(out), flush=True)

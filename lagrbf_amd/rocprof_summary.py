@@ -1,0 +1,122 @@
+"""Summaries of rocprofv3 output for this library's kernels.
+
+  python -m lagrbf_amd.rocprof_summary stats DIR
+      per-kernel table from the *_kernel_stats.csv that `--kernel-trace --stats`
+      writes (calls, average / total duration, share).
+  python -m lagrbf_amd.rocprof_summary traffic FETCH_DIR WRITE_DIR --copy-mib M [--kernel SUBSTR] [--json OUT --key K]
+      HBM bytes per dispatch from two `--pmc FETCH_SIZE` / `--pmc WRITE_SIZE` runs
+      (counter values in KiB per dispatch).  The runs must include a plain copy of
+      M MiB (bench.py --copy-calibration M launches torch.neg over float64, which
+      reads M MiB and writes M MiB); each counter is scaled by known bytes /
+      counter bytes of that copy.
+"""
+import argparse
+import csv
+import glob
+import json
+import os
+import sys
+from collections import defaultdict
+
+
+def _find(d, pattern):
+    hits = sorted(glob.glob(os.path.join(d, "**", pattern), recursive=True))
+    if not hits:
+        raise SystemExit(f"no {pattern} under {d}")
+    return hits
+
+
+def short_name(name):
+    n = name.strip()
+    if n.startswith("void "):
+        n = n[5:]
+    p = n.find("(")
+    return n[:p] if p > 0 else n
+
+
+def stats(d, top=25, out=sys.stdout):
+    rows = []
+    for f in _find(d, "*kernel_stats.csv"):
+        with open(f) as fh:
+            rows += list(csv.DictReader(fh))
+    rows.sort(key=lambda r: -float(r["TotalDurationNs"]))
+    tot = sum(float(r["TotalDurationNs"]) for r in rows)
+    print(f"{'kernel':<58} {'calls':>7} {'avg us':>11} {'total ms':>10} {'%':>6}", file=out)
+    for r in rows[:top]:
+        print(f"{short_name(r['Name'])[:58]:<58} {int(r['Calls']):>7} {float(r['AverageNs']) / 1e3:>11.2f} "
+              f"{float(r['TotalDurationNs']) / 1e6:>10.2f} {100 * float(r['TotalDurationNs']) / tot:>6.2f}", file=out)
+    print(f"{'all kernels':<58} {sum(int(r['Calls']) for r in rows):>7} {'':>11} {tot / 1e6:>10.2f}", file=out)
+
+
+def counters(d, counter):
+    """kernel short name -> list of per-dispatch counter values (KiB)."""
+    per = defaultdict(lambda: defaultdict(float))
+    for f in _find(d, "*counter_collection.csv"):
+        with open(f) as fh:
+            for r in csv.DictReader(fh):
+                if r["Counter_Name"] != counter:
+                    continue
+                per[(short_name(r["Kernel_Name"]), r["Dispatch_Id"])]["v"] += float(r["Counter_Value"])
+    out = defaultdict(list)
+    for (k, _), v in per.items():
+        out[k].append(v["v"])
+    return out
+
+
+def traffic(fdir, wdir, copy_mib, kernel_sub):
+    fetch, write = counters(fdir, "FETCH_SIZE"), counters(wdir, "WRITE_SIZE")
+    copy_bytes = copy_mib * (1 << 20)
+    ck = [k for k in fetch if "neg" in k.lower()]
+    if not ck:
+        raise SystemExit("copy calibration kernel (torch.neg) not found in the FETCH_SIZE run")
+    cf = max(fetch[ck[0]]) * 1024.0
+    cw = max(write[ck[0]]) * 1024.0
+    sf, sw = copy_bytes / cf, copy_bytes / cw
+    res = {"copy_kernel": ck[0], "copy_bytes": copy_bytes, "fetch_scale": sf, "write_scale": sw, "kernels": {}}
+    for k in sorted(fetch):
+        if kernel_sub and kernel_sub not in k:
+            continue
+        fv, wv = fetch[k], write.get(k, [0.0])
+        res["kernels"][k] = {"dispatches": len(fv), "fetch_bytes": sf * 1024.0 * sum(fv) / len(fv),
+                             "write_bytes": sw * 1024.0 * sum(wv) / len(wv)}
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    sub = ap.add_subparsers(dest="cmd", required=True)
+    s = sub.add_parser("stats")
+    s.add_argument("dir")
+    s.add_argument("--top", type=int, default=25)
+    t = sub.add_parser("traffic")
+    t.add_argument("fetch_dir")
+    t.add_argument("write_dir")
+    t.add_argument("--copy-mib", type=int, required=True)
+    t.add_argument("--kernel", default="")
+    t.add_argument("--json", default="")
+    t.add_argument("--key", default="")
+    a = ap.parse_args()
+    if a.cmd == "stats":
+        stats(a.dir, a.top)
+        return
+    res = traffic(a.fetch_dir, a.write_dir, a.copy_mib, a.kernel)
+    print(f"copy kernel {res['copy_kernel']}: scale fetch {res['fetch_scale']:.4f}, write {res['write_scale']:.4f}")
+    for k, v in res["kernels"].items():
+        print(f"{k:<58} {v['dispatches']:>6} fetch {v['fetch_bytes'] / 1e6:>12.3f} MB  write {v['write_bytes'] / 1e6:>12.3f} MB")
+    if a.json:
+        data = {}
+        if os.path.exists(a.json):
+            with open(a.json) as fh:
+                data = json.load(fh)
+        ev = [v for k, v in res["kernels"].items() if "k_eval_cell" in k]
+        if ev:
+            big = max(ev, key=lambda v: v["fetch_bytes"])
+            data[a.key] = {"fetch_bytes": big["fetch_bytes"], "write_bytes": big["write_bytes"],
+                           "fetch_scale": res["fetch_scale"], "write_scale": res["write_scale"],
+                           "copy_bytes": res["copy_bytes"]}
+            with open(a.json, "w") as fh:
+                json.dump(data, fh, indent=1)
+
+
+if __name__ == "__main__":
+    main()

@@ -166,6 +166,9 @@ class OracleCtx:
         self._keep = []
 
     def __del__(self):
+        self.destroy()
+
+    def destroy(self):
         if getattr(self, "ctx", None):
             lib().orc_destroy(self.ctx)
             self.ctx = None
