@@ -117,6 +117,11 @@ struct Ctx {
   DBuf<int32_t> pcell, alive;
   DBuf<int32_t> perm, pstart, pscratch;    // particles grouped by cell
   bool sorted_valid = false;
+  // the swarm is stored in cell order at the start of every advection step;
+  // porig[s] = index (fill order) of the particle held in slot s
+  bool reorder = true;
+  DBuf<int32_t> porig, porig_alt, pcell_alt, alive_alt;
+  DBuf<double> px_alt, vel_alt, fields_alt, w_alt;
   std::vector<int64_t> h_ppid;
   DBuf<unsigned long long> counters;   // [0] removed, [1] reduction bits, [2] count
   // patches
@@ -244,6 +249,8 @@ static int ctx_from_options(Ctx& c, const OptView& o) {
     else if (ev == "ref") c.eval_mode = 2;
     else return set_err(E_WRONG, fn, "-lagrbf_eval must be auto, cell or ref (found %s)", ev.c_str());
   }
+  o.get_bool("lagrbf_reorder", c.reorder, bad);
+  if (bad) return set_err(E_WRONG, fn, "malformed -lagrbf_reorder option");
   return 0;
 }
 
@@ -361,8 +368,9 @@ static int alloc_particles(Ctx& c) {
   HIPCHK(c.pcell.reserve(ld));
   HIPCHK(c.alive.reserve(ld));
   HIPCHK(c.perm.reserve(ld));
-  HIPCHK(c.pstart.reserve((size_t)c.g.C + 1));
-  HIPCHK(c.pscratch.reserve((size_t)c.g.C + 1 + (size_t)(c.g.C + 1) / 1024 + 64));
+  HIPCHK(c.pstart.reserve((size_t)c.g.C + 2));
+  HIPCHK(c.pscratch.reserve((size_t)c.g.C + 2 + (size_t)(c.g.C + 2) / 1024 + 64));
+  HIPCHK(c.porig.reserve(ld));
   c.sorted_valid = false;
   HIPCHK(c.vel.reserve(ld * dim));
   if (c.advection) {
@@ -398,6 +406,7 @@ static int fill_points(Ctx& c, int n, const double* coords, const double* values
     HIPCHK(hipMemsetAsync(c.w.p, 0, sizeof(double) * c.ld * c.g.nen, c.s));
     HIPCHK(hipMemsetAsync(c.vel.p, 0, sizeof(double) * c.ld * dim, c.s));
     hipLaunchKernelGGL(k_fill_i32, dim3((n + 255) / 256), dim3(256), 0, c.s, c.alive.p, n, 1);
+    launch_iota(c.porig.p, n, c.s);
   }
   c.h_ppid.resize(n);
   for (int i = 0; i < n; i++) c.h_ppid[i] = pidx ? pidx[i] : (int64_t)i + 1;
@@ -731,9 +740,50 @@ static void stage(Ctx& c, const double* x0, const StageArgs& a) {
                c.w.p, a, c.s);
 }
 
+template <typename T>
+static void swap_buf(DBuf<T>& a, DBuf<T>& b) {
+  std::swap(a.p, b.p);
+  std::swap(a.cap, b.cap);
+}
+
+// Store the swarm in the cell order of the last sort (alive particles grouped by
+// cell, dead ones at the tail): the evaluation's particle reads and writes become
+// contiguous.  Positions, the step's start velocity, interpolated fields, weights,
+// cell ids, alive flags and the slot -> fill index map move together; nothing a
+// particle computes depends on its slot.
+static int reorder_particles(Ctx& c) {
+  if (!c.reorder || !c.sorted_valid || c.Np == 0) return 0;
+  const size_t ld = (size_t)c.ld;
+  const int dim = c.g.dim, nf = std::max(c.nf, 1);
+  HIPCHK(c.px_alt.reserve(ld * dim));
+  HIPCHK(c.vel_alt.reserve(ld * dim));
+  HIPCHK(c.fields_alt.reserve(ld * nf));
+  HIPCHK(c.w_alt.reserve(ld * c.g.nen));
+  HIPCHK(c.pcell_alt.reserve(ld));
+  HIPCHK(c.alive_alt.reserve(ld));
+  HIPCHK(c.porig_alt.reserve(ld));
+  ReorderSet r{};
+  r.src[0] = c.px.p; r.dst[0] = c.px_alt.p; r.rows[0] = dim;
+  r.src[1] = c.vel.p; r.dst[1] = c.vel_alt.p; r.rows[1] = dim;
+  r.src[2] = c.fields.p; r.dst[2] = c.fields_alt.p; r.rows[2] = c.nf;
+  r.src[3] = c.w.p; r.dst[3] = c.w_alt.p; r.rows[3] = c.g.nen;
+  r.nd = 4;
+  r.isrc[0] = c.pcell.p; r.idst[0] = c.pcell_alt.p;
+  r.isrc[1] = c.alive.p; r.idst[1] = c.alive_alt.p;
+  r.isrc[2] = c.porig.p; r.idst[2] = c.porig_alt.p;
+  r.ni = 3;
+  launch_reorder(r, c.perm.p, c.Np, c.ld, c.s);
+  swap_buf(c.px, c.px_alt); swap_buf(c.vel, c.vel_alt); swap_buf(c.fields, c.fields_alt); swap_buf(c.w, c.w_alt);
+  swap_buf(c.pcell, c.pcell_alt); swap_buf(c.alive, c.alive_alt); swap_buf(c.porig, c.porig_alt);
+  // the cell groups now sit at pstart[c] .. pstart[c+1] in slot order
+  launch_iota(c.perm.p, c.Np, c.s);
+  return 0;
+}
+
 static int advect_swarm(Ctx& c, double dt) {
   const int dim = c.g.dim;
   const size_t bytes = sizeof(double) * dim * c.ld;
+  CHK(reorder_particles(c));
   switch (c.scheme) {
     case 0: {  // runge_kutta_1.c:37-75
       const double* v[1] = {c.vel.p};
@@ -922,22 +972,24 @@ static int output(Ctx& c, bool is_data, int n, double* coords, double* fields, i
     else return set_err(E_WRONG, fn, "unknown field '%s'", fieldname);
   }
   std::vector<double> xs((size_t)dim * c.ld), fs((size_t)std::max(nfl, 1) * c.ld);
-  std::vector<int32_t> al(std::max(Np, 1));
+  std::vector<int32_t> al(std::max(Np, 1)), orig(std::max(Np, 1));
   if (Np > 0) {
     if (coords) HIPCHK(hipMemcpy(xs.data(), c.px.p, sizeof(double) * dim * c.ld, hipMemcpyDeviceToHost));
     if (fields && nfl > 0) HIPCHK(hipMemcpy(fs.data(), src, sizeof(double) * nfl * c.ld, hipMemcpyDeviceToHost));
     HIPCHK(hipMemcpy(al.data(), c.alive.p, sizeof(int32_t) * Np, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(orig.data(), c.porig.p, sizeof(int32_t) * Np, hipMemcpyDeviceToHost));
   }
   if (coords) memset(coords, 0, sizeof(double) * (size_t)n * dim);
   if (fields) memset(fields, 0, sizeof(double) * (size_t)n * nfl);
   if (pidx) memset(pidx, 0, sizeof(int64_t) * (size_t)n);
   for (int p = 0; p < Np; p++) {
     if (!al[p]) continue;
-    int64_t g = c.h_ppid[p] - 1;
+    const int64_t pid = c.h_ppid[orig[p]];
+    int64_t g = pid - 1;
     if (g < 0 || g >= n) return set_err(E_RANGE, fn, "pid %lld out of range [0, %d)", (long long)g, n);
     if (coords) for (int d = 0; d < dim; d++) coords[g * dim + d] = xs[(size_t)d * c.ld + p];
     if (fields) for (int f = 0; f < nfl; f++) fields[g * nfl + f] = fs[(size_t)f * c.ld + p];
-    if (pidx) pidx[g] = c.h_ppid[p];
+    if (pidx) pidx[g] = pid;
   }
   return 0;
 }
@@ -1128,7 +1180,14 @@ int64_t lagrbf_debug_get(void* ctx, const char* what, int32_t index, void* out, 
   } else return -set_err(E_WRONG, "lagrbf_debug_get", "unknown item '%s'", what);
   if (out && n > 0 && src) {
     if (cap < n) return -set_err(E_SIZ, "lagrbf_debug_get", "buffer too small (%lld < %lld)", (long long)cap, (long long)n);
-    if (hipMemcpy(out, src, (size_t)(n * esz), hipMemcpyDeviceToHost) != hipSuccess) return -E_LIB;
+    if (w == "cellid_points" || w == "alive") {
+      // per-particle arrays are returned in fill order (the swarm is stored in cell order)
+      std::vector<int32_t> v((size_t)n), orig((size_t)n);
+      if (hipMemcpy(v.data(), src, (size_t)n * 4, hipMemcpyDeviceToHost) != hipSuccess) return -E_LIB;
+      if (hipMemcpy(orig.data(), c->porig.p, (size_t)n * 4, hipMemcpyDeviceToHost) != hipSuccess) return -E_LIB;
+      int32_t* o = static_cast<int32_t*>(out);
+      for (int64_t q = 0; q < n; q++) o[orig[q]] = v[q];
+    } else if (hipMemcpy(out, src, (size_t)(n * esz), hipMemcpyDeviceToHost) != hipSuccess) return -E_LIB;
   }
   return n;
 }

@@ -606,27 +606,86 @@ void launch_eval_ref(const Grid& g, int ktype, const double* px, int64_t ld, int
 // the member order of interpolation.c:123-129.  Node coordinates and coefficients
 // are wave-uniform loads (scalar cache / SGPR operands).
 // --------------------------------------------------------------------------
-__global__ void k_hist_alive(const int32_t* __restrict__ cell, const int32_t* __restrict__ alive, int n,
-                             int32_t* __restrict__ counts) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n && alive[i]) atomicAdd(&counts[cell[i]], 1);
+// Particles grouped by cell for the cell-ordered evaluation; dead particles go to
+// the tail bucket C.  Lanes of a wave holding the same key in a row (the common
+// case once the swarm is stored in cell order) share one atomic.
+struct WaveRun {
+  int head;   // lane that starts this lane's run
+  int len;    // run length (valid at the head lane)
+};
+
+__device__ __forceinline__ WaveRun wave_run(int key) {
+  const int lane = threadIdx.x & (WAVE - 1);
+  const int prev = __shfl_up(key, 1);
+  const bool head = lane == 0 || prev != key;
+  const unsigned long long heads = __ballot(head);
+  const unsigned long long upto = lane == WAVE - 1 ? ~0ull : ((1ull << (lane + 1)) - 1);
+  WaveRun r;
+  r.head = WAVE - 1 - __clzll((long long)(heads & upto));
+  const unsigned long long above = lane == WAVE - 1 ? 0ull : (heads & ~upto);
+  r.len = (above ? __ffsll((long long)above) - 1 : WAVE) - lane;
+  return r;
 }
 
-__global__ void k_pscatter(const int32_t* __restrict__ cell, const int32_t* __restrict__ alive, int n,
+__device__ __forceinline__ int particle_key(const int32_t* cell, const int32_t* alive, int n, int C, int i) {
+  if (i >= n) return -1;
+  return alive[i] ? cell[i] : C;
+}
+
+__global__ void k_hist_alive(const int32_t* __restrict__ cell, const int32_t* __restrict__ alive, int n, int C,
+                             int32_t* __restrict__ counts) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int key = particle_key(cell, alive, n, C, i);
+  const WaveRun r = wave_run(key);
+  const int lane = threadIdx.x & (WAVE - 1);
+  if (key >= 0 && r.head == lane) atomicAdd(&counts[key], r.len);
+}
+
+__global__ void k_pscatter(const int32_t* __restrict__ cell, const int32_t* __restrict__ alive, int n, int C,
                            int32_t* __restrict__ cursor, int32_t* __restrict__ perm) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n && alive[i]) perm[atomicAdd(&cursor[cell[i]], 1)] = i;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int key = particle_key(cell, alive, n, C, i);
+  const WaveRun r = wave_run(key);
+  const int lane = threadIdx.x & (WAVE - 1);
+  int base = 0;
+  if (key >= 0 && r.head == lane) base = atomicAdd(&cursor[key], r.len);
+  base = __shfl(base, r.head);
+  if (key >= 0) perm[base + lane - r.head] = i;
 }
 
 void launch_sort_particles(const int32_t* cell, const int32_t* alive, int Np, int C, int32_t* pstart, int32_t* perm,
                            int32_t* scratch, hipStream_t s) {
+  // pstart: C + 2 entries (cells, dead bucket, end); scratch: C + 2 + scan blocks
   int32_t* cursor = scratch;
-  int32_t* tmp = scratch + (C + 1);
-  hipMemsetAsync(pstart, 0, sizeof(int32_t) * (C + 1), s);
-  if (Np > 0) hipLaunchKernelGGL(k_hist_alive, dim3((Np + 255) / 256), dim3(256), 0, s, cell, alive, Np, pstart);
-  exclusive_scan(pstart, C + 1, tmp, s);
-  hipMemcpyAsync(cursor, pstart, sizeof(int32_t) * (C + 1), hipMemcpyDeviceToDevice, s);
-  if (Np > 0) hipLaunchKernelGGL(k_pscatter, dim3((Np + 255) / 256), dim3(256), 0, s, cell, alive, Np, cursor, perm);
+  int32_t* tmp = scratch + (C + 2);
+  hipMemsetAsync(pstart, 0, sizeof(int32_t) * (C + 2), s);
+  if (Np > 0) hipLaunchKernelGGL(k_hist_alive, dim3((Np + 255) / 256), dim3(256), 0, s, cell, alive, Np, C, pstart);
+  exclusive_scan(pstart, C + 2, tmp, s);
+  hipMemcpyAsync(cursor, pstart, sizeof(int32_t) * (C + 2), hipMemcpyDeviceToDevice, s);
+  if (Np > 0) hipLaunchKernelGGL(k_pscatter, dim3((Np + 255) / 256), dim3(256), 0, s, cell, alive, Np, C, cursor, perm);
+}
+
+// Store the swarm in the order of perm (slot s receives particle perm[s]).
+__global__ void k_reorder(ReorderSet r, const int32_t* __restrict__ perm, int n, int64_t ld) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const int src = perm[s];
+  for (int a = 0; a < r.nd; a++)
+    for (int row = 0; row < r.rows[a]; row++) r.dst[a][(size_t)row * ld + s] = r.src[a][(size_t)row * ld + src];
+  for (int a = 0; a < r.ni; a++) r.idst[a][s] = r.isrc[a][src];
+}
+
+__global__ void k_iota(int32_t* __restrict__ a, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = i;
+}
+
+void launch_reorder(const ReorderSet& r, const int32_t* perm, int n, int64_t ld, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_reorder, dim3((n + 255) / 256), dim3(256), 0, s, r, perm, n, ld);
+}
+
+void launch_iota(int32_t* a, int n, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_iota, dim3((n + 255) / 256), dim3(256), 0, s, a, n);
 }
 
 __global__ void k_sorted_coords(const double* __restrict__ x, int n, int dim, const int32_t* __restrict__ cn,

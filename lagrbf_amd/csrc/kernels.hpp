@@ -66,9 +66,23 @@ void launch_eval_ref(const Grid& g, int ktype, const double* px, int64_t ld, int
                      const int32_t* alive, const double* w, const PatchView& pv, const double* xdata,
                      const int32_t* cell_start, int nfields, double* out, hipStream_t s);
 
-// cell-ordered evaluation: particles grouped by cell, nodes in cell-sorted SoA
-void launch_sort_particles(const int32_t* cell, const int32_t* alive, int Np, int C, int32_t* pstart, int32_t* perm,
-                           int32_t* scratch /* >= C+1 + scan blocks */, hipStream_t s);
+// cell-ordered evaluation: particles grouped by cell (pstart[c]..pstart[c+1]), dead
+// particles in the tail bucket C, so perm covers the whole swarm
+void launch_sort_particles(const int32_t* cell, const int32_t* alive, int Np, int C,
+                           int32_t* pstart /* C+2 */, int32_t* perm,
+                           int32_t* scratch /* >= C+2 + scan blocks */, hipStream_t s);
+// gather of SoA particle arrays (rows of ld elements) into the order of perm
+struct ReorderSet {
+  const double* src[6];
+  double* dst[6];
+  int rows[6];
+  int nd;
+  const int32_t* isrc[4];
+  int32_t* idst[4];
+  int ni;
+};
+void launch_reorder(const ReorderSet& r, const int32_t* perm, int n, int64_t ld, hipStream_t s);
+void launch_iota(int32_t* a, int n, hipStream_t s);
 void launch_sorted_coords(const double* x, int n, int dim, const int32_t* cell_nodes, double* xs, hipStream_t s);
 // (eval_cell.hip) fp32: coef/zeros/xs are float (cell-relative coordinates)
 bool eval_f32_supported(int ktype);

@@ -146,6 +146,36 @@ def test_mixed_rk4_positions():
     assert err < 1e-8
 
 
+def test_reorder_bit_identical():
+    """The swarm stored in cell order (default) and in fill order give bit-identical
+    positions, fields, pids and cell ids through RK4 steps that remove particles;
+    both match the oracle."""
+    opts, X, V, P = cases.fourier_3d(nlat=24, Np=4000, n=6)
+    print("case: reorder on/off, RK4 25 steps dt=4e-3", flush=True)
+    res = []
+    for reorder in (True, False):
+        o = dict(opts, lagrbf_reorder=reorder)
+        g = L.LagRBFCtx(petsc_options=o)
+        g.fill_data(X, V); g.fill_points(P); g.interpolate_c(); g.advect_steps_c(25, 4e-3)
+        xs = np.zeros_like(P); vals = np.zeros((P.shape[0], 3)); pid = np.zeros(P.shape[0], dtype=np.int64)
+        g.output(xs, vals, pid)
+        res.append((xs, vals, pid, g.debug_get("cellid_points"), g.debug_get("alive"), g.info("removed")))
+    a, b = res
+    assert a[5] == b[5] and a[5] > 0, (a[5], b[5])
+    assert np.array_equal(a[4], b[4])
+    for k in range(3):
+        assert np.array_equal(a[k], b[k]), k
+    live = a[4].astype(bool)
+    assert np.array_equal(a[3][live], b[3][live])
+    from oracle.oracle import OracleCtx
+    oc = OracleCtx(L.LagRBFCtx(petsc_options=dict(opts)).options)
+    oc.fill_data(X, V); oc.fill_points(P); oc.interpolate_c(); oc.advect_steps_c(25, 4e-3)
+    xo = np.zeros_like(P); oc.output(coords=xo)
+    assert oc.removed_count() == a[5]
+    assert rel_err(a[0], xo) < TOL_POS
+    assert np.array_equal(a[3][live], oc.cellids(False)[live])
+
+
 def test_binning_bit_exact():
     opts, X, V, P = cases.fourier_3d(nlat=16, Np=3000, n=5)
     print("case: binning 3D", flush=True)
