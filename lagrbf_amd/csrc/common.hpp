@@ -24,12 +24,13 @@ __device__ __forceinline__ T rbf(T r, T eps) {
   if constexpr (K == K_LINEAR) {
     return eps * r;
   } else if constexpr (K == K_WC0) {
-    T u = T(1) - eps * r;
-    return u < T(0) ? T(0) : u * u;
+    // max(u, 0): u <= 0 gives exactly 0, as the reference's cut-off
+    const T u = fmax(T(1) - eps * r, T(0));
+    return u * u;
   } else if constexpr (K == K_WC2) {
-    T u = T(1) - eps * r;
-    T u2 = u * u;
-    return u < T(0) ? T(0) : (u2 * u2) * (T(5) - T(4) * u);
+    const T u = fmax(T(1) - eps * r, T(0));
+    const T u2 = u * u;
+    return (u2 * u2) * (T(5) - T(4) * u);
   } else if constexpr (K == K_BUMP) {
     T u = T(1) - eps * r;
     return u < T(0) ? T(0) : exp(T(-1) / (u * (T(2) - u)));
@@ -61,14 +62,15 @@ __device__ __forceinline__ T rbf_rt(int k, T r, T eps) {
 
 // sqrt for the hot loops: v_rsq_f64 (about 23 correct bits) refined by two
 // Newton steps on sqrt (r <- r + (x - r^2) / (2 sqrt x)), within 1 ulp for
-// normal x >= 0; the IEEE-rounded libm sequence costs twice the instructions.
+// x >= 1e-300; the IEEE-rounded libm sequence costs twice the instructions.
+// x = 0 gives exactly 0 (rsq of the clamped argument is finite, r = x * y = 0).
 __device__ __forceinline__ double sqrt_fast(double x) {
-  const double y = __builtin_amdgcn_rsq(x);
+  const double y = __builtin_amdgcn_rsq(fmax(x, 1e-300));
   const double hy = 0.5 * y;
   double r = x * y;
   r = fma(hy, fma(-r, r, x), r);
   r = fma(hy, fma(-r, r, x), r);
-  return x > 0.0 ? r : 0.0;
+  return r;
 }
 __device__ __forceinline__ float sqrt_fast(float x) { return __builtin_amdgcn_sqrtf(x); }
 

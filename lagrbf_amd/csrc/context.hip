@@ -105,6 +105,10 @@ struct Ctx {
   std::vector<int32_t> h_pcount;
   DBuf<double> xs_sorted, zeros;           // cell-sorted node coordinates (SoA), zero coefficients
   DBuf<float> xs32, zeros32, coef32_pool;  // mixed path: cell-relative FP32 coordinates, FP32 coefficients
+  // cell-major node table of the evaluation (coordinates + corner-patch coefficients per node)
+  DBuf<double> blk64;
+  DBuf<float> blk32;
+  bool blk_valid = false, blk_f32 = false;
   std::vector<int32_t> h_cell_start_prev, h_cell_nodes_prev;   // cell lists the frozen patches were built from
   bool memb_current = true;                // every set-up patch's members == current cell lists
   int eval_mode = 0;                       // 0 auto (cell-ordered when possible), 1 cell-ordered, 2 reference order
@@ -310,6 +314,7 @@ static int fill_data(Ctx& c, int n, const double* coords, const double* values, 
   const char* fn = "FillDataSwarm";
   const int dim = c.g.dim, nf = c.nf;
   CHK(check_bbox(c, n, coords, fn));
+  c.blk_valid = false;
   if (c.setup_N >= 0 && c.solve_method != 2 && n != c.setup_N && c.mem_used > 0)
     return set_err(E_SIZ, fn, "solve_method %d keeps patch factors across data fills; node count changed from %d to %d "
                    "(call with solve_method 2 or destroy the context)", c.solve_method, c.setup_N, n);
@@ -588,6 +593,7 @@ static int partitions_setup(Ctx& c) {
     if (a != c.h_active[p]) { c.h_active[p] = a; changed = true; }
   }
   if (changed) HIPCHK(hipMemcpyAsync(c.p_active.p, c.h_active.data(), P, hipMemcpyHostToDevice, c.s));
+  if (changed || !solvel.empty()) c.blk_valid = false;
   HIPCHK(hipEventRecord(e1, c.s));
   HIPCHK(hipStreamSynchronize(c.s));
   float ms = 0.f;
@@ -601,6 +607,7 @@ static int partitions_setup(Ctx& c) {
 // LagRBF_PartitionsReset, partition.c:494-524
 static int partitions_reset(Ctx& c) {
   const int P = c.g.P;
+  c.blk_valid = false;
   if (c.solve_method == 2) {
     std::fill(c.h_setup.begin(), c.h_setup.end(), 0);
     std::fill(c.h_solved.begin(), c.h_solved.end(), 0);
@@ -618,6 +625,20 @@ static int partitions_reset(Ctx& c) {
 }
 
 // RBFInterpolation(ctx, swi, swd, name), interpolation.c:178-272
+// Cell-major node table for the evaluation, rebuilt after the data or any patch changed.
+static int ensure_blk(Ctx& c, bool f32) {
+  if (c.blk_valid && c.blk_f32 == f32) return 0;
+  const size_t n = ((size_t)c.N + 1) * blk_row(c.g.dim, c.nf);   // N rows + padding row
+  if (f32) HIPCHK(c.blk32.reserve(n));
+  else HIPCHK(c.blk64.reserve(n));
+  if (c.N > 0)
+    launch_build_blk(c.g, f32, c.dx.p, c.N, c.cell_nodes.p, c.dcell.p, c.cell_start.p, c.p_active.p, c.p_mem_off.p,
+                     c.coef_pool.p, c.nf, f32 ? (void*)c.blk32.p : (void*)c.blk64.p, c.s);
+  c.blk_valid = true;
+  c.blk_f32 = f32;
+  return 0;
+}
+
 static int rbf_interpolation(Ctx& c, double* out, int nfields) {
   if (nfields <= 0 || c.Np == 0) return 0;
   PatchView pv{c.p_n.p, c.p_active.p, c.p_mem_off.p, c.mem_pool.p, c.coef_pool.p, c.nf};
@@ -635,10 +656,19 @@ static int rbf_interpolation(Ctx& c, double* out, int nfields) {
     c.sorted_valid = true;
   }
   if (!c.zeros.p) { HIPCHK(c.zeros.reserve(1)); HIPCHK(hipMemset(c.zeros.p, 0, sizeof(double))); }
+  const bool use_blk = cell_order && eval_blk_supported(c.nf);
+  const bool blk32 = c.precision && eval_f32_supported(c.ktype);
+  if (use_blk) CHK(ensure_blk(c, blk32));
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0)); HIPCHK(hipEventCreate(&e1));
   HIPCHK(hipEventRecord(e0, c.s));
-  if (cell_order) {
+  if (use_blk) {
+    const int per_cell = (c.Np + c.g.C - 1) / std::max(c.g.C, 1);
+    const int block = std::min(256, std::max(64, (per_cell + 63) / 64 * 64));
+    launch_eval_blk(c.g, c.ktype, blk32, c.px.p, c.ld, c.perm.p, c.pstart.p, c.w.p,
+                    blk32 ? (const void*)c.blk32.p : (const void*)c.blk64.p, c.nf, c.cell_start.p, nfields, out, block,
+                    c.s);
+  } else if (cell_order) {
     const int per_cell = (c.Np + c.g.C - 1) / std::max(c.g.C, 1);
     const int block = std::min(256, std::max(64, (per_cell + 63) / 64 * 64));
     const bool f32 = c.precision && eval_f32_supported(c.ktype) && c.xs32.p && c.coef32_pool.p;
@@ -892,6 +922,7 @@ static int advect_interpolate(Ctx& c, double t, double t_next) {
     // CellInfoReset + DMSwarmSetLocalSizes(swarm_data,0,-1), time_loop.c:94-98
     c.N = 0;
     c.data_filled = false;
+    c.blk_valid = false;
     HIPCHK(hipMemsetAsync(c.cell_start.p, 0, sizeof(int32_t) * (c.g.C + 1), c.s));
     HIPCHK(hipStreamSynchronize(c.s));
   }

@@ -1,0 +1,294 @@
+// Cell-ordered PU evaluation over a cell-major node table (interpolation.c:39-154
+// restated for the GPU).
+//
+// Table: one row per data node, in cell-sorted order (row s = s-th entry of the
+// per-cell node lists), BS = 4 + 2^d * nft elements:
+//   [x, y, (z), pad | coefficients of the node in the patches of the 2^d corners
+//    of its cell, corner order Q1, nft fields each]
+// Each (patch, member) coefficient appears exactly once: a member of patch p lies
+// in one cell of p, and p is the patch of one corner of that cell.  An inactive
+// patch (not set up) holds zeros.
+//
+// Evaluation: one workgroup per particle cell c (XCD-aware order), one lane per
+// particle, waves past the cell's particle count exit at once.  The 3^d
+// neighbour cells q are visited in ascending cell index and their nodes in
+// ascending node index; phi(|x - x_j|) is computed once per (particle, node) and
+// accumulated into every corner patch of c that holds q, so each patch's
+// members are summed in member order (patch = concatenation of its cells' node
+// lists, partition.c:337-371).  Which row entries a neighbour contributes to
+// which corner is fixed at compile time, so the row pointer is the only address
+// the node loop computes: coordinates and coefficients are wave-uniform scalar
+// loads at constant offsets, used as SGPR operands.
+//
+// T = double: absolute FP64 coordinates.  T = float (mixed precision): particle
+// and node coordinates relative to their own cell origin (shifted in FP64 before
+// rounding), FP32 coefficients, FP64 Shepard combination.
+#include "kernels.hpp"
+
+namespace lrb {
+
+// Q1 corner index of offset (a, b, c) in {0,1}^3 (DMDAGetElements order)
+__host__ __device__ constexpr int q1_index(int a, int b, int c) { return (b ? (a ? 2 : 3) : (a ? 1 : 0)) + 4 * c; }
+
+// For particle-cell corner k and neighbour cell nbi (di,dj,dk in {-1,0,1}, i fastest):
+// the corner index of that patch's node within the neighbour cell, or -1 when the
+// corner's patch does not hold the neighbour cell.
+template <int DIM>
+__host__ __device__ constexpr int corner_in_nbr(int nbi, int k) {
+  const int di = nbi % 3 - 1, dj = (nbi / 3) % 3 - 1, dk = DIM == 3 ? nbi / 9 - 1 : 0;
+  const int oi = ((k & 3) == 1 || (k & 3) == 2) ? 1 : 0, oj = ((k & 3) >= 2) ? 1 : 0, ok = k >> 2;
+  const int a = oi - di, b = oj - dj, c = ok - dk;
+  const bool in = a >= 0 && a <= 1 && b >= 0 && b <= 1 && c >= 0 && c <= 1;
+  return in ? q1_index(a, b, c) : -1;
+}
+
+template <int KT>
+__host__ __device__ constexpr bool compact_support_blk() { return KT == K_WC0 || KT == K_WC2 || KT == K_BUMP; }
+
+template <int DIM, int NFT>
+__host__ __device__ constexpr int blk_stride() { return 4 + (1 << DIM) * NFT; }
+
+template <int DIM, int KT, int NFT, typename T, int NBI>
+__device__ __forceinline__ void accum_blk(T xp, T yp, T zp, const T* __restrict__ B, int m, T sx, T sy, T sz,
+                                          T (&acc)[1 << DIM][NFT], T eps, T rcut2) {
+  constexpr int NEN = 1 << DIM, BS = blk_stride<DIM, NFT>();
+  // the next row's coordinates are loaded one node ahead (the table ends with a
+  // padding row, so row m is always readable)
+  T n0 = B[0], n1 = B[1], n2 = DIM == 3 ? B[2] : T(0);
+  for (int r = 0; r < m; r++) {
+    const T* Br = B + (size_t)r * BS;
+    const T c0 = n0, c1 = n1, c2 = n2;
+    n0 = Br[BS]; n1 = Br[BS + 1];
+    if constexpr (DIM == 3) n2 = Br[BS + 2];
+    T d0, d1, d2;
+    if constexpr (sizeof(T) == 4) {
+      d0 = xp - (c0 + sx); d1 = yp - (c1 + sy);
+      d2 = DIM == 3 ? zp - (c2 + sz) : T(0);
+    } else {
+      d0 = xp - c0; d1 = yp - c1;
+      d2 = DIM == 3 ? zp - c2 : T(0);
+    }
+    const T r2 = d0 * d0 + d1 * d1 + d2 * d2;
+    if constexpr (compact_support_blk<KT>()) {
+      // every lane beyond the support: phi = 0 exactly, nothing to add
+      if (!__any(r2 < rcut2)) continue;
+    }
+    const T phi = rbf<KT, T>(sqrt_fast(r2), eps);
+#pragma unroll
+    for (int k = 0; k < NEN; k++) {
+      const int kp = corner_in_nbr<DIM>(NBI, k);
+      if (kp >= 0) {
+#pragma unroll
+        for (int f = 0; f < NFT; f++) acc[k][f] += phi * Br[4 + kp * NFT + f];
+      }
+    }
+  }
+}
+
+template <int DIM, int KT, int NFT, typename T>
+__global__ __launch_bounds__(256) void k_eval_blk(Grid g, const double* __restrict__ px, int64_t ld,
+                                                  const int32_t* __restrict__ perm, const int32_t* __restrict__ pstart,
+                                                  const double* __restrict__ w, const T* __restrict__ blk,
+                                                  const int32_t* __restrict__ cs, int nfields,
+                                                  double* __restrict__ out) {
+  constexpr int NEN = 1 << DIM;
+  constexpr int NNB = DIM == 3 ? 27 : 9;
+  constexpr int BS = blk_stride<DIM, NFT>();
+  // XCD-aware order: block b runs on XCD b % 8, which gets a contiguous range of cells
+  const int per = (gridDim.x + 7) / 8;
+  const int b = blockIdx.x;
+  const int c = (b % 8) * per + b / 8;
+  if (c >= g.C) return;
+  const int pb = pstart[c], pe = pstart[c + 1];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  if (pb + wv * 64 >= pe) return;
+  const int ci = c % g.nc[0], cj = (c / g.nc[0]) % g.nc[1], ck = DIM == 3 ? c / (g.nc[0] * g.nc[1]) : 0;
+  // an empty 3^d neighbourhood leaves the particles' values untouched (interpolation.c:89)
+  int nnb = 0;
+  for (int nbi = 0; nbi < NNB; nbi++) {
+    const int a = ci + nbi % 3 - 1, bb = cj + (nbi / 3) % 3 - 1, e = ck + (DIM == 3 ? nbi / 9 - 1 : 0);
+    if (a < 0 || a >= g.nc[0] || bb < 0 || bb >= g.nc[1] || (DIM == 3 && (e < 0 || e >= g.nc[2]))) continue;
+    const int q = a + g.nc[0] * (bb + g.nc[1] * e);
+    nnb += cs[q + 1] - cs[q];
+  }
+  if (nnb == 0) return;
+  const T eps = (T)g.eps_r;
+  const double rc = 1.0 / g.eps_r;
+  const T rcut2 = (T)(rc * rc * (sizeof(T) == 4 ? (1.0 + 1e-5) : (1.0 + 1e-12)));
+  // FP32: coordinates relative to the particle cell's origin
+  const double ox = sizeof(T) == 4 ? g.O[0] + ci * g.h[0] : 0.0;
+  const double oy = sizeof(T) == 4 ? g.O[1] + cj * g.h[1] : 0.0;
+  const double oz = sizeof(T) == 4 && DIM == 3 ? g.O[2] + ck * g.h[2] : 0.0;
+  for (int s0 = pb + wv * 64; s0 < pe; s0 += nw * 64) {
+    const int s = s0 + lane;
+    const bool live = s < pe;
+    const int i = live ? perm[s] : perm[s0];
+    T xp, yp, zp;
+    if constexpr (sizeof(T) == 4) {
+      xp = (T)(px[i] - ox); yp = (T)(px[ld + i] - oy); zp = DIM == 3 ? (T)(px[2 * ld + i] - oz) : T(0);
+    } else {
+      xp = px[i]; yp = px[ld + i]; zp = DIM == 3 ? px[2 * ld + i] : T(0);
+    }
+    T acc[NEN][NFT];
+#pragma unroll
+    for (int k = 0; k < NEN; k++)
+#pragma unroll
+      for (int f = 0; f < NFT; f++) acc[k][f] = T(0);
+#pragma unroll 1
+    for (int nbi = 0; nbi < NNB; nbi++) {
+      const int di = nbi % 3 - 1, dj = (nbi / 3) % 3 - 1, dk = DIM == 3 ? nbi / 9 - 1 : 0;
+      const int a = ci + di, bb = cj + dj, e = ck + dk;
+      if (a < 0 || a >= g.nc[0] || bb < 0 || bb >= g.nc[1] || (DIM == 3 && (e < 0 || e >= g.nc[2]))) continue;
+      const int q = a + g.nc[0] * (bb + g.nc[1] * e);
+      const int nb = cs[q], ne = cs[q + 1];
+      if (nb == ne) continue;
+      const T sx = sizeof(T) == 4 ? (T)(di * g.h[0]) : T(0);
+      const T sy = sizeof(T) == 4 ? (T)(dj * g.h[1]) : T(0);
+      const T sz = sizeof(T) == 4 && DIM == 3 ? (T)(dk * g.h[2]) : T(0);
+      const T* B = blk + (size_t)nb * BS;
+      switch (nbi) {
+#define LRB_CASE(V) case V: accum_blk<DIM, KT, NFT, T, V>(xp, yp, zp, B, ne - nb, sx, sy, sz, acc, eps, rcut2); break;
+        LRB_CASE(0) LRB_CASE(1) LRB_CASE(2) LRB_CASE(3) LRB_CASE(4) LRB_CASE(5) LRB_CASE(6) LRB_CASE(7) LRB_CASE(8)
+        LRB_CASE(9) LRB_CASE(10) LRB_CASE(11) LRB_CASE(12) LRB_CASE(13) LRB_CASE(14) LRB_CASE(15) LRB_CASE(16)
+        LRB_CASE(17) LRB_CASE(18) LRB_CASE(19) LRB_CASE(20) LRB_CASE(21) LRB_CASE(22) LRB_CASE(23) LRB_CASE(24)
+        LRB_CASE(25) LRB_CASE(26)
+#undef LRB_CASE
+        default: break;
+      }
+    }
+    if (!live) continue;
+    double val[NFT];
+#pragma unroll
+    for (int f = 0; f < NFT; f++) val[f] = 0.0;
+#pragma unroll
+    for (int k = 0; k < NEN; k++) {
+      const double wk = w[(size_t)k * ld + i];
+#pragma unroll
+      for (int f = 0; f < NFT; f++) val[f] += (double)acc[k][f] * wk;
+    }
+#pragma unroll
+    for (int f = 0; f < NFT; f++) if (f < nfields) out[(size_t)f * ld + i] = val[f];
+  }
+}
+
+// Table row s: node cn[s] of cell q = dcell[cn[s]], r = s - cs[q] its rank in q;
+// row N is zero padding.
+// Coefficient of corner kp: patch p = node kp of q; member index = (nodes of the
+// cells of p before q, ascending cell index) + r.
+template <typename T>
+__global__ void k_build_blk(Grid g, const double* __restrict__ x, int N, const int32_t* __restrict__ cn,
+                            const int32_t* __restrict__ dcell, const int32_t* __restrict__ cs,
+                            const uint8_t* __restrict__ active, const int64_t* __restrict__ mem_off,
+                            const double* __restrict__ coef, int nft, T* __restrict__ blk) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s > N) return;
+  const int dim = g.dim, BS = 4 + g.nen * nft;
+  if (s == N) {
+    for (int e = 0; e < BS; e++) blk[(size_t)N * BS + e] = T(0);
+    return;
+  }
+  const int j = cn[s], q = dcell[j], r = s - cs[q];
+  const int qi = q % g.nc[0], qj = (q / g.nc[0]) % g.nc[1], qk = dim == 3 ? q / (g.nc[0] * g.nc[1]) : 0;
+  const int qq[3] = {qi, qj, qk};
+  T* row = blk + (size_t)s * BS;
+  for (int d = 0; d < 4; d++) {
+    double v = 0.0;
+    if (d < dim) v = sizeof(T) == 4 ? x[(size_t)j * dim + d] - (g.O[d] + qq[d] * g.h[d]) : x[(size_t)j * dim + d];
+    row[d] = (T)v;
+  }
+  for (int kp = 0; kp < g.nen; kp++) {
+    int a, b, c;
+    corner_offset(kp, a, b, c);
+    const int pi = qi + a, pj = qj + b, pk = qk + c;
+    const int p = pi + g.n[0] * (pj + g.n[1] * pk);
+    T* dst = row + 4 + kp * nft;
+    if (!active[p]) {
+      for (int f = 0; f < nft; f++) dst[f] = T(0);
+      continue;
+    }
+    int off = 0;
+    for (int e2 = 0; e2 <= (dim == 3 ? 1 : 0); e2++)
+      for (int e1 = 0; e1 <= 1; e1++)
+        for (int e0 = 0; e0 <= 1; e0++) {
+          const int ii = pi - 1 + e0, jj = pj - 1 + e1, kk = dim == 3 ? pk - 1 + e2 : 0;
+          if (ii < 0 || ii >= g.nc[0] || jj < 0 || jj >= g.nc[1] || (dim == 3 && (kk < 0 || kk >= g.nc[2]))) continue;
+          const int c2 = ii + g.nc[0] * (jj + g.nc[1] * kk);
+          if (c2 < q) off += cs[c2 + 1] - cs[c2];
+        }
+    const double* src = coef + (size_t)(mem_off[p] + off + r) * nft;
+    for (int f = 0; f < nft; f++) dst[f] = (T)src[f];
+  }
+}
+
+bool eval_blk_supported(int nft) { return nft >= 1 && nft <= 4; }
+size_t blk_row(int dim, int nft) { return 4 + ((size_t)1 << dim) * (size_t)nft; }
+
+void launch_build_blk(const Grid& g, bool fp32, const double* x, int N, const int32_t* cn, const int32_t* dcell,
+                      const int32_t* cs, const uint8_t* active, const int64_t* mem_off, const double* coef, int nft,
+                      void* blk, hipStream_t s) {
+  if (N <= 0) return;
+  // N + 1 threads: rows 0..N-1 and the padding row N
+  if (fp32)
+    hipLaunchKernelGGL(k_build_blk<float>, dim3((N + 256) / 256), dim3(256), 0, s, g, x, N, cn, dcell, cs, active,
+                       mem_off, coef, nft, (float*)blk);
+  else
+    hipLaunchKernelGGL(k_build_blk<double>, dim3((N + 256) / 256), dim3(256), 0, s, g, x, N, cn, dcell, cs, active,
+                       mem_off, coef, nft, (double*)blk);
+}
+
+template <int DIM, int KT, typename T>
+static void eval_blk_k(const Grid& g, const double* px, int64_t ld, const int32_t* perm, const int32_t* pstart,
+                       const double* w, const T* blk, int nft, const int32_t* cs, int nfields, double* out, int block,
+                       hipStream_t s) {
+  const int per = (g.C + 7) / 8;
+  const dim3 grid(per * 8), blkd(block);
+#define LRB_EB(NFT) hipLaunchKernelGGL((k_eval_blk<DIM, KT, NFT, T>), grid, blkd, 0, s, g, px, ld, perm, pstart, w, blk, cs, nfields, out)
+  switch (nft) {
+    case 1: LRB_EB(1); break;
+    case 2: LRB_EB(2); break;
+    case 3: LRB_EB(3); break;
+    default: LRB_EB(4); break;
+  }
+#undef LRB_EB
+}
+
+template <int DIM, typename T>
+static void eval_blk_dim(const Grid& g, int ktype, const double* px, int64_t ld, const int32_t* perm,
+                         const int32_t* pstart, const double* w, const T* blk, int nft, const int32_t* cs, int nfields,
+                         double* out, int block, hipStream_t s) {
+#define LRB_K(K) eval_blk_k<DIM, K, T>(g, px, ld, perm, pstart, w, blk, nft, cs, nfields, out, block, s)
+  if constexpr (sizeof(T) == 8) {
+    switch (ktype) {
+      case K_LINEAR: LRB_K(K_LINEAR); break;
+      case K_WC0: LRB_K(K_WC0); break;
+      case K_WC2: LRB_K(K_WC2); break;
+      case K_BUMP: LRB_K(K_BUMP); break;
+      case K_MQ: LRB_K(K_MQ); break;
+      case K_IMQ: LRB_K(K_IMQ); break;
+      default: LRB_K(K_GAUSS); break;
+    }
+  } else {
+    switch (ktype) {
+      case K_WC0: LRB_K(K_WC0); break;
+      case K_WC2: LRB_K(K_WC2); break;
+      case K_IMQ: LRB_K(K_IMQ); break;
+      default: LRB_K(K_GAUSS); break;
+    }
+  }
+#undef LRB_K
+}
+
+void launch_eval_blk(const Grid& g, int ktype, bool fp32, const double* px, int64_t ld, const int32_t* perm,
+                     const int32_t* pstart, const double* w, const void* blk, int nft, const int32_t* cs, int nfields,
+                     double* out, int block, hipStream_t s) {
+  if (nfields <= 0 || g.C <= 0) return;
+  if (fp32) {
+    if (g.dim == 3) eval_blk_dim<3, float>(g, ktype, px, ld, perm, pstart, w, (const float*)blk, nft, cs, nfields, out, block, s);
+    else eval_blk_dim<2, float>(g, ktype, px, ld, perm, pstart, w, (const float*)blk, nft, cs, nfields, out, block, s);
+  } else {
+    if (g.dim == 3) eval_blk_dim<3, double>(g, ktype, px, ld, perm, pstart, w, (const double*)blk, nft, cs, nfields, out, block, s);
+    else eval_blk_dim<2, double>(g, ktype, px, ld, perm, pstart, w, (const double*)blk, nft, cs, nfields, out, block, s);
+  }
+}
+
+}  // namespace lrb

@@ -463,12 +463,16 @@ __device__ __forceinline__ void weights_point(const Grid& g, const double* __res
                                               const double* __restrict__ puz, double x, double y, double z, int cell,
                                               double* w) {
   const int ci = cell % g.nc[0], cj = (cell / g.nc[0]) % g.nc[1], ck = g.dim == 3 ? cell / (g.nc[0] * g.nc[1]) : 0;
-  const double R = 1.0 / g.eps_w;
+  // PU nodes with a nonzero weight lie strictly inside (x - R, x + R) per axis; the
+  // floor / ceil window (R widened by 1e-12) holds every one of them, and a node it
+  // could miss by rounding sits at the support edge where W_C2 < 1e-60 cannot move
+  // the sum, so the sum equals the reference's sum over all PU nodes.
+  const double R = (1.0 / g.eps_w) * (1.0 + 1e-12);
   int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
   const double xs[3] = {x, y, z};
   for (int d = 0; d < g.dim; d++) {
-    double a = floor((xs[d] - g.O[d] - R) / g.h[d]) - 1.0;
-    double b = ceil((xs[d] - g.O[d] + R) / g.h[d]) + 1.0;
+    double a = floor((xs[d] - g.O[d] - R) / g.h[d]);
+    double b = ceil((xs[d] - g.O[d] + R) / g.h[d]);
     lo[d] = a < 0.0 ? 0 : (a > g.n[d] - 1 ? g.n[d] - 1 : (int)a);
     hi[d] = b < 0.0 ? 0 : (b > g.n[d] - 1 ? g.n[d] - 1 : (int)b);
   }

@@ -3,6 +3,8 @@
   python -m lagrbf_amd.rocprof_summary stats DIR
       per-kernel table from the *_kernel_stats.csv that `--kernel-trace --stats`
       writes (calls, average / total duration, share).
+  python -m lagrbf_amd.rocprof_summary pmc DIR [--kernel SUBSTR]
+      mean per-dispatch value of every counter of a `--pmc` run, per kernel.
   python -m lagrbf_amd.rocprof_summary traffic FETCH_DIR WRITE_DIR --copy-mib M [--kernel SUBSTR] [--json OUT --key K]
       HBM bytes per dispatch from two `--pmc FETCH_SIZE` / `--pmc WRITE_SIZE` runs
       (counter values in KiB per dispatch).  The runs must include a plain copy of
@@ -63,6 +65,31 @@ def counters(d, counter):
     return out
 
 
+def pmc(d, kernel_sub="", out=sys.stdout):
+    """Mean per-dispatch value of every collected counter, per kernel."""
+    per = defaultdict(lambda: defaultdict(float))
+    names = set()
+    for f in _find(d, "*counter_collection.csv"):
+        with open(f) as fh:
+            for r in csv.DictReader(fh):
+                k = short_name(r["Kernel_Name"])
+                if kernel_sub and kernel_sub not in k:
+                    continue
+                per[(k, r["Dispatch_Id"])][r["Counter_Name"]] += float(r["Counter_Value"])
+                names.add(r["Counter_Name"])
+    agg = defaultdict(lambda: defaultdict(list))
+    for (k, _), cv in per.items():
+        for n, v in cv.items():
+            agg[k][n].append(v)
+    names = sorted(names)
+    for k in sorted(agg):
+        print(k, file=out)
+        for n in names:
+            v = agg[k].get(n)
+            if v:
+                print(f"    {n:<28} {sum(v) / len(v):>18.1f}   ({len(v)} dispatches)", file=out)
+
+
 def traffic(fdir, wdir, copy_mib, kernel_sub):
     fetch, write = counters(fdir, "FETCH_SIZE"), counters(wdir, "WRITE_SIZE")
     copy_bytes = copy_mib * (1 << 20)
@@ -88,6 +115,9 @@ def main():
     s = sub.add_parser("stats")
     s.add_argument("dir")
     s.add_argument("--top", type=int, default=25)
+    c = sub.add_parser("pmc")
+    c.add_argument("dir")
+    c.add_argument("--kernel", default="")
     t = sub.add_parser("traffic")
     t.add_argument("fetch_dir")
     t.add_argument("write_dir")
@@ -98,6 +128,9 @@ def main():
     a = ap.parse_args()
     if a.cmd == "stats":
         stats(a.dir, a.top)
+        return
+    if a.cmd == "pmc":
+        pmc(a.dir, a.kernel)
         return
     res = traffic(a.fetch_dir, a.write_dir, a.copy_mib, a.kernel)
     print(f"copy kernel {res['copy_kernel']}: scale fetch {res['fetch_scale']:.4f}, write {res['write_scale']:.4f}")

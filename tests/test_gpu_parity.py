@@ -200,6 +200,27 @@ def test_binning_bit_exact():
     assert rel_err(wg, wo) < 1e-13
 
 
+@pytest.mark.parametrize("dim,overlap", [(3, 0.9), (3, 1.0), (3, 1.7), (2, 1.0), (2, 3.0)])
+def test_weights_vs_brute_force(dim, overlap):
+    """Shepard weights from the support window against the oracle's sum over every
+    PU node (swarm_interp.c:283-396): a node missing from the window would show."""
+    if dim == 3:
+        opts, X, V, P = cases.fourier_3d(nlat=8, Np=3000, n=7, overlap=overlap)
+    else:
+        opts, X, V, P = cases.rotation_2d(N=100, Np=3000, n=9, advection=False)
+        opts["pu_mesh_overlap"] = overlap
+    print(f"case: weights dim={dim} overlap={overlap}", flush=True)
+    from oracle.oracle import OracleCtx
+    g = L.LagRBFCtx(petsc_options=dict(opts))
+    o = OracleCtx(dict(g.options, oracle_weights_brute=True))
+    g.fill_points(P); o.fill_points(P)
+    wg = np.zeros((P.shape[0], 1 << dim))
+    g.output(fvals=wg, fieldname="weights")
+    wo = o.weights()
+    assert np.array_equal(wg == 0.0, wo == 0.0)
+    assert rel_err(wg, wo) < 1e-13
+
+
 def test_points_on_faces_and_edges():
     """x == O and x == L on every axis: located in the first / last cell."""
     opts, X, V, _ = cases.rotation_2d(N=500, Np=10, n=5, advection=False)
