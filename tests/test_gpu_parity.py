@@ -293,3 +293,40 @@ def test_analytical_advection():
     ex = np.stack([0.5 + c * (P[:, 0] - 0.5) - s * (P[:, 1] - 0.5), 0.5 + s * (P[:, 0] - 0.5) + c * (P[:, 1] - 0.5)], axis=1)
     # RK4 truncation at CFL 0.3 (dt ~ 0.05): ~2e-6 closure error
     assert np.max(np.abs(xg - ex)) < 1e-5
+
+
+from tests.golden import make_golden  # noqa: E402
+
+_REG = make_golden.regression_cases()
+
+
+@pytest.mark.parametrize("case", _REG, ids=[c[0] for c in _REG])
+def test_gpu_vs_golden_regression(case):
+    """The GPU library against the committed oracle vectors (tests/golden/oracle_regression.npz):
+    binning bit-exact, weights 1e-13, values 1e-12 (1e-11 / 1e-9 for the worse-conditioned
+    LINEAR, bump and multiquadric patches), positions 1e-9."""
+    import os
+    name, opts, X, V, P, action = case
+    print(f"case: golden {name}", flush=True)
+    g = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "oracle_regression.npz"),
+                allow_pickle=False)
+    c = L.LagRBFCtx(petsc_options=dict(opts))
+    c.fill_data(X, V); c.fill_points(P); c.interpolate_c()
+    if action[0] == "steps":
+        c.advect_steps_c(action[1], action[2])
+    xs = np.zeros_like(P); vals = np.zeros((P.shape[0], c.nfields)); w = np.zeros((P.shape[0], 1 << P.shape[1]))
+    c.output(xs, vals)
+    c.output(fvals=w, fieldname="weights")
+    assert np.array_equal(c.debug_get("cellid_points"), g[f"{name}/cellid_points"])
+    assert np.array_equal(c.debug_get("cellid_data"), g[f"{name}/cellid_data"])
+    assert np.array_equal(c.debug_get("cell_start"), g[f"{name}/cell_start"])
+    assert np.array_equal(c.debug_get("cell_nodes"), g[f"{name}/cell_nodes"])
+    assert c.info("removed") == g[f"{name}/removed"][0]
+    assert rel_err(w, g[f"{name}/weights"]) < 1e-13
+    if action[0] == "steps":
+        assert rel_err(xs, g[f"{name}/coords"]) < TOL_POS
+        assert rel_err(vals, g[f"{name}/values"]) < 1e-10
+    else:
+        k = int(opts["kernel_type"])
+        tol = {4: 1e-9, 0: 1e-11, 3: 1e-11}.get(k, TOL_VAL)
+        assert rel_err(vals, g[f"{name}/values"]) < tol
