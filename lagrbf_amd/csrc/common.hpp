@@ -63,9 +63,10 @@ __device__ __forceinline__ T rbf_rt(int k, T r, T eps) {
 // sqrt for the hot loops: v_rsq_f64 (about 23 correct bits) refined by two
 // Newton steps on sqrt (r <- r + (x - r^2) / (2 sqrt x)), within 1 ulp for
 // x >= 1e-300; the IEEE-rounded libm sequence costs twice the instructions.
-// x = 0 gives exactly 0 (rsq of the clamped argument is finite, r = x * y = 0).
+// x = 0 gives exactly 0 (rsq of x + 1e-300 is finite, r = x * y = 0); for x >= 1e-284
+// the added 1e-300 is below half an ulp and changes nothing.
 __device__ __forceinline__ double sqrt_fast(double x) {
-  const double y = __builtin_amdgcn_rsq(fmax(x, 1e-300));
+  const double y = __builtin_amdgcn_rsq(x + 1e-300);
   const double hy = 0.5 * y;
   double r = x * y;
   r = fma(hy, fma(-r, r, x), r);

@@ -48,8 +48,10 @@ __host__ __device__ constexpr bool compact_support_blk() { return KT == K_WC0 ||
 template <int DIM, int NFT>
 __host__ __device__ constexpr int blk_stride() { return 4 + (1 << DIM) * NFT; }
 
+// xp, yp, zp: particle coordinates in the frame of the neighbour cell's node rows
+// (FP64: absolute; FP32: relative to the neighbour cell's origin).
 template <int DIM, int KT, int NFT, typename T, int NBI>
-__device__ __forceinline__ void accum_blk(T xp, T yp, T zp, const T* __restrict__ B, int m, T sx, T sy, T sz,
+__device__ __forceinline__ void accum_blk(T xp, T yp, T zp, const T* __restrict__ B, int m,
                                           T (&acc)[1 << DIM][NFT], T eps, T rcut2) {
   constexpr int NEN = 1 << DIM, BS = blk_stride<DIM, NFT>();
   // the next row's coordinates are loaded one node ahead (the table ends with a
@@ -60,14 +62,7 @@ __device__ __forceinline__ void accum_blk(T xp, T yp, T zp, const T* __restrict_
     const T c0 = n0, c1 = n1, c2 = n2;
     n0 = Br[BS]; n1 = Br[BS + 1];
     if constexpr (DIM == 3) n2 = Br[BS + 2];
-    T d0, d1, d2;
-    if constexpr (sizeof(T) == 4) {
-      d0 = xp - (c0 + sx); d1 = yp - (c1 + sy);
-      d2 = DIM == 3 ? zp - (c2 + sz) : T(0);
-    } else {
-      d0 = xp - c0; d1 = yp - c1;
-      d2 = DIM == 3 ? zp - c2 : T(0);
-    }
+    const T d0 = xp - c0, d1 = yp - c1, d2 = DIM == 3 ? zp - c2 : T(0);
     const T r2 = d0 * d0 + d1 * d1 + d2 * d2;
     if constexpr (compact_support_blk<KT>()) {
       // every lane beyond the support: phi = 0 exactly, nothing to add
@@ -85,8 +80,16 @@ __device__ __forceinline__ void accum_blk(T xp, T yp, T zp, const T* __restrict_
   }
 }
 
+// occupancy floor (waves per SIMD) of the evaluation kernels
+#ifndef LRB_EVAL_WAVES_F64
+#define LRB_EVAL_WAVES_F64 5
+#endif
+#ifndef LRB_EVAL_WAVES_F32
+#define LRB_EVAL_WAVES_F32 8
+#endif
 template <int DIM, int KT, int NFT, typename T>
-__global__ __launch_bounds__(256) void k_eval_blk(Grid g, const double* __restrict__ px, int64_t ld,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? LRB_EVAL_WAVES_F64 : LRB_EVAL_WAVES_F32)))
+void k_eval_blk(Grid g, const double* __restrict__ px, int64_t ld,
                                                   const int32_t* __restrict__ perm, const int32_t* __restrict__ pstart,
                                                   const double* __restrict__ w, const T* __restrict__ blk,
                                                   const int32_t* __restrict__ cs, int nfields,
@@ -142,12 +145,16 @@ __global__ __launch_bounds__(256) void k_eval_blk(Grid g, const double* __restri
       const int q = a + g.nc[0] * (bb + g.nc[1] * e);
       const int nb = cs[q], ne = cs[q + 1];
       if (nb == ne) continue;
-      const T sx = sizeof(T) == 4 ? (T)(di * g.h[0]) : T(0);
-      const T sy = sizeof(T) == 4 ? (T)(dj * g.h[1]) : T(0);
-      const T sz = sizeof(T) == 4 && DIM == 3 ? (T)(dk * g.h[2]) : T(0);
+      // FP32: particle relative to the neighbour cell's origin (node rows are relative
+      // to their own cell's origin); FP64: absolute coordinates throughout
+      T qx = xp, qy = yp, qz = zp;
+      if constexpr (sizeof(T) == 4) {
+        qx = xp - (T)(di * g.h[0]); qy = yp - (T)(dj * g.h[1]);
+        if constexpr (DIM == 3) qz = zp - (T)(dk * g.h[2]);
+      }
       const T* B = blk + (size_t)nb * BS;
       switch (nbi) {
-#define LRB_CASE(V) case V: accum_blk<DIM, KT, NFT, T, V>(xp, yp, zp, B, ne - nb, sx, sy, sz, acc, eps, rcut2); break;
+#define LRB_CASE(V) case V: accum_blk<DIM, KT, NFT, T, V>(qx, qy, qz, B, ne - nb, acc, eps, rcut2); break;
         LRB_CASE(0) LRB_CASE(1) LRB_CASE(2) LRB_CASE(3) LRB_CASE(4) LRB_CASE(5) LRB_CASE(6) LRB_CASE(7) LRB_CASE(8)
         LRB_CASE(9) LRB_CASE(10) LRB_CASE(11) LRB_CASE(12) LRB_CASE(13) LRB_CASE(14) LRB_CASE(15) LRB_CASE(16)
         LRB_CASE(17) LRB_CASE(18) LRB_CASE(19) LRB_CASE(20) LRB_CASE(21) LRB_CASE(22) LRB_CASE(23) LRB_CASE(24)

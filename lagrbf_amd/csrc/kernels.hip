@@ -459,35 +459,42 @@ void launch_solve_bk(const int32_t* list, int nlist, const int32_t* pn, const in
 // support contribute exact zeros, so the sum runs over the support window in
 // ascending node order.
 // --------------------------------------------------------------------------
+// PU node coordinate O + i*h, as the host builds the node arrays (DMDASetUniformCoordinates,
+// partition_of_unity.c:143): product and sum rounded separately.
+__device__ __forceinline__ double pu_coord(const Grid& g, int d, int i) {
+#pragma clang fp contract(off)
+  return g.O[d] + (double)i * g.h[d];
+}
+
 __device__ __forceinline__ void weights_point(const Grid& g, const double* __restrict__ pux, const double* __restrict__ puy,
                                               const double* __restrict__ puz, double x, double y, double z, int cell,
                                               double* w) {
+  // Every PU node with a nonzero weight lies within R of x, so inside the window of
+  // cells ci - m .. ci + 1 + m per axis, m = ceil(R / h) (R widened by 1e-12); the
+  // nodes outside it add exact zeros to the reference's sum over all PU nodes
+  // (swarm_interp.c:327-372), which is accumulated here in the same ascending
+  // node order.  The window depends on the cell only, so lanes of a wave that
+  // share a cell run the same trip counts.
   const int ci = cell % g.nc[0], cj = (cell / g.nc[0]) % g.nc[1], ck = g.dim == 3 ? cell / (g.nc[0] * g.nc[1]) : 0;
-  // PU nodes with a nonzero weight lie strictly inside (x - R, x + R) per axis; the
-  // floor / ceil window (R widened by 1e-12) holds every one of them, and a node it
-  // could miss by rounding sits at the support edge where W_C2 < 1e-60 cannot move
-  // the sum, so the sum equals the reference's sum over all PU nodes.
   const double R = (1.0 / g.eps_w) * (1.0 + 1e-12);
+  const int cc[3] = {ci, cj, ck};
   int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
-  const double xs[3] = {x, y, z};
   for (int d = 0; d < g.dim; d++) {
-    double a = floor((xs[d] - g.O[d] - R) / g.h[d]);
-    double b = ceil((xs[d] - g.O[d] + R) / g.h[d]);
-    lo[d] = a < 0.0 ? 0 : (a > g.n[d] - 1 ? g.n[d] - 1 : (int)a);
-    hi[d] = b < 0.0 ? 0 : (b > g.n[d] - 1 ? g.n[d] - 1 : (int)b);
+    const int m = (int)ceil(R / g.h[d]);
+    lo[d] = max(cc[d] - m, 0);
+    hi[d] = min(cc[d] + 1 + m, g.n[d] - 1);
   }
   double sum = 0.0;
   for (int k = 0; k < g.nen; k++) w[k] = 0.0;
   for (int kk = lo[2]; kk <= hi[2]; kk++) {
-    const double dz = g.dim == 3 ? z - puz[kk] : 0.0;
+    const double dz = g.dim == 3 ? z - pu_coord(g, 2, kk) : 0.0;
     const int ek = kk - ck;
     for (int j = lo[1]; j <= hi[1]; j++) {
-      const double dy = y - puy[j];
+      const double dy = y - pu_coord(g, 1, j);
       const int ej = j - cj;
       for (int i = lo[0]; i <= hi[0]; i++) {
-        const double dx = x - pux[i];
-        double r = sqrt(dx * dx + dy * dy + dz * dz);
-        double wt = rbf<K_WC2, double>(r, g.eps_w);
+        const double dx = x - pu_coord(g, 0, i);
+        const double wt = rbf<K_WC2, double>(sqrt_fast(dx * dx + dy * dy + dz * dz), g.eps_w);
         sum += wt;
         const int ei = i - ci;
         if ((unsigned)ei <= 1u && (unsigned)ej <= 1u && (unsigned)ek <= 1u) {
@@ -497,7 +504,8 @@ __device__ __forceinline__ void weights_point(const Grid& g, const double* __res
       }
     }
   }
-  for (int k = 0; k < g.nen; k++) w[k] = sum > 0.0 ? w[k] / sum : 0.0;
+  const double inv = sum > 0.0 ? 1.0 / sum : 0.0;
+  for (int k = 0; k < g.nen; k++) w[k] *= inv;
 }
 
 __global__ void k_weights(Grid g, const double* __restrict__ pux, const double* __restrict__ puy, const double* __restrict__ puz,
