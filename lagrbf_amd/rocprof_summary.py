@@ -6,8 +6,8 @@
   python -m lagrbf_amd.rocprof_summary pmc DIR [--kernel SUBSTR]
       mean per-dispatch value of every counter of a `--pmc` run, per kernel.
   python -m lagrbf_amd.rocprof_summary traffic FETCH_DIR WRITE_DIR --copy-mib M [--kernel SUBSTR] [--json OUT --key K]
-      HBM bytes per dispatch from two `--pmc FETCH_SIZE` / `--pmc WRITE_SIZE` runs
-      (counter values in KiB per dispatch).  The runs must include a plain copy of
+      HBM bytes per dispatch (median over a kernel's dispatches) from two
+      `--pmc FETCH_SIZE` / `--pmc WRITE_SIZE` runs (counter values in KiB per dispatch).  The runs must include a plain copy of
       M MiB (bench.py --copy-calibration M launches torch.neg over float64, which
       reads M MiB and writes M MiB); each counter is scaled by known bytes /
       counter bytes of that copy.
@@ -104,9 +104,17 @@ def traffic(fdir, wdir, copy_mib, kernel_sub):
         if kernel_sub and kernel_sub not in k:
             continue
         fv, wv = fetch[k], write.get(k, [0.0])
-        res["kernels"][k] = {"dispatches": len(fv), "fetch_bytes": sf * 1024.0 * sum(fv) / len(fv),
-                             "write_bytes": sw * 1024.0 * sum(wv) / len(wv)}
+        # median over dispatches: steady state (a kernel's first dispatch may see a cold
+        # or unsorted input)
+        res["kernels"][k] = {"dispatches": len(fv), "fetch_bytes": sf * 1024.0 * median(fv),
+                             "write_bytes": sw * 1024.0 * median(wv)}
     return res
+
+
+def median(v):
+    v = sorted(v)
+    n = len(v)
+    return v[n // 2] if n % 2 else 0.5 * (v[n // 2 - 1] + v[n // 2])
 
 
 def main():
@@ -141,7 +149,7 @@ def main():
         if os.path.exists(a.json):
             with open(a.json) as fh:
                 data = json.load(fh)
-        ev = [v for k, v in res["kernels"].items() if "k_eval_cell" in k]
+        ev = [v for k, v in res["kernels"].items() if "k_eval_" in k]
         if ev:
             big = max(ev, key=lambda v: v["fetch_bytes"])
             data[a.key] = {"fetch_bytes": big["fetch_bytes"], "write_bytes": big["write_bytes"],

@@ -11,9 +11,10 @@
  * the LagRBF source tree).  Numerics: strict IEEE double, no fast-math, no FMA
  * contraction (built with -ffp-contract=off).
  *
- * Pinning (what checks each function, see tests/test_oracle_*.py):
+ * Pinning (what checks each function, see tests/test_oracle.py and
+ * tests/test_golden.py):
  *   orc_kernel          closed forms + hand values                       (pinned)
- *   orc_sytrf/sytrs     LAPACK dsytrf/dsytrs via scipy, residual checks  (pinned)
+ *   orc_sytrf/sytrs     LAPACK dsytrf/dsytrs via scipy + committed vectors (pinned)
  *   orc_potrf/potrs     LAPACK dpotrf via scipy                          (pinned)
  *   orc_locate          brute-force interval search on tiny grids        (pinned)
  *   orc_cell_sort       numpy stable argsort                             (pinned)
