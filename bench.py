@@ -139,7 +139,7 @@ def run_path(L, precision, opts, X, V, P, steps, warmup, dt, sample):
         "value": P.shape[0] * steps / wall, "ms_per_step": wall / max(steps, 1) * 1e3,
         "end_to_end": P.shape[0] * steps / (wall + t_setup), "setup_s": t_setup, "removed_particles": removed,
         "fallback_patches": ctx.info("fallback"), "refine_iters_max": ctx.info("refine_iters"),
-        "roofline": {"bound": "valu", "kernel": "pu_evaluation (k_eval_cell)",
+        "roofline": {"bound": "valu", "kernel": "pu_evaluation (k_eval_blk)",
                      "achieved": achieved / 1e12, "peak": PEAK[precision] / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / PEAK[precision], "traffic": traffic,
                      "flops_per_launch": flops, "avg_launch_ms": avg_ms, "launches_timed": int(n_launch)},
@@ -182,8 +182,8 @@ def main():
     ap.add_argument("--particles", type=int, default=4194304)
     ap.add_argument("--precision", default="both", choices=["both", "fp64", "mixed"])
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=2)
-    ap.add_argument("--cpu-particles", type=int, default=32768)
+    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--cpu-particles", type=int, default=65536)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="launch a plain copy of MIB MiB first (scale for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
