@@ -111,6 +111,48 @@ int64_t lagrbf_debug_get(void *ctx, const char *what, int32_t index, void *out, 
 /* Copy the last error message (NUL-terminated, truncated to len). */
 int lagrbf_last_error(char *buf, int32_t len);
 
+/* ---------------- serving (no reference counterpart) ----------------
+ * Continuous batching of advection requests against device-resident datasets.
+ * A dataset is filled once and all its patches are set up and solved (factors
+ * are then released; the evaluation table stays).  Requests on a dataset and
+ * scheme share one particle batch: each tick admits queued requests while the
+ * HBM budget allows, advances every active request by one step (own dt; DOPRI
+ * step size controlled per request) and retires the finished ones.  A
+ * request's result does not depend on the other requests in the batch. */
+
+/* New server; hbm_budget_bytes <= 0 selects 200e9. */
+void *lagrbf_serve_create(double hbm_budget_bytes);
+void lagrbf_serve_destroy(void *srv);
+
+/* Dataset from an options string ("-pu_mesh_dim 3 -pu_mesh_n 32,32,32 ...", the
+ * names lagrbf_set_option accepts; n_interpolation_fields must equal the
+ * dimension: values are the velocity) and n nodes (coords [n*dim], values
+ * [n*dim]).  Returns the dataset id (>= 0) or a negative error code. */
+int32_t lagrbf_serve_add_dataset(void *srv, const char *options, int32_t n, double *coords, double *values);
+
+/* Queue a request: n particles (coords [n*dim], inside the dataset's box), scheme
+ * 0..3 (RK1, RK2, RK4, DOPRI), nsteps steps of size dt (DOPRI: initial dt).
+ * Returns the request id (>= 0) or a negative error code. */
+int64_t lagrbf_serve_submit(void *srv, int32_t dataset, int32_t scheme, int32_t nsteps, double dt, int32_t n,
+                            const double *coords);
+
+/* One tick (admission, one step of every active request, retirement).  Returns
+ * the number of queued + active requests, or a negative error code. */
+int32_t lagrbf_serve_step(void *srv);
+
+/* 1 when the request is done (final positions copied to out [n*dim], rows of
+ * particles that left the box are zero), 0 while pending, negative on error.
+ * times (may be NULL): submit, admit, done, in seconds since lagrbf_serve_create
+ * (-1 while not reached). */
+int32_t lagrbf_serve_poll(void *srv, int64_t id, double *out, double *times);
+
+/* Drop a finished request's host copy. */
+int lagrbf_serve_release(void *srv, int64_t id);
+
+/* "completed", "particle_steps", "ticks", "reserved" (bytes), "queued",
+ * "max_active", "now", "datasets". */
+int lagrbf_serve_info(void *srv, const char *key, double *value);
+
 #ifdef __cplusplus
 }
 #endif

@@ -8,6 +8,7 @@ from .utils import PointContainer, FunctionData, FunctionPointer, halton_sequenc
 from .context import LagRBFCtx
 from .interpolation import LagRBFInterpolator
 from .advection import Advection, AdvectionAnalytical, AdvectionInterpolation
+from .serve import Server
 
 __all__ = ["PointContainer", "FunctionData", "FunctionPointer", "halton_sequence", "structured_points", "halton_points",
-           "LagRBFCtx", "LagRBFInterpolator", "Advection", "AdvectionAnalytical", "AdvectionInterpolation"]
+           "LagRBFCtx", "LagRBFInterpolator", "Advection", "AdvectionAnalytical", "AdvectionInterpolation", "Server"]

@@ -126,6 +126,22 @@ def register_functions(clib):
     c.restype = ctypes.c_int
     c.argtypes = [ctypes.c_char_p, PetscInt]
     f["lagrbf_last_error"] = c
+    # serving
+    D = ndpointer(REAL, flags="C_CONTIGUOUS")
+    for name, res, args in (
+            ("lagrbf_serve_create", ctypes.c_void_p, [ctypes.c_double]),
+            ("lagrbf_serve_destroy", None, [ctypes.c_void_p]),
+            ("lagrbf_serve_add_dataset", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_char_p, PetscInt, D, D]),
+            ("lagrbf_serve_submit", ctypes.c_int64, [ctypes.c_void_p, PetscInt, PetscInt, PetscInt, PetscReal, PetscInt, D]),
+            ("lagrbf_serve_step", ctypes.c_int32, [ctypes.c_void_p]),
+            ("lagrbf_serve_poll", ctypes.c_int32, [ctypes.c_void_p, ctypes.c_int64, wrap_ndpointer(REAL, flags="C_CONTIGUOUS"),
+                                                   wrap_ndpointer(REAL, flags="C_CONTIGUOUS")]),
+            ("lagrbf_serve_release", ctypes.c_int, [ctypes.c_void_p, ctypes.c_int64]),
+            ("lagrbf_serve_info", ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p, ctypes.POINTER(ctypes.c_double)])):
+        c = getattr(clib, name)
+        c.restype = res
+        c.argtypes = args
+        f[name] = c
     return f
 
 
@@ -149,4 +165,6 @@ EXPORTED_SYMBOLS = [
     "lagrbf_advection_function_batch", "lagrbf_reset_coefficients", "lagrbf_interpolate_driver",
     "lagrbf_set_option", "lagrbf_clear_options", "lagrbf_options_insert_file", "lagrbf_advect_steps",
     "lagrbf_get_info", "lagrbf_debug_get", "lagrbf_last_error",
+    "lagrbf_serve_create", "lagrbf_serve_destroy", "lagrbf_serve_add_dataset", "lagrbf_serve_submit",
+    "lagrbf_serve_step", "lagrbf_serve_poll", "lagrbf_serve_release", "lagrbf_serve_info",
 ]

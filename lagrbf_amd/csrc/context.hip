@@ -14,17 +14,13 @@
 #include <vector>
 
 #include "../../include/lagrbf.h"
-#include "kernels.hpp"
-#include "options.hpp"
+#include "ctx.hpp"
 
 namespace lrb {
 
-// PETSc error codes returned for the same conditions as the reference
-enum { E_MEM = 55, E_SUP = 56, E_SIZ = 60, E_WRONG = 62, E_RANGE = 63, E_STATE = 73, E_LIB = 76, E_NULL = 85 };
-
 static thread_local std::string g_last_error;
 
-static int set_err(int code, const char* fn, const char* fmt, ...) {
+int set_err(int code, const char* fn, const char* fmt, ...) {
   char buf[1024];
   va_list ap;
   va_start(ap, fmt);
@@ -37,128 +33,8 @@ static int set_err(int code, const char* fn, const char* fmt, ...) {
   return code;
 }
 
-#define HIPCHK(call)                                                                   \
-  do {                                                                                 \
-    hipError_t e_ = (call);                                                            \
-    if (e_ != hipSuccess) return set_err(E_LIB, __func__, "%s: %s", #call, hipGetErrorString(e_)); \
-  } while (0)
-
-#define CHK(call)              \
-  do {                         \
-    int r_ = (call);           \
-    if (r_) return r_;         \
-  } while (0)
-
-// Device buffer with grow-only capacity.
-template <typename T>
-struct DBuf {
-  T* p = nullptr;
-  size_t cap = 0;
-  ~DBuf() { release(); }
-  void release() { if (p) (void)hipFree(p); p = nullptr; cap = 0; }
-  hipError_t reserve(size_t n) {
-    if (n <= cap && p) return hipSuccess;
-    release();
-    size_t m = n ? n : 1;
-    hipError_t e = hipMalloc(&p, m * sizeof(T));
-    if (e == hipSuccess) cap = m;
-    return e;
-  }
-  // grow keeping the first `keep` elements
-  hipError_t grow(size_t n, size_t keep, hipStream_t s) {
-    if (n <= cap && p) return hipSuccess;
-    T* q = nullptr;
-    hipError_t e = hipMalloc(&q, (n ? n : 1) * sizeof(T));
-    if (e != hipSuccess) return e;
-    if (p && keep) {
-      e = hipMemcpyAsync(q, p, keep * sizeof(T), hipMemcpyDeviceToDevice, s);
-      if (e != hipSuccess) { (void)hipFree(q); return e; }
-      e = hipStreamSynchronize(s);
-      if (e != hipSuccess) { (void)hipFree(q); return e; }
-    }
-    release();
-    p = q;
-    cap = n ? n : 1;
-    return hipSuccess;
-  }
-};
-
-struct Ctx {
-  Grid g{};
-  int ktype = K_LINEAR, solve_method = 0, nf = 0;
-  double rmin = 0.0, rmax = DBL_MAX, cfl_fac = 0.4;
-  bool advection = false;
-  int scheme = 3, order = 0;
-  double adv_dt = 1e-10;
-  double dopri_fac = 0.9, dopri_fac_min = 1e-2, dopri_fac_max = 1e2, dopri_atol = 1e-3, dopri_rtol = 1e-3;
-  double dopri_err = 0.0;
-  std::string velocity_name = "velocity";
-  hipStream_t s = nullptr;
-  int device = 0;
-
-  DBuf<double> pux, puy, puz;
-  // data swarm
-  int N = 0;
-  bool data_filled = false;
-  DBuf<double> dx, dval;
-  DBuf<int32_t> dcell, cell_start, cell_nodes, scratch, pcount;
-  std::vector<int32_t> h_pcount;
-  DBuf<double> xs_sorted, zeros;           // cell-sorted node coordinates (SoA), zero coefficients
-  DBuf<float> xs32, zeros32, coef32_pool;  // mixed path: cell-relative FP32 coordinates, FP32 coefficients
-  // cell-major node table of the evaluation (coordinates + corner-patch coefficients per node)
-  DBuf<double> blk64;
-  DBuf<float> blk32;
-  bool blk_valid = false, blk_f32 = false;
-  std::vector<int32_t> h_cell_start_prev, h_cell_nodes_prev;   // cell lists the frozen patches were built from
-  bool memb_current = true;                // every set-up patch's members == current cell lists
-  int eval_mode = 0;                       // 0 auto (cell-ordered when possible), 1 cell-ordered, 2 reference order
-  std::vector<int64_t> h_dpid;
-  // particle swarm
-  int Np = 0;
-  int64_t ld = 0;
-  bool points_filled = false;
-  DBuf<double> px, w, fields, vel, x0, vadv[7];
-  DBuf<int32_t> pcell, alive;
-  DBuf<int32_t> perm, pstart, pscratch;    // particles grouped by cell
-  bool sorted_valid = false;
-  // the swarm is stored in cell order at the start of every advection step;
-  // porig[s] = index (fill order) of the particle held in slot s
-  bool reorder = true;
-  DBuf<int32_t> porig, porig_alt, pcell_alt, alive_alt;
-  DBuf<double> px_alt, vel_alt, fields_alt, w_alt;
-  std::vector<int64_t> h_ppid;
-  DBuf<unsigned long long> counters;   // [0] removed, [1] reduction bits, [2] count
-  // patches
-  std::vector<uint8_t> h_setup, h_solved, h_active;
-  std::vector<int32_t> h_n;
-  std::vector<int64_t> h_mem_off, h_fac_off;
-  size_t mem_used = 0, fac_used = 0;
-  int setup_N = -1;
-  DBuf<int32_t> p_n, mem_pool, piv_pool, list, info;
-  DBuf<uint8_t> p_active, touched;
-  DBuf<int64_t> p_mem_off, p_fac_off;
-  DBuf<double> fac_pool, coef_pool;
-  DBuf<float> fac32_pool;
-  size_t fac32_used = 0;
-  std::vector<uint8_t> h_method;           // per patch: M_BK64, M_CHOL64, M_CHOL32
-  DBuf<int32_t> pstatus, piters;
-  int precision = 0;                       // 0 FP64, 1 FP32 factor + FP64 refinement
-  int factor_mode = 0;                     // 0 auto (Cholesky for PD kernels), 1 Bunch-Kaufman always
-  int refine_max_iter = 10;
-  double refine_tol = 1e-12;
-  double fallback = 0, refine_iters_max = 0, refine_iters_sum = 0, refined_patches = 0;
-  // analytical velocity
-  int (*vfn)(double*, double*, void*) = nullptr;
-  int (*vfn_batch)(int32_t, int32_t, double*, double*, void*) = nullptr;
-  void* vuser = nullptr;
-  bool analytical = false;
-  // stats
-  double removed = 0, patches_setup = 0, setup_ms = 0, eval_ms = 0, eval_count = 0;
-  int max_patch = 0;
-};
-
 static const char* kVelocity = "velocity";
-enum { M_BK64 = 0, M_CHOL64 = 1, M_CHOL32 = 2 };
+
 
 // ---------------------------------------------------------------------------
 static int ctx_from_options(Ctx& c, const OptView& o) {
@@ -310,7 +186,7 @@ static int check_bbox(const Ctx& c, int n, const double* x, const char* fn) {
   return 0;
 }
 
-static int fill_data(Ctx& c, int n, const double* coords, const double* values, const int64_t* pidx) {
+int fill_data(Ctx& c, int n, const double* coords, const double* values, const int64_t* pidx) {
   const char* fn = "FillDataSwarm";
   const int dim = c.g.dim, nf = c.nf;
   CHK(check_bbox(c, n, coords, fn));
@@ -520,14 +396,15 @@ static int solve_list(Ctx& c, const std::vector<int32_t>& sl) {
 // LagRBF_PartitionsSetUp (partition.c:417-478): touch the corner patches of every
 // particle's cell, set up (gather + assemble + factor) the new ones, solve the
 // unsolved ones.  Batched: one launch per phase over the patch list.
-static int partitions_setup(Ctx& c) {
+int partitions_setup(Ctx& c, bool all_patches) {
   const char* fn = "LagRBF_PartitionsSetUp";
   const int P = c.g.P;
+  if (c.factors_released) return set_err(E_STATE, fn, "patch factors were released");
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0)); HIPCHK(hipEventCreate(&e1));
   HIPCHK(hipEventRecord(e0, c.s));
-  HIPCHK(hipMemsetAsync(c.touched.p, 0, P, c.s));
-  launch_touch(c.g, c.pcell.p, c.alive.p, c.Np, c.touched.p, c.s);
+  HIPCHK(hipMemsetAsync(c.touched.p, all_patches ? 1 : 0, P, c.s));
+  if (!all_patches) launch_touch(c.g, c.pcell.p, c.alive.p, c.Np, c.touched.p, c.s);
   std::vector<uint8_t> t(P);
   HIPCHK(hipMemcpyAsync(t.data(), c.touched.p, P, hipMemcpyDeviceToHost, c.s));
   HIPCHK(hipStreamSynchronize(c.s));
@@ -604,6 +481,12 @@ static int partitions_setup(Ctx& c) {
   return 0;
 }
 
+void release_factors(Ctx& c) {
+  (void)hipStreamSynchronize(c.s);
+  c.fac_pool.release(); c.fac32_pool.release(); c.piv_pool.release();
+  c.factors_released = true;
+}
+
 // LagRBF_PartitionsReset, partition.c:494-524
 static int partitions_reset(Ctx& c) {
   const int P = c.g.P;
@@ -626,7 +509,7 @@ static int partitions_reset(Ctx& c) {
 
 // RBFInterpolation(ctx, swi, swd, name), interpolation.c:178-272
 // Cell-major node table for the evaluation, rebuilt after the data or any patch changed.
-static int ensure_blk(Ctx& c, bool f32) {
+int ensure_blk(Ctx& c, bool f32) {
   if (c.blk_valid && c.blk_f32 == f32) return 0;
   const size_t n = ((size_t)c.N + 1) * blk_row(c.g.dim, c.nf);   // N rows + padding row
   if (f32) HIPCHK(c.blk32.reserve(n));
@@ -770,11 +653,6 @@ static void stage(Ctx& c, const double* x0, const StageArgs& a) {
                c.w.p, a, c.s);
 }
 
-template <typename T>
-static void swap_buf(DBuf<T>& a, DBuf<T>& b) {
-  std::swap(a.p, b.p);
-  std::swap(a.cap, b.cap);
-}
 
 // Store the swarm in the cell order of the last sort (alive particles grouped by
 // cell, dead ones at the tail): the evaluation's particle reads and writes become
@@ -1025,7 +903,7 @@ static int output(Ctx& c, bool is_data, int n, double* coords, double* fields, i
   return 0;
 }
 
-static Ctx* create_from_view(const OptView& o) {
+Ctx* create_from_view(const OptView& o) {
   Ctx* c = new Ctx();
   if (ctx_from_options(*c, o) || ctx_init_device(*c)) {
     if (c->s) (void)hipStreamDestroy(c->s);
@@ -1035,7 +913,7 @@ static Ctx* create_from_view(const OptView& o) {
   return c;
 }
 
-static OptView current_options() {
+OptView current_options() {
   OptView v;
   Options env;
   if (const char* e = std::getenv("LAGRBF_OPTIONS")) env.insert_string(e);

@@ -1,0 +1,160 @@
+// Device state of a LagRBF context (shared by context.hip and serve.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstdint>
+#include <string>
+#include <utility>
+#include <vector>
+
+#include "kernels.hpp"
+#include "options.hpp"
+
+namespace lrb {
+
+// PETSc error codes returned for the same conditions as the reference
+enum { E_MEM = 55, E_SUP = 56, E_SIZ = 60, E_WRONG = 62, E_RANGE = 63, E_STATE = 73, E_LIB = 76, E_NULL = 85 };
+
+int set_err(int code, const char* fn, const char* fmt, ...);
+
+#define HIPCHK(call)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (call);                                                            \
+    if (e_ != hipSuccess) return set_err(E_LIB, __func__, "%s: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+#define CHK(call)              \
+  do {                         \
+    int r_ = (call);           \
+    if (r_) return r_;         \
+  } while (0)
+
+// Device buffer with grow-only capacity.
+template <typename T>
+struct DBuf {
+  T* p = nullptr;
+  size_t cap = 0;
+  ~DBuf() { release(); }
+  void release() { if (p) (void)hipFree(p); p = nullptr; cap = 0; }
+  hipError_t reserve(size_t n) {
+    if (n <= cap && p) return hipSuccess;
+    release();
+    size_t m = n ? n : 1;
+    hipError_t e = hipMalloc(&p, m * sizeof(T));
+    if (e == hipSuccess) cap = m;
+    return e;
+  }
+  // grow keeping the first `keep` elements
+  hipError_t grow(size_t n, size_t keep, hipStream_t s) {
+    if (n <= cap && p) return hipSuccess;
+    T* q = nullptr;
+    hipError_t e = hipMalloc(&q, (n ? n : 1) * sizeof(T));
+    if (e != hipSuccess) return e;
+    if (p && keep) {
+      e = hipMemcpyAsync(q, p, keep * sizeof(T), hipMemcpyDeviceToDevice, s);
+      if (e != hipSuccess) { (void)hipFree(q); return e; }
+      e = hipStreamSynchronize(s);
+      if (e != hipSuccess) { (void)hipFree(q); return e; }
+    }
+    release();
+    p = q;
+    cap = n ? n : 1;
+    return hipSuccess;
+  }
+};
+
+struct Ctx {
+  Grid g{};
+  int ktype = K_LINEAR, solve_method = 0, nf = 0;
+  double rmin = 0.0, rmax = DBL_MAX, cfl_fac = 0.4;
+  bool advection = false;
+  int scheme = 3, order = 0;
+  double adv_dt = 1e-10;
+  double dopri_fac = 0.9, dopri_fac_min = 1e-2, dopri_fac_max = 1e2, dopri_atol = 1e-3, dopri_rtol = 1e-3;
+  double dopri_err = 0.0;
+  std::string velocity_name = "velocity";
+  hipStream_t s = nullptr;
+  int device = 0;
+
+  DBuf<double> pux, puy, puz;
+  // data swarm
+  int N = 0;
+  bool data_filled = false;
+  DBuf<double> dx, dval;
+  DBuf<int32_t> dcell, cell_start, cell_nodes, scratch, pcount;
+  std::vector<int32_t> h_pcount;
+  DBuf<double> xs_sorted, zeros;           // cell-sorted node coordinates (SoA), zero coefficients
+  DBuf<float> xs32, zeros32, coef32_pool;  // mixed path: cell-relative FP32 coordinates, FP32 coefficients
+  // cell-major node table of the evaluation (coordinates + corner-patch coefficients per node)
+  DBuf<double> blk64;
+  DBuf<float> blk32;
+  bool blk_valid = false, blk_f32 = false;
+  std::vector<int32_t> h_cell_start_prev, h_cell_nodes_prev;   // cell lists the frozen patches were built from
+  bool memb_current = true;                // every set-up patch's members == current cell lists
+  int eval_mode = 0;                       // 0 auto (cell-ordered when possible), 1 cell-ordered, 2 reference order
+  std::vector<int64_t> h_dpid;
+  // particle swarm
+  int Np = 0;
+  int64_t ld = 0;
+  bool points_filled = false;
+  DBuf<double> px, w, fields, vel, x0, vadv[7];
+  DBuf<int32_t> pcell, alive;
+  DBuf<int32_t> perm, pstart, pscratch;    // particles grouped by cell
+  bool sorted_valid = false;
+  // the swarm is stored in cell order at the start of every advection step;
+  // porig[s] = index (fill order) of the particle held in slot s
+  bool reorder = true;
+  DBuf<int32_t> porig, porig_alt, pcell_alt, alive_alt;
+  DBuf<double> px_alt, vel_alt, fields_alt, w_alt;
+  std::vector<int64_t> h_ppid;
+  DBuf<unsigned long long> counters;   // [0] removed, [1] reduction bits, [2] count
+  // patches
+  std::vector<uint8_t> h_setup, h_solved, h_active;
+  std::vector<int32_t> h_n;
+  std::vector<int64_t> h_mem_off, h_fac_off;
+  size_t mem_used = 0, fac_used = 0;
+  int setup_N = -1;
+  DBuf<int32_t> p_n, mem_pool, piv_pool, list, info;
+  DBuf<uint8_t> p_active, touched;
+  DBuf<int64_t> p_mem_off, p_fac_off;
+  DBuf<double> fac_pool, coef_pool;
+  DBuf<float> fac32_pool;
+  size_t fac32_used = 0;
+  std::vector<uint8_t> h_method;           // per patch: M_BK64, M_CHOL64, M_CHOL32
+  DBuf<int32_t> pstatus, piters;
+  int precision = 0;                       // 0 FP64, 1 FP32 factor + FP64 refinement
+  int factor_mode = 0;                     // 0 auto (Cholesky for PD kernels), 1 Bunch-Kaufman always
+  bool factors_released = false;           // serving datasets keep coefficients only
+  int refine_max_iter = 10;
+  double refine_tol = 1e-12;
+  double fallback = 0, refine_iters_max = 0, refine_iters_sum = 0, refined_patches = 0;
+  // analytical velocity
+  int (*vfn)(double*, double*, void*) = nullptr;
+  int (*vfn_batch)(int32_t, int32_t, double*, double*, void*) = nullptr;
+  void* vuser = nullptr;
+  bool analytical = false;
+  // stats
+  double removed = 0, patches_setup = 0, setup_ms = 0, eval_ms = 0, eval_count = 0;
+  int max_patch = 0;
+};
+
+enum { M_BK64 = 0, M_CHOL64 = 1, M_CHOL32 = 2 };
+
+template <typename T>
+inline void swap_buf(DBuf<T>& a, DBuf<T>& b) {
+  std::swap(a.p, b.p);
+  std::swap(a.cap, b.cap);
+}
+
+// context.hip
+Ctx* create_from_view(const OptView& o);
+OptView current_options();
+int fill_data(Ctx& c, int n, const double* coords, const double* values, const int64_t* pidx);
+// all_patches: set up every patch (not only the corners of particle cells)
+int partitions_setup(Ctx& c, bool all_patches = false);
+int ensure_blk(Ctx& c, bool f32);
+// free the factor pools once every patch is solved (coefficients stay)
+void release_factors(Ctx& c);
+
+}  // namespace lrb

@@ -740,11 +740,12 @@ __global__ void k_stage(Grid g, const double* __restrict__ pux, const double* __
   const int dim = g.dim;
   double x[3] = {0.0, 0.0, 0.0};
   for (int d = 0; d < dim; d++) x[d] = px[d * ld + i];
+  const double scale = a.dt_req ? (a.dt_req[a.req[i]] * a.mul) / a.div : a.scale;
   for (int d = 0; d < dim; d++) {
     double v = 0.0;
     for (int j = 0; j < a.nv; j++) v += a.a[j] * a.v[j][d * ld + i];
     if (a.write_vel) vel[d * ld + i] = v;
-    x[d] = x0[d * ld + i] + a.scale * v;
+    x[d] = x0[d * ld + i] + scale * v;
     if (a.proj_in_loop) project_point(x, dim, a.rmin, a.rmax);
   }
   if (a.project && !a.proj_in_loop) project_point(x, dim, a.rmin, a.rmax);
@@ -850,6 +851,74 @@ void launch_dopri_err(const double* px, const double* x0, const double* const* v
   VPtrs vp;
   for (int j = 0; j < 7; j++) vp.v[j] = vadv[j];
   if (Np > 0) hipLaunchKernelGGL(k_dopri_err, dim3((Np + 255) / 256), dim3(256), 0, s, px, x0, vp, ld, Np, dim, alive, dt, atol, rtol, maxb, count);
+}
+
+// Per-request form of k_dopri_err (serving): each particle uses its request's dt and
+// reduces into its request's maximum.
+__global__ __launch_bounds__(256) void k_dopri_err_req(const double* __restrict__ px, const double* __restrict__ x0,
+                                                       VPtrs vp, int64_t ld, int Np, int dim,
+                                                       const int32_t* __restrict__ alive, const int32_t* __restrict__ req,
+                                                       const double* __restrict__ dt_req, double atol, double rtol,
+                                                       unsigned long long* maxb_req) {
+#pragma clang fp contract(off)
+  const double Bs[7] = {5179.0 / 57600.0, 0, 7571.0 / 16695.0, 393.0 / 640.0, -92097.0 / 339200.0, 187.0 / 2100.0, 1.0 / 40.0};
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= Np || !alive[i]) return;
+  const int r = req[i];
+  const double dt = dt_req[r];
+  double E = 0.0;
+  for (int d = 0; d < dim; d++) {
+    double vl = 0.0;
+    for (int j = 0; j < 7; j++) vl += Bs[j] * vp.v[j][d * ld + i];
+    const double a0 = fabs(x0[d * ld + i]), a1 = fabs(px[d * ld + i]);
+    const double mv = a0 < a1 ? a1 : a0;
+    const double tol = atol + rtol * mv;
+    const double xl = x0[d * ld + i] + dt * vl;
+    const double e = px[d * ld + i] - xl;
+    E += (e * e) / (tol * tol);
+  }
+  E = sqrt(1.0 / dim * E);
+  if (E >= 0.0) atomicMax(&maxb_req[r], (unsigned long long)__double_as_longlong(E));
+}
+
+void launch_dopri_err_req(const double* px, const double* x0, const double* const* vadv, int64_t ld, int Np, int dim,
+                          const int32_t* alive, const int32_t* req, const double* dt_req, double atol, double rtol,
+                          unsigned long long* maxb_req, hipStream_t s) {
+  VPtrs vp;
+  for (int j = 0; j < 7; j++) vp.v[j] = vadv[j];
+  if (Np > 0) hipLaunchKernelGGL(k_dopri_err_req, dim3((Np + 255) / 256), dim3(256), 0, s, px, x0, vp, ld, Np, dim, alive,
+                                 req, dt_req, atol, rtol, maxb_req);
+}
+
+// Particles of finished requests: final position into the request's output rows
+// (row = the particle's index within its request), then out of the swarm.
+__global__ void k_retire(const double* __restrict__ px, int64_t ld, int Np, int dim, int32_t* __restrict__ alive,
+                         const int32_t* __restrict__ req, const int32_t* __restrict__ porig,
+                         const uint8_t* __restrict__ done, double* const* __restrict__ out_req) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= Np) return;
+  const int r = req[i];
+  if (!done[r]) return;
+  if (alive[i]) {
+    double* o = out_req[r] + (size_t)porig[i] * dim;
+    for (int d = 0; d < dim; d++) o[d] = px[d * ld + i];
+  }
+  alive[i] = 0;
+}
+
+void launch_retire(const double* px, int64_t ld, int Np, int dim, int32_t* alive, const int32_t* req,
+                   const int32_t* porig, const uint8_t* done, double* const* out_req, hipStream_t s) {
+  if (Np > 0) hipLaunchKernelGGL(k_retire, dim3((Np + 255) / 256), dim3(256), 0, s, px, ld, Np, dim, alive, req, porig,
+                                 done, out_req);
+}
+
+__global__ void k_fill_i32_v(int32_t* a, int n, int32_t v) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
+}
+
+void launch_fill_i32(int32_t* a, int n, int32_t v, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_fill_i32_v, dim3((n + 255) / 256), dim3(256), 0, s, a, n, v);
 }
 
 // --------------------------------------------------------------------------

@@ -16,6 +16,10 @@ struct StageArgs {
   int proj_in_loop;   // 1: projection inside the per-axis loop (RK4 combine)
   int write_vel;      // 1: velocity field receives the stage combination
   double rmin, rmax;
+  // per-request step sizes (serving): scale_i = (dt_req[req[i]] * mul) / div
+  const double* dt_req;
+  const int32_t* req;
+  double mul, div;
 };
 
 // Read-only patch tables used by evaluation.
@@ -118,6 +122,14 @@ void launch_cfl(const double* vel, int64_t ld, int Np, int dim, const int32_t* a
 void launch_dopri_err(const double* px, const double* x0, const double* const* vadv, int64_t ld, int Np,
                       int dim, const int32_t* alive, double dt, double atol, double rtol,
                       unsigned long long* max_bits, unsigned int* count, hipStream_t s);
+
+// ---- serving: per-request reductions and retirement ----
+void launch_dopri_err_req(const double* px, const double* x0, const double* const* vadv, int64_t ld, int Np, int dim,
+                          const int32_t* alive, const int32_t* req, const double* dt_req, double atol, double rtol,
+                          unsigned long long* maxb_req, hipStream_t s);
+void launch_retire(const double* px, int64_t ld, int Np, int dim, int32_t* alive, const int32_t* req,
+                   const int32_t* porig, const uint8_t* done, double* const* out_req, hipStream_t s);
+void launch_fill_i32(int32_t* a, int n, int32_t v, hipStream_t s);
 
 // ---- layout conversion ----
 void launch_aos_to_soa(const double* aos, int n, int bs, double* soa, int64_t ld, hipStream_t s);

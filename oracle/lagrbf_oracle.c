@@ -604,6 +604,7 @@ static int partitions_setup(orc_ctx *ctx)
     if (!ctx->alive[p]) continue;
     for (k = 0; k < ctx->nen; k++) touch[ctx->corner[ctx->pcell[p]][k]] = 1;
   }
+  if (ctx->cfg.setup_all) memset(touch, 1, (size_t)ctx->P);
   for (p = 0; p < ctx->P; p++) if (touch[p]) list[cnt++] = p;
 #pragma omp parallel for schedule(dynamic, 1)
   for (k = 0; k < cnt; k++) {

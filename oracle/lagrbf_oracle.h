@@ -61,6 +61,8 @@ typedef struct {
   double dopri_dt_fac, dopri_dt_fac_min, dopri_dt_fac_max, dopri_atol, dopri_rtol;
   int    weights_brute;   /* 1: O(Np*P) Shepard sum as swarm_interp.c:335-360 */
   int    nthreads;        /* OpenMP threads (0 = runtime default) */
+  int    setup_all;       /* 1: every patch is set up (serving datasets), not only the
+                             corners of particle cells */
 } orc_config;
 
 typedef struct orc_ctx orc_ctx;

@@ -37,7 +37,7 @@ class OrcConfig(ctypes.Structure):
         ("dopri_dt_fac", ctypes.c_double), ("dopri_dt_fac_min", ctypes.c_double),
         ("dopri_dt_fac_max", ctypes.c_double), ("dopri_atol", ctypes.c_double),
         ("dopri_rtol", ctypes.c_double),
-        ("weights_brute", ctypes.c_int), ("nthreads", ctypes.c_int),
+        ("weights_brute", ctypes.c_int), ("nthreads", ctypes.c_int), ("setup_all", ctypes.c_int),
     ]
 
 
@@ -149,6 +149,7 @@ def config_from_options(options):
     c.dopri_atol = float(o.get("dopri_atol", c.dopri_atol))
     c.dopri_rtol = float(o.get("dopri_rtol", c.dopri_rtol))
     c.weights_brute = int(_bool(o.get("oracle_weights_brute", False)))
+    c.setup_all = int(_bool(o.get("oracle_setup_all", False)))
     return c
 
 
