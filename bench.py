@@ -174,6 +174,73 @@ def cpu_baseline(opts, X, V, P, steps, dt, nsample, threads):
             "cutoff_value": rates["cutoff"][0]}
 
 
+def serve_datasets(seed=20261203 + 4):
+    """The serving configuration's four datasets: 2D 65,536 uniform nodes under the
+    solid-body rotation (PU 45^2); 3D jittered lattices with the 16-mode Fourier
+    field of 2^20 nodes (PU 32^3), 2,000,000 nodes (PU 40^3) and 4,000,000 nodes
+    (PU 50^3); Wendland C2, about 34 nodes per PU cell."""
+    from tests import cases
+    rng = np.random.default_rng(seed)
+    X = rng.random((65536, 2))
+    V = np.stack([-(X[:, 1] - 0.5), X[:, 0] - 0.5], axis=1)
+    out = [(dict(pu_mesh_dim=2, pu_mesh_O=[0.0, 0.0], pu_mesh_L=[1.0, 1.0], pu_mesh_n=[45, 45], pu_mesh_overlap=1.0,
+                 pu_mesh_shape_factor=1.0, kernel_type=2), X, V, 1e-2)]
+    for nlat, N, n in ((102, 1 << 20, 32), (128, 2000000, 40), (160, 4000000, 50)):
+        o, X, V, _ = cases.fourier_3d(nlat=nlat, N=N, Np=1, n=n, overlap=1.0)
+        keep = ("pu_mesh_dim", "pu_mesh_O", "pu_mesh_L", "pu_mesh_n", "pu_mesh_overlap", "pu_mesh_shape_factor",
+                "kernel_type")
+        out.append(({k: o[k] for k in keep}, X, V, 1e-3))
+    return out
+
+
+def serve_bench(L, precision, dsets, nreq, rate, seed=20261203 + 5, budget=200e9):
+    """Poisson arrivals at `rate` req/s of `nreq` requests: particle count log-uniform
+    in [1e3, 1e6], dataset uniform, RK4 or DOPRI with equal odds, 10..200 steps.
+    Latency = completion - scheduled arrival; throughput = particle-steps / wall."""
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, nreq))
+    ts = time.perf_counter()
+    srv = L.Server(hbm_budget=budget)
+    for o, X, V, _ in dsets:
+        srv.add_dataset(dict(o, lagrbf_precision=precision), X, V)
+    t_setup = time.perf_counter() - ts
+    reqs = []
+    for k in range(nreq):
+        ds = int(rng.integers(len(dsets)))
+        dim = dsets[ds][1].shape[1]
+        n = int(10 ** rng.uniform(3, 6))
+        scheme = "rk4" if rng.random() < 0.5 else "dopri"
+        nsteps = int(rng.integers(10, 201))
+        P = 0.05 + 0.9 * rng.random((n, dim))
+        reqs.append((ds, scheme, nsteps, dsets[ds][3], P))
+    t0 = srv.info("now")
+    w0 = time.perf_counter()
+    i, pending, lat, work = 0, {}, [], 0.0
+    while i < nreq or pending:
+        now = time.perf_counter() - w0
+        while i < nreq and arrivals[i] <= now:
+            ds, scheme, nsteps, dt, P = reqs[i]
+            pending[srv.submit(ds, scheme, nsteps, dt, P)] = (t0 + arrivals[i], P.shape[0] * nsteps)
+            i += 1
+        if not pending:
+            time.sleep(max(0.0, arrivals[i] - (time.perf_counter() - w0)))
+            continue
+        srv.step()
+        for rid in list(pending):
+            done, _, t = srv.poll(rid, want_result=False)
+            if done:
+                arr, w = pending.pop(rid)
+                lat.append(t[2] - arr)
+                work += w
+                srv.release(rid)
+    wall = time.perf_counter() - w0
+    res = {"precision": precision, "requests": nreq, "rate_hz": rate, "wall_s": wall, "setup_s": t_setup,
+           "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
+           "particle_steps_per_s": work / wall, "ticks": srv.info("ticks"), "max_active": srv.info("max_active")}
+    srv.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -184,6 +251,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-steps", type=int, default=4)
     ap.add_argument("--cpu-particles", type=int, default=65536)
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
+    ap.add_argument("--serve-requests", type=int, default=64)
+    ap.add_argument("--serve-rate", type=float, default=200.0)
+    ap.add_argument("--serve-only", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="launch a plain copy of MIB MiB first (scale for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -196,10 +267,16 @@ def main():
     import lagrbf_amd as L
     if args.copy_calibration > 0:
         copy_calibration(args.copy_calibration)
+    if args.serve_only:
+        dsets = serve_datasets()
+        out = {p: serve_bench(L, p, dsets, args.serve_requests, args.serve_rate) for p in ("mixed", "fp64")}
+        print(json.dumps({"serve": out}), flush=True)
+        return
     dt = 1e-3
     t0 = time.perf_counter()
     opts, X, V, P = build_workload(args.particles)
-    log(f"workload built: {X.shape[0]} nodes, {P.shape[0]} particles ({time.perf_counter() - t0:.1f}s)")
+    n_nodes, n_particles = int(X.shape[0]), int(P.shape[0])
+    log(f"workload built: {n_nodes} nodes, {n_particles} particles ({time.perf_counter() - t0:.1f}s)")
     n_pu = int(np.prod(opts["pu_mesh_n"]))
     sample = np.linspace(0, n_pu - 1, 512).astype(np.int64)
     paths = ["mixed", "fp64"] if args.precision == "both" else [args.precision]
@@ -226,18 +303,32 @@ def main():
         log(f"cpu baseline ({thr} threads) ...")
         cpu = cpu_baseline(opts, X, V, P, args.cpu_steps, dt, args.cpu_particles, thr)
         log(f"cpu baseline: {cpu['value']:.3e} {cpu['unit']} ({cpu['sample']}); cutoff mode {cpu['cutoff_value']:.3e}")
+    serve = None
+    if not args.no_serve:
+        del X, V, P
+        dsets = serve_datasets()
+        serve = {}
+        for p in ("mixed", "fp64"):
+            serve[p] = serve_bench(L, p, dsets, args.serve_requests, args.serve_rate)
+            log(f"serve [{p}]: {args.serve_requests} requests at {args.serve_rate:.0f}/s: p50 {serve[p]['p50_ms']:.1f} ms, "
+                f"p99 {serve[p]['p99_ms']:.1f} ms, {serve[p]['particle_steps_per_s']:.3e} particle-steps/s, "
+                f"wall {serve[p]['wall_s']:.1f} s, set-up {serve[p]['setup_s']:.1f} s")
+        serve["config"] = ("4 device-resident datasets (2D 65,536 nodes; 3D 2^20, 2,000,000, 4,000,000 nodes), "
+                           f"{args.serve_requests} requests, Poisson arrivals at {args.serve_rate:.0f}/s, particles "
+                           "log-uniform in [1e3, 1e6], RK4 or DOPRI, 10-200 steps, HBM budget 200 GB")
     out = {
         "metric": "particle-RK4 steps/s, FP32+refine vs FP64 @1e-10 rel err; serve p99 ms",
         "value": r["value"], "unit": "particle-RK4-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32+f64" if main_path == "mixed" else "f64", "data": "synthetic",
-        "config": {"workload": WORKLOAD, "nodes": int(X.shape[0]), "particles": int(P.shape[0]), "pu_n": 32,
+        "config": {"workload": WORKLOAD, "nodes": n_nodes, "particles": n_particles, "pu_n": 32,
                    "kernel": "wendland_c2", "precision": main_path},
         "roofline": r["roofline"],
         "cpu_baseline": cpu,
         "setup_s": r["setup_s"], "end_to_end": r["end_to_end"], "removed_particles": r["removed_particles"],
         "paths": {p: {k: v for k, v in res[p].items()} for p in paths},
         "mixed_vs_fp64": cmp,
+        "serve": serve,
     }
     if rank == 0:
         print(json.dumps(out), flush=True)

@@ -550,7 +550,7 @@ static int rbf_interpolation(Ctx& c, double* out, int nfields) {
     const int block = std::min(256, std::max(64, (per_cell + 63) / 64 * 64));
     launch_eval_blk(c.g, c.ktype, blk32, c.px.p, c.ld, c.perm.p, c.pstart.p, c.w.p,
                     blk32 ? (const void*)c.blk32.p : (const void*)c.blk64.p, c.nf, c.cell_start.p, nfields, out, block,
-                    c.s);
+                    c.Np, nullptr, nullptr, c.s);
   } else if (cell_order) {
     const int per_cell = (c.Np + c.g.C - 1) / std::max(c.g.C, 1);
     const int block = std::min(256, std::max(64, (per_cell + 63) / 64 * 64));
@@ -617,6 +617,15 @@ static int eval_callback(Ctx& c) {
 static int eval_velocity(Ctx& c) {
   if (c.analytical) return eval_callback(c);
   return rbf_interpolation(c, c.vel.p, c.g.dim);
+}
+
+// Velocity at the stage point into an RK stage array (v_adv_s): interpolated
+// straight into it, or through the velocity field for a callback.
+static int eval_velocity_into(Ctx& c, double* dst) {
+  if (!c.analytical) return rbf_interpolation(c, dst, c.g.dim);
+  CHK(eval_callback(c));
+  HIPCHK(hipMemcpyAsync(dst, c.vel.p, sizeof(double) * c.g.dim * c.ld, hipMemcpyDeviceToDevice, c.s));
+  return 0;
 }
 
 // AdvectionInitializeTimestepVelocity, advection.c:353-373
@@ -690,7 +699,6 @@ static int reorder_particles(Ctx& c) {
 
 static int advect_swarm(Ctx& c, double dt) {
   const int dim = c.g.dim;
-  const size_t bytes = sizeof(double) * dim * c.ld;
   CHK(reorder_particles(c));
   switch (c.scheme) {
     case 0: {  // runge_kutta_1.c:37-75
@@ -716,8 +724,7 @@ static int advect_swarm(Ctx& c, double dt) {
         const double* v[1] = {c.vadv[s].p};
         const double a[1] = {1.0};
         stage(c, c.x0.p, stage_args(1, v, a, cc[s] * dt, 1, 0, 1, c));
-        CHK(eval_velocity(c));
-        HIPCHK(hipMemcpyAsync(c.vadv[s + 1].p, c.vel.p, bytes, hipMemcpyDeviceToDevice, c.s));
+        CHK(eval_velocity_into(c, c.vadv[s + 1].p));
       }
       const double* v[4] = {c.vadv[0].p, c.vadv[1].p, c.vadv[2].p, c.vadv[3].p};
       stage(c, c.x0.p, stage_args(4, v, b, dt / 6.0, 1, 1, 0, c));
@@ -737,8 +744,7 @@ static int advect_swarm(Ctx& c, double dt) {
         const double* v[7];
         for (int j = 0; j <= s; j++) v[j] = c.vadv[j].p;
         stage(c, c.x0.p, stage_args(s + 1, v, A[s], dtd, 1, 0, 1, c));
-        CHK(eval_velocity(c));
-        HIPCHK(hipMemcpyAsync(c.vadv[s + 1].p, c.vel.p, bytes, hipMemcpyDeviceToDevice, c.s));
+        CHK(eval_velocity_into(c, c.vadv[s + 1].p));
       }
       const double* vv[7];
       for (int j = 0; j < 7; j++) vv[j] = c.vadv[j].p;

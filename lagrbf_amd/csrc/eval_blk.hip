@@ -25,6 +25,8 @@
 // rounding), FP32 coefficients, FP64 Shepard combination.
 #include "kernels.hpp"
 
+#include <algorithm>
+
 namespace lrb {
 
 // Q1 corner index of offset (a, b, c) in {0,1}^3 (DMDAGetElements order)
@@ -124,17 +126,30 @@ void k_eval_blk(Grid g, const double* __restrict__ px, int64_t ld,
                                                   const int32_t* __restrict__ perm, const int32_t* __restrict__ pstart,
                                                   const double* __restrict__ w, const T* __restrict__ blk,
                                                   const int32_t* __restrict__ cs, int nfields,
-                                                  double* __restrict__ out) {
+                                                  double* __restrict__ out, const int32_t* __restrict__ cell_list,
+                                                  const int32_t* __restrict__ list_count) {
   constexpr int NEN = 1 << DIM;
   constexpr int NNB = DIM == 3 ? 27 : 9;
   constexpr int BS = blk_stride<DIM, NFT>();
-  // XCD-aware order: block b runs on XCD b % 8, which gets a contiguous range of cells
-  const int per = (gridDim.x + 7) / 8;
+  // dense swarm: one block per cell, XCD-aware order (block b runs on XCD b % 8,
+  // which gets a contiguous range of cells); sparse swarm: one block per listed
+  // non-empty cell
   const int b = blockIdx.x;
-  const int c = (b % 8) * per + b / 8;
-  if (c >= g.C) return;
-  const int pb = pstart[c], pe = pstart[c + 1];
+  int c;
+  if (cell_list) {
+    if (b >= *list_count) return;
+    c = cell_list[b];
+  } else {
+    const int per = (gridDim.x + 7) / 8;
+    c = (b % 8) * per + b / 8;
+    if (c >= g.C) return;
+  }
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  const T eps = (T)g.eps_r;
+  const double rc = 1.0 / g.eps_r;
+  const T rcut2 = (T)(rc * rc * (sizeof(T) == 4 ? (1.0 + 1e-5) : (1.0 + 1e-12)));
+  {
+  const int pb = pstart[c], pe = pstart[c + 1];
   if (pb + wv * 64 >= pe) return;
   const int ci = c % g.nc[0], cj = (c / g.nc[0]) % g.nc[1], ck = DIM == 3 ? c / (g.nc[0] * g.nc[1]) : 0;
   // an empty 3^d neighbourhood leaves the particles' values untouched (interpolation.c:89)
@@ -146,9 +161,6 @@ void k_eval_blk(Grid g, const double* __restrict__ px, int64_t ld,
     nnb += cs[q + 1] - cs[q];
   }
   if (nnb == 0) return;
-  const T eps = (T)g.eps_r;
-  const double rc = 1.0 / g.eps_r;
-  const T rcut2 = (T)(rc * rc * (sizeof(T) == 4 ? (1.0 + 1e-5) : (1.0 + 1e-12)));
   // FP32: coordinates relative to the particle cell's origin
   const double ox = sizeof(T) == 4 ? g.O[0] + ci * g.h[0] : 0.0;
   const double oy = sizeof(T) == 4 ? g.O[1] + cj * g.h[1] : 0.0;
@@ -209,6 +221,7 @@ void k_eval_blk(Grid g, const double* __restrict__ px, int64_t ld,
 #pragma unroll
     for (int f = 0; f < NFT; f++) if (f < nfields) out[(size_t)f * ld + i] = val[f];
   }
+  }
 }
 
 // Table row s: node cn[s] of cell q = dcell[cn[s]], r = s - cs[q] its rank in q;
@@ -260,6 +273,172 @@ __global__ void k_build_blk(Grid g, const double* __restrict__ x, int N, const i
   }
 }
 
+// ---------------------------------------------------------------------------
+// Sparse swarms (a few particles per cell): a wave serves 64/G cells at once, one
+// group of G lanes per listed non-empty cell, each lane one particle of its cell
+// (cells with more than G particles are served in chunks of G).  Groups walk the
+// same neighbour offsets in the same order, so the corner mapping stays a
+// compile-time property; each group reads its own cell's rows (vector loads,
+// shared addresses inside a group).  Per particle the arithmetic and its order
+// are those of k_eval_blk.
+template <int DIM, int KT, int NFT, typename T, int NBI>
+__device__ __forceinline__ void accum_sparse(T xp, T yp, T zp, const T* __restrict__ B, int m, int mmax,
+                                             T (&acc)[1 << DIM][NFT], T eps, T rcut2) {
+  constexpr int NEN = 1 << DIM, BS = blk_stride<DIM, NFT>();
+  for (int r = 0; r < mmax; r++) {
+    const bool act = r < m;
+    T r2 = rcut2 * T(4);
+    const T* Br = B + (size_t)r * BS;
+    if (act) {
+      const T d0 = xp - Br[0], d1 = yp - Br[1], d2 = DIM == 3 ? zp - Br[2] : T(0);
+      r2 = d0 * d0 + d1 * d1 + d2 * d2;
+    }
+    if constexpr (compact_support_blk<KT>()) {
+      if (!__any(r2 < rcut2)) continue;
+    }
+    if (act) {
+      const T phi = rbf<KT, T>(sqrt_fast(r2), eps);
+#pragma unroll
+      for (int k = 0; k < NEN; k++) {
+        const int kp = corner_in_nbr<DIM>(NBI, k);
+        if (kp >= 0) {
+#pragma unroll
+          for (int f = 0; f < NFT; f++) acc[k][f] += phi * Br[4 + kp * NFT + f];
+        }
+      }
+    }
+  }
+}
+
+__device__ __forceinline__ int wave_max_int(int v) {
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+  return v;
+}
+
+template <int DIM, int KT, int NFT, typename T, int G>
+__global__ __launch_bounds__(256) void k_eval_sparse(Grid g, const double* __restrict__ px, int64_t ld,
+                                                     const int32_t* __restrict__ perm, const int32_t* __restrict__ pstart,
+                                                     const double* __restrict__ w, const T* __restrict__ blk,
+                                                     const int32_t* __restrict__ cs, int nfields, double* __restrict__ out,
+                                                     const int32_t* __restrict__ cell_list,
+                                                     const int32_t* __restrict__ list_count) {
+  constexpr int NG = 64 / G, NEN = 1 << DIM, NNB = DIM == 3 ? 27 : 9, BS = blk_stride<DIM, NFT>();
+  const int lane = threadIdx.x & 63, wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int gi = lane / G, mlane = lane % G;
+  const int cnt = *list_count;
+  if (wave * NG >= cnt) return;
+  const int item = wave * NG + gi;
+  const bool gvalid = item < cnt;
+  const int c = cell_list[gvalid ? item : wave * NG];
+  const int ci = c % g.nc[0], cj = (c / g.nc[0]) % g.nc[1], ck = DIM == 3 ? c / (g.nc[0] * g.nc[1]) : 0;
+  int nnb = 0;
+  for (int nbi = 0; nbi < NNB; nbi++) {
+    const int a = ci + nbi % 3 - 1, bb = cj + (nbi / 3) % 3 - 1, e = ck + (DIM == 3 ? nbi / 9 - 1 : 0);
+    if (a < 0 || a >= g.nc[0] || bb < 0 || bb >= g.nc[1] || (DIM == 3 && (e < 0 || e >= g.nc[2]))) continue;
+    const int q = a + g.nc[0] * (bb + g.nc[1] * e);
+    nnb += cs[q + 1] - cs[q];
+  }
+  // an empty 3^d neighbourhood leaves the particles' values untouched (interpolation.c:89)
+  const int pb = pstart[c], pe = (gvalid && nnb > 0) ? pstart[c + 1] : pb;
+  const int nchunk = wave_max_int((pe - pb + G - 1) / G);
+  const T eps = (T)g.eps_r;
+  const double rc = 1.0 / g.eps_r;
+  const T rcut2 = (T)(rc * rc * (sizeof(T) == 4 ? (1.0 + 1e-5) : (1.0 + 1e-12)));
+  const double ox = sizeof(T) == 4 ? g.O[0] + ci * g.h[0] : 0.0;
+  const double oy = sizeof(T) == 4 ? g.O[1] + cj * g.h[1] : 0.0;
+  const double oz = sizeof(T) == 4 && DIM == 3 ? g.O[2] + ck * g.h[2] : 0.0;
+  for (int k = 0; k < nchunk; k++) {
+    const int s = pb + k * G + mlane;
+    const bool live = s < pe;
+    const int i = perm[live ? s : 0];
+    T xp, yp, zp;
+    if constexpr (sizeof(T) == 4) {
+      xp = (T)(px[i] - ox); yp = (T)(px[ld + i] - oy); zp = DIM == 3 ? (T)(px[2 * ld + i] - oz) : T(0);
+    } else {
+      xp = px[i]; yp = px[ld + i]; zp = DIM == 3 ? px[2 * ld + i] : T(0);
+    }
+    T acc[NEN][NFT];
+#pragma unroll
+    for (int kk = 0; kk < NEN; kk++)
+#pragma unroll
+      for (int f = 0; f < NFT; f++) acc[kk][f] = T(0);
+#pragma unroll 1
+    for (int nbi = 0; nbi < NNB; nbi++) {
+      const int di = nbi % 3 - 1, dj = (nbi / 3) % 3 - 1, dk = DIM == 3 ? nbi / 9 - 1 : 0;
+      const int a = ci + di, bb = cj + dj, e = ck + dk;
+      int nb = 0, ne = 0;
+      if (live && !(a < 0 || a >= g.nc[0] || bb < 0 || bb >= g.nc[1] || (DIM == 3 && (e < 0 || e >= g.nc[2])))) {
+        const int q = a + g.nc[0] * (bb + g.nc[1] * e);
+        nb = cs[q]; ne = cs[q + 1];
+      }
+      const int mq = ne - nb;
+      const int mmax = wave_max_int(mq);
+      if (mmax == 0) continue;
+      T qx = xp, qy = yp, qz = zp;
+      if constexpr (sizeof(T) == 4) {
+        qx = xp - (T)(di * g.h[0]); qy = yp - (T)(dj * g.h[1]);
+        if constexpr (DIM == 3) qz = zp - (T)(dk * g.h[2]);
+      }
+      const T* B = blk + (size_t)nb * BS;
+      switch (nbi) {
+#define LRB_CASE(V) case V: accum_sparse<DIM, KT, NFT, T, V>(qx, qy, qz, B, mq, mmax, acc, eps, rcut2); break;
+        LRB_CASE(0) LRB_CASE(1) LRB_CASE(2) LRB_CASE(3) LRB_CASE(4) LRB_CASE(5) LRB_CASE(6) LRB_CASE(7) LRB_CASE(8)
+        LRB_CASE(9) LRB_CASE(10) LRB_CASE(11) LRB_CASE(12) LRB_CASE(13) LRB_CASE(14) LRB_CASE(15) LRB_CASE(16)
+        LRB_CASE(17) LRB_CASE(18) LRB_CASE(19) LRB_CASE(20) LRB_CASE(21) LRB_CASE(22) LRB_CASE(23) LRB_CASE(24)
+        LRB_CASE(25) LRB_CASE(26)
+#undef LRB_CASE
+        default: break;
+      }
+    }
+    if (!live) continue;
+    double val[NFT];
+#pragma unroll
+    for (int f = 0; f < NFT; f++) val[f] = 0.0;
+#pragma unroll
+    for (int kk = 0; kk < NEN; kk++) {
+      const double wk = w[(size_t)kk * ld + i];
+#pragma unroll
+      for (int f = 0; f < NFT; f++) val[f] += (double)acc[kk][f] * wk;
+    }
+#pragma unroll
+    for (int f = 0; f < NFT; f++) if (f < nfields) out[(size_t)f * ld + i] = val[f];
+  }
+}
+
+template <int DIM, int KT, int NFT, typename T>
+static void launch_sparse_g(int G, const Grid& g, const double* px, int64_t ld, const int32_t* perm, const int32_t* pstart,
+                            const double* w, const T* blk, const int32_t* cs, int nfields, double* out,
+                            const int32_t* cell_list, const int32_t* list_count, int np, hipStream_t s) {
+  // waves: one per 64/G listed cells (at most min(C, np) cells are non-empty)
+  const int ncell = std::min(g.C, np);
+  const int waves = (ncell + 64 / G - 1) / (64 / G);
+  const dim3 grid((waves + 3) / 4), blkd(256);
+#define LRB_SP(GG) hipLaunchKernelGGL((k_eval_sparse<DIM, KT, NFT, T, GG>), grid, blkd, 0, s, g, px, ld, perm, pstart, w, blk, cs, nfields, out, cell_list, list_count)
+  if (G == 8) LRB_SP(8);
+  else if (G == 16) LRB_SP(16);
+  else LRB_SP(32);
+#undef LRB_SP
+}
+
+// Non-empty particle cells of a grouped swarm, compacted in any order (one atomic
+// per wave); list_count must be zero on entry.
+__global__ void k_nonempty_cells(const int32_t* __restrict__ pstart, int C, int32_t* __restrict__ list,
+                                 int32_t* __restrict__ count) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool ne = c < C && pstart[c + 1] > pstart[c];
+  const unsigned long long m = __ballot(ne);
+  const int lane = threadIdx.x & 63;
+  int base = 0;
+  if (lane == 0 && m) base = atomicAdd(count, (int)__popcll(m));
+  base = __shfl(base, 0);
+  if (ne) list[base + (int)__popcll(m & ((1ull << lane) - 1ull))] = c;
+}
+
+void launch_nonempty_cells(const int32_t* pstart, int C, int32_t* list, int32_t* count, hipStream_t s) {
+  (void)hipMemsetAsync(count, 0, sizeof(int32_t), s);
+  if (C > 0) hipLaunchKernelGGL(k_nonempty_cells, dim3((C + 255) / 256), dim3(256), 0, s, pstart, C, list, count);
+}
+
 bool eval_blk_supported(int nft) { return nft >= 1 && nft <= 4; }
 size_t blk_row(int dim, int nft) { return 4 + ((size_t)1 << dim) * (size_t)nft; }
 
@@ -279,10 +458,19 @@ void launch_build_blk(const Grid& g, bool fp32, const double* x, int N, const in
 template <int DIM, int KT, typename T>
 static void eval_blk_k(const Grid& g, const double* px, int64_t ld, const int32_t* perm, const int32_t* pstart,
                        const double* w, const T* blk, int nft, const int32_t* cs, int nfields, double* out, int block,
-                       hipStream_t s) {
-  const int per = (g.C + 7) / 8;
-  const dim3 grid(per * 8), blkd(block);
-#define LRB_EB(NFT) hipLaunchKernelGGL((k_eval_blk<DIM, KT, NFT, T>), grid, blkd, 0, s, g, px, ld, perm, pstart, w, blk, cs, nfields, out)
+                       int np, const int32_t* cell_list, const int32_t* list_count, hipStream_t s) {
+  if (cell_list && nft == DIM) {
+    // sparse velocity evaluation: lane groups of G >= the mean particles per
+    // non-empty cell
+    const double mean = (double)np / std::max(1, std::min(g.C, np));
+    const int G = mean <= 6.0 ? 8 : (mean <= 12.0 ? 16 : 32);
+    launch_sparse_g<DIM, KT, DIM, T>(G, g, px, ld, perm, pstart, w, blk, cs, nfields, out, cell_list, list_count, np, s);
+    return;
+  }
+  // dense: one block per cell; listed cells: one block per listed non-empty cell
+  const int nblk = cell_list ? std::min(g.C, np) : (g.C + 7) / 8 * 8;
+  const dim3 grid(nblk), blkd(block);
+#define LRB_EB(NFT) hipLaunchKernelGGL((k_eval_blk<DIM, KT, NFT, T>), grid, blkd, 0, s, g, px, ld, perm, pstart, w, blk, cs, nfields, out, cell_list, list_count)
   switch (nft) {
     case 1: LRB_EB(1); break;
     case 2: LRB_EB(2); break;
@@ -295,8 +483,9 @@ static void eval_blk_k(const Grid& g, const double* px, int64_t ld, const int32_
 template <int DIM, typename T>
 static void eval_blk_dim(const Grid& g, int ktype, const double* px, int64_t ld, const int32_t* perm,
                          const int32_t* pstart, const double* w, const T* blk, int nft, const int32_t* cs, int nfields,
-                         double* out, int block, hipStream_t s) {
-#define LRB_K(K) eval_blk_k<DIM, K, T>(g, px, ld, perm, pstart, w, blk, nft, cs, nfields, out, block, s)
+                         double* out, int block, int np, const int32_t* cell_list, const int32_t* list_count,
+                         hipStream_t s) {
+#define LRB_K(K) eval_blk_k<DIM, K, T>(g, px, ld, perm, pstart, w, blk, nft, cs, nfields, out, block, np, cell_list, list_count, s)
   if constexpr (sizeof(T) == 8) {
     switch (ktype) {
       case K_LINEAR: LRB_K(K_LINEAR); break;
@@ -319,16 +508,21 @@ static void eval_blk_dim(const Grid& g, int ktype, const double* px, int64_t ld,
 }
 
 void launch_eval_blk(const Grid& g, int ktype, bool fp32, const double* px, int64_t ld, const int32_t* perm,
-                     const int32_t* pstart, const double* w, const void* blk, int nft, const int32_t* cs, int nfields,
-                     double* out, int block, hipStream_t s) {
-  if (nfields <= 0 || g.C <= 0) return;
+                     const int32_t* pstart, const double* w, const void* blk_, int nft, const int32_t* cs, int nfields,
+                     double* out, int block, int np, const int32_t* cell_list, const int32_t* list_count,
+                     hipStream_t s) {
+  if (nfields <= 0 || g.C <= 0 || np <= 0) return;
+#define LRB_ARGS px, ld, perm, pstart, w, blk, nft, cs, nfields, out, block, np, cell_list, list_count, s
   if (fp32) {
-    if (g.dim == 3) eval_blk_dim<3, float>(g, ktype, px, ld, perm, pstart, w, (const float*)blk, nft, cs, nfields, out, block, s);
-    else eval_blk_dim<2, float>(g, ktype, px, ld, perm, pstart, w, (const float*)blk, nft, cs, nfields, out, block, s);
+    const float* blk = (const float*)blk_;
+    if (g.dim == 3) eval_blk_dim<3, float>(g, ktype, LRB_ARGS);
+    else eval_blk_dim<2, float>(g, ktype, LRB_ARGS);
   } else {
-    if (g.dim == 3) eval_blk_dim<3, double>(g, ktype, px, ld, perm, pstart, w, (const double*)blk, nft, cs, nfields, out, block, s);
-    else eval_blk_dim<2, double>(g, ktype, px, ld, perm, pstart, w, (const double*)blk, nft, cs, nfields, out, block, s);
+    const double* blk = (const double*)blk_;
+    if (g.dim == 3) eval_blk_dim<3, double>(g, ktype, LRB_ARGS);
+    else eval_blk_dim<2, double>(g, ktype, LRB_ARGS);
   }
+#undef LRB_ARGS
 }
 
 }  // namespace lrb

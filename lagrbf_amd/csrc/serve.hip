@@ -54,9 +54,9 @@ struct Batch {
   hipStream_t s = nullptr;
   int Np = 0;
   int64_t ld = 0;
-  DBuf<double> px, x0, vel, fields, w, vadv[7], stage;
-  DBuf<int32_t> cell, alive, req, porig, perm, pstart, pscratch;
-  DBuf<double> px_b, vel_b, fields_b, w_b;
+  DBuf<double> px, x0, vel, w, vadv[7], stage;
+  DBuf<int32_t> cell, alive, req, porig, perm, pstart, pscratch, cell_list, list_count;
+  DBuf<double> px_b, vel_b, w_b;
   DBuf<int32_t> cell_b, alive_b, req_b, porig_b;
   DBuf<unsigned long long> counters, err_req;
   DBuf<double> dt_req;
@@ -91,7 +91,7 @@ struct Server {
 // reorder buffers, v_adv stages of the scheme) plus its output rows.
 size_t request_bytes(int dim, int nen, int scheme, int n) {
   const int stages = scheme == 3 ? 7 : (scheme == 2 ? 4 : (scheme == 1 ? 1 : 0));
-  const size_t dbl = (size_t)dim * (2 + 2 + 2 + 1 + stages) + (size_t)2 * nen;   // px, vel, fields (x2), x0, v_adv, w (x2)
+  const size_t dbl = (size_t)dim * (2 + 2 + 1 + stages) + (size_t)2 * nen;   // px, vel (x2), x0, v_adv, w (x2)
   const size_t ints = 9;                                                         // cell, alive, req, porig (x2), perm
   return (size_t)n * (8 * dbl + 4 * ints + 8 * dim);
 }
@@ -115,12 +115,11 @@ int grow_rows(DBuf<T>& b, int rows, int64_t old_ld, int64_t new_ld, int n, hipSt
 int batch_reserve(Batch& b, int need) {
   if (need <= b.ld) return 0;
   const int64_t nl = std::max<int64_t>({(int64_t)need, 2 * b.ld, 4096}) / 64 * 64 + 64;
-  const int dim = b.ds->g.dim, nen = b.ds->g.nen, nf = dim;
+  const int dim = b.ds->g.dim, nen = b.ds->g.nen;
   const int64_t ol = b.ld;
   const int n = b.Np;
   CHK(grow_rows(b.px, dim, ol, nl, n, b.s));
   CHK(grow_rows(b.vel, dim, ol, nl, n, b.s));
-  CHK(grow_rows(b.fields, nf, ol, nl, n, b.s));
   CHK(grow_rows(b.w, nen, ol, nl, n, b.s));
   CHK(grow_rows(b.cell, 1, ol, nl, n, b.s));
   CHK(grow_rows(b.alive, 1, ol, nl, n, b.s));
@@ -130,7 +129,7 @@ int batch_reserve(Batch& b, int need) {
   HIPCHK(b.x0.reserve((size_t)nl * dim));
   for (int j = 0; j < b.order(); j++) HIPCHK(b.vadv[j].reserve((size_t)nl * dim));
   HIPCHK(b.px_b.reserve((size_t)nl * dim)); HIPCHK(b.vel_b.reserve((size_t)nl * dim));
-  HIPCHK(b.fields_b.reserve((size_t)nl * nf)); HIPCHK(b.w_b.reserve((size_t)nl * nen));
+  HIPCHK(b.w_b.reserve((size_t)nl * nen));
   HIPCHK(b.cell_b.reserve(nl)); HIPCHK(b.alive_b.reserve(nl)); HIPCHK(b.req_b.reserve(nl)); HIPCHK(b.porig_b.reserve(nl));
   HIPCHK(b.perm.reserve(nl));
   b.ld = nl;
@@ -146,7 +145,8 @@ Batch* get_batch(Server& sv, int ds, int scheme) {
   b->scheme = scheme;
   if (hipSetDevice(b->ds->device) != hipSuccess || hipStreamCreate(&b->s) != hipSuccess) return nullptr;
   const int C = b->ds->g.C;
-  if (b->pstart.reserve((size_t)C + 2) != hipSuccess ||
+  if (b->pstart.reserve((size_t)C + 2) != hipSuccess || b->cell_list.reserve((size_t)C) != hipSuccess ||
+      b->list_count.reserve(1) != hipSuccess ||
       b->pscratch.reserve((size_t)C + 2 + (size_t)(C + 2) / 1024 + 64) != hipSuccess ||
       b->counters.reserve(4) != hipSuccess || b->err_req.reserve(RSLOTS) != hipSuccess ||
       b->dt_req.reserve(RSLOTS) != hipSuccess || b->done_req.reserve(RSLOTS) != hipSuccess ||
@@ -172,7 +172,8 @@ int admit(Server& sv, Batch& b, ServeReq& r) {
   const int dim = d.g.dim, off = b.Np, n = r.n;
   CHK(batch_reserve(b, off + n));
   HIPCHK(b.stage.reserve((size_t)n * dim));
-  HIPCHK(hipMalloc(&r.d_out, sizeof(double) * (size_t)n * dim));
+  // stream-ordered allocation: no device-wide synchronisation when it is freed
+  HIPCHK(hipMallocAsync((void**)&r.d_out, sizeof(double) * (size_t)n * dim, b.s));
   HIPCHK(hipMemsetAsync(r.d_out, 0, sizeof(double) * (size_t)n * dim, b.s));
   HIPCHK(hipMemcpyAsync(b.stage.p, r.coords.data(), sizeof(double) * (size_t)n * dim, hipMemcpyHostToDevice, b.s));
   launch_aos_to_soa(b.stage.p, n, dim, b.px.p + off, b.ld, b.s);
@@ -193,17 +194,24 @@ int admit(Server& sv, Batch& b, ServeReq& r) {
   return 0;
 }
 
+// a batch with fewer than 8 particles per cell on average is evaluated over its
+// non-empty cells only
+bool sparse(const Batch& b) { return (int64_t)b.Np < 8 * (int64_t)b.ds->g.C; }
+
 void eval_batch(Batch& b, double* out) {
   Ctx& d = *b.ds;
   const bool f32 = d.precision && eval_f32_supported(d.ktype);
   const int per_cell = (b.Np + d.g.C - 1) / std::max(d.g.C, 1);
   const int block = std::min(256, std::max(64, (per_cell + 63) / 64 * 64));
+  const bool sp = sparse(b);
   launch_eval_blk(d.g, d.ktype, f32, b.px.p, b.ld, b.perm.p, b.pstart.p, b.w.p,
-                  f32 ? (const void*)d.blk32.p : (const void*)d.blk64.p, d.nf, d.cell_start.p, d.g.dim, out, block, b.s);
+                  f32 ? (const void*)d.blk32.p : (const void*)d.blk64.p, d.nf, d.cell_start.p, d.g.dim, out, block,
+                  b.Np, sp ? b.cell_list.p : nullptr, sp ? b.list_count.p : nullptr, b.s);
 }
 
 void group(Batch& b) {
   launch_sort_particles(b.cell.p, b.alive.p, b.Np, b.ds->g.C, b.pstart.p, b.perm.p, b.pscratch.p, b.s);
+  if (sparse(b)) launch_nonempty_cells(b.pstart.p, b.ds->g.C, b.cell_list.p, b.list_count.p, b.s);
 }
 
 StageArgs serve_stage_args(const Batch& b, int nv, const double* const* v, const double* a, double mul, double div,
@@ -228,35 +236,32 @@ void stage(Batch& b, const StageArgs& a) {
 int batch_tick(Batch& b) {
   Ctx& d = *b.ds;
   const int dim = d.g.dim, C = d.g.C;
-  const size_t bytes = sizeof(double) * dim * b.ld;
   // cell order (dead particles at the tail, counted for the cut after the tick)
   group(b);
   ReorderSet r{};
   r.src[0] = b.px.p; r.dst[0] = b.px_b.p; r.rows[0] = dim;
   r.src[1] = b.vel.p; r.dst[1] = b.vel_b.p; r.rows[1] = dim;
-  r.src[2] = b.fields.p; r.dst[2] = b.fields_b.p; r.rows[2] = dim;
-  r.src[3] = b.w.p; r.dst[3] = b.w_b.p; r.rows[3] = d.g.nen;
-  r.nd = 4;
+  r.src[2] = b.w.p; r.dst[2] = b.w_b.p; r.rows[2] = d.g.nen;
+  r.nd = 3;
   r.isrc[0] = b.cell.p; r.idst[0] = b.cell_b.p;
   r.isrc[1] = b.alive.p; r.idst[1] = b.alive_b.p;
   r.isrc[2] = b.req.p; r.idst[2] = b.req_b.p;
   r.isrc[3] = b.porig.p; r.idst[3] = b.porig_b.p;
   r.ni = 4;
   launch_reorder(r, b.perm.p, b.Np, b.ld, b.s);
-  swap_buf(b.px, b.px_b); swap_buf(b.vel, b.vel_b); swap_buf(b.fields, b.fields_b); swap_buf(b.w, b.w_b);
+  swap_buf(b.px, b.px_b); swap_buf(b.vel, b.vel_b); swap_buf(b.w, b.w_b);
   swap_buf(b.cell, b.cell_b); swap_buf(b.alive, b.alive_b); swap_buf(b.req, b.req_b); swap_buf(b.porig, b.porig_b);
   launch_iota(b.perm.p, b.Np, b.s);
   HIPCHK(hipMemcpyAsync(b.h_count, b.pstart.p + C, sizeof(int32_t), hipMemcpyDeviceToHost, b.s));
   if (b.appended) {
     // velocity at the start for the admitted particles (the others recompute the
     // same values: a particle's velocity depends on its position only)
-    eval_batch(b, b.fields.p);
-    launch_copy_rows(b.fields.p, b.ld, b.vel.p, b.ld, dim, b.Np, b.alive.p, b.s);
+    eval_batch(b, b.vel.p);
     b.appended = false;
   }
   switch (b.scheme) {
     case 0: {   // runge_kutta_1.c:37-75 (x0 = the current position)
-      HIPCHK(hipMemcpyAsync(b.x0.p, b.px.p, bytes, hipMemcpyDeviceToDevice, b.s));
+      launch_save(b.px.p, b.x0.p, b.vel.p, nullptr, b.ld, b.Np, dim, b.alive.p, b.s);
       const double* v[1] = {b.vel.p};
       const double a[1] = {1.0};
       stage(b, serve_stage_args(b, 1, v, a, 1.0, 1.0, 0, 0, 0));
@@ -281,8 +286,7 @@ int batch_tick(Batch& b) {
         const double a[1] = {1.0};
         stage(b, serve_stage_args(b, 1, v, a, cc[s], 1.0, 1, 0, 1));
         group(b);
-        eval_batch(b, b.vel.p);
-        HIPCHK(hipMemcpyAsync(b.vadv[s + 1].p, b.vel.p, bytes, hipMemcpyDeviceToDevice, b.s));
+        eval_batch(b, b.vadv[s + 1].p);
       }
       const double* v[4] = {b.vadv[0].p, b.vadv[1].p, b.vadv[2].p, b.vadv[3].p};
       stage(b, serve_stage_args(b, 4, v, bw, 1.0, 6.0, 1, 1, 0));
@@ -302,8 +306,7 @@ int batch_tick(Batch& b) {
         for (int j = 0; j <= s; j++) v[j] = b.vadv[j].p;
         stage(b, serve_stage_args(b, s + 1, v, A[s], 1.0, 1.0, 1, 0, 1));
         group(b);
-        eval_batch(b, b.vel.p);
-        HIPCHK(hipMemcpyAsync(b.vadv[s + 1].p, b.vel.p, bytes, hipMemcpyDeviceToDevice, b.s));
+        eval_batch(b, b.vadv[s + 1].p);
       }
       const double* vv[7];
       for (int j = 0; j < 7; j++) vv[j] = b.vadv[j].p;
@@ -316,8 +319,7 @@ int batch_tick(Batch& b) {
   }
   // velocity at the end of the step (the next step's start)
   group(b);
-  eval_batch(b, b.fields.p);
-  launch_copy_rows(b.fields.p, b.ld, b.vel.p, b.ld, dim, b.Np, b.alive.p, b.s);
+  eval_batch(b, b.vel.p);
   return 0;
 }
 
@@ -367,7 +369,7 @@ int batch_finish(Server& sv, Batch& b) {
     for (int k = 0; k < RSLOTS; k++) {
       if (!done[k]) continue;
       ServeReq* r = b.slots[k];
-      (void)hipFree(r->d_out);
+      (void)hipFreeAsync(r->d_out, b.s);
       r->d_out = nullptr;
       r->done = true;
       r->t_done = t;
@@ -433,6 +435,7 @@ void* lagrbf_serve_create(double hbm_budget_bytes) {
 void lagrbf_serve_destroy(void* srv) {
   auto* sv = static_cast<Server*>(srv);
   if (!sv) return;
+  (void)hipDeviceSynchronize();
   for (auto& kv : sv->reqs)
     if (kv.second->d_out) (void)hipFree(kv.second->d_out);
   delete sv;
