@@ -129,6 +129,9 @@ static int ctx_from_options(Ctx& c, const OptView& o) {
     else if (ev == "ref") c.eval_mode = 2;
     else return set_err(E_WRONG, fn, "-lagrbf_eval must be auto, cell or ref (found %s)", ev.c_str());
   }
+  c.chol_big_min = chol_max_n();
+  o.get_int("lagrbf_chol_big_min", c.chol_big_min, bad);
+  c.chol_big_min = std::max(0, std::min(c.chol_big_min, chol_max_n()));
   o.get_bool("lagrbf_reorder", c.reorder, bad);
   if (bad) return set_err(E_WRONG, fn, "malformed -lagrbf_reorder option");
   return 0;
@@ -160,7 +163,7 @@ static int ctx_init_device(Ctx& c) {
   HIPCHK(c.pcount.reserve(P));
   c.h_method.assign(P, 0);
   HIPCHK(c.pstatus.reserve(P)); HIPCHK(c.piters.reserve(P));
-  HIPCHK(c.list.reserve((size_t)3 * P));
+  HIPCHK(c.list.reserve((size_t)N_METHODS * P));
   HIPCHK(hipMemset(c.p_n.p, 0, sizeof(int32_t) * P));
   HIPCHK(hipMemset(c.p_active.p, 0, P));
   HIPCHK(hipMemset(c.p_mem_off.p, 0, sizeof(int64_t) * P));
@@ -305,10 +308,10 @@ static int fill_points(Ctx& c, int n, const double* coords, const double* values
 // the FP32 pool (FP32 Cholesky of the mixed path).
 static int alloc_factor_slots(Ctx& c, const std::vector<int32_t>& l, int m) {
   if (l.empty()) return 0;
-  size_t& used = m == M_CHOL32 ? c.fac32_used : c.fac_used;
+  size_t& used = method_fp32(m) ? c.fac32_used : c.fac_used;
   size_t need = used;
   for (int p : l) { c.h_fac_off[p] = (int64_t)need; need += (size_t)c.h_n[p] * c.h_n[p]; }
-  if (m == M_CHOL32) HIPCHK(c.fac32_pool.grow(need + 1, used, c.s));
+  if (method_fp32(m)) HIPCHK(c.fac32_pool.grow(need + 1, used, c.s));
   else HIPCHK(c.fac_pool.grow(need + 1, used, c.s));
   used = need;
   return 0;
@@ -323,21 +326,24 @@ static int upload_list(Ctx& c, const std::vector<int32_t>& l, int seg) {
 // refactored with Bunch-Kaufman in FP64 (counted in "fallback").
 static int factor_lists(Ctx& c, std::vector<int32_t>* fl) {
   const int P = c.g.P;
-  for (int m = 0; m < 3; m++) if (!fl[m].empty()) CHK(upload_list(c, fl[m], m));
+  for (int m = 0; m < N_METHODS; m++) if (!fl[m].empty()) CHK(upload_list(c, fl[m], m));
   if (!fl[M_BK64].empty())
     launch_factor_bk(c.g, c.ktype, c.list.p, (int)fl[M_BK64].size(), c.p_n.p, c.p_mem_off.p, c.p_fac_off.p, c.mem_pool.p,
                      c.dx.p, c.fac_pool.p, c.piv_pool.p, c.info.p, c.s);
-  for (int m = M_CHOL64; m <= M_CHOL32; m++)
-    if (!fl[m].empty())
-      launch_factor_chol(c.g, c.ktype, m == M_CHOL32, c.list.p + (size_t)m * P, (int)fl[m].size(), c.p_n.p, c.p_mem_off.p,
-                         c.p_fac_off.p, c.mem_pool.p, c.dx.p, m == M_CHOL32 ? (void*)c.fac32_pool.p : (void*)c.fac_pool.p,
-                         c.pstatus.p, c.s);
-  if (fl[M_CHOL64].empty() && fl[M_CHOL32].empty()) return 0;
+  bool any_chol = false;
+  for (int m = M_CHOL64; m < N_METHODS; m++)
+    if (!fl[m].empty()) {
+      any_chol = true;
+      launch_factor_chol(c.g, c.ktype, method_fp32(m), c.list.p + (size_t)m * P, (int)fl[m].size(), c.p_n.p,
+                         c.p_mem_off.p, c.p_fac_off.p, c.mem_pool.p, c.dx.p,
+                         method_fp32(m) ? (void*)c.fac32_pool.p : (void*)c.fac_pool.p, c.pstatus.p, method_big(m), c.s);
+    }
+  if (!any_chol) return 0;
   std::vector<int32_t> st(P);
   HIPCHK(hipMemcpyAsync(st.data(), c.pstatus.p, sizeof(int32_t) * P, hipMemcpyDeviceToHost, c.s));
   HIPCHK(hipStreamSynchronize(c.s));
   std::vector<int32_t> redo;
-  for (int m = M_CHOL64; m <= M_CHOL32; m++)
+  for (int m = M_CHOL64; m < N_METHODS; m++)
     for (int p : fl[m]) if (st[p] != 0) { redo.push_back(p); c.h_method[p] = M_BK64; }
   if (redo.empty()) return 0;
   c.fallback += (double)redo.size();
@@ -355,26 +361,33 @@ static int factor_lists(Ctx& c, std::vector<int32_t>* fl) {
 // Bunch-Kaufman factor and solved again (counted in "fallback").
 static int solve_list(Ctx& c, const std::vector<int32_t>& sl) {
   const int P = c.g.P;
-  std::vector<int32_t> l[3];
+  std::vector<int32_t> l[N_METHODS];
   for (int p : sl) l[c.h_method[p]].push_back(p);
-  for (int m = 0; m < 3; m++) if (!l[m].empty()) CHK(upload_list(c, l[m], m));
+  for (int m = 0; m < N_METHODS; m++) if (!l[m].empty()) CHK(upload_list(c, l[m], m));
   if (!l[M_BK64].empty())
     launch_solve_bk(c.list.p, (int)l[M_BK64].size(), c.p_n.p, c.p_mem_off.p, c.p_fac_off.p, c.mem_pool.p, c.fac_pool.p,
                     c.piv_pool.p, c.dval.p, c.nf, c.coef_pool.p, c.s);
-  if (c.nf > 8 && (!l[M_CHOL64].empty() || !l[M_CHOL32].empty()))
+  bool any_chol = false;
+  for (int m = M_CHOL64; m < N_METHODS; m++) any_chol = any_chol || !l[m].empty();
+  if (c.nf > 8 && any_chol)
     return set_err(E_SUP, "PartitionSolve", "Cholesky solve handles up to 8 interpolated fields (found %d)", c.nf);
-  for (int m = M_CHOL64; m <= M_CHOL32; m++)
+  if (!l[M_CHOL64B].empty() || !l[M_CHOL32B].empty())
+    HIPCHK(c.solve_ws.reserve((c.mem_used + 1) * 2 * (size_t)std::max(c.nf, 1)));
+  for (int m = M_CHOL64; m < N_METHODS; m++)
     if (!l[m].empty())
-      launch_solve_chol(c.g, c.ktype, m == M_CHOL32, c.list.p + (size_t)m * P, (int)l[m].size(), c.p_n.p, c.p_mem_off.p,
-                        c.p_fac_off.p, c.mem_pool.p, c.dx.p, m == M_CHOL32 ? (const void*)c.fac32_pool.p : (const void*)c.fac_pool.p,
-                        c.dval.p, c.nf, c.coef_pool.p, c.refine_max_iter, c.refine_tol, c.pstatus.p, c.piters.p, c.s);
-  if (l[M_CHOL32].empty()) return 0;
+      launch_solve_chol(c.g, c.ktype, method_fp32(m), c.list.p + (size_t)m * P, (int)l[m].size(), c.p_n.p, c.p_mem_off.p,
+                        c.p_fac_off.p, c.mem_pool.p, c.dx.p, method_fp32(m) ? (const void*)c.fac32_pool.p : (const void*)c.fac_pool.p,
+                        c.dval.p, c.nf, c.coef_pool.p, c.refine_max_iter, c.refine_tol, c.pstatus.p, c.piters.p,
+                        method_big(m) ? c.solve_ws.p : nullptr, c.s);
+  if (l[M_CHOL32].empty() && l[M_CHOL32B].empty()) return 0;
   std::vector<int32_t> st(P), its(P);
   HIPCHK(hipMemcpyAsync(st.data(), c.pstatus.p, sizeof(int32_t) * P, hipMemcpyDeviceToHost, c.s));
   HIPCHK(hipMemcpyAsync(its.data(), c.piters.p, sizeof(int32_t) * P, hipMemcpyDeviceToHost, c.s));
   HIPCHK(hipStreamSynchronize(c.s));
   std::vector<int32_t> redo;
-  for (int p : l[M_CHOL32]) {
+  std::vector<int32_t> refined(l[M_CHOL32]);
+  refined.insert(refined.end(), l[M_CHOL32B].begin(), l[M_CHOL32B].end());
+  for (int p : refined) {
     c.refine_iters_max = std::max(c.refine_iters_max, (double)its[p]);
     c.refine_iters_sum += its[p];
     c.refined_patches += 1;
@@ -409,7 +422,7 @@ int partitions_setup(Ctx& c, bool all_patches) {
   HIPCHK(hipMemcpyAsync(t.data(), c.touched.p, P, hipMemcpyDeviceToHost, c.s));
   HIPCHK(hipStreamSynchronize(c.s));
   std::vector<int32_t> newl, solvel;
-  std::vector<int32_t> fl[3];   // new patches per factor method
+  std::vector<int32_t> fl[N_METHODS];   // new patches per factor method
   size_t need_mem = c.mem_used;
   for (int p = 0; p < P; p++) {
     if (!t[p]) continue;
@@ -421,7 +434,10 @@ int partitions_setup(Ctx& c, bool all_patches) {
       newl.push_back(p);
       c.max_patch = std::max(c.max_patch, n);
       int m = M_BK64;
-      if (c.factor_mode != 1 && chol_kernel_supported(c.ktype) && n <= chol_max_n()) m = c.precision ? M_CHOL32 : M_CHOL64;
+      if (c.factor_mode != 1 && chol_kernel_supported(c.ktype)) {
+        if (n <= c.chol_big_min) m = c.precision ? M_CHOL32 : M_CHOL64;
+        else m = c.precision ? M_CHOL32B : M_CHOL64B;
+      }
       c.h_method[p] = (uint8_t)m;
       if (n > 0) fl[m].push_back(p);
     }
@@ -432,7 +448,7 @@ int partitions_setup(Ctx& c, bool all_patches) {
     HIPCHK(c.coef_pool.grow((need_mem + 1) * std::max(c.nf, 1), c.mem_used * std::max(c.nf, 1), c.s));
     if (c.precision) HIPCHK(c.coef32_pool.grow((need_mem + 1) * std::max(c.nf, 1), c.mem_used * std::max(c.nf, 1), c.s));
     c.mem_used = need_mem;
-    for (int m = 0; m < 3; m++) CHK(alloc_factor_slots(c, fl[m], m));
+    for (int m = 0; m < N_METHODS; m++) CHK(alloc_factor_slots(c, fl[m], m));
     HIPCHK(hipMemcpyAsync(c.p_n.p, c.h_n.data(), sizeof(int32_t) * P, hipMemcpyHostToDevice, c.s));
     HIPCHK(hipMemcpyAsync(c.p_mem_off.p, c.h_mem_off.data(), sizeof(int64_t) * P, hipMemcpyHostToDevice, c.s));
     HIPCHK(hipMemcpyAsync(c.p_fac_off.p, c.h_fac_off.data(), sizeof(int64_t) * P, hipMemcpyHostToDevice, c.s));

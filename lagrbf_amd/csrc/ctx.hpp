@@ -121,11 +121,13 @@ struct Ctx {
   DBuf<double> fac_pool, coef_pool;
   DBuf<float> fac32_pool;
   size_t fac32_used = 0;
-  std::vector<uint8_t> h_method;           // per patch: M_BK64, M_CHOL64, M_CHOL32
+  std::vector<uint8_t> h_method;           // per patch: M_BK64, M_CHOL64, M_CHOL32, M_CHOL64B, M_CHOL32B
+  DBuf<double> solve_ws;                   // solve workspace of the large patches (2 nf per member)
   DBuf<int32_t> pstatus, piters;
   int precision = 0;                       // 0 FP64, 1 FP32 factor + FP64 refinement
   int factor_mode = 0;                     // 0 auto (Cholesky for PD kernels), 1 Bunch-Kaufman always
   bool factors_released = false;           // serving datasets keep coefficients only
+  int chol_big_min = 512;                  // patches above this size use the large-patch Cholesky kernels
   int refine_max_iter = 10;
   double refine_tol = 1e-12;
   double fallback = 0, refine_iters_max = 0, refine_iters_sum = 0, refined_patches = 0;
@@ -139,7 +141,11 @@ struct Ctx {
   int max_patch = 0;
 };
 
-enum { M_BK64 = 0, M_CHOL64 = 1, M_CHOL32 = 2 };
+// factor methods: Bunch-Kaufman FP64; Cholesky FP64 / FP32 (n <= chol_max_n());
+// the same for larger patches (kernels without n-sized LDS)
+enum { M_BK64 = 0, M_CHOL64 = 1, M_CHOL32 = 2, M_CHOL64B = 3, M_CHOL32B = 4, N_METHODS = 5 };
+inline bool method_fp32(int m) { return m == M_CHOL32 || m == M_CHOL32B; }
+inline bool method_big(int m) { return m == M_CHOL64B || m == M_CHOL32B; }
 
 template <typename T>
 inline void swap_buf(DBuf<T>& a, DBuf<T>& b) {

@@ -52,16 +52,18 @@ void launch_solve_bk(const int32_t* list, int nlist, const int32_t* pn, const in
                      const double* values, int nf, double* coef, hipStream_t s);
 
 // blocked Cholesky (positive definite kernels), FP64 or FP32 factor; the FP32
-// solve refines FP64 coefficients against the FP64 matrix (patch_chol.hip)
+// solve refines FP64 coefficients against the FP64 matrix (patch_chol.hip).
+// big: patches above chol_max_n() (no n-sized LDS); their solve needs a workspace
+// ws of 2 nf doubles per member, indexed like the member pool.
 bool chol_kernel_supported(int ktype);
 int chol_max_n();
 void launch_factor_chol(const Grid& g, int ktype, bool fp32, const int32_t* list, int nlist, const int32_t* pn,
                         const int64_t* mem_off, const int64_t* fac_off, const int32_t* mem_pool, const double* xd,
-                        void* fac, int32_t* status, hipStream_t s);
+                        void* fac, int32_t* status, bool big, hipStream_t s);
 void launch_solve_chol(const Grid& g, int ktype, bool fp32, const int32_t* list, int nlist, const int32_t* pn,
                        const int64_t* mem_off, const int64_t* fac_off, const int32_t* mem_pool, const double* xd,
                        const void* fac, const double* values, int nf, double* coef, int max_iter, double tol,
-                       int32_t* status, int32_t* iters, hipStream_t s);
+                       int32_t* status, int32_t* iters, double* ws, hipStream_t s);
 
 // ---- weights / evaluation ----
 void launch_weights(const Grid& g, const double* pux, const double* puy, const double* puz, const double* px,

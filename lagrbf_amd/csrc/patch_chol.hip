@@ -231,6 +231,227 @@ __global__ __launch_bounds__(CNT) void k_solve_chol(Grid g, const int32_t* __res
   }
 }
 
+// ---------------------------------------------------------------------------
+// Patches larger than CNMAX: the same left-looking blocked Cholesky without any
+// n-sized LDS.  Per panel J: the w x w diagonal block is assembled, updated with
+// the factored columns and factored in LDS; every row below it (in chunks of CNT,
+// one row per thread) is assembled, updated and solved against the diagonal block
+// (l_r = a_r L_JJ^-T, forward substitution).  Member coordinates are gathered
+// from the node array and shifted to the patch node in FP64.
+template <typename T, int KT>
+__global__ __launch_bounds__(CNT) void k_factor_chol_big(Grid g, const int32_t* __restrict__ list,
+                                                         const int32_t* __restrict__ pn,
+                                                         const int64_t* __restrict__ mem_off,
+                                                         const int64_t* __restrict__ fac_off,
+                                                         const int32_t* __restrict__ mem_pool,
+                                                         const double* __restrict__ xd, T* __restrict__ fac,
+                                                         int32_t* __restrict__ status) {
+  __shared__ T Lr[CNB * CTT];
+  __shared__ T D[CNB * CNB];
+  __shared__ T colx[3][CNB];
+  __shared__ int s_fail;
+  const int p = list[blockIdx.x];
+  const int n = pn[p];
+  const int tid = threadIdx.x, dim = g.dim;
+  if (n == 0) { if (tid == 0) status[p] = 0; return; }
+  const int32_t* mem = mem_pool + mem_off[p];
+  T* L = fac + fac_off[p];
+  const double cx = patch_centre(g, p, 0), cy = patch_centre(g, p, 1), cz = dim == 3 ? patch_centre(g, p, 2) : 0.0;
+  if (tid == 0) s_fail = 0;
+  const T eps = (T)g.eps_r;
+  for (int J = 0; J < n; J += CNB) {
+    const int w = min(CNB, n - J), rows = n - J;
+    __syncthreads();
+    if (tid < CNB) {
+      const int gj = J + (tid < w ? tid : 0);
+      const double* x = xd + (size_t)mem[gj] * dim;
+      colx[0][tid] = (T)(x[0] - cx); colx[1][tid] = (T)(x[1] - cy); colx[2][tid] = dim == 3 ? (T)(x[2] - cz) : T(0);
+    }
+    __syncthreads();
+    for (int r0 = 0; r0 < rows; r0 += CNT) {
+      const int r = r0 + tid;
+      const bool live = r < rows;
+      const int gi = J + (live ? r : 0);
+      const double* x = xd + (size_t)mem[gi] * dim;
+      const T xi0 = (T)(x[0] - cx), xi1 = (T)(x[1] - cy), xi2 = dim == 3 ? (T)(x[2] - cz) : T(0);
+      T acc[CNB];
+#pragma unroll
+      for (int c = 0; c < CNB; c++) {
+        const T d0 = xi0 - colx[0][c], d1 = xi1 - colx[1][c], d2 = xi2 - colx[2][c];
+        acc[c] = rbf<KT, T>(sqrt_fast(d0 * d0 + d1 * d1 + d2 * d2), eps);
+      }
+      for (int t0 = 0; t0 < J; t0 += CTT) {
+        const int tw = min(CTT, J - t0);
+        __syncthreads();
+        for (int e = tid; e < CNB * CTT; e += CNT) {
+          const int c = e / CTT, tt = e % CTT;
+          Lr[e] = (c < w && tt < tw) ? L[(size_t)(t0 + tt) * n + J + c] : T(0);
+        }
+        __syncthreads();
+        if (live) {
+          for (int tt = 0; tt < tw; tt++) {
+            const T lit = L[(size_t)(t0 + tt) * n + gi];
+#pragma unroll
+            for (int c = 0; c < CNB; c++) acc[c] -= lit * Lr[c * CTT + tt];
+          }
+        }
+      }
+      if (r0 == 0) {
+        // diagonal block: rows 0..w-1 of the first chunk, factored in LDS
+        if (live && r < w)
+#pragma unroll
+          for (int c = 0; c < CNB; c++) if (c < w) D[c * CNB + r] = acc[c];
+        __syncthreads();
+        for (int c = 0; c < w; c++) {
+          const T pcc = D[c * CNB + c];
+          if (!(pcc > T(0)) && tid == 0 && s_fail == 0) s_fail = J + c + 1;
+          const T dd = sqrt(pcc > T(0) ? pcc : T(1));
+          __syncthreads();
+          if (tid > c && tid < w) D[c * CNB + tid] /= dd;
+          __syncthreads();
+          if (tid == 0) D[c * CNB + c] = dd;
+          if (tid > c && tid < w) {
+            const T lrc = D[c * CNB + tid];
+            for (int c2 = c + 1; c2 <= tid; c2++) D[c2 * CNB + tid] -= lrc * D[c * CNB + c2];
+          }
+          __syncthreads();
+        }
+        for (int e = tid; e < w * w; e += CNT) {
+          const int c = e / w, rr = e % w;
+          if (rr >= c) L[(size_t)(J + c) * n + J + rr] = D[c * CNB + rr];
+        }
+      }
+      __syncthreads();
+      // rows below the diagonal block: l_r = a_r L_JJ^-T
+      if (live && r >= w) {
+#pragma unroll
+        for (int c = 0; c < CNB; c++) {
+          if (c < w) {
+            T v = acc[c];
+            for (int c2 = 0; c2 < c; c2++) v -= acc[c2] * D[c2 * CNB + c];
+            acc[c] = v / D[c * CNB + c];
+            L[(size_t)(J + c) * n + gi] = acc[c];
+          }
+        }
+      }
+    }
+  }
+  __syncthreads();
+  if (tid == 0) status[p] = s_fail;
+}
+
+// Solve and FP64 refinement for patches larger than CNMAX; right-hand side and
+// work vectors live in a global workspace (2 nf doubles per member, indexed by
+// the patch's member offset).
+template <typename TL, int KT>
+__global__ __launch_bounds__(CNT) void k_solve_chol_big(Grid g, const int32_t* __restrict__ list,
+                                                        const int32_t* __restrict__ pn,
+                                                        const int64_t* __restrict__ mem_off,
+                                                        const int64_t* __restrict__ fac_off,
+                                                        const int32_t* __restrict__ mem_pool,
+                                                        const double* __restrict__ xd, const TL* __restrict__ fac,
+                                                        const double* __restrict__ values, int nf, double* __restrict__ coef,
+                                                        int max_iter, double tol, double* __restrict__ ws,
+                                                        int32_t* __restrict__ status, int32_t* __restrict__ iters) {
+  __shared__ double red[CNT / CW][8];
+  __shared__ double zk_s[8];
+  const int p = list[blockIdx.x];
+  const int n = pn[p];
+  const int tid = threadIdx.x, lane = tid & (CW - 1), wid = tid / CW;
+  if (n == 0 || nf == 0) return;
+  const int dim = g.dim;
+  const int32_t* mem = mem_pool + mem_off[p];
+  const TL* L = fac + fac_off[p];
+  double* X = coef + mem_off[p] * nf;
+  double* b = ws + (size_t)mem_off[p] * 2 * nf;     // [nf][n]
+  double* y = b + (size_t)nf * n;                   // [nf][n]
+  const double cx = patch_centre(g, p, 0), cy = patch_centre(g, p, 1), cz = dim == 3 ? patch_centre(g, p, 2) : 0.0;
+  for (int i = tid; i < n; i += CNT)
+    for (int f = 0; f < nf; f++) { b[(size_t)f * n + i] = values[(size_t)mem[i] * nf + f]; X[(size_t)i * nf + f] = 0.0; }
+  __syncthreads();
+  const bool refine = sizeof(TL) == sizeof(float);
+  int it = 0, converged = 0;
+  double dprev = 1e300;
+  for (; it <= (refine ? max_iter : 0); it++) {
+    if (it > 0) {
+      for (int i = tid; i < n; i += CNT) {
+        double sres[8];
+        FOR_F sres[f] = values[(size_t)mem[i] * nf + f];
+        const double* xi = xd + (size_t)mem[i] * dim;
+        const double xi0 = xi[0] - cx, xi1 = xi[1] - cy, xi2 = dim == 3 ? xi[2] - cz : 0.0;
+        for (int j = 0; j < n; j++) {
+          const double* xj = xd + (size_t)mem[j] * dim;
+          const double d0 = xi0 - (xj[0] - cx), d1 = xi1 - (xj[1] - cy), d2 = dim == 3 ? xi2 - (xj[2] - cz) : 0.0;
+          const double a = rbf<KT, double>(sqrt_fast(d0 * d0 + d1 * d1 + d2 * d2), g.eps_r);
+          FOR_F sres[f] -= a * X[(size_t)j * nf + f];
+        }
+        FOR_F b[(size_t)f * n + i] = sres[f];
+      }
+      __syncthreads();
+    }
+    for (int f = 0; f < nf; f++)
+      for (int i = tid; i < n; i += CNT) y[(size_t)f * n + i] = b[(size_t)f * n + i];
+    __syncthreads();
+    for (int k = 0; k < n; k++) {
+      if (tid == 0) {
+        const double lkk = (double)L[(size_t)k * n + k];
+        FOR_F { zk_s[f] = y[(size_t)f * n + k] / lkk; y[(size_t)f * n + k] = zk_s[f]; }
+      }
+      __syncthreads();
+      double zk[8];
+      FOR_F zk[f] = zk_s[f];
+      for (int i = k + 1 + tid; i < n; i += CNT) {
+        const double lik = (double)L[(size_t)k * n + i];
+        FOR_F y[(size_t)f * n + i] -= lik * zk[f];
+      }
+      __syncthreads();
+    }
+    for (int k = n - 1; k >= 0; k--) {
+      double sacc[8];
+      FOR_F sacc[f] = 0.0;
+      for (int i = k + 1 + tid; i < n; i += CNT) {
+        const double lik = (double)L[(size_t)k * n + i];
+        FOR_F sacc[f] += lik * y[(size_t)f * n + i];
+      }
+      FOR_F {
+        double v = sacc[f];
+        for (int o = CW / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
+        if (lane == 0) red[wid][f] = v;
+      }
+      __syncthreads();
+      if (tid < nf) {
+        double t = red[0][tid];
+        for (int w2 = 1; w2 < CNT / CW; w2++) t += red[w2][tid];
+        y[(size_t)tid * n + k] = (y[(size_t)tid * n + k] - t) / (double)L[(size_t)k * n + k];
+      }
+      __syncthreads();
+    }
+    double dmax = 0.0, xmax = 0.0;
+    for (int i = tid; i < n; i += CNT)
+      for (int f = 0; f < nf; f++) {
+        const double d = y[(size_t)f * n + i];
+        const double xn = X[(size_t)i * nf + f] + d;
+        X[(size_t)i * nf + f] = xn;
+        dmax = fmax(dmax, fabs(d));
+        xmax = fmax(xmax, fabs(xn));
+      }
+    for (int o = CW / 2; o > 0; o >>= 1) { dmax = fmax(dmax, __shfl_xor(dmax, o)); xmax = fmax(xmax, __shfl_xor(xmax, o)); }
+    if (lane == 0) { red[wid][0] = dmax; red[wid][1] = xmax; }
+    __syncthreads();
+    dmax = red[0][0]; xmax = red[0][1];
+    for (int w2 = 1; w2 < CNT / CW; w2++) { dmax = fmax(dmax, red[w2][0]); xmax = fmax(xmax, red[w2][1]); }
+    __syncthreads();
+    if (!refine) { converged = 1; break; }
+    if (it > 0 && dmax <= tol * xmax) { converged = 1; break; }
+    if (it > 1 && dmax > 0.5 * dprev && dmax <= 100.0 * tol * xmax) { converged = 1; break; }
+    dprev = dmax;
+  }
+  if (tid == 0) {
+    if (status) status[p] = converged ? 0 : 1;
+    if (iters) iters[p] = it;
+  }
+}
+
 static size_t factor_smem(size_t esz) { return esz * (3 * CNMAX + (size_t)CNMAX * CNB + CNB * CTT); }
 static size_t solve_smem() { return sizeof(double) * (3 * CNMAX + 16 * (size_t)CNMAX); }
 
@@ -239,7 +460,12 @@ int chol_max_n() { return CNMAX; }
 template <typename T, int KT>
 static void factor_chol_k(const Grid& g, const int32_t* list, int nlist, const int32_t* pn, const int64_t* mem_off,
                           const int64_t* fac_off, const int32_t* mem_pool, const double* xd, T* fac, int32_t* status,
-                          hipStream_t s) {
+                          bool big, hipStream_t s) {
+  if (big) {
+    hipLaunchKernelGGL((k_factor_chol_big<T, KT>), dim3(nlist), dim3(CNT), 0, s, g, list, pn, mem_off, fac_off, mem_pool,
+                       xd, fac, status);
+    return;
+  }
   static bool attr = false;
   const size_t sm = factor_smem(sizeof(T));
   if (!attr) { (void)hipFuncSetAttribute((const void*)k_factor_chol<T, KT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm); attr = true; }
@@ -250,7 +476,12 @@ template <typename TL, int KT>
 static void solve_chol_k(const Grid& g, const int32_t* list, int nlist, const int32_t* pn, const int64_t* mem_off,
                          const int64_t* fac_off, const int32_t* mem_pool, const double* xd, const TL* fac,
                          const double* values, int nf, double* coef, int max_iter, double tol, int32_t* status,
-                         int32_t* iters, hipStream_t s) {
+                         int32_t* iters, double* ws, hipStream_t s) {
+  if (ws) {
+    hipLaunchKernelGGL((k_solve_chol_big<TL, KT>), dim3(nlist), dim3(CNT), 0, s, g, list, pn, mem_off, fac_off, mem_pool,
+                       xd, fac, values, nf, coef, max_iter, tol, ws, status, iters);
+    return;
+  }
   static bool attr = false;
   const size_t sm = solve_smem();
   if (!attr) { (void)hipFuncSetAttribute((const void*)k_solve_chol<TL, KT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm); attr = true; }
@@ -263,43 +494,43 @@ bool chol_kernel_supported(int ktype) { return ktype == K_WC0 || ktype == K_WC2 
 template <typename T>
 static void factor_chol_dispatch(const Grid& g, int ktype, const int32_t* list, int nlist, const int32_t* pn,
                                  const int64_t* mem_off, const int64_t* fac_off, const int32_t* mem_pool, const double* xd,
-                                 T* fac, int32_t* status, hipStream_t s) {
+                                 T* fac, int32_t* status, bool big, hipStream_t s) {
   switch (ktype) {
-    case K_WC0: factor_chol_k<T, K_WC0>(g, list, nlist, pn, mem_off, fac_off, mem_pool, xd, fac, status, s); break;
-    case K_WC2: factor_chol_k<T, K_WC2>(g, list, nlist, pn, mem_off, fac_off, mem_pool, xd, fac, status, s); break;
-    case K_IMQ: factor_chol_k<T, K_IMQ>(g, list, nlist, pn, mem_off, fac_off, mem_pool, xd, fac, status, s); break;
-    default: factor_chol_k<T, K_GAUSS>(g, list, nlist, pn, mem_off, fac_off, mem_pool, xd, fac, status, s); break;
+    case K_WC0: factor_chol_k<T, K_WC0>(g, list, nlist, pn, mem_off, fac_off, mem_pool, xd, fac, status, big, s); break;
+    case K_WC2: factor_chol_k<T, K_WC2>(g, list, nlist, pn, mem_off, fac_off, mem_pool, xd, fac, status, big, s); break;
+    case K_IMQ: factor_chol_k<T, K_IMQ>(g, list, nlist, pn, mem_off, fac_off, mem_pool, xd, fac, status, big, s); break;
+    default: factor_chol_k<T, K_GAUSS>(g, list, nlist, pn, mem_off, fac_off, mem_pool, xd, fac, status, big, s); break;
   }
 }
 
 void launch_factor_chol(const Grid& g, int ktype, bool fp32, const int32_t* list, int nlist, const int32_t* pn,
                         const int64_t* mem_off, const int64_t* fac_off, const int32_t* mem_pool, const double* xd,
-                        void* fac, int32_t* status, hipStream_t s) {
+                        void* fac, int32_t* status, bool big, hipStream_t s) {
   if (nlist <= 0) return;
-  if (fp32) factor_chol_dispatch<float>(g, ktype, list, nlist, pn, mem_off, fac_off, mem_pool, xd, (float*)fac, status, s);
-  else factor_chol_dispatch<double>(g, ktype, list, nlist, pn, mem_off, fac_off, mem_pool, xd, (double*)fac, status, s);
+  if (fp32) factor_chol_dispatch<float>(g, ktype, list, nlist, pn, mem_off, fac_off, mem_pool, xd, (float*)fac, status, big, s);
+  else factor_chol_dispatch<double>(g, ktype, list, nlist, pn, mem_off, fac_off, mem_pool, xd, (double*)fac, status, big, s);
 }
 
 template <typename TL>
 static void solve_chol_dispatch(const Grid& g, int ktype, const int32_t* list, int nlist, const int32_t* pn,
                                 const int64_t* mem_off, const int64_t* fac_off, const int32_t* mem_pool, const double* xd,
                                 const TL* fac, const double* values, int nf, double* coef, int max_iter, double tol,
-                                int32_t* status, int32_t* iters, hipStream_t s) {
+                                int32_t* status, int32_t* iters, double* ws, hipStream_t s) {
   switch (ktype) {
-    case K_WC0: solve_chol_k<TL, K_WC0>(g, list, nlist, pn, mem_off, fac_off, mem_pool, xd, fac, values, nf, coef, max_iter, tol, status, iters, s); break;
-    case K_WC2: solve_chol_k<TL, K_WC2>(g, list, nlist, pn, mem_off, fac_off, mem_pool, xd, fac, values, nf, coef, max_iter, tol, status, iters, s); break;
-    case K_IMQ: solve_chol_k<TL, K_IMQ>(g, list, nlist, pn, mem_off, fac_off, mem_pool, xd, fac, values, nf, coef, max_iter, tol, status, iters, s); break;
-    default: solve_chol_k<TL, K_GAUSS>(g, list, nlist, pn, mem_off, fac_off, mem_pool, xd, fac, values, nf, coef, max_iter, tol, status, iters, s); break;
+    case K_WC0: solve_chol_k<TL, K_WC0>(g, list, nlist, pn, mem_off, fac_off, mem_pool, xd, fac, values, nf, coef, max_iter, tol, status, iters, ws, s); break;
+    case K_WC2: solve_chol_k<TL, K_WC2>(g, list, nlist, pn, mem_off, fac_off, mem_pool, xd, fac, values, nf, coef, max_iter, tol, status, iters, ws, s); break;
+    case K_IMQ: solve_chol_k<TL, K_IMQ>(g, list, nlist, pn, mem_off, fac_off, mem_pool, xd, fac, values, nf, coef, max_iter, tol, status, iters, ws, s); break;
+    default: solve_chol_k<TL, K_GAUSS>(g, list, nlist, pn, mem_off, fac_off, mem_pool, xd, fac, values, nf, coef, max_iter, tol, status, iters, ws, s); break;
   }
 }
 
 void launch_solve_chol(const Grid& g, int ktype, bool fp32, const int32_t* list, int nlist, const int32_t* pn,
                        const int64_t* mem_off, const int64_t* fac_off, const int32_t* mem_pool, const double* xd,
                        const void* fac, const double* values, int nf, double* coef, int max_iter, double tol,
-                       int32_t* status, int32_t* iters, hipStream_t s) {
+                       int32_t* status, int32_t* iters, double* ws, hipStream_t s) {
   if (nlist <= 0) return;
-  if (fp32) solve_chol_dispatch<float>(g, ktype, list, nlist, pn, mem_off, fac_off, mem_pool, xd, (const float*)fac, values, nf, coef, max_iter, tol, status, iters, s);
-  else solve_chol_dispatch<double>(g, ktype, list, nlist, pn, mem_off, fac_off, mem_pool, xd, (const double*)fac, values, nf, coef, max_iter, tol, status, iters, s);
+  if (fp32) solve_chol_dispatch<float>(g, ktype, list, nlist, pn, mem_off, fac_off, mem_pool, xd, (const float*)fac, values, nf, coef, max_iter, tol, status, iters, ws, s);
+  else solve_chol_dispatch<double>(g, ktype, list, nlist, pn, mem_off, fac_off, mem_pool, xd, (const double*)fac, values, nf, coef, max_iter, tol, status, iters, ws, s);
 }
 
 }  // namespace lrb

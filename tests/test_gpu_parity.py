@@ -354,3 +354,77 @@ def test_c1_rotation_full_size():
           f"mean {ret.mean():.4e}; removed {g.info('removed'):.0f}")
     assert err < TOL_POS
     assert g.info("removed") == o.removed_count() == 0
+
+
+def clustered_3d(n_uniform=6000, n_cluster=1500, sigma=0.03, Np=3000, n=6, seed=5):
+    """Uniform nodes plus one Gaussian cluster (patches of several hundred to a few
+    thousand members), the Fourier velocity, particles uniform in [0.1, 0.9]^3."""
+    rng = np.random.default_rng(cases.SEED + 40 + seed)
+    X = np.concatenate([rng.random((n_uniform, 3)),
+                        np.clip(0.5 + sigma * rng.standard_normal((n_cluster, 3)), 0.0, 1.0)])
+    V = cases.fourier_velocity(X, cases.fourier_modes())
+    P = 0.1 + 0.8 * rng.random((Np, 3))
+    opts = dict(pu_mesh_dim=3, pu_mesh_O=[0.0] * 3, pu_mesh_L=[1.0] * 3, pu_mesh_n=[n] * 3, n_interpolation_fields=3,
+                pu_mesh_overlap=1.0, pu_mesh_shape_factor=1.0, kernel_type=2, solve_method=0,
+                advection=True, advection_scheme=2, cfl_fac=1e6, dt=1e-3)
+    return opts, np.ascontiguousarray(X), np.ascontiguousarray(V), P
+
+
+def _wc2(r, eps):
+    u = np.maximum(1.0 - eps * r, 0.0)
+    return u ** 4 * (4.0 * eps * r + 1.0)
+
+
+@pytest.mark.parametrize("precision,big_min,tol", [("fp64", 16, 1e-12), ("mixed", 16, 1e-10), ("fp64", 512, 1e-12),
+                                                   ("mixed", 512, 1e-10)])
+def test_large_patch_cholesky(precision, big_min, tol):
+    """Uneven patch sizes (uniform nodes + a cluster: 40 to ~1900 members) through the
+    large-patch Cholesky kernels (all patches above big_min).  Patches up to 512
+    members: coefficients against the oracle's dsytrf solve (forward error, tol).
+    The cluster patches are ill-conditioned (dense nodes inside the Wendland
+    support): there the bar is the backward error, |A x - b| / (|A| |x|) < 1e-14
+    with A assembled independently in numpy, and the forward error against the
+    oracle is reported with cond(A)."""
+    opts, X, V, P = clustered_3d()
+    opts["lagrbf_precision"] = precision
+    opts["lagrbf_chol_big_min"] = big_min
+    print(f"case: large patches {precision} big_min={big_min}", flush=True)
+    g, o = pair(opts)
+    for c in (g, o):
+        c.fill_data(X, V); c.fill_points(P); c.interpolate_c()
+    eps = g.info("eps_r")
+    sizes = np.array([int(g.debug_get("patch_flags", p)[2]) for p in range(6 ** 3)])
+    small_err, big_bwd, big_fwd = 0.0, 0.0, 0.0
+    for p in range(6 ** 3):
+        xo = o.patch_coeffs(p)
+        if xo.size == 0:
+            continue
+        xg = g.debug_get("patch_coeffs", p, dtype=np.float64).reshape(xo.shape)
+        fwd = np.max(np.abs(xg - xo)) / np.max(np.abs(xo))
+        if sizes[p] <= 512:
+            small_err = max(small_err, fwd)
+            continue
+        m = g.debug_get("patch_members", p)
+        xs = X[m]
+        A = _wc2(np.sqrt(((xs[:, None, :] - xs[None, :, :]) ** 2).sum(-1)), eps)
+        b = V[m]
+        bwd = np.max(np.abs(A @ xg - b)) / (np.max(np.abs(A).sum(1)) * np.max(np.abs(xg)))
+        big_bwd = max(big_bwd, bwd)
+        big_fwd = max(big_fwd, fwd)
+    big = int((sizes > 512).sum())
+    pbig = int(np.argmax(sizes))
+    mb = g.debug_get("patch_members", pbig)
+    xs = X[mb]
+    cond = np.linalg.cond(_wc2(np.sqrt(((xs[:, None, :] - xs[None, :, :]) ** 2).sum(-1)), eps))
+    print(f"patch sizes {sizes.min()}..{sizes.max()}, {big} above 512; small patches: max rel coefficient err "
+          f"{small_err:.3e}; large patches: backward err {big_bwd:.3e}, forward err {big_fwd:.3e} (cond of the largest "
+          f"{cond:.2e}); fallback {g.info('fallback'):.0f}; refine iters max {g.info('refine_iters'):.0f}")
+    assert big > 0
+    if precision == "fp64":
+        assert g.info("fallback") == 0     # FP32 factors of the dense cluster may fall back to FP64 (counted)
+    assert small_err < tol
+    assert big_bwd < 1e-14
+    assert big_fwd < max(tol, 100 * cond * 1e-16)
+    vg = np.zeros((P.shape[0], 3)); vo = np.zeros_like(vg)
+    g.output(fvals=vg); o.output(fvals=vo)
+    assert rel_err(vg, vo) < (max(TOL_VAL, 100 * cond * 1e-16) if precision == "fp64" else 1e-5)
