@@ -241,6 +241,39 @@ def serve_bench(L, precision, dsets, nreq, rate, seed=20261203 + 5, budget=200e9
     return res
 
 
+def c3_bench(L, precision, steps, warmup):
+    """Uneven batching: 1,000,000 uniform nodes + 64 Gaussian clusters of 15,625
+    (sigma in [0.03, 0.05]), PU 49^3, Wendland C2, 8,388,608 particles, RK2 dt=1e-3."""
+    from tests import cases
+    opts, X, V, P = cases.clustered_c3()
+    ctx = L.LagRBFCtx(petsc_options=dict(opts), precision=precision)
+    ctx.fill_data(X, V)
+    ctx.fill_points(P)
+    ts = time.perf_counter()
+    ctx.interpolate_c()
+    t_setup = time.perf_counter() - ts
+    ctx.advect_steps_c(warmup, 1e-3)
+    ev0, n0 = ctx.info("eval_ms"), ctx.info("eval_count")
+    t1 = time.perf_counter()
+    ctx.advect_steps_c(steps, 1e-3)
+    wall = time.perf_counter() - t1
+    ev1, n1 = ctx.info("eval_ms"), ctx.info("eval_count")
+    cs = ctx.debug_get("cell_start")
+    nc = opts["pu_mesh_n"][0] - 1
+    sizes = patch_sizes(cs, nc)
+    res = {"precision": precision, "nodes": int(X.shape[0]), "particles": int(P.shape[0]), "setup_s": t_setup,
+           "setup_ms_patches": ctx.info("setup_ms"), "patches_setup": ctx.info("patches_setup"),
+           "patch_size": {"min": int(sizes.min()), "p50": float(np.median(sizes)), "p99": float(np.percentile(sizes, 99)),
+                          "max": int(sizes.max()), "above_512": int((sizes > 512).sum())},
+           "fallback_patches": ctx.info("fallback"), "refine_iters_max": ctx.info("refine_iters"),
+           "particle_rk2_steps_per_s": P.shape[0] * steps / wall, "ms_per_step": wall / steps * 1e3,
+           "eval_ms": (ev1 - ev0) / max(n1 - n0, 1), "removed": ctx.info("removed")}
+    ctx.destroy()
+    log(f"c3 [{precision}]: set-up {t_setup:.2f}s, patch sizes {res['patch_size']}, fallback {res['fallback_patches']:.0f}, "
+        f"{res['particle_rk2_steps_per_s']:.3e} particle-RK2-steps/s ({res['ms_per_step']:.1f} ms/step)")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -255,6 +288,8 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=64)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-only", action="store_true")
+    ap.add_argument("--c3-only", action="store_true", help="the uneven-batching configuration only")
+    ap.add_argument("--no-c3", action="store_true", help="skip the uneven-batching measurement")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="launch a plain copy of MIB MiB first (scale for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -267,6 +302,10 @@ def main():
     import lagrbf_amd as L
     if args.copy_calibration > 0:
         copy_calibration(args.copy_calibration)
+    if args.c3_only:
+        out = {p: c3_bench(L, p, args.steps, args.warmup) for p in ("mixed", "fp64")}
+        print(json.dumps({"c3": out}), flush=True)
+        return
     if args.serve_only:
         dsets = serve_datasets()
         out = {p: serve_bench(L, p, dsets, args.serve_requests, args.serve_rate) for p in ("mixed", "fp64")}
@@ -303,19 +342,32 @@ def main():
         log(f"cpu baseline ({thr} threads) ...")
         cpu = cpu_baseline(opts, X, V, P, args.cpu_steps, dt, args.cpu_particles, thr)
         log(f"cpu baseline: {cpu['value']:.3e} {cpu['unit']} ({cpu['sample']}); cutoff mode {cpu['cutoff_value']:.3e}")
-    serve = None
+    serve, c3 = None, None
+    del X, V, P
     if not args.no_serve:
-        del X, V, P
-        dsets = serve_datasets()
-        serve = {}
-        for p in ("mixed", "fp64"):
-            serve[p] = serve_bench(L, p, dsets, args.serve_requests, args.serve_rate)
-            log(f"serve [{p}]: {args.serve_requests} requests at {args.serve_rate:.0f}/s: p50 {serve[p]['p50_ms']:.1f} ms, "
-                f"p99 {serve[p]['p99_ms']:.1f} ms, {serve[p]['particle_steps_per_s']:.3e} particle-steps/s, "
-                f"wall {serve[p]['wall_s']:.1f} s, set-up {serve[p]['setup_s']:.1f} s")
-        serve["config"] = ("4 device-resident datasets (2D 65,536 nodes; 3D 2^20, 2,000,000, 4,000,000 nodes), "
-                           f"{args.serve_requests} requests, Poisson arrivals at {args.serve_rate:.0f}/s, particles "
-                           "log-uniform in [1e3, 1e6], RK4 or DOPRI, 10-200 steps, HBM budget 200 GB")
+        try:
+            dsets = serve_datasets()
+            serve = {}
+            for p in ("mixed", "fp64"):
+                serve[p] = serve_bench(L, p, dsets, args.serve_requests, args.serve_rate)
+                log(f"serve [{p}]: {args.serve_requests} requests at {args.serve_rate:.0f}/s: p50 {serve[p]['p50_ms']:.1f} ms, "
+                    f"p99 {serve[p]['p99_ms']:.1f} ms, {serve[p]['particle_steps_per_s']:.3e} particle-steps/s, "
+                    f"wall {serve[p]['wall_s']:.1f} s, set-up {serve[p]['setup_s']:.1f} s")
+            serve["config"] = ("4 device-resident datasets (2D 65,536 nodes; 3D 2^20, 2,000,000, 4,000,000 nodes), "
+                               f"{args.serve_requests} requests, Poisson arrivals at {args.serve_rate:.0f}/s, particles "
+                               "log-uniform in [1e3, 1e6], RK4 or DOPRI, 10-200 steps, HBM budget 200 GB")
+            del dsets
+        except Exception as e:       # the headline measurement above stands on its own
+            serve = {"error": str(e)}
+            log(f"serve failed: {e}")
+    if not args.no_c3:
+        try:
+            c3 = {p: c3_bench(L, p, 5, 1) for p in ("mixed", "fp64")}
+            c3["config"] = ("uneven batching: 1,000,000 uniform + 64 Gaussian clusters x 15,625 nodes (sigma 0.03-0.05), "
+                            "PU 49^3, Wendland C2, 8,388,608 particles, RK2 dt=1e-3, 5 steps")
+        except Exception as e:
+            c3 = {"error": str(e)}
+            log(f"c3 failed: {e}")
     out = {
         "metric": "particle-RK4 steps/s, FP32+refine vs FP64 @1e-10 rel err; serve p99 ms",
         "value": r["value"], "unit": "particle-RK4-steps/s", "n_gpus": 1, "steps": args.steps,
@@ -329,6 +381,7 @@ def main():
         "paths": {p: {k: v for k, v in res[p].items()} for p in paths},
         "mixed_vs_fp64": cmp,
         "serve": serve,
+        "uneven_batching": c3,
     }
     if rank == 0:
         print(json.dumps(out), flush=True)
