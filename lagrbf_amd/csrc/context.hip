@@ -194,6 +194,7 @@ int fill_data(Ctx& c, int n, const double* coords, const double* values, const i
   const int dim = c.g.dim, nf = c.nf;
   CHK(check_bbox(c, n, coords, fn));
   c.blk_valid = false;
+  c.tuner = EvalTuner();
   if (c.setup_N >= 0 && c.solve_method != 2 && n != c.setup_N && c.mem_used > 0)
     return set_err(E_SIZ, fn, "solve_method %d keeps patch factors across data fills; node count changed from %d to %d "
                    "(call with solve_method 2 or destroy the context)", c.solve_method, c.setup_N, n);
@@ -255,6 +256,8 @@ static int alloc_particles(Ctx& c) {
   HIPCHK(c.pstart.reserve((size_t)c.g.C + 2));
   HIPCHK(c.pscratch.reserve((size_t)c.g.C + 2 + (size_t)(c.g.C + 2) / 1024 + 64));
   HIPCHK(c.porig.reserve(ld));
+  HIPCHK(c.cell_list.reserve((size_t)std::max(c.g.C, 1)));
+  HIPCHK(c.list_count.reserve(1));
   c.sorted_valid = false;
   HIPCHK(c.vel.reserve(ld * dim));
   if (c.advection) {
@@ -564,9 +567,19 @@ static int rbf_interpolation(Ctx& c, double* out, int nfields) {
   if (use_blk) {
     const int per_cell = (c.Np + c.g.C - 1) / std::max(c.g.C, 1);
     const int block = std::min(256, std::max(64, (per_cell + 63) / 64 * 64));
-    launch_eval_blk(c.g, c.ktype, blk32, c.px.p, c.ld, c.perm.p, c.pstart.p, c.w.p,
-                    blk32 ? (const void*)c.blk32.p : (const void*)c.blk64.p, c.nf, c.cell_start.p, nfields, out, block,
-                    c.Np, nullptr, nullptr, c.s);
+    const int bucket = density_bucket(c.Np, c.g.C);
+    auto it = c.tuner.choice.find(bucket);
+    if (it == c.tuner.choice.end() || it->second > 0)
+      launch_nonempty_cells(c.pstart.p, c.g.C, c.cell_list.p, c.list_count.p, c.s);
+    auto launch = [&](int v) {
+      bool listed;
+      int group;
+      eval_variant(v, listed, group);
+      launch_eval_blk(c.g, c.ktype, blk32, c.px.p, c.ld, c.perm.p, c.pstart.p, c.w.p,
+                      blk32 ? (const void*)c.blk32.p : (const void*)c.blk64.p, c.nf, c.cell_start.p, nfields, out,
+                      block, c.Np, listed ? c.cell_list.p : nullptr, listed ? c.list_count.p : nullptr, group, c.s);
+    };
+    c.tuner.run(bucket, c.nf == c.g.dim && nfields == c.nf, c.s, launch);
   } else if (cell_order) {
     const int per_cell = (c.Np + c.g.C - 1) / std::max(c.g.C, 1);
     const int block = std::min(256, std::max(64, (per_cell + 63) / 64 * 64));

@@ -10,6 +10,7 @@
 
 #include "kernels.hpp"
 #include "options.hpp"
+#include "tuner.hpp"
 
 namespace lrb {
 
@@ -90,6 +91,7 @@ struct Ctx {
   DBuf<double> blk64;
   DBuf<float> blk32;
   bool blk_valid = false, blk_f32 = false;
+  EvalTuner tuner;                         // evaluation launch shape per particle density (this data)
   std::vector<int32_t> h_cell_start_prev, h_cell_nodes_prev;   // cell lists the frozen patches were built from
   bool memb_current = true;                // every set-up patch's members == current cell lists
   int eval_mode = 0;                       // 0 auto (cell-ordered when possible), 1 cell-ordered, 2 reference order
@@ -101,6 +103,7 @@ struct Ctx {
   DBuf<double> px, w, fields, vel, x0, vadv[7];
   DBuf<int32_t> pcell, alive;
   DBuf<int32_t> perm, pstart, pscratch;    // particles grouped by cell
+  DBuf<int32_t> cell_list, list_count;     // non-empty particle cells (listed evaluation shapes)
   bool sorted_valid = false;
   // the swarm is stored in cell order at the start of every advection step;
   // porig[s] = index (fill order) of the particle held in slot s

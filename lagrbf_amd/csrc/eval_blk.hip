@@ -458,13 +458,11 @@ void launch_build_blk(const Grid& g, bool fp32, const double* x, int N, const in
 template <int DIM, int KT, typename T>
 static void eval_blk_k(const Grid& g, const double* px, int64_t ld, const int32_t* perm, const int32_t* pstart,
                        const double* w, const T* blk, int nft, const int32_t* cs, int nfields, double* out, int block,
-                       int np, const int32_t* cell_list, const int32_t* list_count, hipStream_t s) {
-  if (cell_list && nft == DIM) {
-    // sparse velocity evaluation: lane groups of G >= the mean particles per
-    // non-empty cell
-    const double mean = (double)np / std::max(1, std::min(g.C, np));
-    const int G = mean <= 6.0 ? 8 : (mean <= 12.0 ? 16 : 32);
-    launch_sparse_g<DIM, KT, DIM, T>(G, g, px, ld, perm, pstart, w, blk, cs, nfields, out, cell_list, list_count, np, s);
+                       int np, const int32_t* cell_list, const int32_t* list_count, int group, hipStream_t s) {
+  if (cell_list && group > 0 && nft == DIM) {
+    // sparse velocity evaluation: lane groups of G particles, 64/G cells per wave
+    launch_sparse_g<DIM, KT, DIM, T>(group, g, px, ld, perm, pstart, w, blk, cs, nfields, out, cell_list, list_count, np,
+                                     s);
     return;
   }
   // dense: one block per cell; listed cells: one block per listed non-empty cell
@@ -484,8 +482,8 @@ template <int DIM, typename T>
 static void eval_blk_dim(const Grid& g, int ktype, const double* px, int64_t ld, const int32_t* perm,
                          const int32_t* pstart, const double* w, const T* blk, int nft, const int32_t* cs, int nfields,
                          double* out, int block, int np, const int32_t* cell_list, const int32_t* list_count,
-                         hipStream_t s) {
-#define LRB_K(K) eval_blk_k<DIM, K, T>(g, px, ld, perm, pstart, w, blk, nft, cs, nfields, out, block, np, cell_list, list_count, s)
+                         int group, hipStream_t s) {
+#define LRB_K(K) eval_blk_k<DIM, K, T>(g, px, ld, perm, pstart, w, blk, nft, cs, nfields, out, block, np, cell_list, list_count, group, s)
   if constexpr (sizeof(T) == 8) {
     switch (ktype) {
       case K_LINEAR: LRB_K(K_LINEAR); break;
@@ -510,9 +508,9 @@ static void eval_blk_dim(const Grid& g, int ktype, const double* px, int64_t ld,
 void launch_eval_blk(const Grid& g, int ktype, bool fp32, const double* px, int64_t ld, const int32_t* perm,
                      const int32_t* pstart, const double* w, const void* blk_, int nft, const int32_t* cs, int nfields,
                      double* out, int block, int np, const int32_t* cell_list, const int32_t* list_count,
-                     hipStream_t s) {
+                     int group, hipStream_t s) {
   if (nfields <= 0 || g.C <= 0 || np <= 0) return;
-#define LRB_ARGS px, ld, perm, pstart, w, blk, nft, cs, nfields, out, block, np, cell_list, list_count, s
+#define LRB_ARGS px, ld, perm, pstart, w, blk, nft, cs, nfields, out, block, np, cell_list, list_count, group, s
   if (fp32) {
     const float* blk = (const float*)blk_;
     if (g.dim == 3) eval_blk_dim<3, float>(g, ktype, LRB_ARGS);

@@ -103,11 +103,13 @@ size_t blk_row(int dim, int nft);
 void launch_build_blk(const Grid& g, bool fp32, const double* x, int N, const int32_t* cn, const int32_t* dcell,
                       const int32_t* cs, const uint8_t* active, const int64_t* mem_off, const double* coef, int nft,
                       void* blk, hipStream_t s);
-// cell_list / list_count (device): evaluate only the listed non-empty cells (sparse
-// swarms); nullptr: one block per cell
+// cell_list / list_count (device): evaluate only the listed non-empty cells, one
+// block per cell (group 0) or 64/group cells per wave in lane groups of
+// group = 8, 16, 32 (needs nft == dim); nullptr: one block per cell.  Every
+// variant performs the same arithmetic per particle.
 void launch_eval_blk(const Grid& g, int ktype, bool fp32, const double* px, int64_t ld, const int32_t* perm,
                      const int32_t* pstart, const double* w, const void* blk, int nft, const int32_t* cs, int nfields,
-                     double* out, int block, int np, const int32_t* cell_list, const int32_t* list_count,
+                     double* out, int block, int np, const int32_t* cell_list, const int32_t* list_count, int group,
                      hipStream_t s);
 void launch_nonempty_cells(const int32_t* pstart, int C, int32_t* list, int32_t* count, hipStream_t s);
 void launch_sorted_coords32(const Grid& g, const double* x, int n, const int32_t* cell_nodes, const int32_t* cell,

@@ -194,24 +194,27 @@ int admit(Server& sv, Batch& b, ServeReq& r) {
   return 0;
 }
 
-// a batch with fewer than 8 particles per cell on average is evaluated over its
-// non-empty cells only
-bool sparse(const Batch& b) { return (int64_t)b.Np < 8 * (int64_t)b.ds->g.C; }
-
+// Evaluation launch shape chosen by the dataset's autotuner for this batch's
+// particle density (the first batch of a density times every shape).
 void eval_batch(Batch& b, double* out) {
   Ctx& d = *b.ds;
   const bool f32 = d.precision && eval_f32_supported(d.ktype);
   const int per_cell = (b.Np + d.g.C - 1) / std::max(d.g.C, 1);
   const int block = std::min(256, std::max(64, (per_cell + 63) / 64 * 64));
-  const bool sp = sparse(b);
-  launch_eval_blk(d.g, d.ktype, f32, b.px.p, b.ld, b.perm.p, b.pstart.p, b.w.p,
-                  f32 ? (const void*)d.blk32.p : (const void*)d.blk64.p, d.nf, d.cell_start.p, d.g.dim, out, block,
-                  b.Np, sp ? b.cell_list.p : nullptr, sp ? b.list_count.p : nullptr, b.s);
+  auto launch = [&](int v) {
+    bool listed;
+    int grp;
+    eval_variant(v, listed, grp);
+    launch_eval_blk(d.g, d.ktype, f32, b.px.p, b.ld, b.perm.p, b.pstart.p, b.w.p,
+                    f32 ? (const void*)d.blk32.p : (const void*)d.blk64.p, d.nf, d.cell_start.p, d.g.dim, out, block,
+                    b.Np, listed ? b.cell_list.p : nullptr, listed ? b.list_count.p : nullptr, grp, b.s);
+  };
+  d.tuner.run(density_bucket(b.Np, d.g.C), d.nf == d.g.dim, b.s, launch);
 }
 
 void group(Batch& b) {
   launch_sort_particles(b.cell.p, b.alive.p, b.Np, b.ds->g.C, b.pstart.p, b.perm.p, b.pscratch.p, b.s);
-  if (sparse(b)) launch_nonempty_cells(b.pstart.p, b.ds->g.C, b.cell_list.p, b.list_count.p, b.s);
+  launch_nonempty_cells(b.pstart.p, b.ds->g.C, b.cell_list.p, b.list_count.p, b.s);
 }
 
 StageArgs serve_stage_args(const Batch& b, int nv, const double* const* v, const double* a, double mul, double div,
