@@ -142,6 +142,38 @@ def register_functions(clib):
         c.restype = res
         c.argtypes = args
         f[name] = c
+    # file formats (include/lagrbf_io.h)
+    CP = ctypes.c_char_p
+    SP = ctypes.POINTER(ctypes.c_char_p)
+    I64 = ctypes.c_int64
+    I32 = ctypes.c_int32
+    VP = ctypes.c_void_p
+    D0 = wrap_ndpointer(REAL, flags="C_CONTIGUOUS")
+    L0 = wrap_ndpointer(INT64, flags="C_CONTIGUOUS")
+    for name, res, args in (
+            ("lagrbf_io_vec_write", I32, [CP, I64, D0, I32, I32]),
+            ("lagrbf_io_vec_read", I32, [CP, I32, ctypes.POINTER(I64), ctypes.POINTER(I32), D0, I64]),
+            ("lagrbf_io_is_write", I32, [CP, I64, L0, I32]),
+            ("lagrbf_io_is_read", I32, [CP, I32, ctypes.POINTER(I64), L0, I64]),
+            ("lagrbf_io_pvd_open", I32, [CP]),
+            ("lagrbf_io_pvd_append", I32, [CP, PetscReal, CP, CP]),
+            ("lagrbf_io_pvd_open_append", I32, [ctypes.c_bool, I64, CP, PetscReal, CP, CP]),
+            ("lagrbf_io_filename", I32, [CP, CP, CP, CP, CP, I64, CP, I32]),
+            ("lagrbf_io_filelist", I32, [CP, I64, CP, I32, ctypes.POINTER(I32)]),
+            ("lagrbf_io_stag_load", VP, [SP, ctypes.POINTER(I32), I32, I32]),
+            ("lagrbf_io_petsc_load", VP, [CP, SP, I32, I32, I32]),
+            ("lagrbf_io_load_data", VP, [I64, I32, I32]),
+            ("lagrbf_io_dataset_info", I32, [VP, ctypes.POINTER(I64), ctypes.POINTER(I32), ctypes.POINTER(I32)]),
+            ("lagrbf_io_dataset_copy", I32, [VP, D0, D0]),
+            ("lagrbf_io_dataset_free", None, [VP]),
+            ("lagrbf_swarm_output", I32, [VP, CP, I64, PetscReal, I32, SP, I32, ctypes.c_bool]),
+            ("lagrbf_swarm_load", I32, [VP, CP, I64, I32, I32, SP]),
+            ("lagrbf_load_data", I32, [VP, I64]),
+            ("lagrbf_fill_swarm_from_dataset", I32, [VP, ctypes.c_bool, VP])):
+        c = getattr(clib, name)
+        c.restype = res
+        c.argtypes = args
+        f[name] = c
     return f
 
 
@@ -167,4 +199,9 @@ EXPORTED_SYMBOLS = [
     "lagrbf_get_info", "lagrbf_debug_get", "lagrbf_last_error",
     "lagrbf_serve_create", "lagrbf_serve_destroy", "lagrbf_serve_add_dataset", "lagrbf_serve_submit",
     "lagrbf_serve_step", "lagrbf_serve_poll", "lagrbf_serve_release", "lagrbf_serve_info",
+    "lagrbf_io_vec_write", "lagrbf_io_vec_read", "lagrbf_io_is_write", "lagrbf_io_is_read", "lagrbf_io_pvd_open",
+    "lagrbf_io_pvd_append", "lagrbf_io_pvd_open_append", "lagrbf_io_filename", "lagrbf_io_filelist",
+    "lagrbf_io_stag_load", "lagrbf_io_petsc_load", "lagrbf_io_load_data", "lagrbf_io_dataset_info",
+    "lagrbf_io_dataset_copy", "lagrbf_io_dataset_free", "lagrbf_swarm_output", "lagrbf_swarm_load",
+    "lagrbf_load_data", "lagrbf_fill_swarm_from_dataset",
 ]

@@ -1,11 +1,18 @@
 """Advection drivers mirroring python_api/lagrbfpy/advection.py:37-496.
 
-The call sequences are the reference's; writing VTP/PVD files (pyvista) is left
-to the caller: ``advect_step`` fills the ``output`` container instead.
+The call sequences, file names and PVD entries are the reference's:
+``advect_step`` fills ``output`` and writes ``step{n:05d}{prefix}.vtp`` into
+``output_path`` plus an entry of the ``pvdfilename`` collection (pyvista is
+replaced by lagrbf_amd.vtp).  ``comm`` is accepted where the reference takes a
+PETSc communicator; there is one rank.  Extension: ``write_output=False`` keeps
+everything in memory.
 """
+import os
+
 import numpy as np
 
 from .context import LagRBFCtx
+from .timeseries import timeseries_write
 
 
 class Advection(LagRBFCtx):
@@ -32,6 +39,22 @@ class Advection(LagRBFCtx):
         self.steps = np.arange(self.nsteps, dtype=np.int32) if steps is None else np.asarray(steps, dtype=np.int32)
         if self.steps.shape[0] != self.nsteps:
             raise ValueError("steps must have the same length as time_steps.")
+        self.pvdfname = kwargs.get("pvdfilename", "timeseries-advection.pvd")
+        self.write_output = bool(kwargs.get("write_output", True))
+
+    def _write_step(self, comm, output, fields, fieldname, file_step, time, prefix):
+        """step{file_step:05d}{prefix}.vtp and its PVD entry (advection.py:247-264, 399-414)."""
+        if not self.write_output:
+            self.output(coords=output.coords, fvals=output.values, pidx=output.pidx, fieldname=fieldname,
+                        is_data=False)
+            return
+        os.makedirs(self.output_path, exist_ok=True)
+        fname = f"step{int(file_step):05d}{prefix}.vtp"
+        self.save(comm, fname=os.path.join(self.output_path, fname), fields_size=fields, output_points=output,
+                  fieldname=fieldname, is_data=False)
+        ts = {"time": [f"{time:.6e}"], "step": [f"{int(file_step):05d}"]}
+        timeseries_write(fname=os.path.join(self.output_path, self.pvdfname), timeseries=ts, prefix=prefix,
+                         extension="vtp", erase=False)
 
     def advect(self, *args, **kwargs):
         raise NotImplementedError("The advect method must be implemented in a subclass.")
@@ -40,7 +63,7 @@ class Advection(LagRBFCtx):
 class AdvectionAnalytical(Advection):
     """Advection with an analytical velocity callback (advection.py:171-320)."""
 
-    def advect_step(self, step, output):
+    def advect_step(self, step, output, comm=None, prefix=""):
         time, time_next = self.time_steps[step], self.time_steps[step + 1]
         func_data = self.get_ref("advection_data")
         if hasattr(func_data, "time"):
@@ -48,22 +71,23 @@ class AdvectionAnalytical(Advection):
         if hasattr(func_data, "step"):
             func_data.step = int(self.steps[step])
         self.advect_analytical_c(time, time_next)
-        self.output(coords=output.coords, fvals=output.values, pidx=output.pidx, fieldname="velocity", is_data=False)
+        self._write_step(comm, output, {"velocity": self.dim}, "velocity", self.steps[step + 1],
+                         self.time_steps[step + 1], prefix)
 
-    def advect(self, points, output):
+    def advect(self, points, output, comm=None, prefix=""):
         try:
             self.get_ref("advection_fn")
         except KeyError as e:
             raise RuntimeError("No advection function has been registered.") from e
         self.fill_points(coords=points.coords, values=points.values, pidx=points.pidx)
         for step in range(self.nsteps - 1):
-            self.advect_step(step, output)
+            self.advect_step(step, output, comm, prefix)
 
 
 class AdvectionInterpolation(Advection):
     """Advection with the velocity interpolated from data points (advection.py:322-496)."""
 
-    def advect_step(self, step, get_data, data_args, output):
+    def advect_step(self, step, get_data, data_args, output, comm=None, fields=None, prefix=""):
         if "step" in data_args:
             data_args["step"] = np.int32(self.steps[step])
         if "time" in data_args:
@@ -73,16 +97,17 @@ class AdvectionInterpolation(Advection):
         data = get_data(**data_args)
         self.fill_data(coords=data.coords, values=data.values, pidx=data.pidx)
         self.interpolate_c()
-        self.output(coords=output.coords, fvals=output.values, pidx=output.pidx, fieldname="fields", is_data=False)
+        fields = {"fields": self.nfields} if fields is None else fields
+        self._write_step(comm, output, fields, "fields", self.steps[step], self.time_steps[step], prefix)
         if step == self.nsteps - 1:
             return
         self.advect_interpolate_c(self.time_steps[step], self.time_steps[step + 1])
 
-    def advect(self, get_data, data_args, points, output, callback=None):
+    def advect(self, get_data, data_args, points, output, comm=None, fields=None, prefix="", callback=None):
         if not callable(get_data):
             raise ValueError("get_data must be a callable function that returns a PointContainer.")
         self.fill_points(coords=points.coords, values=points.values, pidx=points.pidx)
         for step in range(self.start_idx, self.nsteps):
-            self.advect_step(step, get_data, data_args, output)
+            self.advect_step(step, get_data, data_args, output, comm, fields, prefix=prefix)
             if callback is not None:
                 callback(step, output)

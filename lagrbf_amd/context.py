@@ -197,6 +197,62 @@ class LagRBFCtx:
         self._check(cfuncs()["lagrbf_avect_interpolate"](self.ctx, float(time_start), float(time_end)), "lagrbf_advect")
         self.data_filled = False   # the data swarm is cleared at exit (time_loop.c:98)
 
+    def save(self, comm, fname, fields_size, output_points, fieldname="fields", is_data=False):
+        """Output the swarm into ``output_points`` and write it to the .vtp file
+        fname: points, the columns of ``values`` split as ``fields_size``
+        (name -> components) and "pidx" (context.py:459-511).  ``comm`` is
+        accepted for signature compatibility; there is one rank."""
+        from .vtp import write_vtp
+        if not fields_size:
+            raise ValueError("fields_size must name at least one field")
+        if not str(fname).endswith(".vtp"):
+            raise ValueError(f"output file name must end with .vtp, got {fname}")
+        self.output(coords=output_points.coords, fvals=output_points.values, pidx=output_points.pidx,
+                    fieldname=fieldname, is_data=is_data)
+        data, idx = {}, 0
+        for name, bs in fields_size.items():
+            data[name] = output_points.values[:, idx:idx + bs]
+            idx += bs
+        data["pidx"] = output_points.pidx
+        write_vtp(output_points.coords, data, str(fname))
+
+    # -- file formats of the C side (include/lagrbf_io.h) --
+    def swarm_output(self, root, step, time, fields=(), direction=0, permute=False):
+        """SwarmOutput (swarm_output.c:169-234): PETSc binary files of the particle
+        swarm under root/step<step>/ and an entry of root/timeseries_X.pvd.
+        fields: "fields", the velocity name, "weights", "DMSwarm_pid";
+        direction 0 / 1 (forward) / 2 (backward)."""
+        from .fileio import _strings
+        names = list(fields)
+        self._check(cfuncs()["lagrbf_swarm_output"](self.ctx, str(root).encode(), int(step), float(time), len(names),
+                                                    _strings(names), int(direction), bool(permute)), "SwarmOutput")
+
+    def swarm_load(self, dirname, step, fields=(), direction=0):
+        """SwarmLoadFromFiles (swarm_load.c:133-201): refill the particle swarm from
+        the files ``swarm_output`` wrote in dirname (coordinates, plus "fields"
+        and "DMSwarm_pid" when listed)."""
+        from .fileio import _strings
+        names = list(fields)
+        self._check(cfuncs()["lagrbf_swarm_load"](self.ctx, str(dirname).encode(), int(step), int(direction),
+                                                  len(names), _strings(names)), "SwarmLoadFromFiles")
+        self.interp_filled = True
+
+    def load_data(self, step):
+        """SwarmDataLoad (swarm_data.c:161-180): read the data files of ``step``
+        named by the options (data_format, data_fname_* / swarm_data_fname_*)
+        into the data swarm."""
+        self._check(cfuncs()["lagrbf_load_data"](self.ctx, int(step)), "SwarmDataLoad")
+        self.data_filled = True
+
+    def fill_from_dataset(self, dataset, is_data=True):
+        """Fill the data (is_data) or particle swarm from a fileio.Dataset."""
+        self._check(cfuncs()["lagrbf_fill_swarm_from_dataset"](self.ctx, bool(is_data), dataset.handle),
+                    "lagrbf_fill_swarm_from_dataset")
+        if is_data:
+            self.data_filled = True
+        else:
+            self.interp_filled = True
+
     # -- extensions --
     def advect_steps_c(self, nsteps, dt):
         self._check(cfuncs()["lagrbf_advect_steps"](self.ctx, int(nsteps), float(dt)), "lagrbf_advect_steps")

@@ -2,8 +2,10 @@
 // set-up, PU evaluation and the RK stage loop.  All state lives in HBM; the host
 // only keeps per-patch bookkeeping (P entries) and the swarm point ids.
 #include <hip/hip_runtime.h>
+#include <sys/stat.h>
 
 #include <algorithm>
+#include <cerrno>
 #include <cfloat>
 #include <cmath>
 #include <cstdarg>
@@ -14,7 +16,9 @@
 #include <vector>
 
 #include "../../include/lagrbf.h"
+#include "../../include/lagrbf_io.h"
 #include "ctx.hpp"
+#include "io.hpp"
 
 namespace lrb {
 
@@ -883,6 +887,14 @@ static int advect_steps(Ctx& c, int nsteps, double dt) {
   return update_removed(c);
 }
 
+// Particle swarm field by name: "fields", the velocity field, "weights" (SoA, leading dimension ld)
+static int particle_field(Ctx& c, const char* name, const double*& src, int& nfl, const char* fn) {
+  if (!name || !strcmp(name, "fields")) { src = c.fields.p; nfl = c.nf; return 0; }
+  if (!strcmp(name, kVelocity) || (c.advection && c.velocity_name == name)) { src = c.vel.p; nfl = c.g.dim; return 0; }
+  if (!strcmp(name, "weights")) { src = c.w.p; nfl = c.g.nen; return 0; }
+  return set_err(E_WRONG, fn, "unknown field '%s'", name);
+}
+
 // lagrbf_output: rows by pid - 1 (AssembleArray_Allreduce, interface_utils.c:188-263)
 static int output(Ctx& c, bool is_data, int n, double* coords, double* fields, int64_t* pidx, const char* fieldname) {
   const char* fn = "Interface_Output_Allreduce";
@@ -909,12 +921,7 @@ static int output(Ctx& c, bool is_data, int n, double* coords, double* fields, i
   const int Np = c.Np;
   const double* src = nullptr;
   int nfl = 0;
-  if (fields) {
-    if (!fieldname || !strcmp(fieldname, "fields")) { src = c.fields.p; nfl = c.nf; }
-    else if (!strcmp(fieldname, kVelocity) || (c.advection && c.velocity_name == fieldname)) { src = c.vel.p; nfl = dim; }
-    else if (!strcmp(fieldname, "weights")) { src = c.w.p; nfl = c.g.nen; }
-    else return set_err(E_WRONG, fn, "unknown field '%s'", fieldname);
-  }
+  if (fields) CHK(particle_field(c, fieldname, src, nfl, fn));
   std::vector<double> xs((size_t)dim * c.ld), fs((size_t)std::max(nfl, 1) * c.ld);
   std::vector<int32_t> al(std::max(Np, 1)), orig(std::max(Np, 1));
   if (Np > 0) {
@@ -936,6 +943,177 @@ static int output(Ctx& c, bool is_data, int n, double* coords, double* fields, i
     if (pidx) pidx[g] = pid;
   }
   return 0;
+}
+
+// Live particles in fill order: coordinates [n*dim] (when xs), rows of the field
+// src [n*nfl] (when src) and pids (when pid), all AoS on the host.
+static int gather_live(Ctx& c, const double* src, int nfl, std::vector<double>* xs, std::vector<double>* fs,
+                       std::vector<int64_t>* pid) {
+  const int dim = c.g.dim, Np = c.Np;
+  HIPCHK(hipStreamSynchronize(c.s));
+  std::vector<double> hx, hf;
+  std::vector<int32_t> al(std::max(Np, 1)), orig(std::max(Np, 1));
+  if (Np > 0) {
+    if (xs) { hx.resize((size_t)dim * c.ld); HIPCHK(hipMemcpy(hx.data(), c.px.p, sizeof(double) * hx.size(), hipMemcpyDeviceToHost)); }
+    if (src && nfl > 0) { hf.resize((size_t)nfl * c.ld); HIPCHK(hipMemcpy(hf.data(), src, sizeof(double) * hf.size(), hipMemcpyDeviceToHost)); }
+    HIPCHK(hipMemcpy(al.data(), c.alive.p, sizeof(int32_t) * Np, hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(orig.data(), c.porig.p, sizeof(int32_t) * Np, hipMemcpyDeviceToHost));
+  }
+  std::vector<int32_t> slot(std::max(Np, 1), -1);
+  for (int p = 0; p < Np; p++)
+    if (al[p]) slot[orig[p]] = p;
+  if (xs) xs->clear();
+  if (fs) fs->clear();
+  if (pid) pid->clear();
+  for (int i = 0; i < Np; i++) {
+    const int p = slot[i];
+    if (p < 0) continue;
+    if (xs) for (int d = 0; d < dim; d++) xs->push_back(hx[(size_t)d * c.ld + p]);
+    if (fs) for (int f = 0; f < nfl; f++) fs->push_back(hf[(size_t)f * c.ld + p]);
+    if (pid) pid->push_back(c.h_ppid[i]);
+  }
+  return 0;
+}
+
+static int make_dirs(const std::string& path, const char* fn) {
+  std::string cur;
+  size_t b = 0;
+  while (b <= path.size()) {
+    size_t e = path.find('/', b);
+    if (e == std::string::npos) e = path.size();
+    cur = path.substr(0, e);
+    if (!cur.empty()) {
+      struct stat st;
+      if (stat(cur.c_str(), &st) != 0) {
+        if (mkdir(cur.c_str(), 0755) != 0 && errno != EEXIST)
+          return set_err(io::E_FILE_OPEN, fn, "cannot create directory %s: %s", cur.c_str(), strerror(errno));
+      } else if (!S_ISDIR(st.st_mode)) {
+        return set_err(io::E_FILE_OPEN, fn, "%s exists and is not a directory", cur.c_str());
+      }
+    }
+    b = e + 1;
+  }
+  return 0;
+}
+
+// "step%06d", with "_forward" / "_backward" for direction 1 / 2 (swarm_output.c:196-197)
+static std::string step_prefix(int64_t step, int direction) {
+  char sp[64];
+  snprintf(sp, sizeof(sp), "step%06lld", (long long)step);
+  std::string s = sp;
+  if (direction == 1) s += "_forward";
+  if (direction == 2) s += "_backward";
+  return s;
+}
+
+static const char* kCoorField = "DMSwarmPIC_coor";
+static const char* kPidField = "DMSwarm_pid";
+static const char* kInterpName = "swarm_interp";
+
+// SwarmOutput, swarm_output.c:169-234: <root>/step<k>/<prefix>.swarm_interp_<field>.pbvec
+// (coordinates first, then each requested field; "DMSwarm_pid" as an IS .pbis),
+// plus an entry of <root>/timeseries_X.pvd.  Rows are the live particles in fill
+// order.  permute reproduces SwarmOutput_VecCreate's reordering
+// (swarm_output.c:6-43), which reads the row-major field as if it were stored
+// component-major.
+static int swarm_output(Ctx& c, const char* root, int64_t step, double time, int nfields, const char** fields,
+                        int direction, bool permute) {
+  const char* fn = "SwarmOutput";
+  if (!root) return set_err(E_NULL, fn, "output directory is NULL");
+  if (nfields < 0 || (nfields > 0 && !fields)) return set_err(E_NULL, fn, "field list is NULL");
+  if (direction < 0 || direction > 2) return set_err(E_RANGE, fn, "direction must be 0 (none), 1 (forward) or 2 (backward)");
+  const std::string stepdir = "step" + std::to_string((long long)step);
+  const std::string outdir = std::string(root) + "/" + stepdir;
+  CHK(make_dirs(outdir, fn));
+  char sp[64];
+  snprintf(sp, sizeof(sp), "step%06lld", (long long)step);
+  const std::string pvd = std::string(root) + "/timeseries_X.pvd";
+  const std::string entry = std::string(sp) + "_coords.pbvec";
+  CHK(io::pvd_open_append(c.pvd_written, step, pvd.c_str(), time, entry.c_str(), stepdir.c_str()));
+  c.pvd_written = true;
+  const std::string prefix = step_prefix(step, direction);
+  auto fname = [&](const char* field, const char* ext) {
+    return outdir + "/" + prefix + "." + kInterpName + "_" + field + ext;
+  };
+  const int dim = c.g.dim;
+  std::vector<double> xs, fs;
+  std::vector<int64_t> pid;
+  CHK(gather_live(c, nullptr, 0, &xs, nullptr, &pid));
+  const int64_t n = (int64_t)pid.size();
+  CHK(io::vec_write(fname(kCoorField, ".pbvec").c_str(), n * dim, xs.data(), dim, 4));
+  for (int f = 0; f < nfields; f++) {
+    const char* name = fields[f] ? fields[f] : "";
+    if (!strcmp(name, kPidField)) {
+      CHK(io::is_write(fname(name, ".pbis").c_str(), n, pid.data(), 4));
+      continue;
+    }
+    const double* src = nullptr;
+    int nfl = 0;
+    CHK(particle_field(c, name, src, nfl, fn));
+    CHK(gather_live(c, src, nfl, nullptr, &fs, nullptr));
+    if (permute && nfl > 1) {
+      std::vector<double> pv(fs.size());
+      for (int64_t p = 0; p < n; p++)
+        for (int k = 0; k < nfl; k++) pv[(size_t)(p * nfl + k)] = fs[(size_t)(k * n + p)];
+      fs.swap(pv);
+    }
+    CHK(io::vec_write(fname(name, ".pbvec").c_str(), n * nfl, fs.data(), nfl, 4));
+  }
+  return 0;
+}
+
+// SwarmLoadFromFiles, swarm_load.c:133-201, into the particle swarm: coordinates
+// from <dir>/<prefix>.swarm_interp_DMSwarmPIC_coor.pbvec, then "fields" (.pbvec)
+// and "DMSwarm_pid" (.pbis) when listed; the swarm is refilled with them.
+static int swarm_load(Ctx& c, const char* dirname, int64_t step, int direction, int nfields, const char** fields) {
+  const char* fn = "SwarmLoadFromFiles";
+  if (!dirname) return set_err(E_NULL, fn, "directory is NULL");
+  if (nfields < 0 || (nfields > 0 && !fields)) return set_err(E_NULL, fn, "field list is NULL");
+  if (direction < 0 || direction > 2) return set_err(E_RANGE, fn, "direction must be 0 (none), 1 (forward) or 2 (backward)");
+  const std::string base = std::string(dirname) + "/" + step_prefix(step, direction) + "." + kInterpName + "_";
+  const int dim = c.g.dim;
+  std::vector<double> xs, vals;
+  std::vector<int64_t> pid;
+  int bs = 1;
+  const std::string cf = base + kCoorField + ".pbvec";
+  CHK(io::vec_read(cf.c_str(), 4, xs, bs));
+  if (bs != dim) return set_err(E_SIZ, fn, "%s: block size %d differs from the dimension %d", cf.c_str(), bs, dim);
+  const int64_t np = (int64_t)xs.size() / dim;
+  if (np > INT32_MAX) return set_err(E_RANGE, fn, "%lld points exceed the 32-bit swarm size", (long long)np);
+  bool have_vals = false, have_pid = false;
+  for (int f = 0; f < nfields; f++) {
+    const char* name = fields[f] ? fields[f] : "";
+    if (!strcmp(name, kPidField)) {
+      const std::string pf = base + name + ".pbis";
+      CHK(io::is_read(pf.c_str(), 4, pid));
+      if ((int64_t)pid.size() != np)
+        return set_err(E_SIZ, fn, "Loaded point index size %zu does not match swarm size %lld", pid.size(), (long long)np);
+      have_pid = true;
+    } else if (!strcmp(name, "fields")) {
+      const std::string vf = base + name + ".pbvec";
+      CHK(io::vec_read(vf.c_str(), 4, vals, bs));
+      if ((int64_t)vals.size() / bs != np)
+        return set_err(E_SIZ, fn, "Loaded vector size %lld does not match swarm size %lld", (long long)(vals.size() / bs),
+                       (long long)np);
+      if (bs != c.nf) return set_err(E_SIZ, fn, "%s: block size %d differs from n_interpolation_fields %d", vf.c_str(), bs, c.nf);
+      have_vals = true;
+    } else {
+      return set_err(E_SUP, fn, "loading field '%s' is not supported (coordinates, \"fields\" and \"%s\" are)", name, kPidField);
+    }
+  }
+  return fill_points(c, (int)np, xs.data(), have_vals ? vals.data() : nullptr, have_pid ? pid.data() : nullptr);
+}
+
+// fill a swarm from a loaded dataset (StagYY / PETSc binary ingest)
+static int fill_from_dataset(Ctx& c, bool is_data, const io::Dataset& ds) {
+  const char* fn = is_data ? "FillDataSwarm" : "FillInterpSwarm";
+  if (ds.dim != c.g.dim) return set_err(E_SIZ, fn, "dataset dimension %d differs from the context's %d", ds.dim, c.g.dim);
+  if (ds.nfields != c.nf)
+    return set_err(E_SIZ, fn, "dataset has %d fields, n_interpolation_fields is %d", ds.nfields, c.nf);
+  if (ds.n() > INT32_MAX) return set_err(E_RANGE, fn, "%lld points exceed the 32-bit swarm size", (long long)ds.n());
+  const int n = (int)ds.n();
+  const double* v = ds.fields.empty() ? nullptr : ds.fields.data();
+  return is_data ? fill_data(c, n, ds.coords.data(), v, nullptr) : fill_points(c, n, ds.coords.data(), v, nullptr);
 }
 
 Ctx* create_from_view(const OptView& o) {
@@ -1134,6 +1312,31 @@ int64_t lagrbf_debug_get(void* ctx, const char* what, int32_t index, void* out, 
     } else if (hipMemcpy(out, src, (size_t)(n * esz), hipMemcpyDeviceToHost) != hipSuccess) return -E_LIB;
   }
   return n;
+}
+
+int lagrbf_swarm_output(void* ctx, const char* root, int64_t step, double time, int32_t nfields, const char** fields,
+                        int32_t direction, bool permute) {
+  CTX_OR_NULL("SwarmOutput");
+  return swarm_output(*c, root, step, time, nfields, fields, direction, permute);
+}
+
+int lagrbf_swarm_load(void* ctx, const char* dirname, int64_t step, int32_t direction, int32_t nfields,
+                      const char** fields) {
+  CTX_OR_NULL("SwarmLoadFromFiles");
+  return swarm_load(*c, dirname, step, direction, nfields, fields);
+}
+
+int lagrbf_load_data(void* ctx, int64_t step) {
+  CTX_OR_NULL("SwarmDataLoad");
+  io::Dataset ds;
+  CHK(io::load_data_from_options(current_options(), step, c->g.dim, c->nf, ds));
+  return fill_from_dataset(*c, true, ds);
+}
+
+int lagrbf_fill_swarm_from_dataset(void* ctx, bool is_data, void* dataset) {
+  CTX_OR_NULL(is_data ? "FillDataSwarm" : "FillInterpSwarm");
+  if (!dataset) return set_err(E_NULL, "lagrbf_fill_swarm_from_dataset", "dataset is NULL");
+  return fill_from_dataset(*c, is_data, *static_cast<io::Dataset*>(dataset));
 }
 
 int lagrbf_last_error(char* buf, int32_t len) {

@@ -75,6 +75,7 @@ struct Ctx {
   double dopri_fac = 0.9, dopri_fac_min = 1e-2, dopri_fac_max = 1e2, dopri_atol = 1e-3, dopri_rtol = 1e-3;
   double dopri_err = 0.0;
   std::string velocity_name = "velocity";
+  bool pvd_written = false;                // this context has written its PVD collection
   hipStream_t s = nullptr;
   int device = 0;
 

@@ -1,8 +1,9 @@
 """C ABI of liblagrbf.so without a GPU: the library loads, exports every symbol
-include/lagrbf.h declares (and the Python mirror binds), and its host-only parts
+include/*.h declares (and the Python mirror binds), and its host-only parts
 (options channel, option validation, error reporting) behave as the reference's
 (PETSc error codes, messages kept for lagrbf_last_error).  No compute call."""
 import ctypes
+import glob
 import os
 import re
 import subprocess
@@ -10,7 +11,7 @@ import subprocess
 import pytest
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-HEADER = os.path.join(ROOT, "include", "lagrbf.h")
+HEADERS = sorted(glob.glob(os.path.join(ROOT, "include", "*.h")))
 LIB = os.path.join(ROOT, "lagrbf_amd", "liblagrbf.so")
 
 # symbols the reference's Python layer binds (register_cfunc.py:58-195)
@@ -22,9 +23,11 @@ REFERENCE_SYMBOLS = {
 
 
 def header_symbols():
-    text = open(HEADER).read()
-    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    return set(re.findall(r"\b(lagrbf_\w+)\s*\(", text))
+    syms = set()
+    for h in HEADERS:
+        text = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
+        syms |= set(re.findall(r"\b(lagrbf_\w+)\s*\(", text))
+    return syms
 
 
 @pytest.fixture(scope="module")
