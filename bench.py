@@ -148,6 +148,26 @@ def run_path(L, precision, opts, X, V, P, steps, warmup, dt, sample):
     return rec, coefs, pos
 
 
+def run_graph(L, precision, opts, X, V, P, steps, warmup, dt):
+    """The same timed steps with every step after each call's first replayed from a
+    captured HIP graph (option lagrbf_graph): steps/s and the graph share."""
+    ctx = L.LagRBFCtx(petsc_options=dict(opts, lagrbf_graph=True), precision=precision)
+    ctx.fill_data(X, V)
+    ctx.fill_points(P)
+    ctx.interpolate_c()
+    ctx.advect_steps_c(warmup, dt)
+    g0 = ctx.info("graph_steps")
+    t1 = time.perf_counter()
+    ctx.advect_steps_c(steps, dt)
+    wall = time.perf_counter() - t1
+    rec = {"value": P.shape[0] * steps / wall, "ms_per_step": wall / max(steps, 1) * 1e3,
+           "graph_steps": int(ctx.info("graph_steps") - g0), "steps": steps}
+    log(f"[{precision}] graph replay: {steps} steps {wall:.3f}s ({rec['ms_per_step']:.2f} ms/step), "
+        f"{rec['graph_steps']} replayed")
+    ctx.destroy()
+    return rec
+
+
 def cpu_baseline(opts, X, V, P, steps, dt, nsample, threads):
     """The oracle (C restatement of the reference, OpenMP) on a bounded sample:
     the first nsample particles of the workload inside [0.25,0.5]^3, same nodes,
@@ -290,6 +310,7 @@ def main():
     ap.add_argument("--serve-only", action="store_true")
     ap.add_argument("--c3-only", action="store_true", help="the uneven-batching configuration only")
     ap.add_argument("--no-c3", action="store_true", help="skip the uneven-batching measurement")
+    ap.add_argument("--no-graph", action="store_true", help="skip the graph-replay measurement of the fixed-step loop")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="launch a plain copy of MIB MiB first (scale for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -322,6 +343,8 @@ def main():
     res, coefs, pos = {}, {}, {}
     for prec in paths:
         res[prec], coefs[prec], pos[prec] = run_path(L, prec, opts, X, V, P, args.steps, args.warmup, dt, sample)
+        if not args.no_graph:
+            res[prec]["graph"] = run_graph(L, prec, opts, X, V, P, args.steps, args.warmup, dt)
     main_path = paths[0]
     r = res[main_path]
     cmp = None

@@ -92,13 +92,18 @@ int lagrbf_options_insert_file(const char *path);
 
 /* Fixed-step form of the internal advection loop: nsteps steps of size dt with
  * the per-step sequence of time_loop.c:80-90, no CFL, no end-of-call reset.
- * Used for fixed-dt workloads and the throughput benchmark. */
+ * Used for fixed-dt workloads and the throughput benchmark.  With the option
+ * lagrbf_graph, steps after the call's first one are captured as HIP graphs
+ * (one per particle-buffer parity) and replayed, when the step's launches are
+ * fixed: RK1/2/4 with interpolated velocity, every patch set up and solved. */
 int lagrbf_advect_steps(void *ctx, int32_t nsteps, double dt);
 
 /* Runtime counters: "removed" (particles that left the domain), "fallback"
  * (patches refactored in FP64 after FP32 refinement stalled), "refine_iters",
- * "patches_setup", "dt" (current advection dt), "eval_ms", "setup_ms",
- * "n_particles", "n_nodes", "max_patch", "eps_w", "eps_r". */
+ * "patches_setup", "dt" (current advection dt), "eval_ms" and "eval_count"
+ * (evaluation kernel time and launches timed, outside graphs), "graph_ms" and
+ * "graph_steps" (replayed steps and their time), "setup_ms", "n_particles",
+ * "n_nodes", "max_patch", "eps_w", "eps_r". */
 int lagrbf_get_info(void *ctx, const char *key, double *value);
 
 /* Introspection for tests: copies device state into `out` (capacity `cap`

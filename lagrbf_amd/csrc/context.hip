@@ -138,6 +138,8 @@ static int ctx_from_options(Ctx& c, const OptView& o) {
   c.chol_big_min = std::max(0, std::min(c.chol_big_min, chol_max_n()));
   o.get_bool("lagrbf_reorder", c.reorder, bad);
   if (bad) return set_err(E_WRONG, fn, "malformed -lagrbf_reorder option");
+  o.get_bool("lagrbf_graph", c.use_graph, bad);
+  if (bad) return set_err(E_WRONG, fn, "malformed -lagrbf_graph option");
   return 0;
 }
 
@@ -545,6 +547,38 @@ int ensure_blk(Ctx& c, bool f32) {
   return 0;
 }
 
+// Evaluation timing without a host wait per launch: event pairs are recorded
+// around each evaluation and resolved at the next synchronisation point.
+static int event_take(Ctx& c, hipEvent_t& e) {
+  if (!c.ev_pool.empty()) {
+    e = c.ev_pool.back();
+    c.ev_pool.pop_back();
+    return 0;
+  }
+  HIPCHK(hipEventCreate(&e));
+  return 0;
+}
+
+static int flush_eval_timing(Ctx& c) {
+  for (auto& pr : c.ev_pending) {
+    HIPCHK(hipEventSynchronize(pr.second));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, pr.first, pr.second));
+    c.eval_ms += ms;
+    c.eval_count += 1;
+    c.ev_pool.push_back(pr.first);
+    c.ev_pool.push_back(pr.second);
+  }
+  c.ev_pending.clear();
+  return 0;
+}
+
+static void release_events(Ctx& c) {
+  (void)flush_eval_timing(c);
+  for (hipEvent_t e : c.ev_pool) (void)hipEventDestroy(e);
+  c.ev_pool.clear();
+}
+
 static int rbf_interpolation(Ctx& c, double* out, int nfields) {
   if (nfields <= 0 || c.Np == 0) return 0;
   PatchView pv{c.p_n.p, c.p_active.p, c.p_mem_off.p, c.mem_pool.p, c.coef_pool.p, c.nf};
@@ -565,9 +599,16 @@ static int rbf_interpolation(Ctx& c, double* out, int nfields) {
   const bool use_blk = cell_order && eval_blk_supported(c.nf);
   const bool blk32 = c.precision && eval_f32_supported(c.ktype);
   if (use_blk) CHK(ensure_blk(c, blk32));
-  hipEvent_t e0, e1;
-  HIPCHK(hipEventCreate(&e0)); HIPCHK(hipEventCreate(&e1));
-  HIPCHK(hipEventRecord(e0, c.s));
+  // timing: the cell-table evaluation dispatch records its own start and end
+  // (once its launch shape is chosen); the other paths are bracketed by stream
+  // events and resolved at once
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  const bool timed_dispatch = use_blk && c.tuner.choice.count(density_bucket(c.Np, c.g.C)) != 0;
+  if (!c.capturing && (timed_dispatch || !use_blk)) {
+    CHK(event_take(c, e0));
+    CHK(event_take(c, e1));
+    if (!use_blk) HIPCHK(hipEventRecord(e0, c.s));
+  }
   if (use_blk) {
     const int per_cell = (c.Np + c.g.C - 1) / std::max(c.g.C, 1);
     const int block = std::min(256, std::max(64, (per_cell + 63) / 64 * 64));
@@ -583,7 +624,9 @@ static int rbf_interpolation(Ctx& c, double* out, int nfields) {
                       blk32 ? (const void*)c.blk32.p : (const void*)c.blk64.p, c.nf, c.cell_start.p, nfields, out,
                       block, c.Np, listed ? c.cell_list.p : nullptr, listed ? c.list_count.p : nullptr, group, c.s);
     };
+    set_eval_timing_events(e0, e1);
     c.tuner.run(bucket, c.nf == c.g.dim && nfields == c.nf, c.s, launch);
+    set_eval_timing_events(nullptr, nullptr);
   } else if (cell_order) {
     const int per_cell = (c.Np + c.g.C - 1) / std::max(c.g.C, 1);
     const int block = std::min(256, std::max(64, (per_cell + 63) / 64 * 64));
@@ -599,13 +642,11 @@ static int rbf_interpolation(Ctx& c, double* out, int nfields) {
     launch_eval_ref(c.g, c.ktype, c.px.p, c.ld, c.Np, c.pcell.p, c.alive.p, c.w.p, pv, c.dx.p ? c.dx.p : c.pux.p,
                     c.cell_start.p, nfields, out, c.s);
   }
-  HIPCHK(hipEventRecord(e1, c.s));
-  HIPCHK(hipEventSynchronize(e1));
-  float ms = 0.f;
-  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-  c.eval_ms += ms;
-  c.eval_count += 1;
-  HIPCHK(hipEventDestroy(e0)); HIPCHK(hipEventDestroy(e1));
+  if (e0) {
+    if (!use_blk) HIPCHK(hipEventRecord(e1, c.s));
+    c.ev_pending.emplace_back(e0, e1);
+    if (!use_blk || c.ev_pending.size() >= 256) CHK(flush_eval_timing(c));
+  }
   return 0;
 }
 
@@ -696,6 +737,15 @@ static void stage(Ctx& c, const double* x0, const StageArgs& a) {
 }
 
 
+// The reordered swarm becomes the current one (the other set is the next target).
+static void swap_particle_buffers(Ctx& c) {
+  swap_buf(c.px, c.px_alt); swap_buf(c.vel, c.vel_alt); swap_buf(c.fields, c.fields_alt); swap_buf(c.w, c.w_alt);
+  swap_buf(c.pcell, c.pcell_alt); swap_buf(c.alive, c.alive_alt); swap_buf(c.porig, c.porig_alt);
+  c.parity ^= 1;
+}
+
+static bool setup_complete(const Ctx& c);
+
 // Store the swarm in the cell order of the last sort (alive particles grouped by
 // cell, dead ones at the tail): the evaluation's particle reads and writes become
 // contiguous.  Positions, the step's start velocity, interpolated fields, weights,
@@ -723,8 +773,7 @@ static int reorder_particles(Ctx& c) {
   r.isrc[2] = c.porig.p; r.idst[2] = c.porig_alt.p;
   r.ni = 3;
   launch_reorder(r, c.perm.p, c.Np, c.ld, c.s);
-  swap_buf(c.px, c.px_alt); swap_buf(c.vel, c.vel_alt); swap_buf(c.fields, c.fields_alt); swap_buf(c.w, c.w_alt);
-  swap_buf(c.pcell, c.pcell_alt); swap_buf(c.alive, c.alive_alt); swap_buf(c.porig, c.porig_alt);
+  swap_particle_buffers(c);
   // the cell groups now sit at pstart[c] .. pstart[c+1] in slot order
   launch_iota(c.perm.p, c.Np, c.s);
   return 0;
@@ -827,7 +876,7 @@ static int advect_interpolate(Ctx& c, double t, double t_next) {
     c.adv_dt = dt;
     CHK(advect_swarm(c, dt));
     if (!c.analytical) {
-      CHK(partitions_setup(c));
+      if (!setup_complete(c)) CHK(partitions_setup(c));
       CHK(rbf_interpolation(c, c.fields.p, c.nf));
     }
     CHK(init_step_velocity(c));
@@ -868,6 +917,67 @@ static int advect_analytical(Ctx& c, double t, double t_next) {
   return update_removed(c);
 }
 
+// Every patch set up and solved: a lazy set-up pass can change nothing (whatever
+// cells the particles touch), so the per-step pass and its host wait are skipped.
+static bool setup_complete(const Ctx& c) {
+  if (c.factors_released) return true;
+  for (int p = 0; p < c.g.P; p++)
+    if (!c.h_setup[p] || !c.h_solved[p]) return false;
+  return true;
+}
+
+static void destroy_graphs(Ctx& c) {
+  for (auto& g : c.gexec)
+    if (g) {
+      (void)hipGraphExecDestroy(g);
+      g = nullptr;
+    }
+}
+
+// One fixed-dt step: RK stages with the velocity interpolated at each stage point,
+// the lazy set-up, the fields at the new positions and the next step's velocity
+// (InternalForwardAdvectionLoop body, time_loop.c:67-90, at a fixed dt).
+static int fixed_step(Ctx& c, double dt) {
+  c.adv_dt = dt;
+  CHK(advect_swarm(c, dt));
+  if (!c.analytical) {
+    if (!setup_complete(c)) CHK(partitions_setup(c));
+    CHK(rbf_interpolation(c, c.fields.p, c.nf));
+  }
+  return init_step_velocity(c);
+}
+
+// A step can be captured once its launches are fixed: RK1/2/4 with interpolated
+// velocities through the cell-table evaluation, every patch ready, the evaluation
+// shape chosen, and the swarm stored in cell order (so every step swaps the
+// particle buffers and two graphs, one per parity, cover the loop).
+static bool graph_eligible(const Ctx& c) {
+  if (!c.use_graph || c.analytical || c.scheme > 2 || !c.reorder || c.Np == 0) return false;
+  if (c.eval_mode == 2 || (c.eval_mode == 0 && !c.memb_current) || !eval_blk_supported(c.nf) || !c.blk_valid) return false;
+  if (!c.sorted_valid || !c.mem_pool.p) return false;
+  if (c.tuner.choice.find(density_bucket(c.Np, c.g.C)) == c.tuner.choice.end()) return false;
+  return setup_complete(c);
+}
+
+static int capture_step(Ctx& c, double dt, hipGraphExec_t& exec) {
+  const char* fn = "lagrbf_advect_steps";
+  HIPCHK(hipStreamBeginCapture(c.s, hipStreamCaptureModeThreadLocal));
+  c.capturing = true;
+  const int r = fixed_step(c, dt);
+  c.capturing = false;
+  hipGraph_t graph = nullptr;
+  const hipError_t e = hipStreamEndCapture(c.s, &graph);
+  if (r) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return r;
+  }
+  if (e != hipSuccess || !graph) return set_err(E_LIB, fn, "graph capture of the step failed: %s", hipGetErrorString(e));
+  const hipError_t ei = hipGraphInstantiateWithFlags(&exec, graph, 0);
+  (void)hipGraphDestroy(graph);
+  if (ei != hipSuccess) return set_err(E_LIB, fn, "graph instantiation failed: %s", hipGetErrorString(ei));
+  return 0;
+}
+
 static int advect_steps(Ctx& c, int nsteps, double dt) {
   const char* fn = "lagrbf_advect_steps";
   if (!c.advection) return set_err(E_NULL, fn, "Advection context is NULL (set -advection)");
@@ -875,15 +985,52 @@ static int advect_steps(Ctx& c, int nsteps, double dt) {
   if (!c.analytical && c.nf < c.g.dim) return set_err(E_SIZ, fn, "n_interpolation_fields (%d) < dim (%d)", c.nf, c.g.dim);
   if (nsteps < 0) return set_err(E_RANGE, fn, "nsteps must be >= 0");
   CHK(init_step_velocity(c));
-  for (int s = 0; s < nsteps; s++) {
-    c.adv_dt = dt;
-    CHK(advect_swarm(c, dt));
-    if (!c.analytical) {
-      CHK(partitions_setup(c));
-      CHK(rbf_interpolation(c, c.fields.p, c.nf));
+  // graphs live for this call only: nothing outside it can move their buffers
+  hipEvent_t g0 = nullptr, g1 = nullptr;
+  int graph_run = 0;
+  int rc = 0;
+  for (int s = 0; s < nsteps && !rc; s++) {
+    // the call's first step runs eagerly: it allocates what the reorder and the
+    // evaluation need and fixes the evaluation shape before anything is captured
+    if (s == 0 || !graph_eligible(c)) {
+      rc = fixed_step(c, dt);
+      continue;
     }
-    CHK(init_step_velocity(c));
+    // the step's host bookkeeping runs while capturing; the graph then runs the work
+    hipGraphExec_t& ex = c.gexec[c.parity];
+    if (!ex) {
+      rc = capture_step(c, dt, ex);
+      if (rc) break;
+    } else {
+      // replay: on the host a step only swaps the particle buffers (the evaluation
+      // leaves the swarm grouped, as after the captured step)
+      swap_particle_buffers(c);
+      c.sorted_valid = true;
+      c.adv_dt = dt;
+    }
+    if (!g0) {
+      if ((rc = event_take(c, g0)) || (rc = event_take(c, g1))) break;
+      if (hipEventRecord(g0, c.s) != hipSuccess) { rc = set_err(E_LIB, fn, "hipEventRecord failed"); break; }
+    }
+    const hipError_t e = hipGraphLaunch(ex, c.s);
+    if (e != hipSuccess) { rc = set_err(E_LIB, fn, "hipGraphLaunch failed: %s", hipGetErrorString(e)); break; }
+    graph_run++;
   }
+  if (g0) {
+    if (!rc && hipEventRecord(g1, c.s) == hipSuccess && hipEventSynchronize(g1) == hipSuccess) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, g0, g1) == hipSuccess) {
+        c.graph_ms += ms;
+        c.graph_steps += graph_run;
+      }
+    }
+    c.ev_pool.push_back(g0);
+    c.ev_pool.push_back(g1);
+  }
+  HIPCHK(hipStreamSynchronize(c.s));
+  destroy_graphs(c);
+  CHK(rc);
+  CHK(flush_eval_timing(c));
   return update_removed(c);
 }
 
@@ -1156,6 +1303,8 @@ void lagrbf_destroy(void* ctx) {
   if (!c) return;
   (void)hipSetDevice(c->device);
   (void)hipStreamSynchronize(c->s);
+  destroy_graphs(*c);
+  release_events(*c);
   (void)hipStreamDestroy(c->s);
   delete c;
 }
@@ -1261,8 +1410,12 @@ int lagrbf_get_info(void* ctx, const char* key, double* value) {
   else if (k == "patches_setup") *value = c->patches_setup;
   else if (k == "dt") *value = c->adv_dt;
   else if (k == "dopri_err") *value = c->dopri_err;
-  else if (k == "eval_ms") *value = c->eval_ms;
-  else if (k == "eval_count") *value = c->eval_count;
+  else if (k == "eval_ms" || k == "eval_count") {
+    if (int e = flush_eval_timing(*c)) return e;
+    *value = k == "eval_ms" ? c->eval_ms : c->eval_count;
+  }
+  else if (k == "graph_steps") *value = c->graph_steps;
+  else if (k == "graph_ms") *value = c->graph_ms;
   else if (k == "setup_ms") *value = c->setup_ms;
   else if (k == "n_particles") *value = c->Np;
   else if (k == "n_nodes") *value = c->N;

@@ -142,6 +142,15 @@ struct Ctx {
   bool analytical = false;
   // stats
   double removed = 0, patches_setup = 0, setup_ms = 0, eval_ms = 0, eval_count = 0;
+  // evaluation timing: event pairs resolved at the next synchronisation point
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_pending;
+  std::vector<hipEvent_t> ev_pool;
+  // graph capture of the fixed-step loop (lagrbf_advect_steps): one executable
+  // graph per buffer parity (the reorder swaps the particle buffers every step)
+  bool use_graph = false, capturing = false;
+  int parity = 0;
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  double graph_dt = 0.0, graph_steps = 0, graph_ms = 0;
   int max_patch = 0;
 };
 

@@ -25,9 +25,29 @@
 // rounding), FP32 coefficients, FP64 Shepard combination.
 #include "kernels.hpp"
 
+#include <hip/hip_ext.h>
+
 #include <algorithm>
 
 namespace lrb {
+
+// Timing of the evaluation launches: when set, the next evaluation dispatch
+// records its own start and end in these events (kernel-accurate timestamps that
+// need no host wait); unset inside graph capture.
+static thread_local hipEvent_t t_eval_ev0 = nullptr, t_eval_ev1 = nullptr;
+
+void set_eval_timing_events(hipEvent_t start, hipEvent_t stop) {
+  t_eval_ev0 = start;
+  t_eval_ev1 = stop;
+}
+
+#define LRB_LAUNCH_TIMED(KERNEL, GRID, BLOCK, S, ...)                                        \
+  do {                                                                                        \
+    if (t_eval_ev0)                                                                           \
+      hipExtLaunchKernelGGL(KERNEL, GRID, BLOCK, 0, S, t_eval_ev0, t_eval_ev1, 0, __VA_ARGS__); \
+    else                                                                                      \
+      hipLaunchKernelGGL(KERNEL, GRID, BLOCK, 0, S, __VA_ARGS__);                             \
+  } while (0)
 
 // Q1 corner index of offset (a, b, c) in {0,1}^3 (DMDAGetElements order)
 __host__ __device__ constexpr int q1_index(int a, int b, int c) { return (b ? (a ? 2 : 3) : (a ? 1 : 0)) + 4 * c; }
@@ -413,7 +433,7 @@ static void launch_sparse_g(int G, const Grid& g, const double* px, int64_t ld, 
   const int ncell = std::min(g.C, np);
   const int waves = (ncell + 64 / G - 1) / (64 / G);
   const dim3 grid((waves + 3) / 4), blkd(256);
-#define LRB_SP(GG) hipLaunchKernelGGL((k_eval_sparse<DIM, KT, NFT, T, GG>), grid, blkd, 0, s, g, px, ld, perm, pstart, w, blk, cs, nfields, out, cell_list, list_count)
+#define LRB_SP(GG) LRB_LAUNCH_TIMED((k_eval_sparse<DIM, KT, NFT, T, GG>), grid, blkd, s, g, px, ld, perm, pstart, w, blk, cs, nfields, out, cell_list, list_count)
   if (G == 8) LRB_SP(8);
   else if (G == 16) LRB_SP(16);
   else LRB_SP(32);
@@ -435,7 +455,7 @@ __global__ void k_nonempty_cells(const int32_t* __restrict__ pstart, int C, int3
 }
 
 void launch_nonempty_cells(const int32_t* pstart, int C, int32_t* list, int32_t* count, hipStream_t s) {
-  (void)hipMemsetAsync(count, 0, sizeof(int32_t), s);
+  launch_fill_i32(count, 1, 0, s);   // a kernel, not a memset: the sequence may be captured in a graph
   if (C > 0) hipLaunchKernelGGL(k_nonempty_cells, dim3((C + 255) / 256), dim3(256), 0, s, pstart, C, list, count);
 }
 
@@ -468,7 +488,7 @@ static void eval_blk_k(const Grid& g, const double* px, int64_t ld, const int32_
   // dense: one block per cell; listed cells: one block per listed non-empty cell
   const int nblk = cell_list ? std::min(g.C, np) : (g.C + 7) / 8 * 8;
   const dim3 grid(nblk), blkd(block);
-#define LRB_EB(NFT) hipLaunchKernelGGL((k_eval_blk<DIM, KT, NFT, T>), grid, blkd, 0, s, g, px, ld, perm, pstart, w, blk, cs, nfields, out, cell_list, list_count)
+#define LRB_EB(NFT) LRB_LAUNCH_TIMED((k_eval_blk<DIM, KT, NFT, T>), grid, blkd, s, g, px, ld, perm, pstart, w, blk, cs, nfields, out, cell_list, list_count)
   switch (nft) {
     case 1: LRB_EB(1); break;
     case 2: LRB_EB(2); break;

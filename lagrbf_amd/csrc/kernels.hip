@@ -665,15 +665,27 @@ __global__ void k_pscatter(const int32_t* __restrict__ cell, const int32_t* __re
   if (key >= 0) perm[base + lane - r.head] = i;
 }
 
+__global__ void k_zero_i32(int32_t* __restrict__ a, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = 0;
+}
+
+__global__ void k_copy_i32(const int32_t* __restrict__ src, int32_t* __restrict__ dst, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
+// Kernels only (no memset or copy nodes), so the sequence can sit in a captured graph.
 void launch_sort_particles(const int32_t* cell, const int32_t* alive, int Np, int C, int32_t* pstart, int32_t* perm,
                            int32_t* scratch, hipStream_t s) {
   // pstart: C + 2 entries (cells, dead bucket, end); scratch: C + 2 + scan blocks
   int32_t* cursor = scratch;
   int32_t* tmp = scratch + (C + 2);
-  hipMemsetAsync(pstart, 0, sizeof(int32_t) * (C + 2), s);
+  const int n = C + 2;
+  hipLaunchKernelGGL(k_zero_i32, dim3((n + 255) / 256), dim3(256), 0, s, pstart, n);
   if (Np > 0) hipLaunchKernelGGL(k_hist_alive, dim3((Np + 255) / 256), dim3(256), 0, s, cell, alive, Np, C, pstart);
   exclusive_scan(pstart, C + 2, tmp, s);
-  hipMemcpyAsync(cursor, pstart, sizeof(int32_t) * (C + 2), hipMemcpyDeviceToDevice, s);
+  hipLaunchKernelGGL(k_copy_i32, dim3((n + 255) / 256), dim3(256), 0, s, pstart, cursor, n);
   if (Np > 0) hipLaunchKernelGGL(k_pscatter, dim3((Np + 255) / 256), dim3(256), 0, s, cell, alive, Np, C, cursor, perm);
 }
 

@@ -107,6 +107,8 @@ void launch_build_blk(const Grid& g, bool fp32, const double* x, int N, const in
 // block per cell (group 0) or 64/group cells per wave in lane groups of
 // group = 8, 16, 32 (needs nft == dim); nullptr: one block per cell.  Every
 // variant performs the same arithmetic per particle.
+// Events the next launch_eval_blk dispatch records its start and end in (nullptr: none).
+void set_eval_timing_events(hipEvent_t start, hipEvent_t stop);
 void launch_eval_blk(const Grid& g, int ktype, bool fp32, const double* px, int64_t ld, const int32_t* perm,
                      const int32_t* pstart, const double* w, const void* blk, int nft, const int32_t* cs, int nfields,
                      double* out, int block, int np, const int32_t* cell_list, const int32_t* list_count, int group,

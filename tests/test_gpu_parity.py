@@ -428,3 +428,33 @@ def test_large_patch_cholesky(precision, big_min, tol):
     vg = np.zeros((P.shape[0], 3)); vo = np.zeros_like(vg)
     g.output(fvals=vg); o.output(fvals=vo)
     assert rel_err(vg, vo) < (max(TOL_VAL, 100 * cond * 1e-16) if precision == "fp64" else 1e-5)
+
+
+@pytest.mark.parametrize("scheme,precision", [(2, "fp64"), (2, "mixed"), (1, "fp64"), (0, "mixed")])
+def test_graph_steps_match_eager(scheme, precision):
+    """Fixed-step loop replayed from captured graphs (one per buffer parity) against
+    the same loop run eagerly: positions and fields bit-identical after every call,
+    with host reads and synchronisations between the calls' replays."""
+    opts, X, V, P = cases.fourier_3d(nlat=20, Np=20000, n=9, scheme=scheme)
+    print(f"case: graph replay fourier3d scheme={scheme} {precision}", flush=True)
+    ctxs = []
+    for graph in (False, True):
+        o = dict(opts, lagrbf_graph=graph)
+        g = L.LagRBFCtx(petsc_options=o, precision=precision)
+        g.fill_data(X, V)
+        g.fill_points(P)
+        g.interpolate_c()
+        ctxs.append(g)
+    for call in range(4):
+        for g in ctxs:
+            g.advect_steps_c(3, 2e-3)
+        out = []
+        for g in ctxs:
+            xs = np.zeros_like(P); vals = np.zeros((P.shape[0], 3))
+            g.output(xs, vals)
+            out.append((xs, vals, g.info("removed")))
+        assert np.array_equal(out[0][0], out[1][0]), f"positions differ after call {call}"
+        assert np.array_equal(out[0][1], out[1][1]), f"fields differ after call {call}"
+        assert out[0][2] == out[1][2]
+        assert ctxs[1].info("graph_steps") == 2 * (call + 1)
+        assert ctxs[0].info("graph_steps") == 0
