@@ -98,6 +98,17 @@ int lagrbf_options_insert_file(const char *path);
  * fixed: RK1/2/4 with interpolated velocity, every patch set up and solved. */
 int lagrbf_advect_steps(void *ctx, int32_t nsteps, double dt);
 
+/* Per-particle adaptive Dormand-Prince 5(4) from t_start to t_end (no reference
+ * counterpart; the reference's DOPRI uses one dt for the swarm).  Each particle
+ * has its own time and step size (initial -dt), accepts or rejects each attempt
+ * with the DOPRI error norm and controller (-dopri_atol, -dopri_rtol,
+ * -dopri_dt_fac*), and stops at t_end; at most max_attempts attempts.  With
+ * -lagrbf_time_blend (n_interpolation_fields = 2 x dim) the velocity at stage
+ * time t is (1 - tau) v_A + tau v_B, tau = (t - tA) / (tB - tA), v_A the first dim
+ * fields and v_B the next (-lagrbf_blend_times tA,tB, default 0,1).  Counters:
+ * "dp_attempts", "dp_accepted", "dp_rejected", "dp_active" (unfinished). */
+int lagrbf_advect_adaptive(void *ctx, double t_start, double t_end, int32_t max_attempts);
+
 /* Runtime counters: "removed" (particles that left the domain), "fallback"
  * (patches refactored in FP64 after FP32 refinement stalled), "refine_iters",
  * "patches_setup", "dt" (current advection dt), "eval_ms" and "eval_count"

@@ -169,6 +169,7 @@ def register_functions(clib):
             ("lagrbf_swarm_output", I32, [VP, CP, I64, PetscReal, I32, SP, I32, ctypes.c_bool]),
             ("lagrbf_swarm_load", I32, [VP, CP, I64, I32, I32, SP]),
             ("lagrbf_load_data", I32, [VP, I64]),
+            ("lagrbf_advect_adaptive", I32, [VP, PetscReal, PetscReal, I32]),
             ("lagrbf_fill_swarm_from_dataset", I32, [VP, ctypes.c_bool, VP])):
         c = getattr(clib, name)
         c.restype = res
@@ -203,5 +204,5 @@ EXPORTED_SYMBOLS = [
     "lagrbf_io_pvd_append", "lagrbf_io_pvd_open_append", "lagrbf_io_filename", "lagrbf_io_filelist",
     "lagrbf_io_stag_load", "lagrbf_io_petsc_load", "lagrbf_io_load_data", "lagrbf_io_dataset_info",
     "lagrbf_io_dataset_copy", "lagrbf_io_dataset_free", "lagrbf_swarm_output", "lagrbf_swarm_load",
-    "lagrbf_load_data", "lagrbf_fill_swarm_from_dataset",
+    "lagrbf_load_data", "lagrbf_fill_swarm_from_dataset", "lagrbf_advect_adaptive",
 ]

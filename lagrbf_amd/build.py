@@ -7,7 +7,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "liblagrbf.so")
-SOURCES = ["context.hip", "kernels.hip", "eval_cell.hip", "eval_blk.hip", "serve.hip", "patch_chol.hip", "options.cpp", "io.cpp"]
+SOURCES = ["context.hip", "kernels.hip", "eval_cell.hip", "eval_blk.hip", "serve.hip", "patch_chol.hip", "adaptive.hip", "options.cpp", "io.cpp"]
 ARCH = os.environ.get("LAGRBF_OFFLOAD_ARCH", "gfx950")
 
 

@@ -254,6 +254,16 @@ class LagRBFCtx:
             self.interp_filled = True
 
     # -- extensions --
+    def advect_adaptive_c(self, t_start, t_end, max_attempts=100000):
+        """Per-particle adaptive DP45 from t_start to t_end (lagrbf_advect_adaptive);
+        returns {attempts, accepted, rejected, active} of this call."""
+        before = {k: self.info("dp_" + k) for k in ("attempts", "accepted", "rejected")}
+        self._check(cfuncs()["lagrbf_advect_adaptive"](self.ctx, float(t_start), float(t_end), int(max_attempts)),
+                    "lagrbf_advect_adaptive")
+        out = {k: self.info("dp_" + k) - v for k, v in before.items()}
+        out["active"] = self.info("dp_active")
+        return out
+
     def advect_steps_c(self, nsteps, dt):
         self._check(cfuncs()["lagrbf_advect_steps"](self.ctx, int(nsteps), float(dt)), "lagrbf_advect_steps")
 

@@ -140,6 +140,18 @@ static int ctx_from_options(Ctx& c, const OptView& o) {
   if (bad) return set_err(E_WRONG, fn, "malformed -lagrbf_reorder option");
   o.get_bool("lagrbf_graph", c.use_graph, bad);
   if (bad) return set_err(E_WRONG, fn, "malformed -lagrbf_graph option");
+  o.get_bool("lagrbf_setup_all", c.setup_all, bad);
+  o.get_bool("lagrbf_time_blend", c.blend, bad);
+  std::vector<double> bt;
+  if (o.get_real_array("lagrbf_blend_times", bt, bad)) {
+    if (bt.size() != 2 || !(bt[1] != bt[0]))
+      return set_err(E_WRONG, fn, "-lagrbf_blend_times needs two distinct times tA,tB");
+    c.blend_t[0] = bt[0];
+    c.blend_t[1] = bt[1];
+  }
+  if (bad) return set_err(E_WRONG, fn, "malformed -lagrbf_setup_all / -lagrbf_time_blend / -lagrbf_blend_times option");
+  if (c.blend && c.nf != 2 * c.g.dim)
+    return set_err(E_WRONG, fn, "-lagrbf_time_blend needs n_interpolation_fields = 2 x dim (found %d)", c.nf);
   return 0;
 }
 
@@ -199,7 +211,7 @@ int fill_data(Ctx& c, int n, const double* coords, const double* values, const i
   const char* fn = "FillDataSwarm";
   const int dim = c.g.dim, nf = c.nf;
   CHK(check_bbox(c, n, coords, fn));
-  c.blk_valid = false;
+  c.blk_valid = false; c.table_gen++;
   c.tuner = EvalTuner();
   if (c.setup_N >= 0 && c.solve_method != 2 && n != c.setup_N && c.mem_used > 0)
     return set_err(E_SIZ, fn, "solve_method %d keeps patch factors across data fills; node count changed from %d to %d "
@@ -495,7 +507,10 @@ int partitions_setup(Ctx& c, bool all_patches) {
     if (a != c.h_active[p]) { c.h_active[p] = a; changed = true; }
   }
   if (changed) HIPCHK(hipMemcpyAsync(c.p_active.p, c.h_active.data(), P, hipMemcpyHostToDevice, c.s));
-  if (changed || !solvel.empty()) c.blk_valid = false;
+  if (changed || !solvel.empty()) {
+    c.blk_valid = false;
+    c.table_gen++;
+  }
   HIPCHK(hipEventRecord(e1, c.s));
   HIPCHK(hipStreamSynchronize(c.s));
   float ms = 0.f;
@@ -515,7 +530,7 @@ void release_factors(Ctx& c) {
 // LagRBF_PartitionsReset, partition.c:494-524
 static int partitions_reset(Ctx& c) {
   const int P = c.g.P;
-  c.blk_valid = false;
+  c.blk_valid = false; c.table_gen++;
   if (c.solve_method == 2) {
     std::fill(c.h_setup.begin(), c.h_setup.end(), 0);
     std::fill(c.h_solved.begin(), c.h_solved.end(), 0);
@@ -541,7 +556,7 @@ int ensure_blk(Ctx& c, bool f32) {
   else HIPCHK(c.blk64.reserve(n));
   if (c.N > 0)
     launch_build_blk(c.g, f32, c.dx.p, c.N, c.cell_nodes.p, c.dcell.p, c.cell_start.p, c.p_active.p, c.p_mem_off.p,
-                     c.coef_pool.p, c.nf, f32 ? (void*)c.blk32.p : (void*)c.blk64.p, c.s);
+                     c.coef_pool.p, c.nf, c.nf, 0, f32 ? (void*)c.blk32.p : (void*)c.blk64.p, c.s);
   c.blk_valid = true;
   c.blk_f32 = f32;
   return 0;
@@ -579,7 +594,25 @@ static void release_events(Ctx& c) {
   c.ev_pool.clear();
 }
 
-static int rbf_interpolation(Ctx& c, double* out, int nfields) {
+// Table of field group g (fields g*dim .. g*dim+dim-1): the two snapshots of a
+// time-blended velocity, each evaluated with the dim-field kernels.
+static int ensure_blk_group(Ctx& c, bool f32, int g) {
+  if (c.blkg_gen[g] == c.table_gen && c.blkg_f32[g] == f32) return 0;
+  const int dim = c.g.dim;
+  const size_t n = ((size_t)c.N + 1) * blk_row(dim, dim);
+  if (f32) HIPCHK(c.blkg32[g].reserve(n));
+  else HIPCHK(c.blkg64[g].reserve(n));
+  if (c.N > 0)
+    launch_build_blk(c.g, f32, c.dx.p, c.N, c.cell_nodes.p, c.dcell.p, c.cell_start.p, c.p_active.p, c.p_mem_off.p,
+                     c.coef_pool.p, dim, c.nf, g * dim, f32 ? (void*)c.blkg32[g].p : (void*)c.blkg64[g].p, c.s);
+  c.blkg_gen[g] = c.table_gen;
+  c.blkg_f32[g] = f32;
+  return 0;
+}
+
+// group < 0: fields 0 .. nfields-1 of the n_interpolation_fields table; group
+// 0 / 1: the dim fields of that snapshot (cell-table evaluation only).
+static int rbf_interpolation(Ctx& c, double* out, int nfields, int group = -1) {
   if (nfields <= 0 || c.Np == 0) return 0;
   PatchView pv{c.p_n.p, c.p_active.p, c.p_mem_off.p, c.mem_pool.p, c.coef_pool.p, c.nf};
   if (!c.mem_pool.p) {
@@ -596,9 +629,14 @@ static int rbf_interpolation(Ctx& c, double* out, int nfields) {
     c.sorted_valid = true;
   }
   if (!c.zeros.p) { HIPCHK(c.zeros.reserve(1)); HIPCHK(hipMemset(c.zeros.p, 0, sizeof(double))); }
-  const bool use_blk = cell_order && eval_blk_supported(c.nf);
+  const bool use_blk = cell_order && eval_blk_supported(group >= 0 ? c.g.dim : c.nf);
   const bool blk32 = c.precision && eval_f32_supported(c.ktype);
-  if (use_blk) CHK(ensure_blk(c, blk32));
+  if (group >= 0 && !use_blk)
+    return set_err(E_SUP, "RBFInterpolation", "snapshot fields need the cell-table evaluation (patches from the current data)");
+  if (use_blk) CHK(group >= 0 ? ensure_blk_group(c, blk32, group) : ensure_blk(c, blk32));
+  const int nft = group >= 0 ? c.g.dim : c.nf;
+  const void* table = group >= 0 ? (blk32 ? (const void*)c.blkg32[group].p : (const void*)c.blkg64[group].p)
+                                 : (blk32 ? (const void*)c.blk32.p : (const void*)c.blk64.p);
   // timing: the cell-table evaluation dispatch records its own start and end
   // (once its launch shape is chosen); the other paths are bracketed by stream
   // events and resolved at once
@@ -618,14 +656,14 @@ static int rbf_interpolation(Ctx& c, double* out, int nfields) {
       launch_nonempty_cells(c.pstart.p, c.g.C, c.cell_list.p, c.list_count.p, c.s);
     auto launch = [&](int v) {
       bool listed;
-      int group;
-      eval_variant(v, listed, group);
-      launch_eval_blk(c.g, c.ktype, blk32, c.px.p, c.ld, c.perm.p, c.pstart.p, c.w.p,
-                      blk32 ? (const void*)c.blk32.p : (const void*)c.blk64.p, c.nf, c.cell_start.p, nfields, out,
-                      block, c.Np, listed ? c.cell_list.p : nullptr, listed ? c.list_count.p : nullptr, group, c.s);
+      int lanes;
+      eval_variant(v, listed, lanes);
+      launch_eval_blk(c.g, c.ktype, blk32, c.px.p, c.ld, c.perm.p, c.pstart.p, c.w.p, table, nft, c.cell_start.p,
+                      nfields, out, block, c.Np, listed ? c.cell_list.p : nullptr, listed ? c.list_count.p : nullptr,
+                      lanes, c.s);
     };
     set_eval_timing_events(e0, e1);
-    c.tuner.run(bucket, c.nf == c.g.dim && nfields == c.nf, c.s, launch);
+    c.tuner.run(bucket, nft == c.g.dim && nfields == nft, c.s, launch);
     set_eval_timing_events(nullptr, nullptr);
   } else if (cell_order) {
     const int per_cell = (c.Np + c.g.C - 1) / std::max(c.g.C, 1);
@@ -653,7 +691,7 @@ static int rbf_interpolation(Ctx& c, double* out, int nfields) {
 static int interpolation(Ctx& c) {
   const char* fn = "Interface_Interpolation";
   if (!c.points_filled) return set_err(E_NULL, fn, "Interpolation swarm (ctx->swi) is empty: call lagrbf_fill_swarm first");
-  CHK(partitions_setup(c));
+  CHK(partitions_setup(c, c.setup_all));
   CHK(rbf_interpolation(c, c.fields.p, c.nf));
   return 0;
 }
@@ -876,7 +914,7 @@ static int advect_interpolate(Ctx& c, double t, double t_next) {
     c.adv_dt = dt;
     CHK(advect_swarm(c, dt));
     if (!c.analytical) {
-      if (!setup_complete(c)) CHK(partitions_setup(c));
+      if (!setup_complete(c)) CHK(partitions_setup(c, c.setup_all));
       CHK(rbf_interpolation(c, c.fields.p, c.nf));
     }
     CHK(init_step_velocity(c));
@@ -888,7 +926,7 @@ static int advect_interpolate(Ctx& c, double t, double t_next) {
     // CellInfoReset + DMSwarmSetLocalSizes(swarm_data,0,-1), time_loop.c:94-98
     c.N = 0;
     c.data_filled = false;
-    c.blk_valid = false;
+    c.blk_valid = false; c.table_gen++;
     HIPCHK(hipMemsetAsync(c.cell_start.p, 0, sizeof(int32_t) * (c.g.C + 1), c.s));
     HIPCHK(hipStreamSynchronize(c.s));
   }
@@ -941,7 +979,7 @@ static int fixed_step(Ctx& c, double dt) {
   c.adv_dt = dt;
   CHK(advect_swarm(c, dt));
   if (!c.analytical) {
-    if (!setup_complete(c)) CHK(partitions_setup(c));
+    if (!setup_complete(c)) CHK(partitions_setup(c, c.setup_all));
     CHK(rbf_interpolation(c, c.fields.p, c.nf));
   }
   return init_step_velocity(c);
@@ -1030,6 +1068,93 @@ static int advect_steps(Ctx& c, int nsteps, double dt) {
   HIPCHK(hipStreamSynchronize(c.s));
   destroy_graphs(c);
   CHK(rc);
+  CHK(flush_eval_timing(c));
+  return update_removed(c);
+}
+
+// Velocity at the current stage points for stage fraction cst (time t + cst dt of
+// each particle): the interpolated field, or with -lagrbf_time_blend the blend
+// (1 - tau) vA + tau vB of the two snapshots.
+static int adaptive_velocity(Ctx& c, double* dst, double cst) {
+  const int dim = c.g.dim;
+  if (!c.blend) return rbf_interpolation(c, dst, dim);
+  CHK(rbf_interpolation(c, dst, dim, 0));
+  CHK(rbf_interpolation(c, c.vtmp.p, dim, 1));
+  launch_blend(dst, c.vtmp.p, c.ld, c.Np, dim, c.alive.p, c.ptime.p, c.pdt.p, cst, c.blend_t[0], c.blend_t[1], c.s);
+  return 0;
+}
+
+// Per-particle adaptive Dormand-Prince 5(4) from t0 to t1 (no reference
+// counterpart: dormand_prince.c steps the whole swarm with one dt).  Every
+// particle starts at t0 with dt = min(-dt, t1 - t0); each attempt runs the six
+// stages for all unfinished particles with their own dt, then accepts or rejects
+// per particle and adapts its dt (adaptive.hip).  Particles that reach t1 stop;
+// particles that leave the domain are removed as in the fixed-dt loops.  The
+// lazy set-up pass runs between attempts while patches are missing.
+static int advect_adaptive(Ctx& c, double t0, double t1, int max_attempts) {
+  const char* fn = "lagrbf_advect_adaptive";
+  if (!c.advection) return set_err(E_NULL, fn, "Advection context is NULL (set -advection)");
+  if (!c.points_filled) return set_err(E_NULL, fn, "Interpolation swarm (ctx->swi) is NULL");
+  if (!c.data_filled) return set_err(E_NULL, fn, "Data have not been provided");
+  const int dim = c.g.dim, Np = c.Np;
+  if (c.nf < dim) return set_err(E_SIZ, fn, "n_interpolation_fields (%d) < dim (%d)", c.nf, dim);
+  if (!(t1 > t0)) return set_err(E_RANGE, fn, "t_end (%g) must exceed t_start (%g)", t1, t0);
+  if (max_attempts < 0) return set_err(E_RANGE, fn, "max_attempts must be >= 0");
+  if (!(c.adv_dt > 0.0)) return set_err(E_RANGE, fn, "-dt (initial step) must be positive");
+  c.analytical = false;
+  const size_t ld = (size_t)c.ld;
+  HIPCHK(c.x0.reserve(ld * dim));
+  for (int j = 0; j < 7; j++) HIPCHK(c.vadv[j].reserve(ld * dim));
+  if (c.blend) HIPCHK(c.vtmp.reserve(ld * dim));
+  HIPCHK(c.ptime.reserve(ld));
+  HIPCHK(c.pdt.reserve(ld));
+  HIPCHK(c.pfin.reserve(ld));
+  HIPCHK(c.pself.reserve(ld));
+  CHK(partitions_setup(c, c.setup_all));
+  const double dt0 = std::min(c.adv_dt, t1 - t0);
+  launch_dp45_init(Np, c.alive.p, c.ptime.p, c.pdt.p, c.pfin.p, c.pself.p, t0, dt0, c.s);
+  launch_save(c.px.p, c.x0.p, c.vel.p, nullptr, c.ld, Np, dim, c.alive.p, c.s);
+  CHK(adaptive_velocity(c, c.vadv[0].p, 0.0));
+  static const double A[6][6] = {
+      {1.0 / 5.0, 0, 0, 0, 0, 0},
+      {3.0 / 40.0, 9.0 / 40.0, 0, 0, 0, 0},
+      {44.0 / 45.0, -56.0 / 15.0, 32.0 / 9.0, 0, 0, 0},
+      {19372.0 / 6561.0, -25360.0 / 2187.0, 64448.0 / 6561.0, -212.0 / 729.0, 0, 0},
+      {9017.0 / 3168.0, -355.0 / 33.0, 46732.0 / 5247.0, 49.0 / 176.0, -5103.0 / 18656.0, 0},
+      {35.0 / 384.0, 0, 500.0 / 1113.0, 125.0 / 192.0, -2187.0 / 6784.0, 11.0 / 84.0}};
+  static const double Cst[6] = {1.0 / 5.0, 3.0 / 10.0, 4.0 / 5.0, 8.0 / 9.0, 1.0, 1.0};
+  DBuf<unsigned long long> cnt;
+  HIPCHK(cnt.reserve(3));
+  for (int a = 0; a < max_attempts; a++) {
+    for (int s = 0; s < 6; s++) {
+      const double* v[7];
+      for (int j = 0; j <= s; j++) v[j] = c.vadv[j].p;
+      StageArgs sa = stage_args(s + 1, v, A[s], 0.0, 1, 0, 0, c);
+      sa.dt_req = c.pdt.p;
+      sa.req = c.pself.p;
+      sa.mul = 1.0;
+      sa.div = 1.0;
+      stage(c, c.x0.p, sa);
+      CHK(adaptive_velocity(c, c.vadv[s + 1].p, Cst[s]));
+    }
+    HIPCHK(hipMemsetAsync(cnt.p, 0, 3 * sizeof(unsigned long long), c.s));
+    const double* k[7];
+    for (int j = 0; j < 7; j++) k[j] = c.vadv[j].p;
+    launch_dp45_control(c.px.p, c.x0.p, k, c.vadv[0].p, c.ld, Np, dim, c.alive.p, c.pfin.p, c.ptime.p, c.pdt.p, t1,
+                        c.dopri_atol, c.dopri_rtol, c.dopri_fac, c.dopri_fac_min, c.dopri_fac_max, cnt.p, c.s);
+    unsigned long long h[3] = {0, 0, 0};
+    HIPCHK(hipMemcpyAsync(h, cnt.p, sizeof(h), hipMemcpyDeviceToHost, c.s));
+    HIPCHK(hipStreamSynchronize(c.s));
+    c.dp_attempts += 1;
+    c.dp_accepted += (double)h[0];
+    c.dp_rejected += (double)h[1];
+    c.dp_active = (double)h[2];
+    c.sorted_valid = false;
+    if (h[2] == 0) break;
+    if (!setup_complete(c)) CHK(partitions_setup(c, c.setup_all));
+  }
+  launch_dp45_finish(Np, c.alive.p, c.pfin.p, c.s);
+  c.sorted_valid = false;
   CHK(flush_eval_timing(c));
   return update_removed(c);
 }
@@ -1398,6 +1523,11 @@ int lagrbf_advect_steps(void* ctx, int32_t nsteps, double dt) {
   return advect_steps(*c, nsteps, dt);
 }
 
+int lagrbf_advect_adaptive(void* ctx, double t_start, double t_end, int32_t max_attempts) {
+  CTX_OR_NULL("lagrbf_advect_adaptive");
+  return advect_adaptive(*c, t_start, t_end, max_attempts);
+}
+
 int lagrbf_get_info(void* ctx, const char* key, double* value) {
   CTX_OR_NULL("lagrbf_get_info");
   if (!key || !value) return set_err(E_NULL, "lagrbf_get_info", "key/value is NULL");
@@ -1414,6 +1544,10 @@ int lagrbf_get_info(void* ctx, const char* key, double* value) {
     if (int e = flush_eval_timing(*c)) return e;
     *value = k == "eval_ms" ? c->eval_ms : c->eval_count;
   }
+  else if (k == "dp_attempts") *value = c->dp_attempts;
+  else if (k == "dp_accepted") *value = c->dp_accepted;
+  else if (k == "dp_rejected") *value = c->dp_rejected;
+  else if (k == "dp_active") *value = c->dp_active;
   else if (k == "graph_steps") *value = c->graph_steps;
   else if (k == "graph_ms") *value = c->graph_ms;
   else if (k == "setup_ms") *value = c->setup_ms;

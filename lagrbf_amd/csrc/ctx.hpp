@@ -76,6 +76,14 @@ struct Ctx {
   double dopri_err = 0.0;
   std::string velocity_name = "velocity";
   bool pvd_written = false;                // this context has written its PVD collection
+  // time blend of two snapshots (fields 0..dim-1 at tA, dim..2dim-1 at tB)
+  bool setup_all = false;                  // every patch at the first set-up, not only the touched ones
+  bool blend = false;
+  double blend_t[2] = {0.0, 1.0};
+  // per-particle adaptive DP45 state and counters
+  DBuf<double> ptime, pdt, vtmp;
+  DBuf<int32_t> pfin, pself;
+  double dp_attempts = 0, dp_accepted = 0, dp_rejected = 0, dp_active = 0;
   hipStream_t s = nullptr;
   int device = 0;
 
@@ -92,6 +100,13 @@ struct Ctx {
   DBuf<double> blk64;
   DBuf<float> blk32;
   bool blk_valid = false, blk_f32 = false;
+  // tables of one field group (fields g*dim .. g*dim+dim-1) for the time-blended
+  // velocity: rebuilt when table_gen moved on since they were built
+  DBuf<double> blkg64[2];
+  DBuf<float> blkg32[2];
+  int blkg_gen[2] = {-1, -1};
+  bool blkg_f32[2] = {false, false};
+  int table_gen = 0;
   EvalTuner tuner;                         // evaluation launch shape per particle density (this data)
   std::vector<int32_t> h_cell_start_prev, h_cell_nodes_prev;   // cell lists the frozen patches were built from
   bool memb_current = true;                // every set-up patch's members == current cell lists

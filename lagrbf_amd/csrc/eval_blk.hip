@@ -252,7 +252,7 @@ template <typename T>
 __global__ void k_build_blk(Grid g, const double* __restrict__ x, int N, const int32_t* __restrict__ cn,
                             const int32_t* __restrict__ dcell, const int32_t* __restrict__ cs,
                             const uint8_t* __restrict__ active, const int64_t* __restrict__ mem_off,
-                            const double* __restrict__ coef, int nft, T* __restrict__ blk) {
+                            const double* __restrict__ coef, int nft, int cstride, int f0, T* __restrict__ blk) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s > N) return;
   const int dim = g.dim, BS = 4 + g.nen * nft;
@@ -288,7 +288,7 @@ __global__ void k_build_blk(Grid g, const double* __restrict__ x, int N, const i
           const int c2 = ii + g.nc[0] * (jj + g.nc[1] * kk);
           if (c2 < q) off += cs[c2 + 1] - cs[c2];
         }
-    const double* src = coef + (size_t)(mem_off[p] + off + r) * nft;
+    const double* src = coef + (size_t)(mem_off[p] + off + r) * cstride + f0;
     for (int f = 0; f < nft; f++) dst[f] = (T)src[f];
   }
 }
@@ -464,15 +464,15 @@ size_t blk_row(int dim, int nft) { return 4 + ((size_t)1 << dim) * (size_t)nft; 
 
 void launch_build_blk(const Grid& g, bool fp32, const double* x, int N, const int32_t* cn, const int32_t* dcell,
                       const int32_t* cs, const uint8_t* active, const int64_t* mem_off, const double* coef, int nft,
-                      void* blk, hipStream_t s) {
+                      int cstride, int f0, void* blk, hipStream_t s) {
   if (N <= 0) return;
   // N + 1 threads: rows 0..N-1 and the padding row N
   if (fp32)
     hipLaunchKernelGGL(k_build_blk<float>, dim3((N + 256) / 256), dim3(256), 0, s, g, x, N, cn, dcell, cs, active,
-                       mem_off, coef, nft, (float*)blk);
+                       mem_off, coef, nft, cstride, f0, (float*)blk);
   else
     hipLaunchKernelGGL(k_build_blk<double>, dim3((N + 256) / 256), dim3(256), 0, s, g, x, N, cn, dcell, cs, active,
-                       mem_off, coef, nft, (double*)blk);
+                       mem_off, coef, nft, cstride, f0, (double*)blk);
 }
 
 template <int DIM, int KT, typename T>

@@ -839,22 +839,45 @@ __global__ __launch_bounds__(256) void k_dopri_err(const double* __restrict__ px
                                                    double atol, double rtol, unsigned long long* maxb, unsigned int* count) {
 #pragma clang fp contract(off)
   const double Bs[7] = {5179.0 / 57600.0, 0, 7571.0 / 16695.0, 393.0 / 640.0, -92097.0 / 339200.0, 187.0 / 2100.0, 1.0 / 40.0};
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= Np || !alive[i]) return;
-  double E = 0.0;
-  for (int d = 0; d < dim; d++) {
-    double vl = 0.0;
-    for (int j = 0; j < 7; j++) vl += Bs[j] * vp.v[j][d * ld + i];
-    const double a0 = fabs(x0[d * ld + i]), a1 = fabs(px[d * ld + i]);
-    const double mv = a0 < a1 ? a1 : a0;
-    const double tol = atol + rtol * mv;
-    const double xl = x0[d * ld + i] + dt * vl;
-    const double e = px[d * ld + i] - xl;
-    E += (e * e) / (tol * tol);
+  __shared__ unsigned long long red_m[256 / WAVE];
+  __shared__ unsigned int red_c[256 / WAVE];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  // bit pattern of E (>= 0, so bit order = value order); 0 for no contribution
+  unsigned long long eb = 0ull;
+  unsigned int cnt = 0u;
+  if (i < Np && alive[i]) {
+    double E = 0.0;
+    for (int d = 0; d < dim; d++) {
+      double vl = 0.0;
+      for (int j = 0; j < 7; j++) vl += Bs[j] * vp.v[j][d * ld + i];
+      const double a0 = fabs(x0[d * ld + i]), a1 = fabs(px[d * ld + i]);
+      const double mv = a0 < a1 ? a1 : a0;
+      const double tol = atol + rtol * mv;
+      const double xl = x0[d * ld + i] + dt * vl;
+      const double e = px[d * ld + i] - xl;
+      E += (e * e) / (tol * tol);
+    }
+    E = sqrt(1.0 / dim * E);
+    if (E >= 0.0) eb = (unsigned long long)__double_as_longlong(E);
+    cnt = 1u;
   }
-  E = sqrt(1.0 / dim * E);
-  if (E >= 0.0) atomicMax(maxb, (unsigned long long)__double_as_longlong(E));
-  atomicAdd(count, 1u);
+  // block reduction, then one atomic per block (max and count are exact in any order)
+  for (int o = WAVE / 2; o > 0; o >>= 1) {
+    const unsigned long long y = __shfl_xor(eb, o);
+    eb = eb > y ? eb : y;
+    cnt += __shfl_xor(cnt, o);
+  }
+  if ((threadIdx.x & (WAVE - 1)) == 0) { red_m[threadIdx.x / WAVE] = eb; red_c[threadIdx.x / WAVE] = cnt; }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned long long m = red_m[0];
+    unsigned int c = red_c[0];
+    for (int k = 1; k < (int)(blockDim.x / WAVE); k++) { m = m > red_m[k] ? m : red_m[k]; c += red_c[k]; }
+    if (c) {
+      atomicMax(maxb, m);
+      atomicAdd(count, c);
+    }
+  }
 }
 
 void launch_dopri_err(const double* px, const double* x0, const double* const* vadv, int64_t ld, int Np, int dim,
@@ -867,39 +890,50 @@ void launch_dopri_err(const double* px, const double* x0, const double* const* v
 
 // Per-request form of k_dopri_err (serving): each particle uses its request's dt and
 // reduces into its request's maximum.
+// Per-request maxima are reduced in LDS first (one slot per request of the
+// batch), then one global atomic per request present in the block.
 __global__ __launch_bounds__(256) void k_dopri_err_req(const double* __restrict__ px, const double* __restrict__ x0,
                                                        VPtrs vp, int64_t ld, int Np, int dim,
                                                        const int32_t* __restrict__ alive, const int32_t* __restrict__ req,
                                                        const double* __restrict__ dt_req, double atol, double rtol,
-                                                       unsigned long long* maxb_req) {
+                                                       unsigned long long* maxb_req, int nslots) {
 #pragma clang fp contract(off)
+  extern __shared__ unsigned long long smax[];
+  for (int k = threadIdx.x; k < nslots; k += blockDim.x) smax[k] = 0ull;
+  __syncthreads();
   const double Bs[7] = {5179.0 / 57600.0, 0, 7571.0 / 16695.0, 393.0 / 640.0, -92097.0 / 339200.0, 187.0 / 2100.0, 1.0 / 40.0};
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= Np || !alive[i]) return;
-  const int r = req[i];
-  const double dt = dt_req[r];
-  double E = 0.0;
-  for (int d = 0; d < dim; d++) {
-    double vl = 0.0;
-    for (int j = 0; j < 7; j++) vl += Bs[j] * vp.v[j][d * ld + i];
-    const double a0 = fabs(x0[d * ld + i]), a1 = fabs(px[d * ld + i]);
-    const double mv = a0 < a1 ? a1 : a0;
-    const double tol = atol + rtol * mv;
-    const double xl = x0[d * ld + i] + dt * vl;
-    const double e = px[d * ld + i] - xl;
-    E += (e * e) / (tol * tol);
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < Np && alive[i]) {
+    const int r = req[i];
+    const double dt = dt_req[r];
+    double E = 0.0;
+    for (int d = 0; d < dim; d++) {
+      double vl = 0.0;
+      for (int j = 0; j < 7; j++) vl += Bs[j] * vp.v[j][d * ld + i];
+      const double a0 = fabs(x0[d * ld + i]), a1 = fabs(px[d * ld + i]);
+      const double mv = a0 < a1 ? a1 : a0;
+      const double tol = atol + rtol * mv;
+      const double xl = x0[d * ld + i] + dt * vl;
+      const double e = px[d * ld + i] - xl;
+      E += (e * e) / (tol * tol);
+    }
+    E = sqrt(1.0 / dim * E);
+    // E >= 0: the order of the bit patterns is the order of the values
+    if (E >= 0.0) atomicMax(&smax[r], (unsigned long long)__double_as_longlong(E));
   }
-  E = sqrt(1.0 / dim * E);
-  if (E >= 0.0) atomicMax(&maxb_req[r], (unsigned long long)__double_as_longlong(E));
+  __syncthreads();
+  for (int k = threadIdx.x; k < nslots; k += blockDim.x)
+    if (smax[k]) atomicMax(&maxb_req[k], smax[k]);
 }
 
 void launch_dopri_err_req(const double* px, const double* x0, const double* const* vadv, int64_t ld, int Np, int dim,
                           const int32_t* alive, const int32_t* req, const double* dt_req, double atol, double rtol,
-                          unsigned long long* maxb_req, hipStream_t s) {
+                          unsigned long long* maxb_req, int nslots, hipStream_t s) {
   VPtrs vp;
   for (int j = 0; j < 7; j++) vp.v[j] = vadv[j];
-  if (Np > 0) hipLaunchKernelGGL(k_dopri_err_req, dim3((Np + 255) / 256), dim3(256), 0, s, px, x0, vp, ld, Np, dim, alive,
-                                 req, dt_req, atol, rtol, maxb_req);
+  if (Np > 0)
+    hipLaunchKernelGGL(k_dopri_err_req, dim3((Np + 255) / 256), dim3(256), sizeof(unsigned long long) * nslots, s, px, x0,
+                       vp, ld, Np, dim, alive, req, dt_req, atol, rtol, maxb_req, nslots);
 }
 
 // Particles of finished requests: final position into the request's output rows

@@ -102,7 +102,7 @@ bool eval_blk_supported(int nft);
 size_t blk_row(int dim, int nft);
 void launch_build_blk(const Grid& g, bool fp32, const double* x, int N, const int32_t* cn, const int32_t* dcell,
                       const int32_t* cs, const uint8_t* active, const int64_t* mem_off, const double* coef, int nft,
-                      void* blk, hipStream_t s);
+                      int cstride, int f0, void* blk, hipStream_t s);   // fields f0 .. f0+nft-1 of cstride
 // cell_list / list_count (device): evaluate only the listed non-empty cells, one
 // block per cell (group 0) or 64/group cells per wave in lane groups of
 // group = 8, 16, 32 (needs nft == dim); nullptr: one block per cell.  Every
@@ -136,7 +136,16 @@ void launch_dopri_err(const double* px, const double* x0, const double* const* v
 // ---- serving: per-request reductions and retirement ----
 void launch_dopri_err_req(const double* px, const double* x0, const double* const* vadv, int64_t ld, int Np, int dim,
                           const int32_t* alive, const int32_t* req, const double* dt_req, double atol, double rtol,
-                          unsigned long long* maxb_req, hipStream_t s);
+                          unsigned long long* maxb_req, int nslots, hipStream_t s);
+// per-particle adaptive DP45 with time-blended velocity (adaptive.hip)
+void launch_blend(double* va, const double* vb, int64_t ld, int Np, int dim, const int32_t* alive, const double* tp,
+                  const double* dtp, double cst, double tA, double tB, hipStream_t s);
+void launch_dp45_control(double* px, double* x0, const double* const* k, double* k1, int64_t ld, int Np, int dim,
+                         int32_t* alive, int32_t* fin, double* tp, double* dtp, double t_end, double atol, double rtol,
+                         double fac, double fac_min, double fac_max, unsigned long long* counts, hipStream_t s);
+void launch_dp45_init(int Np, const int32_t* alive, double* tp, double* dtp, int32_t* fin, int32_t* self, double t0,
+                      double dt0, hipStream_t s);
+void launch_dp45_finish(int Np, int32_t* alive, const int32_t* fin, hipStream_t s);
 void launch_retire(const double* px, int64_t ld, int Np, int dim, int32_t* alive, const int32_t* req,
                    const int32_t* porig, const uint8_t* done, double* const* out_req, hipStream_t s);
 void launch_fill_i32(int32_t* a, int n, int32_t v, hipStream_t s);

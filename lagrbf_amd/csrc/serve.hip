@@ -315,7 +315,7 @@ int batch_tick(Batch& b) {
       for (int j = 0; j < 7; j++) vv[j] = b.vadv[j].p;
       HIPCHK(hipMemsetAsync(b.err_req.p, 0, sizeof(unsigned long long) * RSLOTS, b.s));
       launch_dopri_err_req(b.px.p, b.x0.p, vv, b.ld, b.Np, dim, b.alive.p, b.req.p, b.dt_req.p, d.dopri_atol,
-                           d.dopri_rtol, b.err_req.p, b.s);
+                           d.dopri_rtol, b.err_req.p, RSLOTS, b.s);
       HIPCHK(hipMemcpyAsync(b.h_err, b.err_req.p, sizeof(unsigned long long) * RSLOTS, hipMemcpyDeviceToHost, b.s));
       break;
     }

@@ -859,6 +859,139 @@ static int advect_swarm(orc_ctx *ctx, double dt)
   }
 }
 
+/* Per-particle adaptive Dormand-Prince 5(4) from t0 to t1, optionally with the
+ * velocity blended in time between two snapshots: fields 0..d-1 at tA and
+ * d..2d-1 at tB, v = (1 - tau) vA + tau vB, tau = (t - tA) / (tB - tA) at each
+ * particle's stage time.  An extension with no reference counterpart (the
+ * reference steps the swarm with one dt, dormand_prince.c:216-310); the tableau,
+ * error norm (dormand_prince.c:37-102) and controller (dormand_prince.c:179-214)
+ * are applied per particle, and an attempt with error norm > 1 is rejected.
+ * Particles are advanced attempt by attempt in lockstep, each with its own t and
+ * dt; one that reaches t1 stops.  counts: attempts, accepted, rejected, active. */
+static void adaptive_eval(orc_ctx *ctx, double *k, double *vals, int blend, const double *tp, const double *dtp,
+                          double cst, double tA, double tB)
+{
+  const int dim = ctx->dim, nv = blend ? 2 * dim : dim;
+  int p, d;
+  rbf_interpolation(ctx, vals, nv);
+  for (p = 0; p < ctx->Np; p++) {
+    if (!ctx->alive[p]) continue;
+    if (blend) {
+      const double t = tp[p] + cst * dtp[p];
+      const double tau = (t - tA) / (tB - tA);
+      for (d = 0; d < dim; d++)
+        k[(size_t)p * dim + d] = (1.0 - tau) * vals[(size_t)p * nv + d] + tau * vals[(size_t)p * nv + dim + d];
+    } else {
+      for (d = 0; d < dim; d++) k[(size_t)p * dim + d] = vals[(size_t)p * nv + d];
+    }
+  }
+}
+
+int orc_advect_adaptive(orc_ctx *ctx, double t0, double t1, int max_attempts, int blend, double tA, double tB,
+                        double *counts)
+{
+  static const double A[6][6] = {
+    {1.0 / 5.0, 0, 0, 0, 0, 0},
+    {3.0 / 40.0, 9.0 / 40.0, 0, 0, 0, 0},
+    {44.0 / 45.0, -56.0 / 15.0, 32.0 / 9.0, 0, 0, 0},
+    {19372.0 / 6561.0, -25360.0 / 2187.0, 64448.0 / 6561.0, -212.0 / 729.0, 0, 0},
+    {9017.0 / 3168.0, -355.0 / 33.0, 46732.0 / 5247.0, 49.0 / 176.0, -5103.0 / 18656.0, 0},
+    {35.0 / 384.0, 0, 500.0 / 1113.0, 125.0 / 192.0, -2187.0 / 6784.0, 11.0 / 84.0}};
+  static const double Cst[6] = {1.0 / 5.0, 3.0 / 10.0, 4.0 / 5.0, 8.0 / 9.0, 1.0, 1.0};
+  static const double Bs[7] = {5179.0 / 57600.0, 0, 7571.0 / 16695.0, 393.0 / 640.0, -92097.0 / 339200.0, 187.0 / 2100.0, 1.0 / 40.0};
+  const int dim = ctx->dim, Np = ctx->Np, nf = ctx->cfg.n_ifields;
+  const size_t n = (size_t)Np * dim + 1;
+  double *k[7], *x0, *tp, *dtp, *vals, dt0;
+  int32_t *fin;
+  int a, s, j, p, d;
+  if (!ctx->data_filled || !ctx->points_filled) return ERR_ARG_NULL;
+  if (!(t1 > t0) || max_attempts < 0 || !(ctx->cfg.dt > 0.0)) return ERR_ARG_RANGE;
+  if ((blend && nf != 2 * dim) || (!blend && nf < dim)) return ERR_ARG_SIZ;
+  for (j = 0; j < 7; j++) k[j] = (double *)calloc(n, sizeof(double));
+  x0 = (double *)calloc(n, sizeof(double));
+  tp = (double *)calloc((size_t)Np + 1, sizeof(double));
+  dtp = (double *)calloc((size_t)Np + 1, sizeof(double));
+  fin = (int32_t *)calloc((size_t)Np + 1, sizeof(int32_t));
+  vals = (double *)calloc((size_t)Np * (blend ? 2 * dim : dim) + 1, sizeof(double));
+  for (j = 0; j < 4; j++) counts[j] = 0.0;
+  partitions_setup(ctx);
+  dt0 = fmin(ctx->cfg.dt, t1 - t0);
+  for (p = 0; p < Np; p++) {
+    tp[p] = t0; dtp[p] = dt0;
+    if (ctx->alive[p]) for (d = 0; d < dim; d++) x0[(size_t)p * dim + d] = ctx->px[(size_t)p * dim + d];
+  }
+  adaptive_eval(ctx, k[0], vals, blend, tp, dtp, 0.0, tA, tB);
+  for (a = 0; a < max_attempts; a++) {
+    double acc = 0, rej = 0, act = 0;
+    for (s = 0; s < 6; s++) {
+      for (p = 0; p < Np; p++) {
+        double *x;
+        if (!ctx->alive[p]) continue;
+        x = ctx->px + (size_t)p * dim;
+        for (d = 0; d < dim; d++) {
+          double v = 0.0;
+          for (j = 0; j <= s; j++) v += A[s][j] * k[j][(size_t)p * dim + d];
+          x[d] = x0[(size_t)p * dim + d] + dtp[p] * v;
+        }
+        project_point(x, dim, ctx->cfg.rmin, ctx->cfg.rmax);
+      }
+      after_move(ctx);
+      adaptive_eval(ctx, k[s + 1], vals, blend, tp, dtp, Cst[s], tA, tB);
+    }
+    for (p = 0; p < Np; p++) {
+      double E = 0.0, t, g, inner, f, fmx, dn, rest; int ok;
+      if (!ctx->alive[p]) continue;
+      for (d = 0; d < dim; d++) {
+        const size_t i = (size_t)p * dim + d;
+        double vl = 0.0, mv, tol, xl, e;
+        for (j = 0; j < 7; j++) vl += Bs[j] * k[j][i];
+        mv = fmax(fabs(x0[i]), fabs(ctx->px[i]));
+        tol = ctx->cfg.dopri_atol + ctx->cfg.dopri_rtol * mv;
+        xl = x0[i] + dtp[p] * vl;
+        e = ctx->px[i] - xl;
+        E += (e * e) / (tol * tol);
+      }
+      E = sqrt(1.0 / dim * E);
+      ok = E <= 1.0;
+      t = tp[p];
+      if (ok) {
+        for (d = 0; d < dim; d++) {
+          const size_t i = (size_t)p * dim + d;
+          x0[i] = ctx->px[i];
+          k[0][i] = k[6][i];
+        }
+        t = t + dtp[p];
+        tp[p] = t;
+        acc += 1;
+      } else {
+        for (d = 0; d < dim; d++) ctx->px[(size_t)p * dim + d] = x0[(size_t)p * dim + d];
+        rej += 1;
+      }
+      fmx = ok ? ctx->cfg.dopri_dt_fac_max : 1.0;
+      g = ctx->cfg.dopri_dt_fac * pow(1.0 / E, 1.0 / 5.0);
+      inner = (ctx->cfg.dopri_dt_fac_min < g) ? g : ctx->cfg.dopri_dt_fac_min;
+      f = (fmx < inner) ? fmx : inner;
+      dn = dtp[p] * f;
+      rest = t1 - t;
+      if (rest <= 1e-12 * fmax(1.0, fabs(t1))) {
+        ctx->alive[p] = 0;
+        fin[p] = 1;
+      } else {
+        if (dn > rest) dn = rest;
+        dtp[p] = dn;
+        act += 1;
+      }
+    }
+    counts[0] += 1; counts[1] += acc; counts[2] += rej; counts[3] = act;
+    if (act == 0) break;
+    partitions_setup(ctx);
+  }
+  for (p = 0; p < Np; p++) if (fin[p]) ctx->alive[p] = 1;
+  for (j = 0; j < 7; j++) free(k[j]);
+  free(x0); free(tp); free(dtp); free(fin); free(vals);
+  return 0;
+}
+
 int orc_set_velocity_batch(orc_ctx *ctx, orc_velocity_batch_fn fn, void *user)
 {
   if (!ctx->cfg.advection) return ERR_ARG_NULL;

@@ -93,6 +93,8 @@ int      orc_set_velocity_batch(orc_ctx *ctx, orc_velocity_batch_fn fn, void *us
 int      orc_advect_interpolate(orc_ctx *ctx, double t, double t_next);
 int      orc_advect_analytical(orc_ctx *ctx, double t, double t_next);
 int      orc_advect_steps(orc_ctx *ctx, int32_t nsteps, double dt);
+int      orc_advect_adaptive(orc_ctx *ctx, double t0, double t1, int max_attempts, int blend, double tA, double tB,
+                             double *counts);
 
 /* ---- introspection (tests) ---- */
 int      orc_get_pu(const orc_ctx *ctx, double *eps_w, double *eps_r, double *pu_coords);

@@ -77,6 +77,8 @@ def lib():
         L.orc_advect_interpolate.argtypes = [V, ctypes.c_double, ctypes.c_double]
         L.orc_advect_analytical.argtypes = [V, ctypes.c_double, ctypes.c_double]
         L.orc_advect_steps.argtypes = [V, ctypes.c_int32, ctypes.c_double]
+        L.orc_advect_adaptive.argtypes = [V, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_double, ctypes.c_double, D]
         L.orc_get_pu.argtypes = [V, D, D, D]
         L.orc_get_cellids.argtypes = [V, ctypes.c_int, I32]
         L.orc_get_cell_lists.argtypes = [V, I32, I32]
@@ -158,6 +160,13 @@ class OracleCtx:
 
     def __init__(self, options):
         self.cfg = config_from_options(options)
+        o = {k.lstrip("-"): v for k, v in dict(options).items()}
+        blend = o.get("lagrbf_time_blend", False)
+        self.blend = blend if isinstance(blend, bool) else str(blend).lower() in ("1", "true", "yes")
+        bt = o.get("lagrbf_blend_times", (0.0, 1.0))
+        if isinstance(bt, str):
+            bt = [float(x) for x in bt.split(",")]
+        self.blend_times = (float(bt[0]), float(bt[1]))
         self.dim = self.cfg.dim
         self.nfields = self.cfg.n_ifields
         self.nen = 1 << self.dim
@@ -228,6 +237,15 @@ class OracleCtx:
 
     def advect_steps_c(self, nsteps, dt):
         self._chk(lib().orc_advect_steps(self.ctx, int(nsteps), float(dt)), "orc_advect_steps")
+
+    def advect_adaptive_c(self, t_start, t_end, max_attempts=100000):
+        """Per-particle adaptive DP45 (time blend from the options); returns
+        {attempts, accepted, rejected, active}."""
+        counts = np.zeros(4)
+        self._chk(lib().orc_advect_adaptive(self.ctx, float(t_start), float(t_end), int(max_attempts), int(self.blend),
+                                            self.blend_times[0], self.blend_times[1], _dp(counts)),
+                  "orc_advect_adaptive")
+        return dict(zip(("attempts", "accepted", "rejected", "active"), counts.tolist()))
 
     # ---- introspection ----
     def pu(self):

@@ -1,0 +1,148 @@
+"""Per-particle adaptive Dormand-Prince 5(4) with a time-blended velocity
+(lagrbf_advect_adaptive; no reference counterpart, the reference's DOPRI takes
+one dt for the swarm).
+
+CPU tests pin the oracle's restatement by closed forms: a blend of v and -v
+between t = 0 and 1 moves every particle along its streamline by the parameter
+s(t) = t - t^2, which returns to 0 at t = 1, so particles come back to their
+start; without a blend the rotation field's flow is a rotation; particles are
+independent (alone or together, bit for bit); a tighter tolerance converges.
+GPU tests compare the library with the oracle: positions 1e-8 relative (a step
+decision that flips near the acceptance threshold moves a position by at most
+the local tolerance), accepted and rejected counts within 1%."""
+import numpy as np
+import pytest
+
+from tests import cases
+
+
+def _rotation_case(blend, rtol=1e-11, n=400, seed=3):
+    opts, X, V, _ = cases.rotation_2d(N=2048, Np=10, n=9, scheme=3)
+    rng = np.random.default_rng(seed)
+    r = 0.3 * np.sqrt(rng.random(n)); th = 2 * np.pi * rng.random(n)
+    P = np.stack([0.5 + r * np.cos(th), 0.5 + r * np.sin(th)], axis=1)
+    # weight radius below the cell size: the corner weights sum to 1 and the
+    # interpolated velocity is continuous (wider radii drop weight, as the reference does)
+    o = dict(opts, pu_mesh_overlap=1.0, dopri_rtol=rtol, dopri_atol=rtol * 1e-3, dt=0.01, oracle_setup_all=True,
+             lagrbf_setup_all=True)
+    if blend:
+        o.update(n_interpolation_fields=4, lagrbf_time_blend=True, lagrbf_blend_times="0.0,1.0")
+        F = np.concatenate([V, -V], axis=1)
+    else:
+        F = V
+    return o, X, F, P
+
+
+def _oracle_run(o, X, F, P, t0=0.0, t1=1.0):
+    from oracle.oracle import OracleCtx
+    c = OracleCtx(o)
+    c.fill_data(X, F)
+    c.fill_points(P)
+    c.interpolate_c()
+    counts = c.advect_adaptive_c(t0, t1)
+    xs = np.zeros_like(P)
+    c.output(coords=xs)
+    c.destroy()
+    return xs, counts
+
+
+def test_oracle_blend_returns_to_start():
+    o, X, F, P = _rotation_case(blend=True)
+    xs, counts = _oracle_run(o, X, F, P)
+    assert counts["active"] == 0 and counts["accepted"] >= P.shape[0]
+    err = np.max(np.abs(xs - P))
+    assert err < 1e-7, err
+
+
+def test_oracle_flow_of_the_interpolant_matches_dop853():
+    """Without a blend the trajectories are those of the interpolated field: an
+    independent integrator (scipy's DOP853 at rtol 1e-12, velocity from the
+    oracle's interpolation at the stage points) agrees to the tolerance."""
+    from scipy.integrate import solve_ivp
+    from oracle.oracle import OracleCtx
+    o, X, F, P = _rotation_case(blend=False, rtol=1e-10, n=16)
+    xs, counts = _oracle_run(o, X, F, P, 0.0, 0.8)
+    ev = OracleCtx(o)
+    ev.fill_data(X, F)
+
+    def rhs(t, y):
+        pts = y.reshape(-1, 2)
+        ev.fill_points(pts)
+        ev.interpolate_c()
+        v = np.zeros_like(pts)
+        ev.output(fvals=v)
+        return v.reshape(-1)
+
+    sol = solve_ivp(rhs, (0.0, 0.8), P.reshape(-1), method="DOP853", rtol=1e-12, atol=1e-14)
+    ev.destroy()
+    ref = sol.y[:, -1].reshape(-1, 2)
+    assert np.max(np.abs(xs - ref)) < 1e-8
+    # and the field is close to the rotation it samples (RBF without polynomial, 2048 nodes)
+    c, s = np.cos(0.8), np.sin(0.8)
+    ex = np.stack([0.5 + c * (P[:, 0] - 0.5) - s * (P[:, 1] - 0.5), 0.5 + s * (P[:, 0] - 0.5) + c * (P[:, 1] - 0.5)], 1)
+    assert np.max(np.abs(xs - ex)) < 0.02
+    # the global error follows the tolerance
+    o2 = dict(o, dopri_rtol=1e-13, dopri_atol=1e-16)
+    xs2, counts2 = _oracle_run(o2, X, F, P, 0.0, 0.8)
+    assert counts2["accepted"] > counts["accepted"]
+    assert np.max(np.abs(xs - xs2)) < 1e-8
+
+
+def test_oracle_particles_independent():
+    o, X, F, P = _rotation_case(blend=True, n=60)
+    xs, _ = _oracle_run(o, X, F, P)
+    for i in (0, 17, 59):
+        one, _ = _oracle_run(o, X, F, P[i:i + 1])
+        assert np.array_equal(one[0], xs[i])
+
+
+def _fourier_blend_case(Np=3000):
+    opts, X, V, P = cases.fourier_3d(nlat=18, Np=Np, n=6, scheme=3)
+    rng = np.random.default_rng(9)
+    P = 0.2 + 0.6 * rng.random((Np, 3))
+    VB = np.roll(V, 1, axis=1) * 0.7
+    o = dict(opts, n_interpolation_fields=6, lagrbf_time_blend=True, lagrbf_blend_times="0.0,1.0", dt=1e-3,
+             dopri_rtol=1e-7, dopri_atol=1e-10, lagrbf_setup_all=True, oracle_setup_all=True)
+    return o, X, np.concatenate([V, VB], axis=1), P
+
+
+@pytest.mark.gpu
+def test_adaptive_gpu_matches_oracle():
+    import lagrbf_amd as L
+    o, X, F, P = _fourier_blend_case()
+    print("case: adaptive DP45 + blend, fourier3d 3000 particles", flush=True)
+    g = L.LagRBFCtx(petsc_options=dict(o))
+    g.fill_data(X, F)
+    g.fill_points(P)
+    g.interpolate_c()
+    cg = g.advect_adaptive_c(0.0, 0.05)
+    xg = np.zeros_like(P)
+    g.output(coords=xg)
+    xo, co = _oracle_run(o, X, F, P, 0.0, 0.05)
+    rel = np.max(np.abs(xg - xo)) / np.max(np.abs(xo))
+    print(f"adaptive gpu vs oracle: rel {rel:.3e}, counts gpu {cg} oracle {co}", flush=True)
+    assert cg["active"] == 0 and co["active"] == 0
+    assert abs(cg["accepted"] - co["accepted"]) <= 0.01 * co["accepted"]
+    assert abs(cg["rejected"] - co["rejected"]) <= 0.01 * max(co["rejected"], 100)
+    assert rel < 1e-8
+
+
+@pytest.mark.gpu
+def test_adaptive_gpu_mixed_and_blend_return():
+    import lagrbf_amd as L
+    print("case: adaptive DP45 blend v/-v return to start, fp64 and mixed", flush=True)
+    out = {}
+    for prec in ("fp64", "mixed"):
+        o, X, F, P = _rotation_case(blend=True, n=2000, rtol=1e-11 if prec == "fp64" else 1e-8)
+        g = L.LagRBFCtx(petsc_options=dict(o), precision=prec)
+        g.fill_data(X, F)
+        g.fill_points(P)
+        g.interpolate_c()
+        c = g.advect_adaptive_c(0.0, 1.0)
+        xs = np.zeros_like(P)
+        g.output(coords=xs)
+        out[prec] = xs
+        assert c["active"] == 0
+    assert np.max(np.abs(out["fp64"] - P)) < 1e-7
+    # FP32 evaluation: ~1e-6 relative velocities
+    assert np.max(np.abs(out["mixed"] - P)) < 1e-4
