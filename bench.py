@@ -294,6 +294,46 @@ def c3_bench(L, precision, steps, warmup):
     return res
 
 
+def c4_sizes(scale):
+    """C4 sizes scaled by `scale` (nodes, particles and PU cells together)."""
+    if scale >= 1.0:
+        return dict(m=126, N=8000000, Np=16777216, pu=38)
+    m = max(8, int(round(126 * scale ** (1 / 3))))
+    return dict(m=m, N=None, Np=max(1024, int(16777216 * scale)), pu=max(4, int(round(38 * scale ** (1 / 3)))))
+
+
+def c4_bench(L, precision, t_end, m=126, N=8000000, Np=16777216, pu=38):
+    """Adaptive DP45 geodynamics-like flow: 8,000,000 perturbed-lattice nodes in a
+    4:1:1 box, two slab-plus-noise velocity snapshots blended linearly in time,
+    16,777,216 particles, per-particle Dormand-Prince 5(4) (rtol 1e-6, atol 1e-9)
+    from t = 0 to t_end."""
+    from tests import cases
+    t0 = time.perf_counter()
+    opts, X, V, P = cases.slab_c4(m=m, N=N, Np=Np, pu=pu)
+    log(f"c4 workload built: {X.shape[0]} nodes, {P.shape[0]} particles ({time.perf_counter() - t0:.1f}s)")
+    ctx = L.LagRBFCtx(petsc_options=dict(opts), precision=precision)
+    ctx.fill_data(X, V)
+    ctx.fill_points(P)
+    ts = time.perf_counter()
+    ctx.interpolate_c()
+    t_setup = time.perf_counter() - ts
+    ev0, n0 = ctx.info("eval_ms"), ctx.info("eval_count")
+    t1 = time.perf_counter()
+    c = ctx.advect_adaptive_c(0.0, t_end)
+    wall = time.perf_counter() - t1
+    ev1, n1 = ctx.info("eval_ms"), ctx.info("eval_count")
+    res = {"precision": precision, "nodes": int(X.shape[0]), "particles": int(P.shape[0]), "t_end": t_end,
+           "setup_s": t_setup, "wall_s": wall, "attempts": c["attempts"], "accepted_steps": c["accepted"],
+           "rejected_steps": c["rejected"], "unfinished": c["active"], "removed": ctx.info("removed"),
+           "particle_steps_per_s": c["accepted"] / wall, "eval_ms": (ev1 - ev0) / max(n1 - n0, 1),
+           "evals": int(n1 - n0)}
+    ctx.destroy()
+    log(f"c4 [{precision}]: set-up {t_setup:.2f}s, {c['attempts']:.0f} attempts, {c['accepted']:.3e} accepted / "
+        f"{c['rejected']:.3e} rejected steps in {wall:.2f}s = {res['particle_steps_per_s']:.3e} particle-steps/s, "
+        f"eval {res['eval_ms']:.2f} ms, removed {res['removed']:.0f}")
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -311,6 +351,10 @@ def main():
     ap.add_argument("--c3-only", action="store_true", help="the uneven-batching configuration only")
     ap.add_argument("--no-c3", action="store_true", help="skip the uneven-batching measurement")
     ap.add_argument("--no-graph", action="store_true", help="skip the graph-replay measurement of the fixed-step loop")
+    ap.add_argument("--c4-only", action="store_true", help="the adaptive DP45 configuration only")
+    ap.add_argument("--no-c4", action="store_true", help="skip the adaptive DP45 measurement")
+    ap.add_argument("--c4-t-end", type=float, default=0.1, help="end time of the adaptive DP45 run")
+    ap.add_argument("--c4-scale", type=float, default=1.0, help="fraction of the C4 node and particle counts")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="launch a plain copy of MIB MiB first (scale for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -326,6 +370,10 @@ def main():
     if args.c3_only:
         out = {p: c3_bench(L, p, args.steps, args.warmup) for p in ("mixed", "fp64")}
         print(json.dumps({"c3": out}), flush=True)
+        return
+    if args.c4_only:
+        out = {p: c4_bench(L, p, args.c4_t_end, **c4_sizes(args.c4_scale)) for p in ("mixed", "fp64")}
+        print(json.dumps({"c4": out}), flush=True)
         return
     if args.serve_only:
         dsets = serve_datasets()
@@ -391,6 +439,16 @@ def main():
         except Exception as e:
             c3 = {"error": str(e)}
             log(f"c3 failed: {e}")
+    c4 = None
+    if not args.no_c4:
+        try:
+            c4 = {p: c4_bench(L, p, args.c4_t_end, **c4_sizes(args.c4_scale)) for p in ("mixed", "fp64")}
+            c4["config"] = ("adaptive DP45: 8,000,000 perturbed-lattice nodes in a 4:1:1 box, slab + Fourier-noise "
+                            "snapshots blended linearly in t, 16,777,216 particles, per-particle Dormand-Prince 5(4) "
+                            f"rtol 1e-6 atol 1e-9, t = 0 .. {args.c4_t_end}")
+        except Exception as e:
+            c4 = {"error": str(e)}
+            log(f"c4 failed: {e}")
     out = {
         "metric": "particle-RK4 steps/s, FP32+refine vs FP64 @1e-10 rel err; serve p99 ms",
         "value": r["value"], "unit": "particle-RK4-steps/s", "n_gpus": 1, "steps": args.steps,
@@ -405,6 +463,7 @@ def main():
         "mixed_vs_fp64": cmp,
         "serve": serve,
         "uneven_batching": c3,
+        "adaptive_dp45": c4,
     }
     if rank == 0:
         print(json.dumps(out), flush=True)

@@ -101,3 +101,49 @@ def clustered_c3(n_uniform=1000000, n_clusters=64, per_cluster=15625, sigma=(0.0
                 pu_mesh_overlap=1.0, pu_mesh_shape_factor=1.0, kernel_type=2, solve_method=0,
                 advection=True, advection_scheme=1, cfl_fac=1e6, dt=1e-3)
     return opts, X, np.ascontiguousarray(V), P
+
+
+def slab_velocity(X, xs, modes, U0=0.5, w=0.3, noise=0.05):
+    """Nondimensional corner-flow-like sinking slab in the x-z plane of a 4:1:1 box
+    plus small divergence-free Fourier noise: stream function
+    psi(x, z) = U0 sin(pi z) (x - xs) exp(-((x - xs)/w)^2), v = (dpsi/dz, 0, -dpsi/dx)
+    (downwelling U0 sin(pi z) at x = xs, no normal flow through z = 0, 1)."""
+    u = X[:, 0] - xs
+    g = u * np.exp(-(u / w) ** 2)
+    dg = (1.0 - 2.0 * (u / w) ** 2) * np.exp(-(u / w) ** 2)
+    z = X[:, 2]
+    V = np.zeros_like(X)
+    V[:, 0] = U0 * np.pi * np.cos(np.pi * z) * g
+    V[:, 2] = -U0 * np.sin(np.pi * z) * dg
+    return V + noise * fourier_velocity(X / np.array([4.0, 1.0, 1.0]), modes)
+
+
+def slab_c4(m=126, N=8000000, Np=16777216, pu=38, seed_off=0):
+    """Adaptive DP45 configuration: a 4m x m x m lattice in [0,4] x [0,1] x [0,1]
+    (nodes at (i+1/2)h, jittered +-0.3 h), the first N of a seeded permutation;
+    velocity snapshots A (slab at x = 1.5, noise seed 1) and B (slab at x = 2.0,
+    noise seed 2), blended linearly between t = 0 and 1 (fields 0-2 and 3-5); PU
+    (4 pu + 1) x (pu + 1)^2 nodes, weight radius 0.97 cell (overlap scaled by the
+    PU node count, as the reference's radius = overlap / P^(1/d) assumes a unit
+    box); Wendland C2; Np particles uniform in the box."""
+    rng = np.random.default_rng(SEED + 3 + seed_off)
+    h = 1.0 / m
+    gx = (np.arange(4 * m) + 0.5) * h
+    g = (np.arange(m) + 0.5) * h
+    X = np.stack(np.meshgrid(gx, g, g, indexing="ij"), axis=-1).reshape(-1, 3)
+    X = X + rng.uniform(-0.3, 0.3, size=X.shape) * h
+    X = np.clip(X, 0.0, [4.0, 1.0, 1.0])
+    if N is not None and N < X.shape[0]:
+        X = X[rng.permutation(X.shape[0])[:N]]
+    X = np.ascontiguousarray(X)
+    VA = slab_velocity(X, 1.5, fourier_modes(seed=SEED + 11))
+    VB = slab_velocity(X, 2.0, fourier_modes(seed=SEED + 12))
+    P = rng.random((Np, 3)) * np.array([4.0, 1.0, 1.0])
+    npu = np.array([4 * pu + 1, pu + 1, pu + 1])
+    hc = 1.0 / pu
+    overlap = 0.97 * hc * float(np.prod(npu)) ** (1.0 / 3.0)
+    opts = dict(pu_mesh_dim=3, pu_mesh_O=[0.0] * 3, pu_mesh_L=[4.0, 1.0, 1.0], pu_mesh_n=npu.tolist(),
+                n_interpolation_fields=6, pu_mesh_overlap=overlap, pu_mesh_shape_factor=1.0, kernel_type=2,
+                solve_method=0, advection=True, advection_scheme=3, dt=1e-3, dopri_rtol=1e-6, dopri_atol=1e-9,
+                lagrbf_time_blend=True, lagrbf_blend_times="0.0,1.0", lagrbf_setup_all=True, oracle_setup_all=True)
+    return opts, X, np.ascontiguousarray(np.concatenate([VA, VB], axis=1)), P

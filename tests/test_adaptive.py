@@ -146,3 +146,27 @@ def test_adaptive_gpu_mixed_and_blend_return():
     assert np.max(np.abs(out["fp64"] - P)) < 1e-7
     # FP32 evaluation: ~1e-6 relative velocities
     assert np.max(np.abs(out["mixed"] - P)) < 1e-4
+
+
+@pytest.mark.gpu
+def test_slab_configuration_matches_oracle():
+    """The adaptive DP45 configuration (4:1:1 box, slab snapshots blended in time)
+    at a small size: library against the oracle, FP64."""
+    import lagrbf_amd as L
+    o, X, F, P = cases.slab_c4(m=16, N=None, Np=3000, pu=5)
+    print("case: slab 4:1:1, 16,384 nodes, 3000 particles, t = 0 .. 0.2", flush=True)
+    g = L.LagRBFCtx(petsc_options=dict(o))
+    g.fill_data(X, F)
+    g.fill_points(P)
+    g.interpolate_c()
+    cg = g.advect_adaptive_c(0.0, 0.2)
+    xg = np.zeros_like(P)
+    g.output(coords=xg)
+    xo, co = _oracle_run(o, X, F, P, 0.0, 0.2)
+    live = np.any(xo != 0, axis=1)
+    rel = np.max(np.abs(xg[live] - xo[live])) / np.max(np.abs(xo[live]))
+    print(f"slab: rel {rel:.3e}, gpu {cg} oracle {co}, removed {g.info('removed'):.0f}", flush=True)
+    assert cg["active"] == 0 and co["active"] == 0
+    assert abs(cg["accepted"] - co["accepted"]) <= 0.01 * co["accepted"]
+    assert np.array_equal(np.any(xg != 0, axis=1), live)
+    assert rel < 1e-8
