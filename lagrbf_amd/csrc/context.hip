@@ -141,6 +141,10 @@ static int ctx_from_options(Ctx& c, const OptView& o) {
   o.get_bool("lagrbf_graph", c.use_graph, bad);
   if (bad) return set_err(E_WRONG, fn, "malformed -lagrbf_graph option");
   o.get_bool("lagrbf_setup_all", c.setup_all, bad);
+  o.get_int("lagrbf_eval_variant", c.eval_variant, bad);
+  if (c.eval_variant < -1 || c.eval_variant >= N_EVAL_VARIANTS)
+    return set_err(E_RANGE, fn, "-lagrbf_eval_variant must be -1 (tune) or 0..%d", N_EVAL_VARIANTS - 1);
+  c.tuner.forced = c.eval_variant;
   o.get_bool("lagrbf_time_blend", c.blend, bad);
   std::vector<double> bt;
   if (o.get_real_array("lagrbf_blend_times", bt, bad)) {
@@ -213,6 +217,7 @@ int fill_data(Ctx& c, int n, const double* coords, const double* values, const i
   CHK(check_bbox(c, n, coords, fn));
   c.blk_valid = false; c.table_gen++;
   c.tuner = EvalTuner();
+  c.tuner.forced = c.eval_variant;
   if (c.setup_N >= 0 && c.solve_method != 2 && n != c.setup_N && c.mem_used > 0)
     return set_err(E_SIZ, fn, "solve_method %d keeps patch factors across data fills; node count changed from %d to %d "
                    "(call with solve_method 2 or destroy the context)", c.solve_method, c.setup_N, n);
@@ -274,7 +279,7 @@ static int alloc_particles(Ctx& c) {
   HIPCHK(c.pstart.reserve((size_t)c.g.C + 2));
   HIPCHK(c.pscratch.reserve((size_t)c.g.C + 2 + (size_t)(c.g.C + 2) / 1024 + 64));
   HIPCHK(c.porig.reserve(ld));
-  HIPCHK(c.cell_list.reserve((size_t)std::max(c.g.C, 1)));
+  HIPCHK(c.cell_list.reserve(nonempty_list_capacity(c.g.C)));
   HIPCHK(c.list_count.reserve(1));
   c.sorted_valid = false;
   HIPCHK(c.vel.reserve(ld * dim));
@@ -641,7 +646,7 @@ static int rbf_interpolation(Ctx& c, double* out, int nfields, int group = -1) {
   // (once its launch shape is chosen); the other paths are bracketed by stream
   // events and resolved at once
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  const bool timed_dispatch = use_blk && c.tuner.choice.count(density_bucket(c.Np, c.g.C)) != 0;
+  const bool timed_dispatch = use_blk && c.tuner.planned(density_bucket(c.Np, c.g.C)) >= 0;
   if (!c.capturing && (timed_dispatch || !use_blk)) {
     CHK(event_take(c, e0));
     CHK(event_take(c, e1));
@@ -651,9 +656,7 @@ static int rbf_interpolation(Ctx& c, double* out, int nfields, int group = -1) {
     const int per_cell = (c.Np + c.g.C - 1) / std::max(c.g.C, 1);
     const int block = std::min(256, std::max(64, (per_cell + 63) / 64 * 64));
     const int bucket = density_bucket(c.Np, c.g.C);
-    auto it = c.tuner.choice.find(bucket);
-    if (it == c.tuner.choice.end() || it->second > 0)
-      launch_nonempty_cells(c.pstart.p, c.g.C, c.cell_list.p, c.list_count.p, c.s);
+    if (c.tuner.planned(bucket) != 0) launch_nonempty_cells(c.pstart.p, c.g.C, c.cell_list.p, c.list_count.p, c.s);
     auto launch = [&](int v) {
       bool listed;
       int lanes;
@@ -993,7 +996,7 @@ static bool graph_eligible(const Ctx& c) {
   if (!c.use_graph || c.analytical || c.scheme > 2 || !c.reorder || c.Np == 0) return false;
   if (c.eval_mode == 2 || (c.eval_mode == 0 && !c.memb_current) || !eval_blk_supported(c.nf) || !c.blk_valid) return false;
   if (!c.sorted_valid || !c.mem_pool.p) return false;
-  if (c.tuner.choice.find(density_bucket(c.Np, c.g.C)) == c.tuner.choice.end()) return false;
+  if (c.tuner.planned(density_bucket(c.Np, c.g.C)) < 0) return false;
   return setup_complete(c);
 }
 

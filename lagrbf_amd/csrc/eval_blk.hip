@@ -157,8 +157,12 @@ void k_eval_blk(Grid g, const double* __restrict__ px, int64_t ld,
   const int b = blockIdx.x;
   int c;
   if (cell_list) {
-    if (b >= *list_count) return;
-    c = cell_list[b];
+    // listed cells (ascending): XCD b % 8 takes a contiguous run of them (the grid
+    // is a multiple of 8 with at least as many blocks as listed cells)
+    const int cnt = *list_count, per = (cnt + 7) / 8;
+    const int pos = (b % 8) * per + b / 8;
+    if (b / 8 >= per || pos >= cnt) return;
+    c = cell_list[pos];
   } else {
     const int per = (gridDim.x + 7) / 8;
     c = (b % 8) * per + b / 8;
@@ -343,9 +347,15 @@ __global__ __launch_bounds__(256) void k_eval_sparse(Grid g, const double* __res
                                                      const int32_t* __restrict__ cell_list,
                                                      const int32_t* __restrict__ list_count) {
   constexpr int NG = 64 / G, NEN = 1 << DIM, NNB = DIM == 3 ? 27 : 9, BS = blk_stride<DIM, NFT>();
-  const int lane = threadIdx.x & 63, wave = (blockIdx.x * blockDim.x + threadIdx.x) >> 6;
+  const int lane = threadIdx.x & 63;
   const int gi = lane / G, mlane = lane % G;
   const int cnt = *list_count;
+  // blocks of 4 waves over the ascending cell list; XCD b % 8 takes a contiguous
+  // run of blocks (the grid is a multiple of 8 with at least the needed blocks)
+  const int need = (cnt + 4 * NG - 1) / (4 * NG), per = (need + 7) / 8;
+  const int lb = (blockIdx.x % 8) * per + blockIdx.x / 8;
+  if ((int)blockIdx.x / 8 >= per || lb >= need) return;
+  const int wave = lb * 4 + (threadIdx.x >> 6);
   if (wave * NG >= cnt) return;
   const int item = wave * NG + gi;
   const bool gvalid = item < cnt;
@@ -432,7 +442,7 @@ static void launch_sparse_g(int G, const Grid& g, const double* px, int64_t ld, 
   // waves: one per 64/G listed cells (at most min(C, np) cells are non-empty)
   const int ncell = std::min(g.C, np);
   const int waves = (ncell + 64 / G - 1) / (64 / G);
-  const dim3 grid((waves + 3) / 4), blkd(256);
+  const dim3 grid(((waves + 3) / 4 + 7) / 8 * 8), blkd(256);
 #define LRB_SP(GG) LRB_LAUNCH_TIMED((k_eval_sparse<DIM, KT, NFT, T, GG>), grid, blkd, s, g, px, ld, perm, pstart, w, blk, cs, nfields, out, cell_list, list_count)
   if (G == 8) LRB_SP(8);
   else if (G == 16) LRB_SP(16);
@@ -440,24 +450,70 @@ static void launch_sparse_g(int G, const Grid& g, const double* px, int64_t ld, 
 #undef LRB_SP
 }
 
-// Non-empty particle cells of a grouped swarm, compacted in any order (one atomic
-// per wave); list_count must be zero on entry.
-__global__ void k_nonempty_cells(const int32_t* __restrict__ pstart, int C, int32_t* __restrict__ list,
-                                 int32_t* __restrict__ count) {
-  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+// Non-empty particle cells of a grouped swarm, in ascending cell order: per-block
+// counts, an exclusive scan of the counts, then an ordered write.  `list` holds C
+// entries followed by ceil(C / 256) + 1 entries of scratch (the block offsets).
+__global__ __launch_bounds__(256) void k_ne_count(const int32_t* __restrict__ pstart, int C,
+                                                  int32_t* __restrict__ bcount) {
+  __shared__ int32_t wc[4];
+  const int c = blockIdx.x * 256 + threadIdx.x;
   const bool ne = c < C && pstart[c + 1] > pstart[c];
   const unsigned long long m = __ballot(ne);
-  const int lane = threadIdx.x & 63;
-  int base = 0;
-  if (lane == 0 && m) base = atomicAdd(count, (int)__popcll(m));
-  base = __shfl(base, 0);
-  if (ne) list[base + (int)__popcll(m & ((1ull << lane) - 1ull))] = c;
+  if ((threadIdx.x & 63) == 0) wc[threadIdx.x >> 6] = (int32_t)__popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) bcount[blockIdx.x] = wc[0] + wc[1] + wc[2] + wc[3];
 }
 
-void launch_nonempty_cells(const int32_t* pstart, int C, int32_t* list, int32_t* count, hipStream_t s) {
-  launch_fill_i32(count, 1, 0, s);   // a kernel, not a memset: the sequence may be captured in a graph
-  if (C > 0) hipLaunchKernelGGL(k_nonempty_cells, dim3((C + 255) / 256), dim3(256), 0, s, pstart, C, list, count);
+// One block: exclusive scan of nb block counts in place, total into *count.
+__global__ __launch_bounds__(256) void k_ne_scan(int32_t* __restrict__ bcount, int nb, int32_t* __restrict__ count) {
+  __shared__ int32_t sh[256];
+  const int t = threadIdx.x;
+  int32_t run = 0;
+  for (int base = 0; base < nb; base += 256) {
+    const int32_t v = base + t < nb ? bcount[base + t] : 0;
+    sh[t] = v;
+    __syncthreads();
+    for (int o = 1; o < 256; o <<= 1) {
+      const int32_t x = t >= o ? sh[t - o] : 0;
+      __syncthreads();
+      sh[t] += x;
+      __syncthreads();
+    }
+    if (base + t < nb) bcount[base + t] = run + sh[t] - v;
+    run += sh[255];
+    __syncthreads();
+  }
+  if (t == 0) *count = run;
 }
+
+__global__ __launch_bounds__(256) void k_ne_write(const int32_t* __restrict__ pstart, int C,
+                                                  const int32_t* __restrict__ boff, int32_t* __restrict__ list) {
+  __shared__ int32_t wc[4];
+  const int c = blockIdx.x * 256 + threadIdx.x;
+  const bool ne = c < C && pstart[c + 1] > pstart[c];
+  const unsigned long long m = __ballot(ne);
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  if (lane == 0) wc[wv] = (int32_t)__popcll(m);
+  __syncthreads();
+  int off = boff[blockIdx.x];
+  for (int k = 0; k < wv; k++) off += wc[k];
+  if (ne) list[off + (int)__popcll(m & ((1ull << lane) - 1ull))] = c;
+}
+
+// Kernels only: the sequence may sit in a captured graph.
+void launch_nonempty_cells(const int32_t* pstart, int C, int32_t* list, int32_t* count, hipStream_t s) {
+  if (C <= 0) {
+    launch_fill_i32(count, 1, 0, s);
+    return;
+  }
+  const int nb = (C + 255) / 256;
+  int32_t* boff = list + C;
+  hipLaunchKernelGGL(k_ne_count, dim3(nb), dim3(256), 0, s, pstart, C, boff);
+  hipLaunchKernelGGL(k_ne_scan, dim3(1), dim3(256), 0, s, boff, nb, count);
+  hipLaunchKernelGGL(k_ne_write, dim3(nb), dim3(256), 0, s, pstart, C, boff, list);
+}
+
+size_t nonempty_list_capacity(int C) { return (size_t)std::max(C, 1) + (size_t)(std::max(C, 1) + 255) / 256 + 1; }
 
 bool eval_blk_supported(int nft) { return nft >= 1 && nft <= 4; }
 size_t blk_row(int dim, int nft) { return 4 + ((size_t)1 << dim) * (size_t)nft; }
@@ -486,7 +542,7 @@ static void eval_blk_k(const Grid& g, const double* px, int64_t ld, const int32_
     return;
   }
   // dense: one block per cell; listed cells: one block per listed non-empty cell
-  const int nblk = cell_list ? std::min(g.C, np) : (g.C + 7) / 8 * 8;
+  const int nblk = cell_list ? (std::min(g.C, np) + 7) / 8 * 8 : (g.C + 7) / 8 * 8;
   const dim3 grid(nblk), blkd(block);
 #define LRB_EB(NFT) LRB_LAUNCH_TIMED((k_eval_blk<DIM, KT, NFT, T>), grid, blkd, s, g, px, ld, perm, pstart, w, blk, cs, nfields, out, cell_list, list_count)
   switch (nft) {

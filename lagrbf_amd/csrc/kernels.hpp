@@ -113,7 +113,9 @@ void launch_eval_blk(const Grid& g, int ktype, bool fp32, const double* px, int6
                      const int32_t* pstart, const double* w, const void* blk, int nft, const int32_t* cs, int nfields,
                      double* out, int block, int np, const int32_t* cell_list, const int32_t* list_count, int group,
                      hipStream_t s);
+// ascending non-empty cells; list needs nonempty_list_capacity(C) entries (the tail is scratch)
 void launch_nonempty_cells(const int32_t* pstart, int C, int32_t* list, int32_t* count, hipStream_t s);
+size_t nonempty_list_capacity(int C);
 void launch_sorted_coords32(const Grid& g, const double* x, int n, const int32_t* cell_nodes, const int32_t* cell,
                             float* xs, hipStream_t s);
 void launch_coef_to_f32(const int32_t* list, int nlist, const int32_t* pn, const int64_t* mem_off, int nf,

@@ -145,7 +145,7 @@ Batch* get_batch(Server& sv, int ds, int scheme) {
   b->scheme = scheme;
   if (hipSetDevice(b->ds->device) != hipSuccess || hipStreamCreate(&b->s) != hipSuccess) return nullptr;
   const int C = b->ds->g.C;
-  if (b->pstart.reserve((size_t)C + 2) != hipSuccess || b->cell_list.reserve((size_t)C) != hipSuccess ||
+  if (b->pstart.reserve((size_t)C + 2) != hipSuccess || b->cell_list.reserve(nonempty_list_capacity(C)) != hipSuccess ||
       b->list_count.reserve(1) != hipSuccess ||
       b->pscratch.reserve((size_t)C + 2 + (size_t)(C + 2) / 1024 + 64) != hipSuccess ||
       b->counters.reserve(4) != hipSuccess || b->err_req.reserve(RSLOTS) != hipSuccess ||

@@ -30,12 +30,28 @@ inline int density_bucket(int64_t np, int ncells) {
 struct EvalTuner {
   std::map<int, int> choice;   // density bucket -> variant
   std::map<int, float> best_ms;
+  int forced = -1;             // a fixed variant (option lagrbf_eval_variant), -1: tune
+
+  // variant run() will launch without timing candidates, or -1 when it will tune
+  int planned(int bucket) const {
+    if (forced >= 0) return forced;
+    auto it = choice.find(bucket);
+    return it == choice.end() ? -1 : it->second;
+  }
 
   // launch(v) enqueues the evaluation with variant v on stream s.  Returns the
   // variant used.  Candidates run back to back on the same inputs; each writes
   // the same values, so the output of the last one stands.
   template <typename F>
   int run(int bucket, bool grouped_ok, hipStream_t s, F&& launch) {
+    if (forced >= 0) {
+      bool listed;
+      int group;
+      eval_variant(forced, listed, group);
+      const int v = (group > 0 && !grouped_ok) ? 1 : forced;
+      launch(v);
+      return v;
+    }
     auto it = choice.find(bucket);
     if (it != choice.end()) {
       launch(it->second);

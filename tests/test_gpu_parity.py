@@ -458,3 +458,34 @@ def test_graph_steps_match_eager(scheme, precision):
         assert out[0][2] == out[1][2]
         assert ctxs[1].info("graph_steps") == 2 * (call + 1)
         assert ctxs[0].info("graph_steps") == 0
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("Np", [600, 60000])
+def test_eval_launch_shapes_identical(precision, Np):
+    """Every evaluation launch shape the autotuner chooses between (one block per
+    cell, per listed non-empty cell, lane groups of 32 / 16 / 8 over the ascending
+    cell list) gives the same values bit for bit, on a sparse and a dense swarm;
+    FP64 against the oracle at 1e-12."""
+    opts, X, V, P = cases.fourier_3d(nlat=20, Np=Np, n=9)
+    opts["advection"] = False
+    print(f"case: launch shapes fourier3d Np={Np} {precision}", flush=True)
+    vals = []
+    for v in range(5):
+        g = L.LagRBFCtx(petsc_options=dict(opts, lagrbf_eval_variant=v), precision=precision)
+        g.fill_data(X, V)
+        g.fill_points(P)
+        g.interpolate_c()
+        out = np.zeros((P.shape[0], 3))
+        g.output(fvals=out)
+        vals.append(out)
+        g.destroy()
+    for v in range(1, 5):
+        assert np.array_equal(vals[0], vals[v]), f"variant {v} differs"
+    if precision == "fp64":
+        from oracle.oracle import OracleCtx
+        o = OracleCtx(L.LagRBFCtx(petsc_options=dict(opts)).options)
+        o.fill_data(X, V); o.fill_points(P); o.interpolate_c()
+        vo = np.zeros_like(vals[0])
+        o.output(None, vo, None)
+        assert rel_err(vals[0], vo) < TOL_VAL
