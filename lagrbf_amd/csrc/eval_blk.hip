@@ -147,7 +147,7 @@ void k_eval_blk(Grid g, const double* __restrict__ px, int64_t ld,
                                                   const double* __restrict__ w, const T* __restrict__ blk,
                                                   const int32_t* __restrict__ cs, int nfields,
                                                   double* __restrict__ out, const int32_t* __restrict__ cell_list,
-                                                  const int32_t* __restrict__ list_count) {
+                                                  const int32_t* __restrict__ list_count, int xcd_runs) {
   constexpr int NEN = 1 << DIM;
   constexpr int NNB = DIM == 3 ? 27 : 9;
   constexpr int BS = blk_stride<DIM, NFT>();
@@ -158,10 +158,16 @@ void k_eval_blk(Grid g, const double* __restrict__ px, int64_t ld,
   int c;
   if (cell_list) {
     // listed cells (ascending): XCD b % 8 takes a contiguous run of them (the grid
-    // is a multiple of 8 with at least as many blocks as listed cells)
-    const int cnt = *list_count, per = (cnt + 7) / 8;
-    const int pos = (b % 8) * per + b / 8;
-    if (b / 8 >= per || pos >= cnt) return;
+    // is a multiple of 8 with at least as many blocks as listed cells), or block b
+    // takes entry b (consecutive cells on consecutive XCDs)
+    const int cnt = *list_count;
+    int pos = b;
+    if (xcd_runs) {
+      const int per = (cnt + 7) / 8;
+      pos = (b % 8) * per + b / 8;
+      if (b / 8 >= per) return;
+    }
+    if (pos >= cnt) return;
     c = cell_list[pos];
   } else {
     const int per = (gridDim.x + 7) / 8;
@@ -535,7 +541,7 @@ template <int DIM, int KT, typename T>
 static void eval_blk_k(const Grid& g, const double* px, int64_t ld, const int32_t* perm, const int32_t* pstart,
                        const double* w, const T* blk, int nft, const int32_t* cs, int nfields, double* out, int block,
                        int np, const int32_t* cell_list, const int32_t* list_count, int group, hipStream_t s) {
-  if (cell_list && group > 0 && nft == DIM) {
+  if (cell_list && group > 0 && nft == DIM) {   // group -1: listed, round-robin blocks
     // sparse velocity evaluation: lane groups of G particles, 64/G cells per wave
     launch_sparse_g<DIM, KT, DIM, T>(group, g, px, ld, perm, pstart, w, blk, cs, nfields, out, cell_list, list_count, np,
                                      s);
@@ -544,7 +550,8 @@ static void eval_blk_k(const Grid& g, const double* px, int64_t ld, const int32_
   // dense: one block per cell; listed cells: one block per listed non-empty cell
   const int nblk = cell_list ? (std::min(g.C, np) + 7) / 8 * 8 : (g.C + 7) / 8 * 8;
   const dim3 grid(nblk), blkd(block);
-#define LRB_EB(NFT) LRB_LAUNCH_TIMED((k_eval_blk<DIM, KT, NFT, T>), grid, blkd, s, g, px, ld, perm, pstart, w, blk, cs, nfields, out, cell_list, list_count)
+  const int xcd_runs = group >= 0 ? 1 : 0;
+#define LRB_EB(NFT) LRB_LAUNCH_TIMED((k_eval_blk<DIM, KT, NFT, T>), grid, blkd, s, g, px, ld, perm, pstart, w, blk, cs, nfields, out, cell_list, list_count, xcd_runs)
   switch (nft) {
     case 1: LRB_EB(1); break;
     case 2: LRB_EB(2); break;

@@ -13,11 +13,13 @@
 
 namespace lrb {
 
-constexpr int N_EVAL_VARIANTS = 5;
+constexpr int N_EVAL_VARIANTS = 6;
 
-// variant -> (evaluate listed cells only, lane group size; 0 = one block per cell)
+// variant -> (evaluate listed cells only, lane group size; 0 = one block per cell,
+// -1 = one block per listed cell with the blocks dealt round-robin over the XCDs
+// instead of in contiguous runs: better balanced when cells differ widely in work)
 inline void eval_variant(int v, bool& listed, int& group) {
-  static const int G[N_EVAL_VARIANTS] = {0, 0, 32, 16, 8};
+  static const int G[N_EVAL_VARIANTS] = {0, 0, 32, 16, 8, -1};
   listed = v > 0;
   group = G[v];
 }

@@ -464,14 +464,15 @@ def test_graph_steps_match_eager(scheme, precision):
 @pytest.mark.parametrize("Np", [600, 60000])
 def test_eval_launch_shapes_identical(precision, Np):
     """Every evaluation launch shape the autotuner chooses between (one block per
-    cell, per listed non-empty cell, lane groups of 32 / 16 / 8 over the ascending
-    cell list) gives the same values bit for bit, on a sparse and a dense swarm;
+    cell, per listed non-empty cell in XCD runs or round-robin, lane groups of
+    32 / 16 / 8 over the ascending cell list) gives the same values bit for bit, on
+    a sparse and a dense swarm;
     FP64 against the oracle at 1e-12."""
     opts, X, V, P = cases.fourier_3d(nlat=20, Np=Np, n=9)
     opts["advection"] = False
     print(f"case: launch shapes fourier3d Np={Np} {precision}", flush=True)
     vals = []
-    for v in range(5):
+    for v in range(6):
         g = L.LagRBFCtx(petsc_options=dict(opts, lagrbf_eval_variant=v), precision=precision)
         g.fill_data(X, V)
         g.fill_points(P)
@@ -480,7 +481,7 @@ def test_eval_launch_shapes_identical(precision, Np):
         g.output(fvals=out)
         vals.append(out)
         g.destroy()
-    for v in range(1, 5):
+    for v in range(1, 6):
         assert np.array_equal(vals[0], vals[v]), f"variant {v} differs"
     if precision == "fp64":
         from oracle.oracle import OracleCtx
