@@ -142,7 +142,8 @@ def run_path(L, precision, opts, X, V, P, steps, warmup, dt, sample):
         "roofline": {"bound": "valu", "kernel": "pu_evaluation (k_eval_blk)",
                      "achieved": achieved / 1e12, "peak": PEAK[precision] / 1e12, "unit": "TFLOP/s",
                      "frac": achieved / PEAK[precision], "traffic": traffic,
-                     "flops_per_launch": flops, "avg_launch_ms": avg_ms, "launches_timed": int(n_launch)},
+                     "flops_per_launch": flops, "avg_launch_ms": avg_ms, "launches_timed": int(n_launch),
+                     "launch_shape": int(ctx.info("eval_variant"))},
     }
     ctx.destroy()
     return rec, coefs, pos

@@ -670,7 +670,7 @@ static int rbf_interpolation(Ctx& c, double* out, int nfields, int group = -1) {
                       lanes, c.s);
     };
     set_eval_timing_events(e0, e1);
-    c.tuner.run(bucket, nft == c.g.dim && nfields == nft, c.s, launch);
+    c.last_variant = c.tuner.run(bucket, nft == c.g.dim && nfields == nft, c.s, launch);
     set_eval_timing_events(nullptr, nullptr);
   } else if (cell_order) {
     const int per_cell = (c.Np + c.g.C - 1) / std::max(c.g.C, 1);
@@ -1555,6 +1555,7 @@ int lagrbf_get_info(void* ctx, const char* key, double* value) {
   else if (k == "dp_accepted") *value = c->dp_accepted;
   else if (k == "dp_rejected") *value = c->dp_rejected;
   else if (k == "dp_active") *value = c->dp_active;
+  else if (k == "eval_variant") *value = c->last_variant;
   else if (k == "graph_steps") *value = c->graph_steps;
   else if (k == "graph_ms") *value = c->graph_ms;
   else if (k == "setup_ms") *value = c->setup_ms;
