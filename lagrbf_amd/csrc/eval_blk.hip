@@ -64,11 +64,25 @@ __host__ __device__ constexpr int corner_in_nbr(int nbi, int k) {
   return in ? q1_index(a, b, c) : -1;
 }
 
+// how many corner patches of the particle cell hold neighbour nbi
+template <int DIM>
+__host__ __device__ constexpr int n_corners_in_nbr(int nbi) {
+  int n = 0;
+  for (int k = 0; k < (1 << DIM); k++) n += corner_in_nbr<DIM>(nbi, k) >= 0 ? 1 : 0;
+  return n;
+}
+
 template <int KT>
 __host__ __device__ constexpr bool compact_support_blk() { return KT == K_WC0 || KT == K_WC2 || KT == K_BUMP; }
 
 template <int DIM, int NFT>
 __host__ __device__ constexpr int blk_stride() { return 4 + (1 << DIM) * NFT; }
+
+// |d|^2 with one fixed rounding: every launch shape computes it through this
+// helper, so they agree bit for bit whatever contraction the compiler would pick
+// for the plain expression (fusing d0*d0 or d1*d1 gives different roundings).
+template <typename T>
+__device__ __forceinline__ T dist2(T d0, T d1, T d2) { return fma(d2, d2, fma(d1, d1, d0 * d0)); }
 
 // xp, yp, zp: particle coordinates in the frame of the neighbour cell's node rows
 // (FP64: absolute; FP32: relative to the neighbour cell's origin).
@@ -85,7 +99,7 @@ __device__ __forceinline__ void accum_blk(T xp, T yp, T zp, const T* __restrict_
     n0 = Br[BS]; n1 = Br[BS + 1];
     if constexpr (DIM == 3) n2 = Br[BS + 2];
     const T d0 = xp - c0, d1 = yp - c1, d2 = DIM == 3 ? zp - c2 : T(0);
-    const T r2 = d0 * d0 + d1 * d1 + d2 * d2;
+    const T r2 = dist2(d0, d1, d2);
     if constexpr (compact_support_blk<KT>()) {
       // every lane beyond the support: phi = 0 exactly, nothing to add
       if (!__any(r2 < rcut2)) continue;
@@ -101,6 +115,97 @@ __device__ __forceinline__ void accum_blk(T xp, T yp, T zp, const T* __restrict_
     }
   }
 }
+
+// Wait for every scalar load in flight (s_waitcnt lgkmcnt(0); vmcnt and expcnt
+// left at their maxima).  Scalar loads return out of order, so this is the only
+// wait there is for them.
+__device__ __forceinline__ void wait_scalar_loads() { __builtin_amdgcn_s_waitcnt(0xC07F); }
+
+// One table row's entries that neighbour NBI uses, as wave-uniform values.
+template <int DIM, int NFT, typename T>
+struct BlkRow {
+  T v[blk_stride<DIM, NFT>()];
+};
+
+template <int DIM, int NFT, typename T, int NBI>
+__device__ __forceinline__ void load_row(BlkRow<DIM, NFT, T>& row, const T* __restrict__ p) {
+  constexpr int NEN = 1 << DIM;
+  // all four leading entries, pad included, so that they arrive as one 16-byte load
+  row.v[0] = p[0];
+  row.v[1] = p[1];
+  row.v[2] = p[2];
+  row.v[3] = p[3];
+#pragma unroll
+  for (int k = 0; k < NEN; k++) {
+    const int kp = corner_in_nbr<DIM>(NBI, k);
+    if (kp >= 0) {
+#pragma unroll
+      for (int f = 0; f < NFT; f++) row.v[4 + kp * NFT + f] = p[4 + kp * NFT + f];
+    }
+  }
+}
+
+template <int DIM, int KT, int NFT, typename T, int NBI>
+__device__ __forceinline__ void node_blk(const BlkRow<DIM, NFT, T>& row, T xp, T yp, T zp, T (&acc)[1 << DIM][NFT],
+                                         T eps, T rcut2) {
+  constexpr int NEN = 1 << DIM;
+  const T d0 = xp - row.v[0], d1 = yp - row.v[1], d2 = DIM == 3 ? zp - row.v[2] : T(0);
+  const T r2 = dist2(d0, d1, d2);
+  // every lane beyond a compact support: phi = 0 exactly, nothing to add
+  if (compact_support_blk<KT>() && !__any(r2 < rcut2)) return;
+  const T phi = rbf<KT, T>(sqrt_fast(r2), eps);
+#pragma unroll
+  for (int k = 0; k < NEN; k++) {
+    const int kp = corner_in_nbr<DIM>(NBI, k);
+    if (kp >= 0) {
+#pragma unroll
+      for (int f = 0; f < NFT; f++) acc[k][f] += phi * row.v[4 + kp * NFT + f];
+    }
+  }
+}
+
+// accum_blk with the scalar loads of row r + 1 in flight while row r is computed:
+// two rows alternate in registers, and the wait for the next row comes after the
+// current one's arithmetic.  Same operations in the same node order as accum_blk,
+// so the same values.  Row m, one past the neighbour's last node, may be loaded:
+// it is the next cell's first row or the table's padding row (row N).
+template <int DIM, int KT, int NFT, typename T, int NBI>
+__device__ __forceinline__ void accum_blk_pipe(T xp, T yp, T zp, const T* __restrict__ B, int m,
+                                               T (&acc)[1 << DIM][NFT], T eps, T rcut2) {
+  constexpr int BS = blk_stride<DIM, NFT>();
+  BlkRow<DIM, NFT, T> ra, rb;
+  load_row<DIM, NFT, T, NBI>(ra, B);
+  wait_scalar_loads();
+  const T* q = B + BS;   // the next row to load
+  int r = m;
+  // the empty asm statements keep the loads where they are written: without them
+  // the compiler sinks a row's coefficient loads into the branch that uses them,
+  // right before their first use
+  while (r >= 2) {
+    load_row<DIM, NFT, T, NBI>(rb, q);
+    q += BS;
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_sched_barrier(0);
+    node_blk<DIM, KT, NFT, T, NBI>(ra, xp, yp, zp, acc, eps, rcut2);
+    __builtin_amdgcn_sched_barrier(0);
+    wait_scalar_loads();
+    asm volatile("" ::"s"(rb.v[2]), "s"(rb.v[3]));   // keeps the 16-byte load whole
+    load_row<DIM, NFT, T, NBI>(ra, q);
+    q += BS;
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_sched_barrier(0);
+    node_blk<DIM, KT, NFT, T, NBI>(rb, xp, yp, zp, acc, eps, rcut2);
+    __builtin_amdgcn_sched_barrier(0);
+    wait_scalar_loads();
+    asm volatile("" ::"s"(ra.v[2]), "s"(ra.v[3]));
+    r -= 2;
+  }
+  if (r == 1) node_blk<DIM, KT, NFT, T, NBI>(ra, xp, yp, zp, acc, eps, rcut2);
+}
+
+#ifndef LRB_EVAL_PIPE
+#define LRB_EVAL_PIPE 1
+#endif
 
 // The 3^d neighbour cells in ascending cell index, as one straight sequence of
 // loops specialised per neighbour (no dispatch between them, so the accumulators
@@ -126,7 +231,12 @@ __device__ __forceinline__ void accum_nbrs(const Grid& g, int ci, int cj, int ck
           qx = xp - (T)(di * g.h[0]); qy = yp - (T)(dj * g.h[1]);
           if constexpr (DIM == 3) qz = zp - (T)(dk * g.h[2]);
         }
-        accum_blk<DIM, KT, NFT, T, NBI>(qx, qy, qz, blk + (size_t)nb * BS, ne - nb, acc, eps, rcut2);
+        // two FP64 rows of the centre neighbour (all 2^d corners) would not fit the
+        // scalar registers: it keeps the single-row loop
+        if constexpr (LRB_EVAL_PIPE && (sizeof(T) == 4 || n_corners_in_nbr<DIM>(NBI) < (1 << DIM)))
+          accum_blk_pipe<DIM, KT, NFT, T, NBI>(qx, qy, qz, blk + (size_t)nb * BS, ne - nb, acc, eps, rcut2);
+        else
+          accum_blk<DIM, KT, NFT, T, NBI>(qx, qy, qz, blk + (size_t)nb * BS, ne - nb, acc, eps, rcut2);
       }
     }
     accum_nbrs<DIM, KT, NFT, T, NBI + 1>(g, ci, cj, ck, xp, yp, zp, blk, cs, acc, eps, rcut2);
@@ -228,7 +338,11 @@ void k_eval_blk(Grid g, const double* __restrict__ px, int64_t ld,
         const T qz = DIM == 3 ? zp - (T)(dk * g.h[2]) : T(0);
         const T* B = blk + (size_t)nb * BS;
         switch (nbi) {
+#if LRB_EVAL_PIPE
+#define LRB_CASE(V) case V: accum_blk_pipe<DIM, KT, NFT, T, V>(qx, qy, qz, B, ne - nb, acc, eps, rcut2); break;
+#else
 #define LRB_CASE(V) case V: accum_blk<DIM, KT, NFT, T, V>(qx, qy, qz, B, ne - nb, acc, eps, rcut2); break;
+#endif
           LRB_CASE(0) LRB_CASE(1) LRB_CASE(2) LRB_CASE(3) LRB_CASE(4) LRB_CASE(5) LRB_CASE(6) LRB_CASE(7) LRB_CASE(8)
           LRB_CASE(9) LRB_CASE(10) LRB_CASE(11) LRB_CASE(12) LRB_CASE(13) LRB_CASE(14) LRB_CASE(15) LRB_CASE(16)
           LRB_CASE(17) LRB_CASE(18) LRB_CASE(19) LRB_CASE(20) LRB_CASE(21) LRB_CASE(22) LRB_CASE(23) LRB_CASE(24)
@@ -321,7 +435,7 @@ __device__ __forceinline__ void accum_sparse(T xp, T yp, T zp, const T* __restri
     const T* Br = B + (size_t)r * BS;
     if (act) {
       const T d0 = xp - Br[0], d1 = yp - Br[1], d2 = DIM == 3 ? zp - Br[2] : T(0);
-      r2 = d0 * d0 + d1 * d1 + d2 * d2;
+      r2 = dist2(d0, d1, d2);
     }
     if constexpr (compact_support_blk<KT>()) {
       if (!__any(r2 < rcut2)) continue;
