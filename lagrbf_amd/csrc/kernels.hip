@@ -473,16 +473,25 @@ __device__ __forceinline__ void weights_point(const Grid& g, const double* __res
   // cells ci - m .. ci + 1 + m per axis, m = ceil(R / h) (R widened by 1e-12); the
   // nodes outside it add exact zeros to the reference's sum over all PU nodes
   // (swarm_interp.c:327-372), which is accumulated here in the same ascending
-  // node order.  The window depends on the cell only, so lanes of a wave that
-  // share a cell run the same trip counts.
+  // node order.  The window is then narrowed per particle to the nodes within R
+  // per axis: with R below the cell size that is 1-2 nodes per axis instead of 4.
   const int ci = cell % g.nc[0], cj = (cell / g.nc[0]) % g.nc[1], ck = g.dim == 3 ? cell / (g.nc[0] * g.nc[1]) : 0;
   const double R = (1.0 / g.eps_w) * (1.0 + 1e-12);
   const int cc[3] = {ci, cj, ck};
+  const double xc[3] = {x, y, z};
   int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
   for (int d = 0; d < g.dim; d++) {
     const int m = (int)ceil(R / g.h[d]);
     lo[d] = max(cc[d] - m, 0);
     hi[d] = min(cc[d] + 1 + m, g.n[d] - 1);
+    // narrowed per particle: node i can lie within R only if |x_d - (O_d + i h_d)| <= R,
+    // i.e. t - R/h <= i <= t + R/h with t = (x_d - O_d) / h_d (bounds widened by 1e-6
+    // of a cell against rounding); the nodes dropped add exact zeros
+    const double t = (xc[d] - g.O[d]) / g.h[d], rh = R / g.h[d];
+    if (fabs(t) < 1e9) {   // a NaN coordinate keeps the cell window (and NaN weights)
+      lo[d] = max(lo[d], (int)ceil(t - rh - 1e-6));
+      hi[d] = min(hi[d], (int)floor(t + rh + 1e-6));
+    }
   }
   double sum = 0.0;
   for (int k = 0; k < g.nen; k++) w[k] = 0.0;
