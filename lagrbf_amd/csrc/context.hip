@@ -556,7 +556,7 @@ static int partitions_reset(Ctx& c) {
 // Cell-major node table for the evaluation, rebuilt after the data or any patch changed.
 int ensure_blk(Ctx& c, bool f32) {
   if (c.blk_valid && c.blk_f32 == f32) return 0;
-  const size_t n = ((size_t)c.N + 1) * blk_row(c.g.dim, c.nf);   // N rows + padding row
+  const size_t n = ((size_t)c.N + BLK_PAD_ROWS) * blk_row(c.g.dim, c.nf);   // N rows + padding rows
   if (f32) HIPCHK(c.blk32.reserve(n));
   else HIPCHK(c.blk64.reserve(n));
   if (c.N > 0)
@@ -608,7 +608,7 @@ static void release_events(Ctx& c) {
 static int ensure_blk_group(Ctx& c, bool f32, int g) {
   if (c.blkg_gen[g] == c.table_gen && c.blkg_f32[g] == f32) return 0;
   const int dim = c.g.dim;
-  const size_t n = ((size_t)c.N + 1) * blk_row(dim, dim);
+  const size_t n = ((size_t)c.N + BLK_PAD_ROWS) * blk_row(dim, dim);
   if (f32) HIPCHK(c.blkg32[g].reserve(n));
   else HIPCHK(c.blkg64[g].reserve(n));
   if (c.N > 0)
@@ -660,17 +660,19 @@ static int rbf_interpolation(Ctx& c, double* out, int nfields, int group = -1) {
     const int per_cell = (c.Np + c.g.C - 1) / std::max(c.g.C, 1);
     const int block = std::min(256, std::max(64, (per_cell + 63) / 64 * 64));
     const int bucket = density_bucket(c.Np, c.g.C);
-    if (c.tuner.planned(bucket) != 0) launch_nonempty_cells(c.pstart.p, c.g.C, c.cell_list.p, c.list_count.p, c.s);
+    const int planned = c.tuner.planned(bucket);
+    if (planned < 0 || eval_variant_listed(planned))
+      launch_nonempty_cells(c.pstart.p, c.g.C, c.cell_list.p, c.list_count.p, c.s);
     auto launch = [&](int v) {
       bool listed;
-      int lanes;
-      eval_variant(v, listed, lanes);
+      int lanes, pipe;
+      eval_variant(v, listed, lanes, pipe);
       launch_eval_blk(c.g, c.ktype, blk32, c.px.p, c.ld, c.perm.p, c.pstart.p, c.w.p, table, nft, c.cell_start.p,
                       nfields, out, block, c.Np, listed ? c.cell_list.p : nullptr, listed ? c.list_count.p : nullptr,
-                      lanes, c.s);
+                      lanes, pipe, c.s);
     };
     set_eval_timing_events(e0, e1);
-    c.last_variant = c.tuner.run(bucket, nft == c.g.dim && nfields == nft, c.s, launch);
+    c.last_variant = c.tuner.run(bucket, nft == c.g.dim && nfields == nft, blk32, c.s, launch);
     set_eval_timing_events(nullptr, nullptr);
   } else if (cell_order) {
     const int per_cell = (c.Np + c.g.C - 1) / std::max(c.g.C, 1);

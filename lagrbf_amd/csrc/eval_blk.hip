@@ -203,6 +203,54 @@ __device__ __forceinline__ void accum_blk_pipe(T xp, T yp, T zp, const T* __rest
   if (r == 1) node_blk<DIM, KT, NFT, T, NBI>(ra, xp, yp, zp, acc, eps, rcut2);
 }
 
+// Two rows per wait, for rows of at most 16 entries: rows j, j + 1 are computed
+// while j + 2, j + 3 load, so each load has two nodes of arithmetic to land in.
+// Same node order and operations as accum_blk.  Up to two rows past the
+// neighbour's last node may be loaded: the next cell's rows or the table's
+// padding rows N, N + 1.
+template <int DIM, int KT, int NFT, typename T, int NBI>
+__device__ __forceinline__ void accum_blk_pipe2(T xp, T yp, T zp, const T* __restrict__ B, int m,
+                                                T (&acc)[1 << DIM][NFT], T eps, T rcut2) {
+  constexpr int BS = blk_stride<DIM, NFT>();
+  BlkRow<DIM, NFT, T> a0, a1, b0, b1;
+  load_row<DIM, NFT, T, NBI>(a0, B);
+  load_row<DIM, NFT, T, NBI>(a1, B + BS);
+  wait_scalar_loads();
+  const T* q = B + 2 * BS;   // the next row to load
+  int r = m;
+  while (r >= 4) {
+    load_row<DIM, NFT, T, NBI>(b0, q);
+    load_row<DIM, NFT, T, NBI>(b1, q + BS);
+    q += 2 * BS;
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_sched_barrier(0);
+    node_blk<DIM, KT, NFT, T, NBI>(a0, xp, yp, zp, acc, eps, rcut2);
+    node_blk<DIM, KT, NFT, T, NBI>(a1, xp, yp, zp, acc, eps, rcut2);
+    __builtin_amdgcn_sched_barrier(0);
+    wait_scalar_loads();
+    asm volatile("" ::"s"(b0.v[2]), "s"(b0.v[3]), "s"(b1.v[2]), "s"(b1.v[3]));
+    load_row<DIM, NFT, T, NBI>(a0, q);
+    load_row<DIM, NFT, T, NBI>(a1, q + BS);
+    q += 2 * BS;
+    asm volatile("" ::: "memory");
+    __builtin_amdgcn_sched_barrier(0);
+    node_blk<DIM, KT, NFT, T, NBI>(b0, xp, yp, zp, acc, eps, rcut2);
+    node_blk<DIM, KT, NFT, T, NBI>(b1, xp, yp, zp, acc, eps, rcut2);
+    __builtin_amdgcn_sched_barrier(0);
+    wait_scalar_loads();
+    asm volatile("" ::"s"(a0.v[2]), "s"(a0.v[3]), "s"(a1.v[2]), "s"(a1.v[3]));
+    r -= 4;
+  }
+  // 0 <= r < 4 rows left; a0, a1 hold the first two of them
+  if (r >= 1) node_blk<DIM, KT, NFT, T, NBI>(a0, xp, yp, zp, acc, eps, rcut2);
+  if (r >= 2) node_blk<DIM, KT, NFT, T, NBI>(a1, xp, yp, zp, acc, eps, rcut2);
+  if (r == 3) {
+    load_row<DIM, NFT, T, NBI>(b0, q);
+    wait_scalar_loads();
+    node_blk<DIM, KT, NFT, T, NBI>(b0, xp, yp, zp, acc, eps, rcut2);
+  }
+}
+
 #ifndef LRB_EVAL_PIPE
 #define LRB_EVAL_PIPE 1
 #endif
@@ -250,7 +298,9 @@ __device__ __forceinline__ void accum_nbrs(const Grid& g, int ci, int cj, int ck
 #ifndef LRB_EVAL_WAVES_F32
 #define LRB_EVAL_WAVES_F32 8
 #endif
-template <int DIM, int KT, int NFT, typename T>
+// PIPE (FP32 only): table rows per wait in the node loops of the non-centre
+// neighbours, 1 or 2 (the autotuner chooses; each suits other launch shapes)
+template <int DIM, int KT, int NFT, typename T, int PIPE>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? LRB_EVAL_WAVES_F64 : LRB_EVAL_WAVES_F32)))
 void k_eval_blk(Grid g, const double* __restrict__ px, int64_t ld,
                                                   const int32_t* __restrict__ perm, const int32_t* __restrict__ pstart,
@@ -339,7 +389,13 @@ void k_eval_blk(Grid g, const double* __restrict__ px, int64_t ld,
         const T* B = blk + (size_t)nb * BS;
         switch (nbi) {
 #if LRB_EVAL_PIPE
-#define LRB_CASE(V) case V: accum_blk_pipe<DIM, KT, NFT, T, V>(qx, qy, qz, B, ne - nb, acc, eps, rcut2); break;
+#define LRB_CASE(V)                                                                                     \
+  case V:                                                                                               \
+    if constexpr (PIPE == 2 && 4 + n_corners_in_nbr<DIM>(V) * NFT <= 16)                                 \
+      accum_blk_pipe2<DIM, KT, NFT, T, V>(qx, qy, qz, B, ne - nb, acc, eps, rcut2);                     \
+    else                                                                                                \
+      accum_blk_pipe<DIM, KT, NFT, T, V>(qx, qy, qz, B, ne - nb, acc, eps, rcut2);                      \
+    break;
 #else
 #define LRB_CASE(V) case V: accum_blk<DIM, KT, NFT, T, V>(qx, qy, qz, B, ne - nb, acc, eps, rcut2); break;
 #endif
@@ -369,7 +425,7 @@ void k_eval_blk(Grid g, const double* __restrict__ px, int64_t ld,
 }
 
 // Table row s: node cn[s] of cell q = dcell[cn[s]], r = s - cs[q] its rank in q;
-// row N is zero padding.
+// rows N .. N + BLK_PAD_ROWS - 1 are zero padding.
 // Coefficient of corner kp: patch p = node kp of q; member index = (nodes of the
 // cells of p before q, ascending cell index) + r.
 template <typename T>
@@ -378,10 +434,10 @@ __global__ void k_build_blk(Grid g, const double* __restrict__ x, int N, const i
                             const uint8_t* __restrict__ active, const int64_t* __restrict__ mem_off,
                             const double* __restrict__ coef, int nft, int cstride, int f0, T* __restrict__ blk) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s > N) return;
+  if (s >= N + BLK_PAD_ROWS) return;
   const int dim = g.dim, BS = 4 + g.nen * nft;
-  if (s == N) {
-    for (int e = 0; e < BS; e++) blk[(size_t)N * BS + e] = T(0);
+  if (s >= N) {
+    for (int e = 0; e < BS; e++) blk[(size_t)s * BS + e] = T(0);
     return;
   }
   const int j = cn[s], q = dcell[j], r = s - cs[q];
@@ -642,19 +698,20 @@ void launch_build_blk(const Grid& g, bool fp32, const double* x, int N, const in
                       const int32_t* cs, const uint8_t* active, const int64_t* mem_off, const double* coef, int nft,
                       int cstride, int f0, void* blk, hipStream_t s) {
   if (N <= 0) return;
-  // N + 1 threads: rows 0..N-1 and the padding row N
+  // N + BLK_PAD_ROWS threads: rows 0..N-1 and the zero padding rows
+  const int nt = N + BLK_PAD_ROWS;
   if (fp32)
-    hipLaunchKernelGGL(k_build_blk<float>, dim3((N + 256) / 256), dim3(256), 0, s, g, x, N, cn, dcell, cs, active,
+    hipLaunchKernelGGL(k_build_blk<float>, dim3((nt + 255) / 256), dim3(256), 0, s, g, x, N, cn, dcell, cs, active,
                        mem_off, coef, nft, cstride, f0, (float*)blk);
   else
-    hipLaunchKernelGGL(k_build_blk<double>, dim3((N + 256) / 256), dim3(256), 0, s, g, x, N, cn, dcell, cs, active,
+    hipLaunchKernelGGL(k_build_blk<double>, dim3((nt + 255) / 256), dim3(256), 0, s, g, x, N, cn, dcell, cs, active,
                        mem_off, coef, nft, cstride, f0, (double*)blk);
 }
 
 template <int DIM, int KT, typename T>
 static void eval_blk_k(const Grid& g, const double* px, int64_t ld, const int32_t* perm, const int32_t* pstart,
                        const double* w, const T* blk, int nft, const int32_t* cs, int nfields, double* out, int block,
-                       int np, const int32_t* cell_list, const int32_t* list_count, int group, hipStream_t s) {
+                       int np, const int32_t* cell_list, const int32_t* list_count, int group, int pipe, hipStream_t s) {
   if (cell_list && group > 0 && nft == DIM) {   // group -1: listed, round-robin blocks
     // sparse velocity evaluation: lane groups of G particles, 64/G cells per wave
     launch_sparse_g<DIM, KT, DIM, T>(group, g, px, ld, perm, pstart, w, blk, cs, nfields, out, cell_list, list_count, np,
@@ -665,12 +722,19 @@ static void eval_blk_k(const Grid& g, const double* px, int64_t ld, const int32_
   const int nblk = cell_list ? (std::min(g.C, np) + 7) / 8 * 8 : (g.C + 7) / 8 * 8;
   const dim3 grid(nblk), blkd(block);
   const int xcd_runs = group >= 0 ? 1 : 0;
-#define LRB_EB(NFT) LRB_LAUNCH_TIMED((k_eval_blk<DIM, KT, NFT, T>), grid, blkd, s, g, px, ld, perm, pstart, w, blk, cs, nfields, out, cell_list, list_count, xcd_runs)
+#define LRB_EB(NFT, P) LRB_LAUNCH_TIMED((k_eval_blk<DIM, KT, NFT, T, P>), grid, blkd, s, g, px, ld, perm, pstart, w, blk, cs, nfields, out, cell_list, list_count, xcd_runs)
+  if constexpr (sizeof(T) == 4) {
+    // two rows per wait: the velocity evaluations (nft = dim) only
+    if (pipe == 2 && nft == DIM) {
+      LRB_EB(DIM, 2);
+      return;
+    }
+  }
   switch (nft) {
-    case 1: LRB_EB(1); break;
-    case 2: LRB_EB(2); break;
-    case 3: LRB_EB(3); break;
-    default: LRB_EB(4); break;
+    case 1: LRB_EB(1, 1); break;
+    case 2: LRB_EB(2, 1); break;
+    case 3: LRB_EB(3, 1); break;
+    default: LRB_EB(4, 1); break;
   }
 #undef LRB_EB
 }
@@ -679,8 +743,8 @@ template <int DIM, typename T>
 static void eval_blk_dim(const Grid& g, int ktype, const double* px, int64_t ld, const int32_t* perm,
                          const int32_t* pstart, const double* w, const T* blk, int nft, const int32_t* cs, int nfields,
                          double* out, int block, int np, const int32_t* cell_list, const int32_t* list_count,
-                         int group, hipStream_t s) {
-#define LRB_K(K) eval_blk_k<DIM, K, T>(g, px, ld, perm, pstart, w, blk, nft, cs, nfields, out, block, np, cell_list, list_count, group, s)
+                         int group, int pipe, hipStream_t s) {
+#define LRB_K(K) eval_blk_k<DIM, K, T>(g, px, ld, perm, pstart, w, blk, nft, cs, nfields, out, block, np, cell_list, list_count, group, pipe, s)
   if constexpr (sizeof(T) == 8) {
     switch (ktype) {
       case K_LINEAR: LRB_K(K_LINEAR); break;
@@ -705,9 +769,9 @@ static void eval_blk_dim(const Grid& g, int ktype, const double* px, int64_t ld,
 void launch_eval_blk(const Grid& g, int ktype, bool fp32, const double* px, int64_t ld, const int32_t* perm,
                      const int32_t* pstart, const double* w, const void* blk_, int nft, const int32_t* cs, int nfields,
                      double* out, int block, int np, const int32_t* cell_list, const int32_t* list_count,
-                     int group, hipStream_t s) {
+                     int group, int pipe, hipStream_t s) {
   if (nfields <= 0 || g.C <= 0 || np <= 0) return;
-#define LRB_ARGS px, ld, perm, pstart, w, blk, nft, cs, nfields, out, block, np, cell_list, list_count, group, s
+#define LRB_ARGS px, ld, perm, pstart, w, blk, nft, cs, nfields, out, block, np, cell_list, list_count, group, pipe, s
   if (fp32) {
     const float* blk = (const float*)blk_;
     if (g.dim == 3) eval_blk_dim<3, float>(g, ktype, LRB_ARGS);

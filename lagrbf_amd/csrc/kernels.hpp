@@ -100,6 +100,9 @@ void launch_eval_cell(const Grid& g, int ktype, bool fp32, const double* px, int
 // the 2^d corner patches of its cell]; evaluation for nft <= 4 fields
 bool eval_blk_supported(int nft);
 size_t blk_row(int dim, int nft);
+// zero rows after the N node rows of the evaluation table (the node loops read up
+// to two rows past a cell's last node)
+constexpr int BLK_PAD_ROWS = 2;
 void launch_build_blk(const Grid& g, bool fp32, const double* x, int N, const int32_t* cn, const int32_t* dcell,
                       const int32_t* cs, const uint8_t* active, const int64_t* mem_off, const double* coef, int nft,
                       int cstride, int f0, void* blk, hipStream_t s);   // fields f0 .. f0+nft-1 of cstride
@@ -112,7 +115,7 @@ void set_eval_timing_events(hipEvent_t start, hipEvent_t stop);
 void launch_eval_blk(const Grid& g, int ktype, bool fp32, const double* px, int64_t ld, const int32_t* perm,
                      const int32_t* pstart, const double* w, const void* blk, int nft, const int32_t* cs, int nfields,
                      double* out, int block, int np, const int32_t* cell_list, const int32_t* list_count, int group,
-                     hipStream_t s);
+                     int pipe, hipStream_t s);
 // ascending non-empty cells; list needs nonempty_list_capacity(C) entries (the tail is scratch)
 void launch_nonempty_cells(const int32_t* pstart, int C, int32_t* list, int32_t* count, hipStream_t s);
 size_t nonempty_list_capacity(int C);

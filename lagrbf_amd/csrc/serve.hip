@@ -203,13 +203,13 @@ void eval_batch(Batch& b, double* out) {
   const int block = std::min(256, std::max(64, (per_cell + 63) / 64 * 64));
   auto launch = [&](int v) {
     bool listed;
-    int grp;
-    eval_variant(v, listed, grp);
+    int grp, pipe;
+    eval_variant(v, listed, grp, pipe);
     launch_eval_blk(d.g, d.ktype, f32, b.px.p, b.ld, b.perm.p, b.pstart.p, b.w.p,
                     f32 ? (const void*)d.blk32.p : (const void*)d.blk64.p, d.nf, d.cell_start.p, d.g.dim, out, block,
-                    b.Np, listed ? b.cell_list.p : nullptr, listed ? b.list_count.p : nullptr, grp, b.s);
+                    b.Np, listed ? b.cell_list.p : nullptr, listed ? b.list_count.p : nullptr, grp, pipe, b.s);
   };
-  d.tuner.run(density_bucket(b.Np, d.g.C), d.nf == d.g.dim, b.s, launch);
+  d.tuner.run(density_bucket(b.Np, d.g.C), d.nf == d.g.dim, f32, b.s, launch);
 }
 
 void group(Batch& b) {

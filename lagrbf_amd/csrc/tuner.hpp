@@ -1,6 +1,7 @@
 // Evaluation autotuner: the PU evaluation has several launch shapes with the same
 // per-particle arithmetic (one block per cell; one block per listed non-empty
-// cell; lane groups of 32, 16 or 8 particles with several cells per wave).
+// cell, in contiguous XCD runs or round-robin; lane groups of 32, 16 or 8
+// particles with several cells per wave; one or two table rows per wait).
 // Which is fastest depends on the particles per cell and the dataset, so the
 // first evaluation in each density bucket (particles per cell, half-octave
 // steps) times every valid shape on the live launch and keeps the fastest.
@@ -13,15 +14,28 @@
 
 namespace lrb {
 
-constexpr int N_EVAL_VARIANTS = 6;
+constexpr int N_EVAL_VARIANTS = 9;
 
 // variant -> (evaluate listed cells only, lane group size; 0 = one block per cell,
 // -1 = one block per listed cell with the blocks dealt round-robin over the XCDs
-// instead of in contiguous runs: better balanced when cells differ widely in work)
-inline void eval_variant(int v, bool& listed, int& group) {
-  static const int G[N_EVAL_VARIANTS] = {0, 0, 32, 16, 8, -1};
-  listed = v > 0;
+// instead of in contiguous runs: better balanced when cells differ widely in work;
+// rows per wait of the FP32 node loop, 1 or 2: variants 6-8 are 0, 1 and 5 with two)
+inline void eval_variant(int v, bool& listed, int& group, int& pipe) {
+  static const int G[N_EVAL_VARIANTS] = {0, 0, 32, 16, 8, -1, 0, 0, -1};
+  static const int L[N_EVAL_VARIANTS] = {0, 1, 1, 1, 1, 1, 0, 1, 1};
+  listed = L[v] != 0;
   group = G[v];
+  pipe = v >= 6 ? 2 : 1;
+}
+
+// the same shape with one row per wait (the FP64 evaluation has no two-row loop)
+inline int eval_variant_pipe1(int v) { return v == 6 ? 0 : v == 7 ? 1 : v == 8 ? 5 : v; }
+
+inline bool eval_variant_listed(int v) {
+  bool listed;
+  int group, pipe;
+  eval_variant(v, listed, group, pipe);
+  return listed;
 }
 
 inline int density_bucket(int64_t np, int ncells) {
@@ -30,7 +44,13 @@ inline int density_bucket(int64_t np, int ncells) {
 }
 
 struct EvalTuner {
+  // Trial rounds per bucket: the first evaluation of a swarm may still see it in
+  // fill order (the swarm is stored in cell order at each RK step), which favours
+  // other shapes than the steady state, so the bucket's next evaluation times the
+  // candidates again and that choice stays.
+  static constexpr int TRIAL_ROUNDS = 2;
   std::map<int, int> choice;   // density bucket -> variant
+  std::map<int, int> rounds;   // density bucket -> trial rounds run
   std::map<int, float> best_ms;
   int forced = -1;             // a fixed variant (option lagrbf_eval_variant), -1: tune
 
@@ -38,26 +58,42 @@ struct EvalTuner {
   int planned(int bucket) const {
     if (forced >= 0) return forced;
     auto it = choice.find(bucket);
-    return it == choice.end() ? -1 : it->second;
+    auto r = rounds.find(bucket);
+    return it == choice.end() || r == rounds.end() || r->second < TRIAL_ROUNDS ? -1 : it->second;
   }
 
   // launch(v) enqueues the evaluation with variant v on stream s.  Returns the
   // variant used.  Candidates run back to back on the same inputs; each writes
   // the same values, so the output of the last one stands.
+  // valid(v): whether variant v applies (lane groups need the velocity fields
+  // only; two rows per wait exist for FP32 only)
+  static bool valid(int v, bool grouped_ok, bool pipe2_ok) {
+    bool listed;
+    int group, pipe;
+    eval_variant(v, listed, group, pipe);
+    return !(group > 0 && !grouped_ok) && !(pipe == 2 && !pipe2_ok);
+  }
+
+  // the variant a forced choice stands for where it does not apply
+  static int fallback(int v, bool grouped_ok, bool pipe2_ok) {
+    if (!pipe2_ok) v = eval_variant_pipe1(v);
+    bool listed;
+    int group, pipe;
+    eval_variant(v, listed, group, pipe);
+    return (group > 0 && !grouped_ok) ? 1 : v;
+  }
+
   template <typename F>
-  int run(int bucket, bool grouped_ok, hipStream_t s, F&& launch) {
+  int run(int bucket, bool grouped_ok, bool pipe2_ok, hipStream_t s, F&& launch) {
     if (forced >= 0) {
-      bool listed;
-      int group;
-      eval_variant(forced, listed, group);
-      const int v = (group > 0 && !grouped_ok) ? 1 : forced;
+      const int v = fallback(forced, grouped_ok, pipe2_ok);
       launch(v);
       return v;
     }
-    auto it = choice.find(bucket);
-    if (it != choice.end()) {
-      launch(it->second);
-      return it->second;
+    const int planned_v = planned(bucket);
+    if (planned_v >= 0) {
+      launch(planned_v);
+      return planned_v;
     }
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
@@ -68,10 +104,7 @@ struct EvalTuner {
     float best = 1e30f;
     int bv = 0;
     for (int v = 0; v < N_EVAL_VARIANTS; v++) {
-      bool listed;
-      int group;
-      eval_variant(v, listed, group);
-      if (group > 0 && !grouped_ok) continue;
+      if (!valid(v, grouped_ok, pipe2_ok)) continue;
       (void)hipEventRecord(e0, s);
       launch(v);
       (void)hipEventRecord(e1, s);
@@ -84,6 +117,7 @@ struct EvalTuner {
     (void)hipEventDestroy(e1);
     choice[bucket] = bv;
     best_ms[bucket] = best;
+    rounds[bucket] += 1;
     return bv;
   }
 };

@@ -462,26 +462,34 @@ def test_graph_steps_match_eager(scheme, precision):
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("Np", [600, 60000])
-def test_eval_launch_shapes_identical(precision, Np):
+@pytest.mark.parametrize("dim", [3, 2])
+def test_eval_launch_shapes_identical(precision, Np, dim):
     """Every evaluation launch shape the autotuner chooses between (one block per
     cell, per listed non-empty cell in XCD runs or round-robin, lane groups of
-    32 / 16 / 8 over the ascending cell list) gives the same values bit for bit, on
-    a sparse and a dense swarm;
-    FP64 against the oracle at 1e-12."""
-    opts, X, V, P = cases.fourier_3d(nlat=20, Np=Np, n=9)
+    32 / 16 / 8 over the ascending cell list, one or two table rows per wait in
+    FP32) gives the same values bit for bit, on a sparse and a dense swarm, in 3D
+    and 2D; FP64 against the oracle at 1e-12.  FP64 has no two-row loop: there
+    the variants 6-8 run as 0, 1 and 5."""
+    if dim == 3:
+        opts, X, V, P = cases.fourier_3d(nlat=20, Np=Np, n=9)
+    else:
+        opts, X, V, P = cases.rotation_2d(N=4096, Np=Np, n=17, advection=False)
     opts["advection"] = False
-    print(f"case: launch shapes fourier3d Np={Np} {precision}", flush=True)
+    print(f"case: launch shapes dim {dim} Np={Np} {precision}", flush=True)
     vals = []
-    for v in range(6):
+    for v in range(9):
         g = L.LagRBFCtx(petsc_options=dict(opts, lagrbf_eval_variant=v), precision=precision)
         g.fill_data(X, V)
         g.fill_points(P)
         g.interpolate_c()
-        out = np.zeros((P.shape[0], 3))
+        out = np.zeros((P.shape[0], dim))
         g.output(fvals=out)
-        vals.append(out)
+        used = g.info("eval_variant")
         g.destroy()
-    for v in range(1, 6):
+        vals.append(out)
+        want = v if precision == "mixed" else {6: 0, 7: 1, 8: 5}.get(v, v)
+        assert used == want, (v, used)
+    for v in range(1, 9):
         assert np.array_equal(vals[0], vals[v]), f"variant {v} differs"
     if precision == "fp64":
         from oracle.oracle import OracleCtx
