@@ -462,20 +462,24 @@ def test_graph_steps_match_eager(scheme, precision):
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("Np", [600, 60000])
-@pytest.mark.parametrize("dim", [3, 2])
-def test_eval_launch_shapes_identical(precision, Np, dim):
+@pytest.mark.parametrize("case", ["3d", "2d", "3d-few-nodes"])
+def test_eval_launch_shapes_identical(precision, Np, case):
     """Every evaluation launch shape the autotuner chooses between (one block per
     cell, per listed non-empty cell in XCD runs or round-robin, lane groups of
     32 / 16 / 8 over the ascending cell list, one or two table rows per wait in
     FP32) gives the same values bit for bit, on a sparse and a dense swarm, in 3D
-    and 2D; FP64 against the oracle at 1e-12.  FP64 has no two-row loop: there
-    the variants 6-8 run as 0, 1 and 5."""
-    if dim == 3:
+    and 2D, and with 0-6 nodes per cell (the node loops' short and empty tails);
+    FP64 against the oracle at 1e-12.  FP64 has no two-row loop: there the
+    variants 6-8 run as 0, 1 and 5."""
+    if case == "3d":
         opts, X, V, P = cases.fourier_3d(nlat=20, Np=Np, n=9)
+    elif case == "3d-few-nodes":
+        opts, X, V, P = cases.fourier_3d(nlat=20, N=1500, Np=Np, n=9)
     else:
         opts, X, V, P = cases.rotation_2d(N=4096, Np=Np, n=17, advection=False)
+    dim = X.shape[1]
     opts["advection"] = False
-    print(f"case: launch shapes dim {dim} Np={Np} {precision}", flush=True)
+    print(f"case: launch shapes {case} Np={Np} {precision}", flush=True)
     vals = []
     for v in range(9):
         g = L.LagRBFCtx(petsc_options=dict(opts, lagrbf_eval_variant=v), precision=precision)
