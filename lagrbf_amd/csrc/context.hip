@@ -277,7 +277,7 @@ static int alloc_particles(Ctx& c) {
   HIPCHK(c.alive.reserve(ld));
   HIPCHK(c.perm.reserve(ld));
   HIPCHK(c.pstart.reserve((size_t)c.g.C + 2));
-  HIPCHK(c.pscratch.reserve((size_t)c.g.C + 2 + (size_t)(c.g.C + 2) / 1024 + 64));
+  HIPCHK(c.pscratch.reserve(sort_particles_scratch(c.g.C, (int)ld)));
   HIPCHK(c.porig.reserve(ld));
   HIPCHK(c.cell_list.reserve(nonempty_list_capacity(c.g.C)));
   HIPCHK(c.list_count.reserve(1));
@@ -1100,6 +1100,40 @@ static int adaptive_velocity(Ctx& c, double* dst, double cst) {
 // per particle and adapts its dt (adaptive.hip).  Particles that reach t1 stop;
 // particles that leave the domain are removed as in the fixed-dt loops.  The
 // lazy set-up pass runs between attempts while patches are missing.
+// Adaptive loop: store the swarm in cell order, finished and removed particles
+// last, moving the per-particle integration state with it (start position x0,
+// first-same-as-last stage k1, time, step size, finished flag; the self index
+// stays the identity).  The per-stage grouping then finds long same-cell runs
+// (one atomic per run) and the evaluation reads contiguous particles.
+static int reorder_adaptive(Ctx& c) {
+  if (!c.reorder || c.Np == 0) return 0;
+  if (!c.sorted_valid) {
+    launch_sort_particles(c.pcell.p, c.alive.p, c.Np, c.g.C, c.pstart.p, c.perm.p, c.pscratch.p, c.s);
+    c.sorted_valid = true;
+  }
+  const size_t ld = (size_t)c.ld;
+  const int dim = c.g.dim;
+  HIPCHK(c.ptime_alt.reserve(ld));
+  HIPCHK(c.pdt_alt.reserve(ld));
+  HIPCHK(c.pfin_alt.reserve(ld));
+  // vadv[1] and vadv[2] are recomputed by every attempt: they take k1 and x0
+  ReorderSet r{};
+  r.src[0] = c.x0.p; r.dst[0] = c.vadv[2].p; r.rows[0] = dim;
+  r.src[1] = c.vadv[0].p; r.dst[1] = c.vadv[1].p; r.rows[1] = dim;
+  r.src[2] = c.ptime.p; r.dst[2] = c.ptime_alt.p; r.rows[2] = 1;
+  r.src[3] = c.pdt.p; r.dst[3] = c.pdt_alt.p; r.rows[3] = 1;
+  r.nd = 4;
+  r.isrc[0] = c.pfin.p; r.idst[0] = c.pfin_alt.p;
+  r.ni = 1;
+  launch_reorder(r, c.perm.p, c.Np, c.ld, c.s);
+  swap_buf(c.x0, c.vadv[2]);
+  swap_buf(c.vadv[0], c.vadv[1]);
+  swap_buf(c.ptime, c.ptime_alt);
+  swap_buf(c.pdt, c.pdt_alt);
+  swap_buf(c.pfin, c.pfin_alt);
+  return reorder_particles(c);
+}
+
 static int advect_adaptive(Ctx& c, double t0, double t1, int max_attempts) {
   const char* fn = "lagrbf_advect_adaptive";
   if (!c.advection) return set_err(E_NULL, fn, "Advection context is NULL (set -advection)");
@@ -1135,6 +1169,7 @@ static int advect_adaptive(Ctx& c, double t0, double t1, int max_attempts) {
   DBuf<unsigned long long> cnt;
   HIPCHK(cnt.reserve(3));
   for (int a = 0; a < max_attempts; a++) {
+    CHK(reorder_adaptive(c));
     for (int s = 0; s < 6; s++) {
       const double* v[7];
       for (int j = 0; j <= s; j++) v[j] = c.vadv[j].p;

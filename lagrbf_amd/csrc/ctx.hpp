@@ -83,6 +83,8 @@ struct Ctx {
   // per-particle adaptive DP45 state and counters
   DBuf<double> ptime, pdt, vtmp;
   DBuf<int32_t> pfin, pself;
+  DBuf<double> ptime_alt, pdt_alt;     // reorder targets of the adaptive state
+  DBuf<int32_t> pfin_alt;
   double dp_attempts = 0, dp_accepted = 0, dp_rejected = 0, dp_active = 0;
   hipStream_t s = nullptr;
   int device = 0;

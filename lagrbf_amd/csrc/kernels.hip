@@ -5,6 +5,7 @@
 // offsets.  Wave size is 64 (CDNA); blocks are multiples of 64.
 #include "kernels.hpp"
 
+#include <algorithm>
 #include <cfloat>
 #include <cmath>
 
@@ -653,26 +654,57 @@ __device__ __forceinline__ int particle_key(const int32_t* cell, const int32_t* 
   return alive[i] ? cell[i] : C;
 }
 
-__global__ void k_hist_alive(const int32_t* __restrict__ cell, const int32_t* __restrict__ alive, int n, int C,
-                             int32_t* __restrict__ counts) {
+// Live particles: per-cell counts, one atomic per same-cell run of a wave.  Dead
+// particles (finished or removed) take no atomics: each block writes its dead
+// count, and a scan of those counts gives every dead particle its tail slot (one
+// counter shared by all dead particles would serialise once most are dead).
+// Blocks of 256 threads.
+__global__ __launch_bounds__(256) void k_hist_alive(const int32_t* __restrict__ cell, const int32_t* __restrict__ alive,
+                                                    int n, int C, int32_t* __restrict__ counts,
+                                                    int32_t* __restrict__ bdead) {
+  __shared__ int32_t wd[4];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int key = particle_key(cell, alive, n, C, i);
-  const WaveRun r = wave_run(key);
-  const int lane = threadIdx.x & (WAVE - 1);
-  if (key >= 0 && r.head == lane) atomicAdd(&counts[key], r.len);
+  const bool dead = key == C;
+  const int lkey = dead ? -1 : key;
+  const WaveRun r = wave_run(lkey);
+  const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
+  if (lkey >= 0 && r.head == lane) atomicAdd(&counts[lkey], r.len);
+  const unsigned long long m = __ballot(dead);
+  if (lane == 0) wd[wv] = (int32_t)__popcll(m);
+  __syncthreads();
+  if (threadIdx.x == 0) bdead[blockIdx.x] = wd[0] + wd[1] + wd[2] + wd[3];
 }
 
-__global__ void k_pscatter(const int32_t* __restrict__ cell, const int32_t* __restrict__ alive, int n, int C,
-                           int32_t* __restrict__ cursor, int32_t* __restrict__ perm) {
+// dead_base: the first dead slot (the live total, pstart[C]); doff: exclusive scan
+// of the blocks' dead counts
+__global__ __launch_bounds__(256) void k_pscatter(const int32_t* __restrict__ cell, const int32_t* __restrict__ alive,
+                                                  int n, int C, int32_t* __restrict__ cursor,
+                                                  const int32_t* __restrict__ dead_base,
+                                                  const int32_t* __restrict__ doff, int32_t* __restrict__ perm) {
+  __shared__ int32_t wd[4];
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int key = particle_key(cell, alive, n, C, i);
-  const WaveRun r = wave_run(key);
-  const int lane = threadIdx.x & (WAVE - 1);
+  const bool dead = key == C;
+  const int lkey = dead ? -1 : key;
+  const WaveRun r = wave_run(lkey);
+  const int lane = threadIdx.x & (WAVE - 1), wv = threadIdx.x / WAVE;
   int base = 0;
-  if (key >= 0 && r.head == lane) base = atomicAdd(&cursor[key], r.len);
+  if (lkey >= 0 && r.head == lane) base = atomicAdd(&cursor[lkey], r.len);
   base = __shfl(base, r.head);
-  if (key >= 0) perm[base + lane - r.head] = i;
+  if (lkey >= 0) perm[base + lane - r.head] = i;
+  const unsigned long long m = __ballot(dead);
+  if (lane == 0) wd[wv] = (int32_t)__popcll(m);
+  __syncthreads();
+  if (dead) {
+    int off = *dead_base + doff[blockIdx.x];
+    for (int k = 0; k < wv; k++) off += wd[k];
+    const unsigned long long below = lane == 0 ? 0ull : (m & ((1ull << lane) - 1ull));
+    perm[off + (int)__popcll(below)] = i;
+  }
 }
+
+__global__ void k_set_i32(int32_t* __restrict__ a, int32_t v) { *a = v; }
 
 __global__ void k_zero_i32(int32_t* __restrict__ a, int n) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -684,18 +716,31 @@ __global__ void k_copy_i32(const int32_t* __restrict__ src, int32_t* __restrict_
   if (i < n) dst[i] = src[i];
 }
 
+size_t sort_particles_scratch(int C, int Np) {
+  const size_t nb = ((size_t)std::max(Np, 1) + 255) / 256;
+  return ((size_t)C + 2) + ((size_t)C + 2) / 1024 + 64 + nb + nb / 1024 + 64;
+}
+
 // Kernels only (no memset or copy nodes), so the sequence can sit in a captured graph.
 void launch_sort_particles(const int32_t* cell, const int32_t* alive, int Np, int C, int32_t* pstart, int32_t* perm,
                            int32_t* scratch, hipStream_t s) {
-  // pstart: C + 2 entries (cells, dead bucket, end); scratch: C + 2 + scan blocks
-  int32_t* cursor = scratch;
-  int32_t* tmp = scratch + (C + 2);
+  // pstart: C + 2 entries (cells, dead bucket, end); scratch (sort_particles_scratch):
+  // cursor C + 2, scan blocks of pstart, per-block dead counts, their scan blocks
   const int n = C + 2;
+  const int nb = (std::max(Np, 1) + 255) / 256;
+  int32_t* cursor = scratch;
+  int32_t* tmp = scratch + n;
+  int32_t* bdead = tmp + (size_t)n / 1024 + 64;
+  int32_t* tmp2 = bdead + nb;
   hipLaunchKernelGGL(k_zero_i32, dim3((n + 255) / 256), dim3(256), 0, s, pstart, n);
-  if (Np > 0) hipLaunchKernelGGL(k_hist_alive, dim3((Np + 255) / 256), dim3(256), 0, s, cell, alive, Np, C, pstart);
-  exclusive_scan(pstart, C + 2, tmp, s);
+  if (Np > 0) hipLaunchKernelGGL(k_hist_alive, dim3(nb), dim3(256), 0, s, cell, alive, Np, C, pstart, bdead);
+  // live counts only: after the scan pstart[C] = pstart[C + 1] = live total
+  exclusive_scan(pstart, n, tmp, s);
+  if (Np > 0) exclusive_scan(bdead, nb, tmp2, s);
+  hipLaunchKernelGGL(k_set_i32, dim3(1), dim3(1), 0, s, pstart + C + 1, Np);
   hipLaunchKernelGGL(k_copy_i32, dim3((n + 255) / 256), dim3(256), 0, s, pstart, cursor, n);
-  if (Np > 0) hipLaunchKernelGGL(k_pscatter, dim3((Np + 255) / 256), dim3(256), 0, s, cell, alive, Np, C, cursor, perm);
+  if (Np > 0)
+    hipLaunchKernelGGL(k_pscatter, dim3(nb), dim3(256), 0, s, cell, alive, Np, C, cursor, pstart + C, bdead, perm);
 }
 
 // Store the swarm in the order of perm (slot s receives particle perm[s]).

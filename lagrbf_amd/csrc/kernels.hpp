@@ -76,7 +76,8 @@ void launch_eval_ref(const Grid& g, int ktype, const double* px, int64_t ld, int
 // particles in the tail bucket C, so perm covers the whole swarm
 void launch_sort_particles(const int32_t* cell, const int32_t* alive, int Np, int C,
                            int32_t* pstart /* C+2 */, int32_t* perm,
-                           int32_t* scratch /* >= C+2 + scan blocks */, hipStream_t s);
+                           int32_t* scratch /* sort_particles_scratch(C, Np) */, hipStream_t s);
+size_t sort_particles_scratch(int C, int Np);
 // gather of SoA particle arrays (rows of ld elements) into the order of perm
 struct ReorderSet {
   const double* src[6];

@@ -132,6 +132,7 @@ int batch_reserve(Batch& b, int need) {
   HIPCHK(b.w_b.reserve((size_t)nl * nen));
   HIPCHK(b.cell_b.reserve(nl)); HIPCHK(b.alive_b.reserve(nl)); HIPCHK(b.req_b.reserve(nl)); HIPCHK(b.porig_b.reserve(nl));
   HIPCHK(b.perm.reserve(nl));
+  HIPCHK(b.pscratch.reserve(sort_particles_scratch(b.ds->g.C, (int)nl)));
   b.ld = nl;
   return 0;
 }
@@ -147,7 +148,7 @@ Batch* get_batch(Server& sv, int ds, int scheme) {
   const int C = b->ds->g.C;
   if (b->pstart.reserve((size_t)C + 2) != hipSuccess || b->cell_list.reserve(nonempty_list_capacity(C)) != hipSuccess ||
       b->list_count.reserve(1) != hipSuccess ||
-      b->pscratch.reserve((size_t)C + 2 + (size_t)(C + 2) / 1024 + 64) != hipSuccess ||
+      b->pscratch.reserve(sort_particles_scratch(C, 1)) != hipSuccess ||
       b->counters.reserve(4) != hipSuccess || b->err_req.reserve(RSLOTS) != hipSuccess ||
       b->dt_req.reserve(RSLOTS) != hipSuccess || b->done_req.reserve(RSLOTS) != hipSuccess ||
       b->out_req.reserve(RSLOTS) != hipSuccess ||
