@@ -323,6 +323,7 @@ static int fill_points(Ctx& c, int n, const double* coords, const double* values
   // DMSwarmPICSetCellId + SwarmInterpSetWeights (interface_utils.c:118-138)
   launch_locate_particles(c.g, c.px.p, c.ld, n, c.pcell.p, c.alive.p, c.counters.p, c.s);
   c.sorted_valid = false;
+  c.cell_ordered = false;
   launch_weights(c.g, c.pux.p, c.puy.p, c.puz.p, c.px.p, c.ld, n, c.pcell.p, c.alive.p, c.w.p, c.s);
   HIPCHK(hipStreamSynchronize(c.s));
   c.points_filled = true;
@@ -621,6 +622,9 @@ static int ensure_blk_group(Ctx& c, bool f32, int g) {
 
 // group < 0: fields 0 .. nfields-1 of the n_interpolation_fields table; group
 // 0 / 1: the dim fields of that snapshot (cell-table evaluation only).
+// Density bucket of the autotuner: particles per cell of the swarm.
+static int eval_bucket(const Ctx& c) { return density_bucket(c.Np, c.g.C); }
+
 static int rbf_interpolation(Ctx& c, double* out, int nfields, int group = -1) {
   if (nfields <= 0 || c.Np == 0) return 0;
   PatchView pv{c.p_n.p, c.p_active.p, c.p_mem_off.p, c.mem_pool.p, c.coef_pool.p, c.nf};
@@ -650,7 +654,7 @@ static int rbf_interpolation(Ctx& c, double* out, int nfields, int group = -1) {
   // (once its launch shape is chosen); the other paths are bracketed by stream
   // events and resolved at once
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  const bool timed_dispatch = use_blk && c.tuner.planned(density_bucket(c.Np, c.g.C)) >= 0;
+  const bool timed_dispatch = use_blk && c.tuner.planned(eval_bucket(c)) >= 0;
   if (!c.capturing && (timed_dispatch || !use_blk)) {
     CHK(event_take(c, e0));
     CHK(event_take(c, e1));
@@ -659,7 +663,7 @@ static int rbf_interpolation(Ctx& c, double* out, int nfields, int group = -1) {
   if (use_blk) {
     const int per_cell = (c.Np + c.g.C - 1) / std::max(c.g.C, 1);
     const int block = std::min(256, std::max(64, (per_cell + 63) / 64 * 64));
-    const int bucket = density_bucket(c.Np, c.g.C);
+    const int bucket = eval_bucket(c);
     const int planned = c.tuner.planned(bucket);
     if (planned < 0 || eval_variant_listed(planned))
       launch_nonempty_cells(c.pstart.p, c.g.C, c.cell_list.p, c.list_count.p, c.s);
@@ -672,7 +676,7 @@ static int rbf_interpolation(Ctx& c, double* out, int nfields, int group = -1) {
                       lanes, pipe, c.s);
     };
     set_eval_timing_events(e0, e1);
-    c.last_variant = c.tuner.run(bucket, nft == c.g.dim && nfields == nft, blk32, c.s, launch);
+    c.last_variant = c.tuner.run(bucket, nft == c.g.dim && nfields == nft, blk32, c.cell_ordered, c.s, launch);
     set_eval_timing_events(nullptr, nullptr);
   } else if (cell_order) {
     const int per_cell = (c.Np + c.g.C - 1) / std::max(c.g.C, 1);
@@ -821,6 +825,7 @@ static int reorder_particles(Ctx& c) {
   r.ni = 3;
   launch_reorder(r, c.perm.p, c.Np, c.ld, c.s);
   swap_particle_buffers(c);
+  c.cell_ordered = true;
   // the cell groups now sit at pstart[c] .. pstart[c+1] in slot order
   launch_iota(c.perm.p, c.Np, c.s);
   return 0;
@@ -1002,7 +1007,7 @@ static bool graph_eligible(const Ctx& c) {
   if (!c.use_graph || c.analytical || c.scheme > 2 || !c.reorder || c.Np == 0) return false;
   if (c.eval_mode == 2 || (c.eval_mode == 0 && !c.memb_current) || !eval_blk_supported(c.nf) || !c.blk_valid) return false;
   if (!c.sorted_valid || !c.mem_pool.p) return false;
-  if (c.tuner.planned(density_bucket(c.Np, c.g.C)) < 0) return false;
+  if (c.tuner.planned(eval_bucket(c)) < 0) return false;
   return setup_complete(c);
 }
 

@@ -84,6 +84,7 @@ struct Ctx {
   DBuf<double> ptime, pdt, vtmp;
   DBuf<int32_t> pfin, pself;
   DBuf<double> ptime_alt, pdt_alt;     // reorder targets of the adaptive state
+  bool cell_ordered = false;              // swarm stored in cell order since the last fill
   DBuf<int32_t> pfin_alt;
   double dp_attempts = 0, dp_accepted = 0, dp_rejected = 0, dp_active = 0;
   hipStream_t s = nullptr;

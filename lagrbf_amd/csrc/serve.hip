@@ -210,7 +210,8 @@ void eval_batch(Batch& b, double* out) {
                     f32 ? (const void*)d.blk32.p : (const void*)d.blk64.p, d.nf, d.cell_start.p, d.g.dim, out, block,
                     b.Np, listed ? b.cell_list.p : nullptr, listed ? b.list_count.p : nullptr, grp, pipe, b.s);
   };
-  d.tuner.run(density_bucket(b.Np, d.g.C), d.nf == d.g.dim, f32, b.s, launch);
+  // the batch is stored in cell order at every tick
+  d.tuner.run(density_bucket(b.Np, d.g.C), d.nf == d.g.dim, f32, true, b.s, launch);
 }
 
 void group(Batch& b) {

@@ -44,13 +44,12 @@ inline int density_bucket(int64_t np, int ncells) {
 }
 
 struct EvalTuner {
-  // Trial rounds per bucket: the first evaluation of a swarm may still see it in
-  // fill order (the swarm is stored in cell order at each RK step), which favours
-  // other shapes than the steady state, so the bucket's next evaluation times the
-  // candidates again and that choice stays.
-  static constexpr int TRIAL_ROUNDS = 2;
+  // A choice timed on a swarm still in fill order (the first evaluation, before an
+  // RK step stores the swarm in cell order) favours other shapes than the steady
+  // state: it is provisional, and the bucket's next evaluation times the
+  // candidates again.  A choice timed on a swarm in cell order stays.
   std::map<int, int> choice;   // density bucket -> variant
-  std::map<int, int> rounds;   // density bucket -> trial rounds run
+  std::map<int, bool> final_;  // density bucket -> choice timed on a swarm in cell order
   std::map<int, float> best_ms;
   int forced = -1;             // a fixed variant (option lagrbf_eval_variant), -1: tune
 
@@ -58,8 +57,8 @@ struct EvalTuner {
   int planned(int bucket) const {
     if (forced >= 0) return forced;
     auto it = choice.find(bucket);
-    auto r = rounds.find(bucket);
-    return it == choice.end() || r == rounds.end() || r->second < TRIAL_ROUNDS ? -1 : it->second;
+    auto f = final_.find(bucket);
+    return it == choice.end() || f == final_.end() || !f->second ? -1 : it->second;
   }
 
   // launch(v) enqueues the evaluation with variant v on stream s.  Returns the
@@ -83,8 +82,9 @@ struct EvalTuner {
     return (group > 0 && !grouped_ok) ? 1 : v;
   }
 
+  // cell_ordered: the swarm is stored in cell order (a choice made now stays)
   template <typename F>
-  int run(int bucket, bool grouped_ok, bool pipe2_ok, hipStream_t s, F&& launch) {
+  int run(int bucket, bool grouped_ok, bool pipe2_ok, bool cell_ordered, hipStream_t s, F&& launch) {
     if (forced >= 0) {
       const int v = fallback(forced, grouped_ok, pipe2_ok);
       launch(v);
@@ -115,9 +115,11 @@ struct EvalTuner {
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+    // the second round stays whatever the order (positions drift between reorders)
+    const bool again = choice.count(bucket) > 0;
     choice[bucket] = bv;
     best_ms[bucket] = best;
-    rounds[bucket] += 1;
+    final_[bucket] = cell_ordered || again;
     return bv;
   }
 };
