@@ -669,14 +669,15 @@ static int rbf_interpolation(Ctx& c, double* out, int nfields, int group = -1) {
       launch_nonempty_cells(c.pstart.p, c.g.C, c.cell_list.p, c.list_count.p, c.s);
     auto launch = [&](int v) {
       bool listed;
-      int lanes, pipe;
-      eval_variant(v, listed, lanes, pipe);
+      int lanes, pipe, waves;
+      eval_variant(v, listed, lanes, pipe, waves);
       launch_eval_blk(c.g, c.ktype, blk32, c.px.p, c.ld, c.perm.p, c.pstart.p, c.w.p, table, nft, c.cell_start.p,
                       nfields, out, block, c.Np, listed ? c.cell_list.p : nullptr, listed ? c.list_count.p : nullptr,
-                      lanes, pipe, c.s);
+                      lanes, pipe, waves, c.s);
     };
     set_eval_timing_events(e0, e1);
-    c.last_variant = c.tuner.run(bucket, nft == c.g.dim && nfields == nft, blk32, c.cell_ordered, c.s, launch);
+    c.last_variant =
+        c.tuner.run(bucket, nft == c.g.dim && nfields == nft, blk32, c.cell_ordered, !c.use_graph, c.s, launch);
     set_eval_timing_events(nullptr, nullptr);
   } else if (cell_order) {
     const int per_cell = (c.Np + c.g.C - 1) / std::max(c.g.C, 1);

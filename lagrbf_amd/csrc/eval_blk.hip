@@ -299,9 +299,11 @@ __device__ __forceinline__ void accum_nbrs(const Grid& g, int ci, int cj, int ck
 #define LRB_EVAL_WAVES_F32 8
 #endif
 // PIPE (FP32 only): table rows per wait in the node loops of the non-centre
-// neighbours, 1 or 2 (the autotuner chooses; each suits other launch shapes)
-template <int DIM, int KT, int NFT, typename T, int PIPE>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sizeof(T) == 8 ? LRB_EVAL_WAVES_F64 : LRB_EVAL_WAVES_F32)))
+// neighbours, 1 or 2; WAVES: occupancy floor in waves per SIMD (the autotuner
+// chooses among the combinations; each suits other launch shapes)
+template <int DIM, int KT, int NFT, typename T, int PIPE,
+          int WAVES = (sizeof(T) == 8 ? LRB_EVAL_WAVES_F64 : LRB_EVAL_WAVES_F32)>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)))
 void k_eval_blk(Grid g, const double* __restrict__ px, int64_t ld,
                                                   const int32_t* __restrict__ perm, const int32_t* __restrict__ pstart,
                                                   const double* __restrict__ w, const T* __restrict__ blk,
@@ -711,7 +713,8 @@ void launch_build_blk(const Grid& g, bool fp32, const double* x, int N, const in
 template <int DIM, int KT, typename T>
 static void eval_blk_k(const Grid& g, const double* px, int64_t ld, const int32_t* perm, const int32_t* pstart,
                        const double* w, const T* blk, int nft, const int32_t* cs, int nfields, double* out, int block,
-                       int np, const int32_t* cell_list, const int32_t* list_count, int group, int pipe, hipStream_t s) {
+                       int np, const int32_t* cell_list, const int32_t* list_count, int group, int pipe, int waves,
+                       hipStream_t s) {
   if (cell_list && group > 0 && nft == DIM) {   // group -1: listed, round-robin blocks
     // sparse velocity evaluation: lane groups of G particles, 64/G cells per wave
     launch_sparse_g<DIM, KT, DIM, T>(group, g, px, ld, perm, pstart, w, blk, cs, nfields, out, cell_list, list_count, np,
@@ -726,7 +729,11 @@ static void eval_blk_k(const Grid& g, const double* px, int64_t ld, const int32_
   if constexpr (sizeof(T) == 4) {
     // two rows per wait: the velocity evaluations (nft = dim) only
     if (pipe == 2 && nft == DIM) {
-      LRB_EB(DIM, 2);
+      if (waves == 6)
+        LRB_LAUNCH_TIMED((k_eval_blk<DIM, KT, DIM, T, 2, 6>), grid, blkd, s, g, px, ld, perm, pstart, w, blk, cs,
+                         nfields, out, cell_list, list_count, xcd_runs);
+      else
+        LRB_EB(DIM, 2);
       return;
     }
   }
@@ -743,8 +750,8 @@ template <int DIM, typename T>
 static void eval_blk_dim(const Grid& g, int ktype, const double* px, int64_t ld, const int32_t* perm,
                          const int32_t* pstart, const double* w, const T* blk, int nft, const int32_t* cs, int nfields,
                          double* out, int block, int np, const int32_t* cell_list, const int32_t* list_count,
-                         int group, int pipe, hipStream_t s) {
-#define LRB_K(K) eval_blk_k<DIM, K, T>(g, px, ld, perm, pstart, w, blk, nft, cs, nfields, out, block, np, cell_list, list_count, group, pipe, s)
+                         int group, int pipe, int waves, hipStream_t s) {
+#define LRB_K(K) eval_blk_k<DIM, K, T>(g, px, ld, perm, pstart, w, blk, nft, cs, nfields, out, block, np, cell_list, list_count, group, pipe, waves, s)
   if constexpr (sizeof(T) == 8) {
     switch (ktype) {
       case K_LINEAR: LRB_K(K_LINEAR); break;
@@ -769,9 +776,9 @@ static void eval_blk_dim(const Grid& g, int ktype, const double* px, int64_t ld,
 void launch_eval_blk(const Grid& g, int ktype, bool fp32, const double* px, int64_t ld, const int32_t* perm,
                      const int32_t* pstart, const double* w, const void* blk_, int nft, const int32_t* cs, int nfields,
                      double* out, int block, int np, const int32_t* cell_list, const int32_t* list_count,
-                     int group, int pipe, hipStream_t s) {
+                     int group, int pipe, int waves, hipStream_t s) {
   if (nfields <= 0 || g.C <= 0 || np <= 0) return;
-#define LRB_ARGS px, ld, perm, pstart, w, blk, nft, cs, nfields, out, block, np, cell_list, list_count, group, pipe, s
+#define LRB_ARGS px, ld, perm, pstart, w, blk, nft, cs, nfields, out, block, np, cell_list, list_count, group, pipe, waves, s
   if (fp32) {
     const float* blk = (const float*)blk_;
     if (g.dim == 3) eval_blk_dim<3, float>(g, ktype, LRB_ARGS);

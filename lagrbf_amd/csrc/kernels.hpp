@@ -116,7 +116,7 @@ void set_eval_timing_events(hipEvent_t start, hipEvent_t stop);
 void launch_eval_blk(const Grid& g, int ktype, bool fp32, const double* px, int64_t ld, const int32_t* perm,
                      const int32_t* pstart, const double* w, const void* blk, int nft, const int32_t* cs, int nfields,
                      double* out, int block, int np, const int32_t* cell_list, const int32_t* list_count, int group,
-                     int pipe, hipStream_t s);
+                     int pipe, int waves, hipStream_t s);
 // ascending non-empty cells; list needs nonempty_list_capacity(C) entries (the tail is scratch)
 void launch_nonempty_cells(const int32_t* pstart, int C, int32_t* list, int32_t* count, hipStream_t s);
 size_t nonempty_list_capacity(int C);

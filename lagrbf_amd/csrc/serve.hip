@@ -204,14 +204,14 @@ void eval_batch(Batch& b, double* out) {
   const int block = std::min(256, std::max(64, (per_cell + 63) / 64 * 64));
   auto launch = [&](int v) {
     bool listed;
-    int grp, pipe;
-    eval_variant(v, listed, grp, pipe);
+    int grp, pipe, waves;
+    eval_variant(v, listed, grp, pipe, waves);
     launch_eval_blk(d.g, d.ktype, f32, b.px.p, b.ld, b.perm.p, b.pstart.p, b.w.p,
                     f32 ? (const void*)d.blk32.p : (const void*)d.blk64.p, d.nf, d.cell_start.p, d.g.dim, out, block,
-                    b.Np, listed ? b.cell_list.p : nullptr, listed ? b.list_count.p : nullptr, grp, pipe, b.s);
+                    b.Np, listed ? b.cell_list.p : nullptr, listed ? b.list_count.p : nullptr, grp, pipe, waves, b.s);
   };
   // the batch is stored in cell order at every tick
-  d.tuner.run(density_bucket(b.Np, d.g.C), d.nf == d.g.dim, f32, true, b.s, launch);
+  d.tuner.run(density_bucket(b.Np, d.g.C), d.nf == d.g.dim, f32, true, true, b.s, launch);
 }
 
 void group(Batch& b) {

@@ -11,30 +11,35 @@
 #include <cmath>
 #include <cstdint>
 #include <map>
+#include <set>
+#include <vector>
 
 namespace lrb {
 
-constexpr int N_EVAL_VARIANTS = 9;
+constexpr int N_EVAL_VARIANTS = 10;
 
 // variant -> (evaluate listed cells only, lane group size; 0 = one block per cell,
 // -1 = one block per listed cell with the blocks dealt round-robin over the XCDs
 // instead of in contiguous runs: better balanced when cells differ widely in work;
-// rows per wait of the FP32 node loop, 1 or 2: variants 6-8 are 0, 1 and 5 with two)
-inline void eval_variant(int v, bool& listed, int& group, int& pipe) {
-  static const int G[N_EVAL_VARIANTS] = {0, 0, 32, 16, 8, -1, 0, 0, -1};
-  static const int L[N_EVAL_VARIANTS] = {0, 1, 1, 1, 1, 1, 0, 1, 1};
+// rows per wait of the FP32 node loop, 1 or 2: variants 6-8 are 0, 1 and 5 with two;
+// FP32 occupancy floor in waves per SIMD, 0 = the default: variant 9 is 8 with 6
+// waves, which leaves the loop its registers without spilling)
+inline void eval_variant(int v, bool& listed, int& group, int& pipe, int& waves) {
+  static const int G[N_EVAL_VARIANTS] = {0, 0, 32, 16, 8, -1, 0, 0, -1, -1};
+  static const int L[N_EVAL_VARIANTS] = {0, 1, 1, 1, 1, 1, 0, 1, 1, 1};
   listed = L[v] != 0;
   group = G[v];
   pipe = v >= 6 ? 2 : 1;
+  waves = v == 9 ? 6 : 0;
 }
 
 // the same shape with one row per wait (the FP64 evaluation has no two-row loop)
-inline int eval_variant_pipe1(int v) { return v == 6 ? 0 : v == 7 ? 1 : v == 8 ? 5 : v; }
+inline int eval_variant_pipe1(int v) { return v == 6 ? 0 : v == 7 ? 1 : v >= 8 ? 5 : v; }
 
 inline bool eval_variant_listed(int v) {
   bool listed;
-  int group, pipe;
-  eval_variant(v, listed, group, pipe);
+  int group, pipe, waves;
+  eval_variant(v, listed, group, pipe, waves);
   return listed;
 }
 
@@ -44,14 +49,51 @@ inline int density_bucket(int64_t np, int ncells) {
 }
 
 struct EvalTuner {
-  // A choice timed on a swarm still in fill order (the first evaluation, before an
-  // RK step stores the swarm in cell order) favours other shapes than the steady
-  // state: it is provisional, and the bucket's next evaluation times the
-  // candidates again.  A choice timed on a swarm in cell order stays.
-  std::map<int, int> choice;   // density bucket -> variant
-  std::map<int, bool> final_;  // density bucket -> choice timed on a swarm in cell order
+  // Per density bucket:
+  //  - a trial times every valid shape back to back on the live launch (two
+  //    interleaved passes, the faster time of each counts);
+  //  - a trial on a swarm still in fill order (the first evaluation, before an RK
+  //    step stores the swarm in cell order) is provisional: the next evaluation
+  //    runs the trial again;
+  //  - back-to-back launches do not see the caches and clocks of the steady state
+  //    (an RK stage runs between two evaluations), so the shapes within 5% of the
+  //    best trial time get a run-off: the bucket's next evaluations use one of them
+  //    each, timed with stream events, and the fastest of those stays.
+  // All shapes write identical values, so none of this changes results.
+  static constexpr float RUNOFF_MARGIN = 1.05f;
+  struct Runoff {
+    std::vector<int> cand;                 // shapes in the run-off
+    std::vector<hipEvent_t> ev;            // start/stop per launched candidate
+    size_t next = 0;
+  };
+  std::map<int, int> choice;    // density bucket -> variant
+  std::map<int, bool> final_;   // density bucket -> choice stays
+  std::map<int, Runoff> runoff; // density bucket -> run-off in progress
   std::map<int, float> best_ms;
-  int forced = -1;             // a fixed variant (option lagrbf_eval_variant), -1: tune
+  int forced = -1;              // a fixed variant (option lagrbf_eval_variant), -1: tune
+
+  EvalTuner() = default;
+  EvalTuner(const EvalTuner&) = delete;
+  EvalTuner& operator=(const EvalTuner&) = delete;
+  EvalTuner& operator=(EvalTuner&& o) noexcept {
+    release();
+    choice = std::move(o.choice);
+    final_ = std::move(o.final_);
+    runoff = std::move(o.runoff);
+    best_ms = std::move(o.best_ms);
+    forced = o.forced;
+    seen_ = std::move(o.seen_);
+    o.runoff.clear();
+    return *this;
+  }
+  ~EvalTuner() { release(); }
+
+  // events of unfinished run-offs
+  void release() {
+    for (auto& kv : runoff)
+      for (hipEvent_t e : kv.second.ev) (void)hipEventDestroy(e);
+    runoff.clear();
+  }
 
   // variant run() will launch without timing candidates, or -1 when it will tune
   int planned(int bucket) const {
@@ -61,15 +103,12 @@ struct EvalTuner {
     return it == choice.end() || f == final_.end() || !f->second ? -1 : it->second;
   }
 
-  // launch(v) enqueues the evaluation with variant v on stream s.  Returns the
-  // variant used.  Candidates run back to back on the same inputs; each writes
-  // the same values, so the output of the last one stands.
   // valid(v): whether variant v applies (lane groups need the velocity fields
   // only; two rows per wait exist for FP32 only)
   static bool valid(int v, bool grouped_ok, bool pipe2_ok) {
     bool listed;
-    int group, pipe;
-    eval_variant(v, listed, group, pipe);
+    int group, pipe, waves;
+    eval_variant(v, listed, group, pipe, waves);
     return !(group > 0 && !grouped_ok) && !(pipe == 2 && !pipe2_ok);
   }
 
@@ -77,14 +116,18 @@ struct EvalTuner {
   static int fallback(int v, bool grouped_ok, bool pipe2_ok) {
     if (!pipe2_ok) v = eval_variant_pipe1(v);
     bool listed;
-    int group, pipe;
-    eval_variant(v, listed, group, pipe);
+    int group, pipe, waves;
+    eval_variant(v, listed, group, pipe, waves);
     return (group > 0 && !grouped_ok) ? 1 : v;
   }
 
-  // cell_ordered: the swarm is stored in cell order (a choice made now stays)
+  // launch(v) enqueues the evaluation with variant v on stream s; returns the
+  // variant used.  cell_ordered: the swarm is stored in cell order; runoff: a
+  // run-off may follow the trial (callers about to capture the step in a graph
+  // take the trial's choice at once).
   template <typename F>
-  int run(int bucket, bool grouped_ok, bool pipe2_ok, bool cell_ordered, hipStream_t s, F&& launch) {
+  int run(int bucket, bool grouped_ok, bool pipe2_ok, bool cell_ordered, bool allow_runoff, hipStream_t s,
+          F&& launch) {
     if (forced >= 0) {
       const int v = fallback(forced, grouped_ok, pipe2_ok);
       launch(v);
@@ -95,32 +138,93 @@ struct EvalTuner {
       launch(planned_v);
       return planned_v;
     }
+    auto ro = runoff.find(bucket);
+    if (ro != runoff.end()) return runoff_step(bucket, ro->second, s, launch);
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
       launch(0);
       return 0;
     }
     launch(0);   // warm caches and code objects before timing
-    float best = 1e30f;
-    int bv = 0;
-    for (int v = 0; v < N_EVAL_VARIANTS; v++) {
-      if (!valid(v, grouped_ok, pipe2_ok)) continue;
-      (void)hipEventRecord(e0, s);
-      launch(v);
-      (void)hipEventRecord(e1, s);
-      (void)hipEventSynchronize(e1);
-      float ms = 0.f;
-      (void)hipEventElapsedTime(&ms, e0, e1);
-      if (ms < best) { best = ms; bv = v; }
+    float tmin[N_EVAL_VARIANTS];
+    for (int v = 0; v < N_EVAL_VARIANTS; v++) tmin[v] = 1e30f;
+    for (int pass = 0; pass < 2; pass++) {
+      for (int v = 0; v < N_EVAL_VARIANTS; v++) {
+        if (!valid(v, grouped_ok, pipe2_ok)) continue;
+        (void)hipEventRecord(e0, s);
+        launch(v);
+        (void)hipEventRecord(e1, s);
+        (void)hipEventSynchronize(e1);
+        float ms = 0.f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (ms < tmin[v]) tmin[v] = ms;
+      }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    // the second round stays whatever the order (positions drift between reorders)
-    const bool again = choice.count(bucket) > 0;
+    float best = 1e30f;
+    int bv = 0;
+    for (int v = 0; v < N_EVAL_VARIANTS; v++)
+      if (tmin[v] < best) { best = tmin[v]; bv = v; }
     choice[bucket] = bv;
     best_ms[bucket] = best;
-    final_[bucket] = cell_ordered || again;
+    if (!cell_ordered && !choice_seen(bucket)) {
+      final_[bucket] = false;   // provisional: timed on a swarm in fill order
+      seen_.insert(bucket);
+      return bv;
+    }
+    Runoff r;
+    if (allow_runoff)
+      for (int v = 0; v < N_EVAL_VARIANTS; v++)
+        if (tmin[v] <= best * RUNOFF_MARGIN && tmin[v] < 1e29f) r.cand.push_back(v);
+    final_[bucket] = r.cand.size() <= 1;
+    if (r.cand.size() > 1) runoff[bucket] = std::move(r);
     return bv;
+  }
+
+ private:
+  std::set<int> seen_;   // buckets with a provisional trial behind them
+  bool choice_seen(int bucket) const { return seen_.count(bucket) > 0; }
+
+  template <typename F>
+  int runoff_step(int bucket, Runoff& r, hipStream_t s, F&& launch) {
+    const int v = r.cand[r.next];
+    hipEvent_t a = nullptr, b = nullptr;
+    if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+      if (a) (void)hipEventDestroy(a);
+      final_[bucket] = true;   // keep the trial's choice
+      for (hipEvent_t e : r.ev) (void)hipEventDestroy(e);
+      runoff.erase(bucket);
+      const int c = choice[bucket];
+      launch(c);
+      return c;
+    }
+    (void)hipEventRecord(a, s);
+    launch(v);
+    (void)hipEventRecord(b, s);
+    r.ev.push_back(a);
+    r.ev.push_back(b);
+    r.next += 1;
+    if (r.next == r.cand.size()) {
+      // every candidate ran once in the steady state: the fastest stays
+      float best = 1e30f;
+      int bv = choice[bucket];
+      for (size_t k = 0; k < r.cand.size(); k++) {
+        (void)hipEventSynchronize(r.ev[2 * k + 1]);
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, r.ev[2 * k], r.ev[2 * k + 1]) == hipSuccess && ms < best) {
+          best = ms;
+          bv = r.cand[k];
+        }
+      }
+      (void)hipGetLastError();
+      for (hipEvent_t e : r.ev) (void)hipEventDestroy(e);
+      runoff.erase(bucket);
+      choice[bucket] = bv;
+      best_ms[bucket] = best;
+      final_[bucket] = true;
+    }
+    return v;
   }
 };
 

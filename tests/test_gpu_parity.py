@@ -466,11 +466,11 @@ def test_graph_steps_match_eager(scheme, precision):
 def test_eval_launch_shapes_identical(precision, Np, case):
     """Every evaluation launch shape the autotuner chooses between (one block per
     cell, per listed non-empty cell in XCD runs or round-robin, lane groups of
-    32 / 16 / 8 over the ascending cell list, one or two table rows per wait in
-    FP32) gives the same values bit for bit, on a sparse and a dense swarm, in 3D
-    and 2D, and with 0-6 nodes per cell (the node loops' short and empty tails);
-    FP64 against the oracle at 1e-12.  FP64 has no two-row loop: there the
-    variants 6-8 run as 0, 1 and 5."""
+    32 / 16 / 8 over the ascending cell list, one or two table rows per wait and
+    two occupancy floors in FP32) gives the same values bit for bit, on a sparse
+    and a dense swarm, in 3D and 2D, and with 0-9 nodes per cell (the node loops'
+    short and empty tails); FP64 against the oracle at 1e-12.  FP64 has no
+    two-row loop: there the variants 6-9 run as 0, 1, 5 and 5."""
     if case == "3d":
         opts, X, V, P = cases.fourier_3d(nlat=20, Np=Np, n=9)
     elif case == "3d-few-nodes":
@@ -481,7 +481,7 @@ def test_eval_launch_shapes_identical(precision, Np, case):
     opts["advection"] = False
     print(f"case: launch shapes {case} Np={Np} {precision}", flush=True)
     vals = []
-    for v in range(9):
+    for v in range(10):
         g = L.LagRBFCtx(petsc_options=dict(opts, lagrbf_eval_variant=v), precision=precision)
         g.fill_data(X, V)
         g.fill_points(P)
@@ -491,9 +491,9 @@ def test_eval_launch_shapes_identical(precision, Np, case):
         used = g.info("eval_variant")
         g.destroy()
         vals.append(out)
-        want = v if precision == "mixed" else {6: 0, 7: 1, 8: 5}.get(v, v)
+        want = v if precision == "mixed" else {6: 0, 7: 1, 8: 5, 9: 5}.get(v, v)
         assert used == want, (v, used)
-    for v in range(1, 9):
+    for v in range(1, 10):
         assert np.array_equal(vals[0], vals[v]), f"variant {v} differs"
     if precision == "fp64":
         from oracle.oracle import OracleCtx
