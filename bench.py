@@ -124,9 +124,15 @@ def run_path(L, precision, opts, X, V, P, steps, warmup, dt, sample):
     cellid = ctx.debug_get("cellid_points")
     nc = opts["pu_mesh_n"][0] - 1
     flops = eval_flops(cs, nc, patch_sizes(cs, nc), cellid, alive)
-    n_launch = max(n_ev1 - n_ev0, 1)
-    avg_ms = (ev1 - ev0) / n_launch
-    achieved = flops / (avg_ms * 1e-3)
+    # evaluations are timed once their launch shape is settled (the autotuner's
+    # trials and run-off are not): with too few warmup steps none may be
+    n_launch = int(n_ev1 - n_ev0)
+    avg_ms = achieved = None
+    if n_launch > 0:
+        avg_ms = (ev1 - ev0) / n_launch
+        achieved = flops / (avg_ms * 1e-3)
+    else:
+        log(f"[{precision}] no settled evaluation in the timed steps: raise --warmup for the roofline figures")
     traffic = load_traffic(precision)
     coefs = {}
     for p in sample:
@@ -140,9 +146,9 @@ def run_path(L, precision, opts, X, V, P, steps, warmup, dt, sample):
         "end_to_end": P.shape[0] * steps / (wall + t_setup), "setup_s": t_setup, "removed_particles": removed,
         "fallback_patches": ctx.info("fallback"), "refine_iters_max": ctx.info("refine_iters"),
         "roofline": {"bound": "valu", "kernel": "pu_evaluation (k_eval_blk)",
-                     "achieved": achieved / 1e12, "peak": PEAK[precision] / 1e12, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK[precision], "traffic": traffic,
-                     "flops_per_launch": flops, "avg_launch_ms": avg_ms, "launches_timed": int(n_launch),
+                     "achieved": achieved / 1e12 if achieved else None, "peak": PEAK[precision] / 1e12,
+                     "unit": "TFLOP/s", "frac": achieved / PEAK[precision] if achieved else None, "traffic": traffic,
+                     "flops_per_launch": flops, "avg_launch_ms": avg_ms, "launches_timed": n_launch,
                      "launch_shape": int(ctx.info("eval_variant"))},
     }
     ctx.destroy()

@@ -677,7 +677,8 @@ static int rbf_interpolation(Ctx& c, double* out, int nfields, int group = -1) {
     };
     set_eval_timing_events(e0, e1);
     c.last_variant =
-        c.tuner.run(bucket, nft == c.g.dim && nfields == nft, blk32, c.cell_ordered, !c.use_graph, c.s, launch);
+        c.tuner.run(bucket, nft == c.g.dim && nfields == nft, blk32, c.cell_ordered || !c.reorder, !c.use_graph, c.s,
+                    launch);
     set_eval_timing_events(nullptr, nullptr);
   } else if (cell_order) {
     const int per_cell = (c.Np + c.g.C - 1) / std::max(c.g.C, 1);

@@ -11,7 +11,6 @@
 #include <cmath>
 #include <cstdint>
 #include <map>
-#include <set>
 #include <vector>
 
 namespace lrb {
@@ -50,17 +49,20 @@ inline int density_bucket(int64_t np, int ncells) {
 
 struct EvalTuner {
   // Per density bucket:
-  //  - a trial times every valid shape back to back on the live launch (two
-  //    interleaved passes, the faster time of each counts);
-  //  - a trial on a swarm still in fill order (the first evaluation, before an RK
-  //    step stores the swarm in cell order) is provisional: the next evaluation
-  //    runs the trial again;
+  //  - a swarm still in fill order (the first evaluation, before an RK step stores
+  //    the swarm in cell order) runs the default shape untimed: timings there
+  //    favour other shapes than the steady state;
+  //  - the first evaluation in cell order times every valid shape back to back on
+  //    the live launch (lane groups only below 64 particles per cell, where they
+  //    can win);
   //  - back-to-back launches do not see the caches and clocks of the steady state
   //    (an RK stage runs between two evaluations), so the shapes within 5% of the
   //    best trial time get a run-off: the bucket's next evaluations use one of them
   //    each, timed with stream events, and the fastest of those stays.
   // All shapes write identical values, so none of this changes results.
   static constexpr float RUNOFF_MARGIN = 1.05f;
+  static constexpr int DEFAULT_VARIANT = 5;
+  static constexpr int GROUPED_MAX_BUCKET = 12;   // bucket of 64 particles per cell
   struct Runoff {
     std::vector<int> cand;                 // shapes in the run-off
     std::vector<hipEvent_t> ev;            // start/stop per launched candidate
@@ -82,7 +84,6 @@ struct EvalTuner {
     runoff = std::move(o.runoff);
     best_ms = std::move(o.best_ms);
     forced = o.forced;
-    seen_ = std::move(o.seen_);
     o.runoff.clear();
     return *this;
   }
@@ -140,51 +141,46 @@ struct EvalTuner {
     }
     auto ro = runoff.find(bucket);
     if (ro != runoff.end()) return runoff_step(bucket, ro->second, s, launch);
+    if (!cell_ordered) {
+      launch(DEFAULT_VARIANT);
+      return DEFAULT_VARIANT;
+    }
+    const bool grouped = grouped_ok && bucket < GROUPED_MAX_BUCKET;
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
-      launch(0);
-      return 0;
+      launch(DEFAULT_VARIANT);
+      return DEFAULT_VARIANT;
     }
-    launch(0);   // warm caches and code objects before timing
+    launch(DEFAULT_VARIANT);   // warm caches and code objects before timing
     float tmin[N_EVAL_VARIANTS];
-    for (int v = 0; v < N_EVAL_VARIANTS; v++) tmin[v] = 1e30f;
-    for (int pass = 0; pass < 2; pass++) {
-      for (int v = 0; v < N_EVAL_VARIANTS; v++) {
-        if (!valid(v, grouped_ok, pipe2_ok)) continue;
-        (void)hipEventRecord(e0, s);
-        launch(v);
-        (void)hipEventRecord(e1, s);
-        (void)hipEventSynchronize(e1);
-        float ms = 0.f;
-        (void)hipEventElapsedTime(&ms, e0, e1);
-        if (ms < tmin[v]) tmin[v] = ms;
-      }
+    float best = 1e30f;
+    int bv = DEFAULT_VARIANT;
+    for (int v = 0; v < N_EVAL_VARIANTS; v++) {
+      tmin[v] = 1e30f;
+      if (!valid(v, grouped, pipe2_ok)) continue;
+      (void)hipEventRecord(e0, s);
+      launch(v);
+      (void)hipEventRecord(e1, s);
+      (void)hipEventSynchronize(e1);
+      float ms = 0.f;
+      (void)hipEventElapsedTime(&ms, e0, e1);
+      tmin[v] = ms;
+      if (ms < best) { best = ms; bv = v; }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    float best = 1e30f;
-    int bv = 0;
-    for (int v = 0; v < N_EVAL_VARIANTS; v++)
-      if (tmin[v] < best) { best = tmin[v]; bv = v; }
     choice[bucket] = bv;
     best_ms[bucket] = best;
-    if (!cell_ordered && !choice_seen(bucket)) {
-      final_[bucket] = false;   // provisional: timed on a swarm in fill order
-      seen_.insert(bucket);
-      return bv;
-    }
     Runoff r;
     if (allow_runoff)
       for (int v = 0; v < N_EVAL_VARIANTS; v++)
-        if (tmin[v] <= best * RUNOFF_MARGIN && tmin[v] < 1e29f) r.cand.push_back(v);
+        if (tmin[v] <= best * RUNOFF_MARGIN) r.cand.push_back(v);
     final_[bucket] = r.cand.size() <= 1;
     if (r.cand.size() > 1) runoff[bucket] = std::move(r);
     return bv;
   }
 
  private:
-  std::set<int> seen_;   // buckets with a provisional trial behind them
-  bool choice_seen(int bucket) const { return seen_.count(bucket) > 0; }
 
   template <typename F>
   int runoff_step(int bucket, Runoff& r, hipStream_t s, F&& launch) {

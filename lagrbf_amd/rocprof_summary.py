@@ -65,8 +65,10 @@ def counters(d, counter):
     return out
 
 
-def pmc(d, kernel_sub="", out=sys.stdout):
-    """Mean per-dispatch value of every collected counter, per kernel."""
+def pmc(d, kernel_sub="", out=sys.stdout, last=0):
+    """Mean per-dispatch value of every collected counter, per kernel (over the
+    last `last` dispatches of each kernel when last > 0: the steady state after
+    the autotuner's trial launches)."""
     per = defaultdict(lambda: defaultdict(float))
     names = set()
     for f in _find(d, "*counter_collection.csv"):
@@ -78,9 +80,13 @@ def pmc(d, kernel_sub="", out=sys.stdout):
                 per[(k, r["Dispatch_Id"])][r["Counter_Name"]] += float(r["Counter_Value"])
                 names.add(r["Counter_Name"])
     agg = defaultdict(lambda: defaultdict(list))
-    for (k, _), cv in per.items():
+    for (k, _), cv in sorted(per.items(), key=lambda kv: (kv[0][0], int(kv[0][1]))):
         for n, v in cv.items():
             agg[k][n].append(v)
+    if last > 0:
+        for k in agg:
+            for n in agg[k]:
+                agg[k][n] = agg[k][n][-last:]
     names = sorted(names)
     for k in sorted(agg):
         print(k, file=out)
@@ -126,6 +132,7 @@ def main():
     c = sub.add_parser("pmc")
     c.add_argument("dir")
     c.add_argument("--kernel", default="")
+    c.add_argument("--last", type=int, default=0, help="average the last N dispatches of each kernel only")
     t = sub.add_parser("traffic")
     t.add_argument("fetch_dir")
     t.add_argument("write_dir")
@@ -138,7 +145,7 @@ def main():
         stats(a.dir, a.top)
         return
     if a.cmd == "pmc":
-        pmc(a.dir, a.kernel)
+        pmc(a.dir, a.kernel, last=a.last)
         return
     res = traffic(a.fetch_dir, a.write_dir, a.copy_mib, a.kernel)
     print(f"copy kernel {res['copy_kernel']}: scale fetch {res['fetch_scale']:.4f}, write {res['write_scale']:.4f}")
