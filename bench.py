@@ -362,6 +362,8 @@ def main():
     ap.add_argument("--no-c4", action="store_true", help="skip the adaptive DP45 measurement")
     ap.add_argument("--c4-t-end", type=float, default=0.1, help="end time of the adaptive DP45 run")
     ap.add_argument("--c4-scale", type=float, default=1.0, help="fraction of the C4 node and particle counts")
+    ap.add_argument("--eval-variant", type=int, default=-1,
+                    help="fix the evaluation launch shape (0..9, for profiling one kernel); -1: autotuned")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="launch a plain copy of MIB MiB first (scale for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -390,6 +392,8 @@ def main():
     dt = 1e-3
     t0 = time.perf_counter()
     opts, X, V, P = build_workload(args.particles)
+    if args.eval_variant >= 0:
+        opts["lagrbf_eval_variant"] = args.eval_variant
     n_nodes, n_particles = int(X.shape[0]), int(P.shape[0])
     log(f"workload built: {n_nodes} nodes, {n_particles} particles ({time.perf_counter() - t0:.1f}s)")
     n_pu = int(np.prod(opts["pu_mesh_n"]))
