@@ -368,7 +368,7 @@ def main():
                     help="launch a plain copy of MIB MiB first (scale for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU only (one GPU's share of the swarm); "
+        print(json.dumps({"error": "this benchmark runs on one GPU only (one GPU's share of the swarm); "
                                    "--gpus must be 1"}), flush=True)
         sys.exit(2)
     rank = int(os.environ.get("RANK", "0"))
