@@ -10,8 +10,9 @@ namespace lrb {
 struct VPtrs7 { const double* v[7]; };
 static constexpr int WAVE = 64;   // wavefront width of gfx950
 
-// v = (1 - tau) vA + tau vB at the particle's stage time t + c dt, tau = (t - tA) / (tB - tA)
-__global__ void k_blend(double* __restrict__ va, const double* __restrict__ vb, int64_t ld, int Np, int dim,
+// out = (1 - tau) vA + tau vB at the particle's stage time t + c dt,
+// tau = (t - tA) / (tB - tA) (out may be vA)
+__global__ void k_blend(double* out, const double* va, const double* __restrict__ vb, int64_t ld, int Np, int dim,
                         const int32_t* __restrict__ alive, const double* __restrict__ tp, const double* __restrict__ dtp,
                         double cst, double tA, double tB) {
 #pragma clang fp contract(off)
@@ -21,13 +22,15 @@ __global__ void k_blend(double* __restrict__ va, const double* __restrict__ vb, 
   const double tau = (t - tA) / (tB - tA);
   for (int d = 0; d < dim; d++) {
     const size_t k = (size_t)d * ld + i;
-    va[k] = (1.0 - tau) * va[k] + tau * vb[k];
+    out[k] = (1.0 - tau) * va[k] + tau * vb[k];
   }
 }
 
-void launch_blend(double* va, const double* vb, int64_t ld, int Np, int dim, const int32_t* alive, const double* tp,
-                  const double* dtp, double cst, double tA, double tB, hipStream_t s) {
-  if (Np > 0) hipLaunchKernelGGL(k_blend, dim3((Np + 255) / 256), dim3(256), 0, s, va, vb, ld, Np, dim, alive, tp, dtp, cst, tA, tB);
+void launch_blend(double* out, const double* va, const double* vb, int64_t ld, int Np, int dim, const int32_t* alive,
+                  const double* tp, const double* dtp, double cst, double tA, double tB, hipStream_t s) {
+  if (Np > 0)
+    hipLaunchKernelGGL(k_blend, dim3((Np + 255) / 256), dim3(256), 0, s, out, va, vb, ld, Np, dim, alive, tp, dtp, cst,
+                       tA, tB);
 }
 
 // Accept/reject of one attempt per particle.  px holds y1 = x0 + dt sum b_j k_j

@@ -146,6 +146,7 @@ static int ctx_from_options(Ctx& c, const OptView& o) {
     return set_err(E_RANGE, fn, "-lagrbf_eval_variant must be -1 (tune) or 0..%d", N_EVAL_VARIANTS - 1);
   c.tuner.forced = c.eval_variant;
   o.get_bool("lagrbf_time_blend", c.blend, bad);
+  o.get_int("lagrbf_blend_fused", c.blend_fused, bad);
   std::vector<double> bt;
   if (o.get_real_array("lagrbf_blend_times", bt, bad)) {
     if (bt.size() != 2 || !(bt[1] != bt[0]))
@@ -642,8 +643,8 @@ static int rbf_interpolation(Ctx& c, double* out, int nfields, int group = -1) {
     c.sorted_valid = true;
   }
   if (!c.zeros.p) { HIPCHK(c.zeros.reserve(1)); HIPCHK(hipMemset(c.zeros.p, 0, sizeof(double))); }
-  const bool use_blk = cell_order && eval_blk_supported(group >= 0 ? c.g.dim : c.nf);
   const bool blk32 = c.precision && eval_f32_supported(c.ktype);
+  const bool use_blk = cell_order && eval_blk_supported(group >= 0 ? c.g.dim : c.nf, c.g.dim, blk32);
   if (group >= 0 && !use_blk)
     return set_err(E_SUP, "RBFInterpolation", "snapshot fields need the cell-table evaluation (patches from the current data)");
   if (use_blk) CHK(group >= 0 ? ensure_blk_group(c, blk32, group) : ensure_blk(c, blk32));
@@ -677,8 +678,8 @@ static int rbf_interpolation(Ctx& c, double* out, int nfields, int group = -1) {
     };
     set_eval_timing_events(e0, e1);
     c.last_variant =
-        c.tuner.run(bucket, nft == c.g.dim && nfields == nft, blk32, c.cell_ordered || !c.reorder, !c.use_graph, c.s,
-                    launch);
+        c.tuner.run(bucket, nft == c.g.dim && nfields == nft, blk32 && nft == c.g.dim, c.cell_ordered || !c.reorder,
+                    !c.use_graph, c.s, launch);
     set_eval_timing_events(nullptr, nullptr);
   } else if (cell_order) {
     const int per_cell = (c.Np + c.g.C - 1) / std::max(c.g.C, 1);
@@ -1007,7 +1008,7 @@ static int fixed_step(Ctx& c, double dt) {
 // particle buffers and two graphs, one per parity, cover the loop).
 static bool graph_eligible(const Ctx& c) {
   if (!c.use_graph || c.analytical || c.scheme > 2 || !c.reorder || c.Np == 0) return false;
-  if (c.eval_mode == 2 || (c.eval_mode == 0 && !c.memb_current) || !eval_blk_supported(c.nf) || !c.blk_valid) return false;
+  if (c.eval_mode == 2 || (c.eval_mode == 0 && !c.memb_current) || !eval_blk_supported(c.nf, c.g.dim, c.precision && eval_f32_supported(c.ktype)) || !c.blk_valid) return false;
   if (!c.sorted_valid || !c.mem_pool.p) return false;
   if (c.tuner.planned(eval_bucket(c)) < 0) return false;
   return setup_complete(c);
@@ -1094,9 +1095,22 @@ static int advect_steps(Ctx& c, int nsteps, double dt) {
 static int adaptive_velocity(Ctx& c, double* dst, double cst) {
   const int dim = c.g.dim;
   if (!c.blend) return rbf_interpolation(c, dst, dim);
+  // -1 (default) and 1: fused where the cell table evaluates 2d fields, which is
+  // the FP32 evaluation (the FP64 one would need 96 accumulator registers per lane)
+  const bool f32 = c.precision && eval_f32_supported(c.ktype);
+  const bool fused = c.blend_fused != 0 && eval_blk_supported(2 * dim, dim, f32);
+  if (fused) {
+    // both snapshots in one evaluation (the 2d fields of the data): phi once per
+    // particle and node; each field's sums are those of the per-snapshot passes
+    CHK(rbf_interpolation(c, c.vtmp.p, 2 * dim));
+    launch_blend(dst, c.vtmp.p, c.vtmp.p + (size_t)dim * c.ld, c.ld, c.Np, dim, c.alive.p, c.ptime.p, c.pdt.p, cst,
+                 c.blend_t[0], c.blend_t[1], c.s);
+    return 0;
+  }
   CHK(rbf_interpolation(c, dst, dim, 0));
   CHK(rbf_interpolation(c, c.vtmp.p, dim, 1));
-  launch_blend(dst, c.vtmp.p, c.ld, c.Np, dim, c.alive.p, c.ptime.p, c.pdt.p, cst, c.blend_t[0], c.blend_t[1], c.s);
+  launch_blend(dst, dst, c.vtmp.p, c.ld, c.Np, dim, c.alive.p, c.ptime.p, c.pdt.p, cst, c.blend_t[0], c.blend_t[1],
+               c.s);
   return 0;
 }
 
@@ -1155,7 +1169,7 @@ static int advect_adaptive(Ctx& c, double t0, double t1, int max_attempts) {
   const size_t ld = (size_t)c.ld;
   HIPCHK(c.x0.reserve(ld * dim));
   for (int j = 0; j < 7; j++) HIPCHK(c.vadv[j].reserve(ld * dim));
-  if (c.blend) HIPCHK(c.vtmp.reserve(ld * dim));
+  if (c.blend) HIPCHK(c.vtmp.reserve(ld * 2 * dim));
   HIPCHK(c.ptime.reserve(ld));
   HIPCHK(c.pdt.reserve(ld));
   HIPCHK(c.pfin.reserve(ld));

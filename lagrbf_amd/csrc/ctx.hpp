@@ -79,6 +79,7 @@ struct Ctx {
   // time blend of two snapshots (fields 0..dim-1 at tA, dim..2dim-1 at tB)
   bool setup_all = false;                  // every patch at the first set-up, not only the touched ones
   bool blend = false;
+  int blend_fused = -1;                    // both snapshots in one evaluation: 1, 0, -1 = FP32 only
   double blend_t[2] = {0.0, 1.0};
   // per-particle adaptive DP45 state and counters
   DBuf<double> ptime, pdt, vtmp;

@@ -72,6 +72,14 @@ __host__ __device__ constexpr int n_corners_in_nbr(int nbi) {
   return n;
 }
 
+// Whether two rows of neighbour nbi's table entries fit the scalar registers next
+// to the rest of the kernel (at most 72 of them: the FP32 centre neighbour of a
+// 3- or 4-field table, the FP64 face neighbours of a 3-field table).
+template <int DIM, int NFT, typename T>
+__host__ __device__ constexpr bool two_rows_fit(int nbi) {
+  return 2 * (4 + n_corners_in_nbr<DIM>(nbi) * NFT) * (int)(sizeof(T) / 4) <= 72;
+}
+
 template <int KT>
 __host__ __device__ constexpr bool compact_support_blk() { return KT == K_WC0 || KT == K_WC2 || KT == K_BUMP; }
 
@@ -281,7 +289,7 @@ __device__ __forceinline__ void accum_nbrs(const Grid& g, int ci, int cj, int ck
         }
         // two FP64 rows of the centre neighbour (all 2^d corners) would not fit the
         // scalar registers: it keeps the single-row loop
-        if constexpr (LRB_EVAL_PIPE && (sizeof(T) == 4 || n_corners_in_nbr<DIM>(NBI) < (1 << DIM)))
+        if constexpr (LRB_EVAL_PIPE && two_rows_fit<DIM, NFT, T>(NBI))
           accum_blk_pipe<DIM, KT, NFT, T, NBI>(qx, qy, qz, blk + (size_t)nb * BS, ne - nb, acc, eps, rcut2);
         else
           accum_blk<DIM, KT, NFT, T, NBI>(qx, qy, qz, blk + (size_t)nb * BS, ne - nb, acc, eps, rcut2);
@@ -395,8 +403,10 @@ void k_eval_blk(Grid g, const double* __restrict__ px, int64_t ld,
   case V:                                                                                               \
     if constexpr (PIPE == 2 && 4 + n_corners_in_nbr<DIM>(V) * NFT <= 16)                                 \
       accum_blk_pipe2<DIM, KT, NFT, T, V>(qx, qy, qz, B, ne - nb, acc, eps, rcut2);                     \
-    else                                                                                                \
+    else if constexpr (two_rows_fit<DIM, NFT, T>(V))                                                    \
       accum_blk_pipe<DIM, KT, NFT, T, V>(qx, qy, qz, B, ne - nb, acc, eps, rcut2);                      \
+    else                                                                                                \
+      accum_blk<DIM, KT, NFT, T, V>(qx, qy, qz, B, ne - nb, acc, eps, rcut2);                           \
     break;
 #else
 #define LRB_CASE(V) case V: accum_blk<DIM, KT, NFT, T, V>(qx, qy, qz, B, ne - nb, acc, eps, rcut2); break;
@@ -693,7 +703,9 @@ void launch_nonempty_cells(const int32_t* pstart, int C, int32_t* list, int32_t*
 
 size_t nonempty_list_capacity(int C) { return (size_t)std::max(C, 1) + (size_t)(std::max(C, 1) + 255) / 256 + 1; }
 
-bool eval_blk_supported(int nft) { return nft >= 1 && nft <= 4; }
+// six fields (two 3D velocity snapshots in one pass) in FP32 only: the FP64 kernel
+// would need 96 accumulator registers per lane
+bool eval_blk_supported(int nft, int dim, bool f32) { return (nft >= 1 && nft <= 4) || (nft == 6 && dim == 3 && f32); }
 size_t blk_row(int dim, int nft) { return 4 + ((size_t)1 << dim) * (size_t)nft; }
 
 void launch_build_blk(const Grid& g, bool fp32, const double* x, int N, const int32_t* cn, const int32_t* dcell,
@@ -741,7 +753,14 @@ static void eval_blk_k(const Grid& g, const double* px, int64_t ld, const int32_
     case 1: LRB_EB(1, 1); break;
     case 2: LRB_EB(2, 1); break;
     case 3: LRB_EB(3, 1); break;
-    default: LRB_EB(4, 1); break;
+    case 4: LRB_EB(4, 1); break;
+    default:
+      // six fields, FP32: two 3D velocity snapshots in one pass (48 accumulators per
+      // lane: 6 waves per SIMD, no spills)
+      if constexpr (DIM == 3 && sizeof(T) == 4)
+        LRB_LAUNCH_TIMED((k_eval_blk<DIM, KT, 6, T, 1, 6>), grid, blkd, s, g, px, ld, perm, pstart, w, blk, cs,
+                         nfields, out, cell_list, list_count, xcd_runs);
+      break;
   }
 #undef LRB_EB
 }

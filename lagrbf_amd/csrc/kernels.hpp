@@ -99,7 +99,7 @@ void launch_eval_cell(const Grid& g, int ktype, bool fp32, const double* px, int
                       int nfields, double* out, int block, hipStream_t s);
 // (eval_blk.hip) cell-major node table: per node row [coords, pad, coefficients in
 // the 2^d corner patches of its cell]; evaluation for nft <= 4 fields
-bool eval_blk_supported(int nft);
+bool eval_blk_supported(int nft, int dim, bool f32);
 size_t blk_row(int dim, int nft);
 // zero rows after the N node rows of the evaluation table (the node loops read up
 // to two rows past a cell's last node)
@@ -144,8 +144,8 @@ void launch_dopri_err_req(const double* px, const double* x0, const double* cons
                           const int32_t* alive, const int32_t* req, const double* dt_req, double atol, double rtol,
                           unsigned long long* maxb_req, int nslots, hipStream_t s);
 // per-particle adaptive DP45 with time-blended velocity (adaptive.hip)
-void launch_blend(double* va, const double* vb, int64_t ld, int Np, int dim, const int32_t* alive, const double* tp,
-                  const double* dtp, double cst, double tA, double tB, hipStream_t s);
+void launch_blend(double* out, const double* va, const double* vb, int64_t ld, int Np, int dim, const int32_t* alive,
+                  const double* tp, const double* dtp, double cst, double tA, double tB, hipStream_t s);
 void launch_dp45_control(double* px, double* x0, const double* const* k, double* k1, int64_t ld, int Np, int dim,
                          int32_t* alive, int32_t* fin, double* tp, double* dtp, double t_end, double atol, double rtol,
                          double fac, double fac_min, double fac_max, unsigned long long* counts, hipStream_t s);

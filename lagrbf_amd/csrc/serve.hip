@@ -457,7 +457,7 @@ int32_t lagrbf_serve_add_dataset(void* srv, const char* options, int32_t n, doub
   Ctx* c = create_from_view(v);
   if (!c) return -E_WRONG;
   std::unique_ptr<Ctx> own(c);
-  if (c->nf != c->g.dim || !eval_blk_supported(c->nf))
+  if (c->nf != c->g.dim || !eval_blk_supported(c->nf, c->g.dim, false))
     return -set_err(E_WRONG, fn, "a serving dataset carries the velocity: n_interpolation_fields must equal the dimension");
   if (hipSetDevice(c->device) != hipSuccess) return -set_err(E_LIB, fn, "hipSetDevice failed");
   int e = fill_data(*c, n, coords, values, nullptr);

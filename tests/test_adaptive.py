@@ -170,3 +170,27 @@ def test_slab_configuration_matches_oracle():
     assert abs(cg["accepted"] - co["accepted"]) <= 0.01 * co["accepted"]
     assert np.array_equal(np.any(xg != 0, axis=1), live)
     assert rel < 1e-8
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_blend_fused_matches_two_passes(precision):
+    """Both snapshots evaluated in one pass (6 fields, the FP32 default) against one
+    evaluation per snapshot: the same sums per field, so the same trajectories and
+    step counts, bit for bit (FP64 runs one pass per snapshot either way)."""
+    import lagrbf_amd as L
+    o, X, F, P = _fourier_blend_case(Np=4000)
+    print(f"case: adaptive blend fused vs two passes, {precision}", flush=True)
+    res = []
+    for fused in (1, 0):
+        g = L.LagRBFCtx(petsc_options=dict(o, lagrbf_blend_fused=fused), precision=precision)
+        g.fill_data(X, F)
+        g.fill_points(P)
+        g.interpolate_c()
+        cnt = g.advect_adaptive_c(0.0, 0.05)
+        xs = np.zeros_like(P)
+        g.output(coords=xs)
+        res.append((xs, cnt))
+        g.destroy()
+    assert res[0][1] == res[1][1]
+    assert np.array_equal(res[0][0], res[1][0])
