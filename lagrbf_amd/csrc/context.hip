@@ -623,8 +623,9 @@ static int ensure_blk_group(Ctx& c, bool f32, int g) {
 
 // group < 0: fields 0 .. nfields-1 of the n_interpolation_fields table; group
 // 0 / 1: the dim fields of that snapshot (cell-table evaluation only).
-// Density bucket of the autotuner: particles per cell of the swarm.
-static int eval_bucket(const Ctx& c) { return density_bucket(c.Np, c.g.C); }
+// Density bucket of the autotuner: particles per cell of the swarm (the whole
+// swarm's while the adaptive loop evaluates its unfinished prefix).
+static int eval_bucket(const Ctx& c) { return density_bucket(c.bucket_np >= 0 ? c.bucket_np : c.Np, c.g.C); }
 
 static int rbf_interpolation(Ctx& c, double* out, int nfields, int group = -1) {
   if (nfields <= 0 || c.Np == 0) return 0;
@@ -1189,8 +1190,21 @@ static int advect_adaptive(Ctx& c, double t0, double t1, int max_attempts) {
   static const double Cst[6] = {1.0 / 5.0, 3.0 / 10.0, 4.0 / 5.0, 8.0 / 9.0, 1.0, 1.0};
   DBuf<unsigned long long> cnt;
   HIPCHK(cnt.reserve(3));
+  // Once the swarm is reordered, the unfinished particles fill slots [0, live):
+  // the attempt's launches cover that prefix only (the reorder itself covers the
+  // whole swarm, so the finished tail moves with it).  The autotuner keeps the
+  // whole swarm's density bucket.
+  struct Prefix {
+    Ctx& c;
+    int np;
+    ~Prefix() { c.Np = np; c.bucket_np = -1; }
+  } restore{c, Np};
+  int live = Np;
   for (int a = 0; a < max_attempts; a++) {
     CHK(reorder_adaptive(c));
+    const int n = c.reorder ? live : Np;
+    c.Np = n;
+    c.bucket_np = Np;
     for (int s = 0; s < 6; s++) {
       const double* v[7];
       for (int j = 0; j <= s; j++) v[j] = c.vadv[j].p;
@@ -1205,11 +1219,14 @@ static int advect_adaptive(Ctx& c, double t0, double t1, int max_attempts) {
     HIPCHK(hipMemsetAsync(cnt.p, 0, 3 * sizeof(unsigned long long), c.s));
     const double* k[7];
     for (int j = 0; j < 7; j++) k[j] = c.vadv[j].p;
-    launch_dp45_control(c.px.p, c.x0.p, k, c.vadv[0].p, c.ld, Np, dim, c.alive.p, c.pfin.p, c.ptime.p, c.pdt.p, t1,
+    launch_dp45_control(c.px.p, c.x0.p, k, c.vadv[0].p, c.ld, n, dim, c.alive.p, c.pfin.p, c.ptime.p, c.pdt.p, t1,
                         c.dopri_atol, c.dopri_rtol, c.dopri_fac, c.dopri_fac_min, c.dopri_fac_max, cnt.p, c.s);
+    c.Np = Np;
+    c.bucket_np = -1;
     unsigned long long h[3] = {0, 0, 0};
     HIPCHK(hipMemcpyAsync(h, cnt.p, sizeof(h), hipMemcpyDeviceToHost, c.s));
     HIPCHK(hipStreamSynchronize(c.s));
+    live = (int)h[2];
     c.dp_attempts += 1;
     c.dp_accepted += (double)h[0];
     c.dp_rejected += (double)h[1];

@@ -86,6 +86,7 @@ struct Ctx {
   DBuf<int32_t> pfin, pself;
   DBuf<double> ptime_alt, pdt_alt;     // reorder targets of the adaptive state
   bool cell_ordered = false;              // swarm stored in cell order since the last fill
+  int bucket_np = -1;                     // swarm size for the tuner's bucket while Np is a prefix
   DBuf<int32_t> pfin_alt;
   double dp_attempts = 0, dp_accepted = 0, dp_rejected = 0, dp_active = 0;
   hipStream_t s = nullptr;
