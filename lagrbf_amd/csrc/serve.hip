@@ -211,12 +211,16 @@ void eval_batch(Batch& b, double* out) {
                     b.Np, listed ? b.cell_list.p : nullptr, listed ? b.list_count.p : nullptr, grp, pipe, waves, b.s);
   };
   // the batch is stored in cell order at every tick
-  d.tuner.run(density_bucket(b.Np, d.g.C), d.nf == d.g.dim, f32, true, true, b.s, launch);
+  const int bucket = density_bucket(b.Np, d.g.C);
+  // the non-empty cell list, for the shapes that use it (and the tuner's trials)
+  const int planned = d.tuner.planned(bucket);
+  if (planned < 0 || eval_variant_listed(planned))
+    launch_nonempty_cells(b.pstart.p, d.g.C, b.cell_list.p, b.list_count.p, b.s);
+  d.tuner.run(bucket, d.nf == d.g.dim, f32, true, true, b.s, launch);
 }
 
 void group(Batch& b) {
   launch_sort_particles(b.cell.p, b.alive.p, b.Np, b.ds->g.C, b.pstart.p, b.perm.p, b.pscratch.p, b.s);
-  launch_nonempty_cells(b.pstart.p, b.ds->g.C, b.cell_list.p, b.list_count.p, b.s);
 }
 
 StageArgs serve_stage_args(const Batch& b, int nv, const double* const* v, const double* a, double mul, double div,
