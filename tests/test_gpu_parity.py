@@ -502,3 +502,35 @@ def test_eval_launch_shapes_identical(precision, Np, case):
         vo = np.zeros_like(vals[0])
         o.output(None, vo, None)
         assert rel_err(vals[0], vo) < TOL_VAL
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_six_field_evaluation(precision):
+    """Six interpolated fields in 3D (two velocity snapshots): the FP32 cell table
+    evaluates all six in one pass (its own kernel), FP64 falls back to the per-cell
+    kernel.  Against the oracle (FP64 1e-12, FP32 evaluation 1e-5 relative), and every
+    FP32 launch shape bit-identical."""
+    opts, X, V, P = cases.fourier_3d(nlat=20, Np=30000, n=9)
+    F = np.concatenate([V, 0.7 * np.roll(V, 1, axis=1)], axis=1)
+    opts = dict(opts, n_interpolation_fields=6, advection=False)
+    print(f"case: six fields fourier3d {precision}", flush=True)
+    from oracle.oracle import OracleCtx
+    o = OracleCtx(L.LagRBFCtx(petsc_options=dict(opts)).options)
+    o.fill_data(X, F); o.fill_points(P); o.interpolate_c()
+    vo = np.zeros((P.shape[0], 6))
+    o.output(None, vo, None)
+    vals = []
+    for v in ((-1,) if precision == "fp64" else (0, 1, 5, 9)):
+        g = L.LagRBFCtx(petsc_options=dict(opts, lagrbf_eval_variant=v), precision=precision)
+        g.fill_data(X, F)
+        g.fill_points(P)
+        g.interpolate_c()
+        out = np.zeros((P.shape[0], 6))
+        g.output(fvals=out)
+        g.destroy()
+        vals.append(out)
+    err = rel_err(vals[0], vo)
+    print(f"six fields {precision}: rel err vs oracle {err:.3e}", flush=True)
+    assert err < (TOL_VAL if precision == "fp64" else 1e-5)
+    for v in vals[1:]:
+        assert np.array_equal(vals[0], v)
