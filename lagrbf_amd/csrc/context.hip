@@ -402,11 +402,14 @@ static int solve_list(Ctx& c, const std::vector<int32_t>& sl) {
   if (!l[M_CHOL64B].empty() || !l[M_CHOL32B].empty())
     HIPCHK(c.solve_ws.reserve((c.mem_used + 1) * 2 * (size_t)std::max(c.nf, 1)));
   for (int m = M_CHOL64; m < N_METHODS; m++)
-    if (!l[m].empty())
+    if (!l[m].empty()) {
+      int nmax = 1;
+      for (int p : l[m]) nmax = std::max(nmax, (int)c.h_n[p]);
       launch_solve_chol(c.g, c.ktype, method_fp32(m), c.list.p + (size_t)m * P, (int)l[m].size(), c.p_n.p, c.p_mem_off.p,
                         c.p_fac_off.p, c.mem_pool.p, c.dx.p, method_fp32(m) ? (const void*)c.fac32_pool.p : (const void*)c.fac_pool.p,
                         c.dval.p, c.nf, c.coef_pool.p, c.refine_max_iter, c.refine_tol, c.pstatus.p, c.piters.p,
-                        method_big(m) ? c.solve_ws.p : nullptr, c.s);
+                        method_big(m) ? c.solve_ws.p : nullptr, nmax, c.s);
+    }
   if (l[M_CHOL32].empty() && l[M_CHOL32B].empty()) return 0;
   std::vector<int32_t> st(P), its(P);
   HIPCHK(hipMemcpyAsync(st.data(), c.pstatus.p, sizeof(int32_t) * P, hipMemcpyDeviceToHost, c.s));

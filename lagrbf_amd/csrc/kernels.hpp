@@ -63,7 +63,7 @@ void launch_factor_chol(const Grid& g, int ktype, bool fp32, const int32_t* list
 void launch_solve_chol(const Grid& g, int ktype, bool fp32, const int32_t* list, int nlist, const int32_t* pn,
                        const int64_t* mem_off, const int64_t* fac_off, const int32_t* mem_pool, const double* xd,
                        const void* fac, const double* values, int nf, double* coef, int max_iter, double tol,
-                       int32_t* status, int32_t* iters, double* ws, hipStream_t s);
+                       int32_t* status, int32_t* iters, double* ws, int nmax /* largest listed patch */, hipStream_t s);
 
 // ---- weights / evaluation ----
 void launch_weights(const Grid& g, const double* pux, const double* puy, const double* puz, const double* px,

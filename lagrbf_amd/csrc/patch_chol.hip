@@ -13,6 +13,8 @@
 // T, so the FP32 matrix carries relative, not absolute, rounding.
 #include "kernels.hpp"
 
+#include <algorithm>
+
 #include <cmath>
 
 namespace lrb {
@@ -119,30 +121,39 @@ __global__ __launch_bounds__(CNT) void k_factor_chol(Grid g, const int32_t* __re
 // values (PartitionMatSetRHS, partition_factor.c:57-90).  With a float factor the
 // FP64 solution is refined: r = B - A X with A recomputed in FP64, X += solve(r),
 // until the correction is below tol * |X| (or max_iter); status 1 = not converged.
-template <typename TL, int KT>
-__global__ __launch_bounds__(CNT) void k_solve_chol(Grid g, const int32_t* __restrict__ list, const int32_t* __restrict__ pn,
+template <typename TL, int KT, int NT>
+__global__ __launch_bounds__(NT) void k_solve_chol(Grid g, const int32_t* __restrict__ list, const int32_t* __restrict__ pn,
                                                     const int64_t* __restrict__ mem_off, const int64_t* __restrict__ fac_off,
                                                     const int32_t* __restrict__ mem_pool, const double* __restrict__ xd,
                                                     const TL* __restrict__ fac, const double* __restrict__ values, int nf,
                                                     double* __restrict__ coef, int max_iter, double tol,
-                                                    int32_t* __restrict__ status, int32_t* __restrict__ iters) {
+                                                    int32_t* __restrict__ status, int32_t* __restrict__ iters, int S) {
+  // S: row stride of the LDS arrays (the largest patch of the launch)
   extern __shared__ __align__(16) unsigned char smem[];
-  double* xr = reinterpret_cast<double*>(smem);      // [3][CNMAX] relative coords (FP64)
-  double* b = xr + 3 * CNMAX;                        // [nf][n] right-hand side / residual
-  double* y = b + (size_t)8 * CNMAX;                 // [nf][n] work vector
-  __shared__ double red[CNT / CW][8];
+  double* xr = reinterpret_cast<double*>(smem);      // [3][S] relative coords (FP64)
+  double* b = xr + (size_t)3 * S;                    // [nf][S] right-hand side / residual
+  double* y = b + (size_t)nf * S;                    // [nf][S] work vector
+  double* xl = y + (size_t)nf * S;                   // [nf][S] the solution, kept here until the end
+  __shared__ double red[NT / CW][8];
   const int p = list[blockIdx.x];
   const int n = pn[p];
   const int tid = threadIdx.x, lane = tid & (CW - 1), wid = tid / CW;
   if (n == 0 || nf == 0) return;
+  if (n > S) {   // not sized for this patch: reported as not converged (refactored in FP64)
+    if (tid == 0) {
+      if (status) status[p] = 1;
+      if (iters) iters[p] = 0;
+    }
+    return;
+  }
   const int dim = g.dim;
   const int32_t* mem = mem_pool + mem_off[p];
   const TL* L = fac + fac_off[p];
   double* X = coef + mem_off[p] * nf;                // row-major [n][nf]
-  for (int i = tid; i < n; i += CNT) {
+  for (int i = tid; i < n; i += NT) {
     const double* x = xd + (size_t)mem[i] * dim;
-    for (int d = 0; d < 3; d++) xr[d * CNMAX + i] = d < dim ? x[d] - patch_centre(g, p, d) : 0.0;
-    for (int f = 0; f < nf; f++) { const double v = values[(size_t)mem[i] * nf + f]; b[f * CNMAX + i] = v; X[(size_t)i * nf + f] = 0.0; }
+    for (int d = 0; d < 3; d++) xr[d * S + i] = d < dim ? x[d] - patch_centre(g, p, d) : 0.0;
+    for (int f = 0; f < nf; f++) { const double v = values[(size_t)mem[i] * nf + f]; b[f * S + i] = v; xl[f * S + i] = 0.0; }
   }
   __syncthreads();
   const bool refine = sizeof(TL) == sizeof(float);
@@ -152,42 +163,42 @@ __global__ __launch_bounds__(CNT) void k_solve_chol(Grid g, const int32_t* __res
   for (; it <= (refine ? max_iter : 0); it++) {
     if (it > 0) {
       // residual b - A X in FP64, A recomputed from the member coordinates
-      for (int i = tid; i < n; i += CNT) {
+      for (int i = tid; i < n; i += NT) {
         double s[8];
         FOR_F s[f] = values[(size_t)mem[i] * nf + f];
-        const double xi0 = xr[i], xi1 = xr[CNMAX + i], xi2 = xr[2 * CNMAX + i];
+        const double xi0 = xr[i], xi1 = xr[S + i], xi2 = xr[2 * S + i];
         for (int j = 0; j < n; j++) {
-          const double d0 = xi0 - xr[j], d1 = xi1 - xr[CNMAX + j], d2 = xi2 - xr[2 * CNMAX + j];
+          const double d0 = xi0 - xr[j], d1 = xi1 - xr[S + j], d2 = xi2 - xr[2 * S + j];
           const double a = rbf<KT, double>(sqrt_fast(d0 * d0 + d1 * d1 + d2 * d2), g.eps_r);
-          FOR_F s[f] -= a * X[(size_t)j * nf + f];
+          FOR_F s[f] -= a * xl[f * S + j];
         }
-        FOR_F b[f * CNMAX + i] = s[f];
+        FOR_F b[f * S + i] = s[f];
       }
       __syncthreads();
     }
     for (int f = 0; f < nf; f++)
-      for (int i = tid; i < n; i += CNT) y[f * CNMAX + i] = b[f * CNMAX + i];
+      for (int i = tid; i < n; i += NT) y[f * S + i] = b[f * S + i];
     __syncthreads();
     // forward: L z = b (column oriented)
     for (int k = 0; k < n; k++) {
       const double lkk = (double)L[(size_t)k * n + k];
       double zk[8];
-      FOR_F zk[f] = y[f * CNMAX + k] / lkk;
-      for (int i = k + 1 + tid; i < n; i += CNT) {
+      FOR_F zk[f] = y[f * S + k] / lkk;
+      for (int i = k + 1 + tid; i < n; i += NT) {
         const double lik = (double)L[(size_t)k * n + i];
-        FOR_F y[f * CNMAX + i] -= lik * zk[f];
+        FOR_F y[f * S + i] -= lik * zk[f];
       }
       __syncthreads();
-      if (tid == 0) FOR_F y[f * CNMAX + k] = zk[f];
+      if (tid == 0) FOR_F y[f * S + k] = zk[f];
     }
     __syncthreads();
     // backward: L^T d = z (dot products down each column)
     for (int k = n - 1; k >= 0; k--) {
       double s[8];
       FOR_F s[f] = 0.0;
-      for (int i = k + 1 + tid; i < n; i += CNT) {
+      for (int i = k + 1 + tid; i < n; i += NT) {
         const double lik = (double)L[(size_t)k * n + i];
-        FOR_F s[f] += lik * y[f * CNMAX + i];
+        FOR_F s[f] += lik * y[f * S + i];
       }
       FOR_F {
         double v = s[f];
@@ -197,18 +208,18 @@ __global__ __launch_bounds__(CNT) void k_solve_chol(Grid g, const int32_t* __res
       __syncthreads();
       if (tid < nf) {
         double t = red[0][tid];
-        for (int w2 = 1; w2 < CNT / CW; w2++) t += red[w2][tid];
-        y[tid * CNMAX + k] = (y[tid * CNMAX + k] - t) / (double)L[(size_t)k * n + k];
+        for (int w2 = 1; w2 < NT / CW; w2++) t += red[w2][tid];
+        y[tid * S + k] = (y[tid * S + k] - t) / (double)L[(size_t)k * n + k];
       }
       __syncthreads();
     }
     // X += d; convergence test on max |d| / max |X|
     double dmax = 0.0, xmax = 0.0;
-    for (int i = tid; i < n; i += CNT)
+    for (int i = tid; i < n; i += NT)
       for (int f = 0; f < nf; f++) {
-        const double d = y[f * CNMAX + i];
-        const double xn = X[(size_t)i * nf + f] + d;
-        X[(size_t)i * nf + f] = xn;
+        const double d = y[f * S + i];
+        const double xn = xl[f * S + i] + d;
+        xl[f * S + i] = xn;
         dmax = fmax(dmax, fabs(d));
         xmax = fmax(xmax, fabs(xn));
       }
@@ -216,7 +227,7 @@ __global__ __launch_bounds__(CNT) void k_solve_chol(Grid g, const int32_t* __res
     if (lane == 0) { red[wid][0] = dmax; red[wid][1] = xmax; }
     __syncthreads();
     dmax = red[0][0]; xmax = red[0][1];
-    for (int w2 = 1; w2 < CNT / CW; w2++) { dmax = fmax(dmax, red[w2][0]); xmax = fmax(xmax, red[w2][1]); }
+    for (int w2 = 1; w2 < NT / CW; w2++) { dmax = fmax(dmax, red[w2][0]); xmax = fmax(xmax, red[w2][1]); }
     __syncthreads();
     if (!refine) { converged = 1; break; }
     // done when the correction is below tol * |X|, or when it has stopped shrinking
@@ -225,6 +236,8 @@ __global__ __launch_bounds__(CNT) void k_solve_chol(Grid g, const int32_t* __res
     if (it > 1 && dmax > 0.5 * dprev && dmax <= 100.0 * tol * xmax) { converged = 1; break; }
     dprev = dmax;
   }
+  for (int i = tid; i < n; i += NT)
+    for (int f = 0; f < nf; f++) X[(size_t)i * nf + f] = xl[f * S + i];
   if (tid == 0) {
     if (status) status[p] = converged ? 0 : 1;
     if (iters) iters[p] = it;
@@ -453,7 +466,10 @@ __global__ __launch_bounds__(CNT) void k_solve_chol_big(Grid g, const int32_t* _
 }
 
 static size_t factor_smem(size_t esz) { return esz * (3 * CNMAX + (size_t)CNMAX * CNB + CNB * CTT); }
-static size_t solve_smem() { return sizeof(double) * (3 * CNMAX + 16 * (size_t)CNMAX); }
+// LDS of the solve: coordinates [3][S]; right-hand side, work vector, solution [nf][S]
+static size_t solve_smem(int S, int nf) { return sizeof(double) * (size_t)S * (3 + 3 * (size_t)nf); }
+static constexpr int SNT = 256;  // threads per patch in the solve (measured against one wave:
+                                 // the residual's parallel work outweighs the cheaper barriers)
 
 int chol_max_n() { return CNMAX; }
 
@@ -476,17 +492,22 @@ template <typename TL, int KT>
 static void solve_chol_k(const Grid& g, const int32_t* list, int nlist, const int32_t* pn, const int64_t* mem_off,
                          const int64_t* fac_off, const int32_t* mem_pool, const double* xd, const TL* fac,
                          const double* values, int nf, double* coef, int max_iter, double tol, int32_t* status,
-                         int32_t* iters, double* ws, hipStream_t s) {
+                         int32_t* iters, double* ws, int nmax, hipStream_t s) {
   if (ws) {
     hipLaunchKernelGGL((k_solve_chol_big<TL, KT>), dim3(nlist), dim3(CNT), 0, s, g, list, pn, mem_off, fac_off, mem_pool,
                        xd, fac, values, nf, coef, max_iter, tol, ws, status, iters);
     return;
   }
   static bool attr = false;
-  const size_t sm = solve_smem();
-  if (!attr) { (void)hipFuncSetAttribute((const void*)k_solve_chol<TL, KT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm); attr = true; }
-  hipLaunchKernelGGL((k_solve_chol<TL, KT>), dim3(nlist), dim3(CNT), sm, s, g, list, pn, mem_off, fac_off, mem_pool, xd, fac,
-                     values, nf, coef, max_iter, tol, status, iters);
+  const int S = std::max(1, std::min(nmax, CNMAX));
+  const size_t sm = solve_smem(S, nf);
+  if (!attr) {
+    (void)hipFuncSetAttribute((const void*)k_solve_chol<TL, KT, SNT>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)solve_smem(CNMAX, 8));
+    attr = true;
+  }
+  hipLaunchKernelGGL((k_solve_chol<TL, KT, SNT>), dim3(nlist), dim3(SNT), sm, s, g, list, pn, mem_off, fac_off, mem_pool,
+                     xd, fac, values, nf, coef, max_iter, tol, status, iters, S);
 }
 
 bool chol_kernel_supported(int ktype) { return ktype == K_WC0 || ktype == K_WC2 || ktype == K_GAUSS || ktype == K_IMQ; }
@@ -515,22 +536,22 @@ template <typename TL>
 static void solve_chol_dispatch(const Grid& g, int ktype, const int32_t* list, int nlist, const int32_t* pn,
                                 const int64_t* mem_off, const int64_t* fac_off, const int32_t* mem_pool, const double* xd,
                                 const TL* fac, const double* values, int nf, double* coef, int max_iter, double tol,
-                                int32_t* status, int32_t* iters, double* ws, hipStream_t s) {
+                                int32_t* status, int32_t* iters, double* ws, int nmax, hipStream_t s) {
   switch (ktype) {
-    case K_WC0: solve_chol_k<TL, K_WC0>(g, list, nlist, pn, mem_off, fac_off, mem_pool, xd, fac, values, nf, coef, max_iter, tol, status, iters, ws, s); break;
-    case K_WC2: solve_chol_k<TL, K_WC2>(g, list, nlist, pn, mem_off, fac_off, mem_pool, xd, fac, values, nf, coef, max_iter, tol, status, iters, ws, s); break;
-    case K_IMQ: solve_chol_k<TL, K_IMQ>(g, list, nlist, pn, mem_off, fac_off, mem_pool, xd, fac, values, nf, coef, max_iter, tol, status, iters, ws, s); break;
-    default: solve_chol_k<TL, K_GAUSS>(g, list, nlist, pn, mem_off, fac_off, mem_pool, xd, fac, values, nf, coef, max_iter, tol, status, iters, ws, s); break;
+    case K_WC0: solve_chol_k<TL, K_WC0>(g, list, nlist, pn, mem_off, fac_off, mem_pool, xd, fac, values, nf, coef, max_iter, tol, status, iters, ws, nmax, s); break;
+    case K_WC2: solve_chol_k<TL, K_WC2>(g, list, nlist, pn, mem_off, fac_off, mem_pool, xd, fac, values, nf, coef, max_iter, tol, status, iters, ws, nmax, s); break;
+    case K_IMQ: solve_chol_k<TL, K_IMQ>(g, list, nlist, pn, mem_off, fac_off, mem_pool, xd, fac, values, nf, coef, max_iter, tol, status, iters, ws, nmax, s); break;
+    default: solve_chol_k<TL, K_GAUSS>(g, list, nlist, pn, mem_off, fac_off, mem_pool, xd, fac, values, nf, coef, max_iter, tol, status, iters, ws, nmax, s); break;
   }
 }
 
 void launch_solve_chol(const Grid& g, int ktype, bool fp32, const int32_t* list, int nlist, const int32_t* pn,
                        const int64_t* mem_off, const int64_t* fac_off, const int32_t* mem_pool, const double* xd,
                        const void* fac, const double* values, int nf, double* coef, int max_iter, double tol,
-                       int32_t* status, int32_t* iters, double* ws, hipStream_t s) {
+                       int32_t* status, int32_t* iters, double* ws, int nmax, hipStream_t s) {
   if (nlist <= 0) return;
-  if (fp32) solve_chol_dispatch<float>(g, ktype, list, nlist, pn, mem_off, fac_off, mem_pool, xd, (const float*)fac, values, nf, coef, max_iter, tol, status, iters, ws, s);
-  else solve_chol_dispatch<double>(g, ktype, list, nlist, pn, mem_off, fac_off, mem_pool, xd, (const double*)fac, values, nf, coef, max_iter, tol, status, iters, ws, s);
+  if (fp32) solve_chol_dispatch<float>(g, ktype, list, nlist, pn, mem_off, fac_off, mem_pool, xd, (const float*)fac, values, nf, coef, max_iter, tol, status, iters, ws, nmax, s);
+  else solve_chol_dispatch<double>(g, ktype, list, nlist, pn, mem_off, fac_off, mem_pool, xd, (const double*)fac, values, nf, coef, max_iter, tol, status, iters, ws, nmax, s);
 }
 
 }  // namespace lrb
