@@ -179,27 +179,66 @@ __global__ __launch_bounds__(NT) void k_solve_chol(Grid g, const int32_t* __rest
     for (int f = 0; f < nf; f++)
       for (int i = tid; i < n; i += NT) y[f * S + i] = b[f * S + i];
     __syncthreads();
-    // forward: L z = b (column oriented)
-    for (int k = 0; k < n; k++) {
-      const double lkk = (double)L[(size_t)k * n + k];
-      double zk[8];
-      FOR_F zk[f] = y[f * S + k] / lkk;
-      for (int i = k + 1 + tid; i < n; i += NT) {
-        const double lik = (double)L[(size_t)k * n + i];
-        FOR_F y[f * S + i] -= lik * zk[f];
+    // forward: L z = b (column oriented).  A thread holds rows k+1+tid and
+    // k+1+tid+NT of column k (n <= 2 NT); the next column's entries are loaded one
+    // step ahead, so the global loads overlap the current step instead of
+    // stalling every one of the n sequential steps.
+    static_assert(2 * NT >= CNMAX, "two rows per thread cover a column");
+    {
+      double dk = (double)L[0];
+      double l0 = 1 + tid < n ? (double)L[1 + tid] : 0.0;
+      double l1 = 1 + tid + NT < n ? (double)L[1 + tid + NT] : 0.0;
+      for (int k = 0; k < n; k++) {
+        double dn = 0.0, n0 = 0.0, n1 = 0.0;
+        if (k + 1 < n) {
+          const TL* c1 = L + (size_t)(k + 1) * n;
+          dn = (double)c1[k + 1];
+          if (k + 2 + tid < n) n0 = (double)c1[k + 2 + tid];
+          if (k + 2 + tid + NT < n) n1 = (double)c1[k + 2 + tid + NT];
+        }
+        double zk[8];
+        FOR_F zk[f] = y[f * S + k] / dk;
+        const int i0 = k + 1 + tid, i1 = i0 + NT;
+        if (i0 < n) {
+          const double lik = l0;
+          FOR_F y[f * S + i0] -= lik * zk[f];
+        }
+        if (i1 < n) {
+          const double lik = l1;
+          FOR_F y[f * S + i1] -= lik * zk[f];
+        }
+        __syncthreads();
+        if (tid == 0) FOR_F y[f * S + k] = zk[f];
+        dk = dn; l0 = n0; l1 = n1;
       }
-      __syncthreads();
-      if (tid == 0) FOR_F y[f * S + k] = zk[f];
     }
     __syncthreads();
-    // backward: L^T d = z (dot products down each column)
+    // backward: L^T d = z (dot products down each column), the previous column's
+    // entries loaded one step ahead
+    double b0 = 0.0, b1 = 0.0;                          // column n-1 has no rows below
+    double dkb = (double)L[(size_t)(n - 1) * n + (n - 1)];
     for (int k = n - 1; k >= 0; k--) {
+      double pn0 = 0.0, pn1 = 0.0, pd = 0.0;
+      if (k >= 1) {
+        const TL* cp = L + (size_t)(k - 1) * n;
+        pd = (double)cp[k - 1];
+        if (k + tid < n) pn0 = (double)cp[k + tid];
+        if (k + tid + NT < n) pn1 = (double)cp[k + tid + NT];
+      }
       double s[8];
       FOR_F s[f] = 0.0;
-      for (int i = k + 1 + tid; i < n; i += NT) {
-        const double lik = (double)L[(size_t)k * n + i];
-        FOR_F s[f] += lik * y[f * S + i];
+      {
+        const int i0 = k + 1 + tid, i1 = i0 + NT;
+        if (i0 < n) {
+          const double lik = b0;
+          FOR_F s[f] += lik * y[f * S + i0];
+        }
+        if (i1 < n) {
+          const double lik = b1;
+          FOR_F s[f] += lik * y[f * S + i1];
+        }
       }
+      b0 = pn0; b1 = pn1;
       FOR_F {
         double v = s[f];
         for (int o = CW / 2; o > 0; o >>= 1) v += __shfl_xor(v, o);
@@ -209,8 +248,9 @@ __global__ __launch_bounds__(NT) void k_solve_chol(Grid g, const int32_t* __rest
       if (tid < nf) {
         double t = red[0][tid];
         for (int w2 = 1; w2 < NT / CW; w2++) t += red[w2][tid];
-        y[tid * S + k] = (y[tid * S + k] - t) / (double)L[(size_t)k * n + k];
+        y[tid * S + k] = (y[tid * S + k] - t) / dkb;
       }
+      dkb = pd;
       __syncthreads();
     }
     // X += d; convergence test on max |d| / max |X|
