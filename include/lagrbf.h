@@ -144,7 +144,10 @@ void lagrbf_serve_destroy(void *srv);
 /* Dataset from an options string ("-pu_mesh_dim 3 -pu_mesh_n 32,32,32 ...", the
  * names lagrbf_set_option accepts; n_interpolation_fields must equal the
  * dimension: values are the velocity) and n nodes (coords [n*dim], values
- * [n*dim]).  Returns the dataset id (>= 0) or a negative error code. */
+ * [n*dim]).  -lagrbf_serve_tune_min_particles m (default 4194304): batch
+ * evaluations of fewer than m particles run one fixed launch shape instead of
+ * the autotuner's trials.  Returns the dataset id (>= 0) or a negative error
+ * code. */
 int32_t lagrbf_serve_add_dataset(void *srv, const char *options, int32_t n, double *coords, double *values);
 
 /* Queue a request: n particles (coords [n*dim], inside the dataset's box), scheme

@@ -114,6 +114,7 @@ struct Ctx {
   int table_gen = 0;
   EvalTuner tuner;                         // evaluation launch shape per particle density (this data)
   int eval_variant = -1;                   // forced launch shape (tests), -1: tuned
+  int serve_tune_min = 0;                  // serving: smaller batch evaluations skip the tuner
   int last_variant = -1;                   // launch shape of the last cell-table evaluation
   std::vector<int32_t> h_cell_start_prev, h_cell_nodes_prev;   // cell lists the frozen patches were built from
   bool memb_current = true;                // every set-up patch's members == current cell lists

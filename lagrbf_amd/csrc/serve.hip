@@ -195,6 +195,9 @@ int admit(Server& sv, Batch& b, ServeReq& r) {
   return 0;
 }
 
+static constexpr int SMALL_BATCH_VARIANT = 1;   // listed cells, contiguous XCD runs
+static constexpr int SERVE_TUNE_MIN_DEFAULT = 1 << 22;
+
 // Evaluation launch shape chosen by the dataset's autotuner for this batch's
 // particle density (the first batch of a density times every shape).
 void eval_batch(Batch& b, double* out) {
@@ -212,11 +215,19 @@ void eval_batch(Batch& b, double* out) {
   };
   // the batch is stored in cell order at every tick
   const int bucket = density_bucket(b.Np, d.g.C);
+  // A batch below serve_tune_min particles runs the listed shape in contiguous
+  // XCD runs untimed: batch sizes wander over many density buckets as requests
+  // come and go, and a trial of every shape there costs more than the fastest
+  // shape saves on so few particles.
+  const bool small = d.tuner.forced < 0 && b.Np < d.serve_tune_min;
+  const int planned = small ? SMALL_BATCH_VARIANT : d.tuner.planned(bucket);
   // the non-empty cell list, for the shapes that use it (and the tuner's trials)
-  const int planned = d.tuner.planned(bucket);
   if (planned < 0 || eval_variant_listed(planned))
     launch_nonempty_cells(b.pstart.p, d.g.C, b.cell_list.p, b.list_count.p, b.s);
-  d.tuner.run(bucket, d.nf == d.g.dim, f32, true, true, b.s, launch);
+  if (small)
+    launch(SMALL_BATCH_VARIANT);
+  else
+    d.tuner.run(bucket, d.nf == d.g.dim, f32, true, true, b.s, launch);
 }
 
 void group(Batch& b) {
@@ -461,6 +472,11 @@ int32_t lagrbf_serve_add_dataset(void* srv, const char* options, int32_t n, doub
   Ctx* c = create_from_view(v);
   if (!c) return -E_WRONG;
   std::unique_ptr<Ctx> own(c);
+  bool bad = false;
+  c->serve_tune_min = SERVE_TUNE_MIN_DEFAULT;
+  v.get_int("lagrbf_serve_tune_min_particles", c->serve_tune_min, bad);
+  if (bad || c->serve_tune_min < 0)
+    return -set_err(E_WRONG, fn, "-lagrbf_serve_tune_min_particles must be a non-negative integer");
   if (c->nf != c->g.dim || !eval_blk_supported(c->nf, c->g.dim, false))
     return -set_err(E_WRONG, fn, "a serving dataset carries the velocity: n_interpolation_fields must equal the dimension");
   if (hipSetDevice(c->device) != hipSuccess) return -set_err(E_LIB, fn, "hipSetDevice failed");

@@ -49,7 +49,8 @@ class Server:
     def add_dataset(self, options, coords, values):
         """Dataset of nodes coords [N, d] carrying the velocity values [N, d]
         (n_interpolation_fields = d); options as for LagRBFCtx (pu_mesh_*, kernel_type,
-        lagrbf_precision, ...)."""
+        lagrbf_precision, ...), and lagrbf_serve_tune_min_particles: batches of fewer
+        particles (default 4194304) skip the autotuner's trials."""
         coords = np.ascontiguousarray(coords, dtype=np.float64)
         values = np.ascontiguousarray(values, dtype=np.float64)
         o = dict(options)

@@ -28,10 +28,10 @@ def particles(dim, n, seed):
     return 0.2 + 0.6 * rng.random((n, 3))
 
 
-def make_server(budget=200e9, precision="fp64"):
+def make_server(budget=200e9, precision="fp64", **extra):
     srv = L.Server(hbm_budget=budget)
     for o, X, V in datasets():
-        srv.add_dataset(dict(o, lagrbf_precision=precision), X, V)
+        srv.add_dataset(dict(o, lagrbf_precision=precision, **extra), X, V)
     return srv
 
 
@@ -103,6 +103,24 @@ def test_rk4_matches_oracle_all_patches():
         print(f"dataset {ds}: served RK4 vs oracle max rel err {err:.3e}")
         assert err < 1e-9
     srv.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_tuned_and_untuned_batches_identical(precision):
+    """Batches below lagrbf_serve_tune_min_particles run one fixed launch shape
+    untimed; with the threshold at 0 every batch density goes through the
+    autotuner's trials and run-off instead.  Every shape computes the same sums,
+    so the results agree bit for bit."""
+    print(f"case: serve tuned vs untuned small batches, {precision}", flush=True)
+    out = []
+    for thr in (None, 0):
+        srv = make_server(precision=precision, **({} if thr is None else {"lagrbf_serve_tune_min_particles": thr}))
+        rids = [submit(srv, spec) for spec in MIX]
+        res = drain(srv, rids)
+        out.append([res[r][0] for r in rids])
+        srv.close()
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)
 
 
 def test_budget_admission_order():
