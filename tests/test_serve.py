@@ -123,6 +123,15 @@ def test_tuned_and_untuned_batches_identical(precision):
         assert np.array_equal(a, b)
 
 
+def test_tune_threshold_option_checked():
+    srv = L.Server()
+    o, X, V = datasets()[0]
+    with pytest.raises(RuntimeError, match="lagrbf_serve_tune_min_particles"):
+        srv.add_dataset(dict(o, lagrbf_serve_tune_min_particles=-1), X, V)
+    assert srv.add_dataset(dict(o, lagrbf_serve_tune_min_particles=0), X, V) == 0
+    srv.close()
+
+
 def test_budget_admission_order():
     """With room for one request at a time, the second is admitted only after the
     first finishes; both results equal their unconstrained runs."""
