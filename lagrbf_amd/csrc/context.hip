@@ -117,6 +117,12 @@ static int ctx_from_options(Ctx& c, const OptView& o) {
     else if (pr == "mixed") c.precision = 1;
     else return set_err(E_WRONG, fn, "-lagrbf_precision must be fp64 or mixed (found %s)", pr.c_str());
   }
+  std::string mf;
+  if (o.get_string("lagrbf_mixed_factor", mf)) {
+    if (mf == "fp32") c.mixed_factor64 = false;
+    else if (mf == "fp64") c.mixed_factor64 = true;
+    else return set_err(E_WRONG, fn, "-lagrbf_mixed_factor must be fp32 or fp64 (found %s)", mf.c_str());
+  }
   std::string fm;
   if (o.get_string("lagrbf_factor", fm)) {
     if (fm == "auto") c.factor_mode = 0;
@@ -466,8 +472,11 @@ int partitions_setup(Ctx& c, bool all_patches) {
       c.max_patch = std::max(c.max_patch, n);
       int m = M_BK64;
       if (c.factor_mode != 1 && chol_kernel_supported(c.ktype)) {
-        if (n <= c.chol_big_min) m = c.precision ? M_CHOL32 : M_CHOL64;
-        else m = c.precision ? M_CHOL32B : M_CHOL64B;
+        // the mixed path factors in FP32 and refines in FP64 unless told to factor
+        // in FP64 (the FP32 evaluation then reads the FP64 solution all the same)
+        const bool f32 = c.precision && !c.mixed_factor64;
+        if (n <= c.chol_big_min) m = f32 ? M_CHOL32 : M_CHOL64;
+        else m = f32 ? M_CHOL32B : M_CHOL64B;
       }
       c.h_method[p] = (uint8_t)m;
       if (n > 0) fl[m].push_back(p);

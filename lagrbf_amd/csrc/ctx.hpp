@@ -152,6 +152,7 @@ struct Ctx {
   DBuf<double> solve_ws;                   // solve workspace of the large patches (2 nf per member)
   DBuf<int32_t> pstatus, piters;
   int precision = 0;                       // 0 FP64, 1 FP32 factor + FP64 refinement
+  bool mixed_factor64 = false;             // mixed: factor in FP64 (no refinement), FP32 evaluation kept
   int factor_mode = 0;                     // 0 auto (Cholesky for PD kernels), 1 Bunch-Kaufman always
   bool factors_released = false;           // serving datasets keep coefficients only
   int chol_big_min = 512;                  // patches above this size use the large-patch Cholesky kernels

@@ -131,6 +131,27 @@ def test_patch_coefficients_vs_oracle(precision, factor, tol):
         assert err < 1e-5      # FP32 evaluation (values), stated tolerance
 
 
+def test_mixed_with_fp64_factor():
+    """lagrbf_mixed_factor fp64: the mixed path factors in FP64 (no refinement) and
+    keeps the FP32 evaluation: coefficients within 1e-12 of the oracle, values within
+    the FP32 evaluation's 1e-5."""
+    opts, X, V, P = cases.fourier_3d(nlat=24, Np=4000, n=6)
+    print("case: mixed precision with FP64 factors", flush=True)
+    with pytest.raises(RuntimeError, match="lagrbf_mixed_factor"):
+        L.LagRBFCtx(petsc_options=dict(opts, lagrbf_mixed_factor="fp16"), precision="mixed")
+    opts = dict(opts, lagrbf_precision="mixed", lagrbf_mixed_factor="fp64")
+    g, o = pair(opts)
+    for c in (g, o):
+        c.fill_data(X, V); c.fill_points(P); c.interpolate_c()
+    errs = _coef_errors(g, o, 6 ** 3)
+    print(f"mixed, FP64 factor: max rel coefficient err {errs.max():.3e}, refine iters {g.info('refine_iters'):.0f}")
+    assert errs.max() < 1e-12
+    assert g.info("refine_iters") == 0
+    vg = np.zeros((P.shape[0], 3)); vo = np.zeros_like(vg)
+    g.output(fvals=vg); o.output(fvals=vo)
+    assert rel_err(vg, vo) < 1e-5
+
+
 def test_mixed_rk4_positions():
     """Mixed path through 10 RK4 steps: positions vs the FP64 oracle (FP32 evaluation
     error accumulates as dt * 1e-7 per step)."""
