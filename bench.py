@@ -296,9 +296,24 @@ def c3_bench(L, precision, steps, warmup):
            "particle_rk2_steps_per_s": P.shape[0] * steps / wall, "ms_per_step": wall / steps * 1e3,
            "eval_ms": (ev1 - ev0) / max(n1 - n0, 1), "removed": ctx.info("removed")}
     ctx.destroy()
+    if precision == "mixed":
+        res["setup_s_fp64_factor"] = setup_with_fp64_factor(L, opts, X, V, P)
     log(f"c3 [{precision}]: set-up {t_setup:.2f}s, patch sizes {res['patch_size']}, fallback {res['fallback_patches']:.0f}, "
         f"{res['particle_rk2_steps_per_s']:.3e} particle-RK2-steps/s ({res['ms_per_step']:.1f} ms/step)")
     return res
+
+
+def setup_with_fp64_factor(L, opts, X, V, P):
+    """Set-up time of the mixed path with lagrbf_mixed_factor fp64 (FP64 factors, no
+    refinement, FP32 evaluation) on the same nodes and particles."""
+    ctx = L.LagRBFCtx(petsc_options=dict(opts, lagrbf_mixed_factor="fp64"), precision="mixed")
+    ctx.fill_data(X, V)
+    ctx.fill_points(P)
+    ts = time.perf_counter()
+    ctx.interpolate_c()
+    t = time.perf_counter() - ts
+    ctx.destroy()
+    return t
 
 
 def c4_sizes(scale):
@@ -335,6 +350,8 @@ def c4_bench(L, precision, t_end, m=126, N=8000000, Np=16777216, pu=38):
            "particle_steps_per_s": c["accepted"] / wall, "eval_ms": (ev1 - ev0) / max(n1 - n0, 1),
            "evals": int(n1 - n0)}
     ctx.destroy()
+    if precision == "mixed":
+        res["setup_s_fp64_factor"] = setup_with_fp64_factor(L, opts, X, V, P)
     log(f"c4 [{precision}]: set-up {t_setup:.2f}s, {c['attempts']:.0f} attempts, {c['accepted']:.3e} accepted / "
         f"{c['rejected']:.3e} rejected steps in {wall:.2f}s = {res['particle_steps_per_s']:.3e} particle-steps/s, "
         f"eval {res['eval_ms']:.2f} ms, removed {res['removed']:.0f}")
