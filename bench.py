@@ -380,7 +380,7 @@ def main():
     ap.add_argument("--c4-t-end", type=float, default=0.1, help="end time of the adaptive DP45 run")
     ap.add_argument("--c4-scale", type=float, default=1.0, help="fraction of the C4 node and particle counts")
     ap.add_argument("--eval-variant", type=int, default=-1,
-                    help="fix the evaluation launch shape (0..9, for profiling one kernel); -1: autotuned")
+                    help="fix the evaluation launch shape (0..13, for profiling one kernel); -1: autotuned")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="launch a plain copy of MIB MiB first (scale for rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()

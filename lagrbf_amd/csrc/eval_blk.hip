@@ -317,7 +317,7 @@ void k_eval_blk(Grid g, const double* __restrict__ px, int64_t ld,
                                                   const double* __restrict__ w, const T* __restrict__ blk,
                                                   const int32_t* __restrict__ cs, int nfields,
                                                   double* __restrict__ out, const int32_t* __restrict__ cell_list,
-                                                  const int32_t* __restrict__ list_count, int xcd_runs) {
+                                                  const int32_t* __restrict__ list_count, int deal) {
   constexpr int NEN = 1 << DIM;
   constexpr int NNB = DIM == 3 ? 27 : 9;
   constexpr int BS = blk_stride<DIM, NFT>();
@@ -327,15 +327,22 @@ void k_eval_blk(Grid g, const double* __restrict__ px, int64_t ld,
   const int b = blockIdx.x;
   int c;
   if (cell_list) {
-    // listed cells (ascending): XCD b % 8 takes a contiguous run of them (the grid
-    // is a multiple of 8 with at least as many blocks as listed cells), or block b
-    // takes entry b (consecutive cells on consecutive XCDs)
+    // listed cells (ascending; the grid is a multiple of 8 with at least as many
+    // blocks as listed cells): deal 0, XCD b % 8 takes a contiguous run of them;
+    // deal K >= 1, runs of K consecutive entries go round-robin over the XCDs
+    // (K = 1: block b takes entry b).  Round-robin balances heavy and light cells
+    // over the XCDs; runs of K keep K neighbouring cells' table rows in one L2.
+    // The last partial round of 8K blocks keeps b -> b, so every entry has one block.
     const int cnt = *list_count;
     int pos = b;
-    if (xcd_runs) {
+    if (deal == 0) {
       const int per = (cnt + 7) / 8;
       pos = (b % 8) * per + b / 8;
       if (b / 8 >= per) return;
+    } else if (deal > 1) {
+      const int r = b / 8, x = b % 8;
+      const int full = ((int)gridDim.x / 8) / deal;
+      if (r < full * deal) pos = (r / deal) * 8 * deal + x * deal + (r % deal);
     }
     if (pos >= cnt) return;
     c = cell_list[pos];
@@ -727,7 +734,7 @@ static void eval_blk_k(const Grid& g, const double* px, int64_t ld, const int32_
                        const double* w, const T* blk, int nft, const int32_t* cs, int nfields, double* out, int block,
                        int np, const int32_t* cell_list, const int32_t* list_count, int group, int pipe, int waves,
                        hipStream_t s) {
-  if (cell_list && group > 0 && nft == DIM) {   // group -1: listed, round-robin blocks
+  if (cell_list && group > 0 && nft == DIM) {   // group < 0: listed cells dealt round-robin
     // sparse velocity evaluation: lane groups of G particles, 64/G cells per wave
     launch_sparse_g<DIM, KT, DIM, T>(group, g, px, ld, perm, pstart, w, blk, cs, nfields, out, cell_list, list_count, np,
                                      s);
@@ -736,14 +743,15 @@ static void eval_blk_k(const Grid& g, const double* px, int64_t ld, const int32_
   // dense: one block per cell; listed cells: one block per listed non-empty cell
   const int nblk = cell_list ? (std::min(g.C, np) + 7) / 8 * 8 : (g.C + 7) / 8 * 8;
   const dim3 grid(nblk), blkd(block);
-  const int xcd_runs = group >= 0 ? 1 : 0;
-#define LRB_EB(NFT, P) LRB_LAUNCH_TIMED((k_eval_blk<DIM, KT, NFT, T, P>), grid, blkd, s, g, px, ld, perm, pstart, w, blk, cs, nfields, out, cell_list, list_count, xcd_runs)
+  // group 0: contiguous XCD runs; group -K: round-robin in runs of K listed cells
+  const int deal = group >= 0 ? 0 : -group;
+#define LRB_EB(NFT, P) LRB_LAUNCH_TIMED((k_eval_blk<DIM, KT, NFT, T, P>), grid, blkd, s, g, px, ld, perm, pstart, w, blk, cs, nfields, out, cell_list, list_count, deal)
   if constexpr (sizeof(T) == 4) {
     // two rows per wait: the velocity evaluations (nft = dim) only
     if (pipe == 2 && nft == DIM) {
       if (waves == 6)
         LRB_LAUNCH_TIMED((k_eval_blk<DIM, KT, DIM, T, 2, 6>), grid, blkd, s, g, px, ld, perm, pstart, w, blk, cs,
-                         nfields, out, cell_list, list_count, xcd_runs);
+                         nfields, out, cell_list, list_count, deal);
       else
         LRB_EB(DIM, 2);
       return;
@@ -759,7 +767,7 @@ static void eval_blk_k(const Grid& g, const double* px, int64_t ld, const int32_
       // lane: 6 waves per SIMD, no spills)
       if constexpr (DIM == 3 && sizeof(T) == 4)
         LRB_LAUNCH_TIMED((k_eval_blk<DIM, KT, 6, T, 1, 6>), grid, blkd, s, g, px, ld, perm, pstart, w, blk, cs,
-                         nfields, out, cell_list, list_count, xcd_runs);
+                         nfields, out, cell_list, list_count, deal);
       break;
   }
 #undef LRB_EB

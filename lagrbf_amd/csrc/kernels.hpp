@@ -108,8 +108,9 @@ void launch_build_blk(const Grid& g, bool fp32, const double* x, int N, const in
                       const int32_t* cs, const uint8_t* active, const int64_t* mem_off, const double* coef, int nft,
                       int cstride, int f0, void* blk, hipStream_t s);   // fields f0 .. f0+nft-1 of cstride
 // cell_list / list_count (device): evaluate only the listed non-empty cells, one
-// block per cell (group 0) or 64/group cells per wave in lane groups of
-// group = 8, 16, 32 (needs nft == dim); nullptr: one block per cell.  Every
+// block per cell (group 0: contiguous runs of cells per XCD; group -K: runs of K
+// cells dealt round-robin over the XCDs) or 64/group cells per wave in lane groups
+// of group = 8, 16, 32 (needs nft == dim); nullptr: one block per cell.  Every
 // variant performs the same arithmetic per particle.
 // Events the next launch_eval_blk dispatch records its start and end in (nullptr: none).
 void set_eval_timing_events(hipEvent_t start, hipEvent_t stop);

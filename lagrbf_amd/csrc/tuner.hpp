@@ -1,7 +1,8 @@
 // Evaluation autotuner: the PU evaluation has several launch shapes with the same
 // per-particle arithmetic (one block per cell; one block per listed non-empty
-// cell, in contiguous XCD runs or round-robin; lane groups of 32, 16 or 8
-// particles with several cells per wave; one or two table rows per wait).
+// cell, in contiguous XCD runs or dealt round-robin singly or in runs of 4 or 16
+// cells; lane groups of 32, 16 or 8 particles with several cells per wave; one or
+// two table rows per wait).
 // Which is fastest depends on the particles per cell and the dataset, so the
 // first evaluation in each density bucket (particles per cell, half-octave
 // steps) times every valid shape on the live launch and keeps the fastest.
@@ -15,25 +16,32 @@
 
 namespace lrb {
 
-constexpr int N_EVAL_VARIANTS = 10;
+constexpr int N_EVAL_VARIANTS = 14;
 
 // variant -> (evaluate listed cells only, lane group size; 0 = one block per cell,
-// -1 = one block per listed cell with the blocks dealt round-robin over the XCDs
-// instead of in contiguous runs: better balanced when cells differ widely in work;
-// rows per wait of the FP32 node loop, 1 or 2: variants 6-8 are 0, 1 and 5 with two;
-// FP32 occupancy floor in waves per SIMD, 0 = the default: variant 9 is 8 with 6
-// waves, which leaves the loop its registers without spilling)
+// -K = one block per listed cell with runs of K consecutive listed cells dealt
+// round-robin over the XCDs instead of one contiguous run per XCD: better balanced
+// when cells differ widely in work, and with K > 1 neighbouring cells still share
+// an L2; rows per wait of the FP32 node loop, 1 or 2: variants 6-8 are 0, 1 and 5
+// with two; FP32 occupancy floor in waves per SIMD, 0 = the default: variant 9 is
+// 8 with 6 waves, which leaves the loop its registers without spilling; variants
+// 10-11 are 5 and 12-13 are 9 in runs of 4 and 16 cells)
 inline void eval_variant(int v, bool& listed, int& group, int& pipe, int& waves) {
-  static const int G[N_EVAL_VARIANTS] = {0, 0, 32, 16, 8, -1, 0, 0, -1, -1};
-  static const int L[N_EVAL_VARIANTS] = {0, 1, 1, 1, 1, 1, 0, 1, 1, 1};
+  static const int G[N_EVAL_VARIANTS] = {0, 0, 32, 16, 8, -1, 0, 0, -1, -1, -4, -16, -4, -16};
+  static const int L[N_EVAL_VARIANTS] = {0, 1, 1, 1, 1, 1, 0, 1, 1, 1, 1, 1, 1, 1};
+  static const int PIPE[N_EVAL_VARIANTS] = {1, 1, 1, 1, 1, 1, 2, 2, 2, 2, 1, 1, 2, 2};
+  static const int WAVES[N_EVAL_VARIANTS] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 6, 0, 0, 6, 6};
   listed = L[v] != 0;
   group = G[v];
-  pipe = v >= 6 ? 2 : 1;
-  waves = v == 9 ? 6 : 0;
+  pipe = PIPE[v];
+  waves = WAVES[v];
 }
 
 // the same shape with one row per wait (the FP64 evaluation has no two-row loop)
-inline int eval_variant_pipe1(int v) { return v == 6 ? 0 : v == 7 ? 1 : v >= 8 ? 5 : v; }
+inline int eval_variant_pipe1(int v) {
+  static const int P1[N_EVAL_VARIANTS] = {0, 1, 2, 3, 4, 5, 0, 1, 5, 5, 10, 11, 10, 11};
+  return P1[v];
+}
 
 inline bool eval_variant_listed(int v) {
   bool listed;
@@ -73,6 +81,7 @@ struct EvalTuner {
   std::map<int, Runoff> runoff; // density bucket -> run-off in progress
   std::map<int, float> best_ms;
   int forced = -1;              // a fixed variant (option lagrbf_eval_variant), -1: tune
+  bool untuned = false;         // run DEFAULT_VARIANT without trials (unless forced)
 
   EvalTuner() = default;
   EvalTuner(const EvalTuner&) = delete;
@@ -84,6 +93,7 @@ struct EvalTuner {
     runoff = std::move(o.runoff);
     best_ms = std::move(o.best_ms);
     forced = o.forced;
+    untuned = o.untuned;
     o.runoff.clear();
     return *this;
   }
@@ -99,6 +109,7 @@ struct EvalTuner {
   // variant run() will launch without timing candidates, or -1 when it will tune
   int planned(int bucket) const {
     if (forced >= 0) return forced;
+    if (untuned) return DEFAULT_VARIANT;
     auto it = choice.find(bucket);
     auto f = final_.find(bucket);
     return it == choice.end() || f == final_.end() || !f->second ? -1 : it->second;
@@ -133,6 +144,10 @@ struct EvalTuner {
       const int v = fallback(forced, grouped_ok, pipe2_ok);
       launch(v);
       return v;
+    }
+    if (untuned) {
+      launch(DEFAULT_VARIANT);
+      return DEFAULT_VARIANT;
     }
     const int planned_v = planned(bucket);
     if (planned_v >= 0) {

@@ -491,7 +491,8 @@ def test_eval_launch_shapes_identical(precision, Np, case):
     two occupancy floors in FP32) gives the same values bit for bit, on a sparse
     and a dense swarm, in 3D and 2D, and with 0-9 nodes per cell (the node loops'
     short and empty tails); FP64 against the oracle at 1e-12.  FP64 has no
-    two-row loop: there the variants 6-9 run as 0, 1, 5 and 5."""
+    two-row loop: there the variants 6-9 run as 0, 1, 5 and 5, and 12-13 (runs of
+    4 and 16 listed cells dealt round-robin) as 10-11."""
     if case == "3d":
         opts, X, V, P = cases.fourier_3d(nlat=20, Np=Np, n=9)
     elif case == "3d-few-nodes":
@@ -502,7 +503,8 @@ def test_eval_launch_shapes_identical(precision, Np, case):
     opts["advection"] = False
     print(f"case: launch shapes {case} Np={Np} {precision}", flush=True)
     vals = []
-    for v in range(10):
+    nvar = 14
+    for v in range(nvar):
         g = L.LagRBFCtx(petsc_options=dict(opts, lagrbf_eval_variant=v), precision=precision)
         g.fill_data(X, V)
         g.fill_points(P)
@@ -512,9 +514,9 @@ def test_eval_launch_shapes_identical(precision, Np, case):
         used = g.info("eval_variant")
         g.destroy()
         vals.append(out)
-        want = v if precision == "mixed" else {6: 0, 7: 1, 8: 5, 9: 5}.get(v, v)
+        want = v if precision == "mixed" else {6: 0, 7: 1, 8: 5, 9: 5, 12: 10, 13: 11}.get(v, v)
         assert used == want, (v, used)
-    for v in range(1, 10):
+    for v in range(1, nvar):
         assert np.array_equal(vals[0], vals[v]), f"variant {v} differs"
     if precision == "fp64":
         from oracle.oracle import OracleCtx
@@ -541,7 +543,7 @@ def test_six_field_evaluation(precision):
     vo = np.zeros((P.shape[0], 6))
     o.output(None, vo, None)
     vals = []
-    for v in ((-1,) if precision == "fp64" else (0, 1, 5, 9)):
+    for v in ((-1,) if precision == "fp64" else (0, 1, 5, 9, 10, 13)):
         g = L.LagRBFCtx(petsc_options=dict(opts, lagrbf_eval_variant=v), precision=precision)
         g.fill_data(X, F)
         g.fill_points(P)
