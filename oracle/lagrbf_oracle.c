@@ -35,6 +35,11 @@ typedef struct {
   int     *ipiv;
   double  *X;          /* coefficients, column-major [nf][n] as coeffs[f*ndata+nd] */
   int      nf;
+  /* extensions: polynomial block of m terms (degree deg), its coefficients
+   * column-major [nf][m], and the patch's shape parameter */
+  int      m, deg;
+  double  *beta;
+  double   eps;
 } orc_patch;
 
 struct orc_ctx {
@@ -73,6 +78,8 @@ void orc_config_default(orc_config *c)
   c->dt = 1e-10;                          /* advection_ctx.c: AdvectionCtxCreate */
   c->dopri_dt_fac = 0.9; c->dopri_dt_fac_min = 1e-2; c->dopri_dt_fac_max = 1e2;  /* dormand_prince.c:318-322 */
   c->dopri_atol = 1e-3; c->dopri_rtol = 1e-3;
+  c->poly_degree = -1;                    /* extension: no augmentation, as the reference */
+  c->local_eps = 0.0;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -88,8 +95,43 @@ double orc_kernel(int ktype, double r, double eps)
   case ORC_KERNEL_MQ:         u = eps * r; return sqrt(1.0 + u * u);
   case ORC_KERNEL_IMQ:        u = eps * r; return 1.0 / sqrt(1.0 + u * u);
   case ORC_KERNEL_GAUSSIAN:   u = eps * r; return exp(-u * u);
+  case ORC_KERNEL_PHS3:       u = eps * r; return u * u * u;   /* extension */
   default: return NAN;
   }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Polynomial augmentation (extension; the reference has none,
+ * python_api/lagrbfpy/interpolation.py:43-45).  The patch system is the saddle
+ * point [A P; P^T 0] [c; beta] = [f; 0] with P_ik = p_k(xi_i), xi = (x - x_p) / h
+ * the member shifted to the patch's PU node x_p and scaled by the largest cell
+ * width h.  The interpolant of patch p is s_p(x) = sum_j c_j phi(|x - x_j|) +
+ * sum_k beta_k p_k(xi(x)).                                                    */
+int orc_poly_terms(int dim, int deg)
+{
+  if (deg < 0) return 0;
+  if (deg == 0) return 1;
+  if (deg == 1) return dim + 1;
+  return (dim + 1) * (dim + 2) / 2;
+}
+
+void orc_poly_eval(int dim, int deg, const double *xi, double *out)
+{
+  int a, b, k = 0;
+  if (deg < 0) return;
+  out[k++] = 1.0;
+  if (deg >= 1) for (a = 0; a < dim; a++) out[k++] = xi[a];
+  if (deg >= 2)
+    for (a = 0; a < dim; a++)
+      for (b = a; b < dim; b++) out[k++] = xi[a] * xi[b];
+}
+
+int orc_patch_degree(int dim, int D, int n)
+{
+  int k, lim = n / 2 > 1 ? n / 2 : 1;
+  if (D < 0 || n <= 0) return -1;
+  for (k = D; k >= 0; k--) if (orc_poly_terms(dim, k) <= lim) return k;
+  return -1;
 }
 
 /* ------------------------------------------------------------------------ */
@@ -333,7 +375,9 @@ orc_ctx *orc_create(const orc_config *cfg)
   orc_ctx *ctx; int d, p; double box = 0.0, ov, radius;
   if (!cfg || cfg->dim < 2 || cfg->dim > 3) return NULL;
   if (cfg->shape_factor <= 0.0) return NULL;
-  if (cfg->kernel_type < 0 || cfg->kernel_type > 6) return NULL;
+  if (cfg->kernel_type < 0 || cfg->kernel_type > ORC_KERNEL_PHS3) return NULL;
+  if (cfg->poly_degree < -1 || cfg->poly_degree > 2) return NULL;
+  if (!(cfg->local_eps >= 0.0)) return NULL;
   if (cfg->solve_method < 0 || cfg->solve_method > 2) return NULL;
   for (d = 0; d < cfg->dim; d++) if (cfg->n[d] < 2 || !(cfg->L[d] > cfg->O[d])) return NULL;
   ctx = (orc_ctx *)calloc(1, sizeof(orc_ctx));
@@ -373,7 +417,7 @@ orc_ctx *orc_create(const orc_config *cfg)
 
 static void patch_clear(orc_patch *p)
 {
-  free(p->members); free(p->A); free(p->ipiv); free(p->X);
+  free(p->members); free(p->A); free(p->ipiv); free(p->X); free(p->beta);
   memset(p, 0, sizeof(*p));
 }
 
@@ -555,29 +599,75 @@ static int patch_fill_members(orc_ctx *ctx, int p, int32_t *out)
   return cnt;
 }
 
+/* Number of grid cells touching PU node p (the cells whose nodes form patch p). */
+static int patch_ncells(const orc_ctx *ctx, int p)
+{
+  const int *n = ctx->cfg.n; int a[3], d, cnt = 1;
+  a[0] = p % n[0]; a[1] = (p / n[0]) % n[1]; a[2] = ctx->dim == 3 ? p / (n[0] * n[1]) : 0;
+  for (d = 0; d < ctx->dim; d++) cnt *= (a[d] > 0 ? 1 : 0) + (a[d] < ctx->nc[d] ? 1 : 0);
+  return cnt;
+}
+
+/* Shape parameter of patch p: eps_r, or with -lagrbf_local_eps c (extension)
+ * c / h_local, h_local = (cells of p x cell volume / members)^(1/d).         */
+static double patch_eps(const orc_ctx *ctx, int p, int n)
+{
+  double vol = 1.0; int d;
+  if (!(ctx->cfg.local_eps > 0.0) || n <= 0) return ctx->eps_r;
+  for (d = 0; d < ctx->dim; d++) vol *= ctx->h[d];
+  vol *= (double)patch_ncells(ctx, p);
+  return ctx->cfg.local_eps / pow(vol / (double)n, 1.0 / ctx->dim);
+}
+
+/* polynomial scale: 1 / largest cell width */
+static double poly_scale(const orc_ctx *ctx)
+{
+  double hm = ctx->h[0]; int d;
+  for (d = 1; d < ctx->dim; d++) if (ctx->h[d] > hm) hm = ctx->h[d];
+  return 1.0 / hm;
+}
+
 /* PartitionSetUpRBFMatrix_*: assemble (partition_solve.c:47-82) and factor
- * (partition_factor.c:37-55; PETSc SeqDense Cholesky without SPD flag = dsytrf) */
+ * (partition_factor.c:37-55; PETSc SeqDense Cholesky without SPD flag = dsytrf).
+ * With a polynomial block (extension) the same dsytrf factors the (n+m) x (n+m)
+ * saddle-point matrix [A P; P^T 0]. */
 static int patch_setup(orc_ctx *ctx, int p)
 {
-  orc_patch *pt = &ctx->patch[p]; int n, i, j, d;
+  orc_patch *pt = &ctx->patch[p]; int n, i, j, d, nt, m;
   const int dim = ctx->dim;
   n = patch_fill_members(ctx, p, NULL);
   pt->n = n;
   pt->members = (int32_t *)malloc(sizeof(int32_t) * ((size_t)n + 1));
   patch_fill_members(ctx, p, pt->members);
   pt->nf = ctx->cfg.n_ifields;
+  pt->eps = patch_eps(ctx, p, n);
+  pt->deg = orc_patch_degree(dim, ctx->cfg.poly_degree, n);
+  pt->m = m = orc_poly_terms(dim, pt->deg);
+  nt = n + m;
   pt->X = (double *)calloc((size_t)n * (pt->nf > 0 ? pt->nf : 1) + 1, sizeof(double));
-  pt->A = (double *)malloc(sizeof(double) * ((size_t)n * n + 1));
-  pt->ipiv = (int *)malloc(sizeof(int) * ((size_t)n + 1));
+  pt->beta = (double *)calloc((size_t)m * (pt->nf > 0 ? pt->nf : 1) + 1, sizeof(double));
+  pt->A = (double *)malloc(sizeof(double) * ((size_t)nt * nt + 1));
+  pt->ipiv = (int *)malloc(sizeof(int) * ((size_t)nt + 1));
   for (i = 0; i < n; i++)
     for (j = i; j < n; j++) {
       double dist = 0.0, v;
       const double *xi = ctx->dx + (size_t)pt->members[i] * dim, *xj = ctx->dx + (size_t)pt->members[j] * dim;
       for (d = 0; d < dim; d++) dist += (xi[d] - xj[d]) * (xi[d] - xj[d]);
-      v = orc_kernel(ctx->cfg.kernel_type, sqrt(dist), ctx->eps_r);
-      pt->A[(size_t)j * n + i] = v; pt->A[(size_t)i * n + j] = v;
+      v = orc_kernel(ctx->cfg.kernel_type, sqrt(dist), pt->eps);
+      pt->A[(size_t)j * nt + i] = v; pt->A[(size_t)i * nt + j] = v;
     }
-  if (n > 0) orc_sytrf_lower(n, pt->A, n, pt->ipiv);
+  if (m > 0) {
+    const double ps = poly_scale(ctx); const double *c = ctx->pu + (size_t)p * dim;
+    for (i = 0; i < n; i++) {
+      double xi[3], pv[10];
+      const double *x = ctx->dx + (size_t)pt->members[i] * dim;
+      for (d = 0; d < dim; d++) xi[d] = (x[d] - c[d]) * ps;
+      orc_poly_eval(dim, pt->deg, xi, pv);
+      for (j = 0; j < m; j++) { pt->A[(size_t)(n + j) * nt + i] = pv[j]; pt->A[(size_t)i * nt + n + j] = pv[j]; }
+    }
+    for (i = 0; i < m; i++) for (j = 0; j < m; j++) pt->A[(size_t)(n + j) * nt + n + i] = 0.0;
+  }
+  if (nt > 0) orc_sytrf_lower(nt, pt->A, nt, pt->ipiv);
   pt->setup = 1;
   return 0;
 }
@@ -585,11 +675,26 @@ static int patch_setup(orc_ctx *ctx, int p)
 /* PartitionSolveLocalCoefficients*: RHS gather (partition_factor.c:57-90) + solve */
 static void patch_solve(orc_ctx *ctx, int p)
 {
-  orc_patch *pt = &ctx->patch[p]; int i, f; const int nf = pt->nf, n = pt->n;
+  orc_patch *pt = &ctx->patch[p]; int i, f; const int nf = pt->nf, n = pt->n, m = pt->m, nt = n + m;
+  double *B;
   if (n == 0 || nf == 0) { pt->solved = 1; return; }
+  if (m == 0) {
+    for (i = 0; i < n; i++)
+      for (f = 0; f < nf; f++) pt->X[(size_t)f * n + i] = ctx->dval[(size_t)pt->members[i] * nf + f];
+    orc_sytrs_lower(n, nf, pt->A, n, pt->ipiv, pt->X, n);
+    pt->solved = 1;
+    return;
+  }
+  /* saddle point: right-hand side [f; 0], solution [c; beta] */
+  B = (double *)calloc((size_t)nt * nf, sizeof(double));
   for (i = 0; i < n; i++)
-    for (f = 0; f < nf; f++) pt->X[(size_t)f * n + i] = ctx->dval[(size_t)pt->members[i] * nf + f];
-  orc_sytrs_lower(n, nf, pt->A, n, pt->ipiv, pt->X, n);
+    for (f = 0; f < nf; f++) B[(size_t)f * nt + i] = ctx->dval[(size_t)pt->members[i] * nf + f];
+  orc_sytrs_lower(nt, nf, pt->A, nt, pt->ipiv, B, nt);
+  for (f = 0; f < nf; f++) {
+    for (i = 0; i < n; i++) pt->X[(size_t)f * n + i] = B[(size_t)f * nt + i];
+    for (i = 0; i < m; i++) pt->beta[(size_t)f * m + i] = B[(size_t)f * nt + n + i];
+  }
+  free(B);
   pt->solved = 1;
 }
 
@@ -665,8 +770,16 @@ static void rbf_interpolation(orc_ctx *ctx, double *values, int nfields)
       for (nd = 0; nd < pt->n; nd++) {
         const double *xj = ctx->dx + (size_t)pt->members[nd] * dim; double dist = 0.0, rbf;
         for (d = 0; d < dim; d++) { double dx = x[d] - xj[d]; dist += dx * dx; }
-        rbf = orc_kernel(ctx->cfg.kernel_type, sqrt(dist), ctx->eps_r);
+        rbf = orc_kernel(ctx->cfg.kernel_type, sqrt(dist), pt->eps);
         for (f = 0; f < nfields; f++) interp[f] += rbf * pt->X[(size_t)f * pt->n + nd];
+      }
+      if (pt->m > 0) {   /* polynomial part (extension) */
+        double xi[3], pv[10]; int q;
+        const double *c = ctx->pu + (size_t)ctx->corner[cell][k] * dim, ps = poly_scale(ctx);
+        for (d = 0; d < dim; d++) xi[d] = (x[d] - c[d]) * ps;
+        orc_poly_eval(dim, pt->deg, xi, pv);
+        for (f = 0; f < nfields; f++)
+          for (q = 0; q < pt->m; q++) interp[f] += pv[q] * pt->beta[(size_t)f * pt->m + q];
       }
       wgt = ctx->w[(size_t)np * nen + k];
       for (f = 0; f < nfields; f++) values[(size_t)np * nfields + f] += interp[f] * wgt;
@@ -1131,6 +1244,15 @@ int orc_patch_coeffs(const orc_ctx *ctx, int p, double *X)
   pt = &ctx->patch[p];
   if (X && pt->X) memcpy(X, pt->X, sizeof(double) * (size_t)pt->n * pt->nf);
   return pt->n;
+}
+int orc_patch_poly(const orc_ctx *ctx, int p, double *beta, double *eps)
+{
+  const orc_patch *pt;
+  if (p < 0 || p >= ctx->P) return -1;
+  pt = &ctx->patch[p];
+  if (eps) *eps = pt->setup ? pt->eps : patch_eps(ctx, p, patch_fill_members((orc_ctx *)ctx, p, NULL));
+  if (beta && pt->beta) memcpy(beta, pt->beta, sizeof(double) * (size_t)pt->m * pt->nf);
+  return pt->m;
 }
 int orc_patch_flags(const orc_ctx *ctx, int p, int *setup, int *solved)
 {

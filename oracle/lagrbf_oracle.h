@@ -14,6 +14,11 @@
  * Pinning (what checks each function, see tests/test_oracle.py and
  * tests/test_golden.py):
  *   orc_kernel          closed forms + hand values                       (pinned)
+ *   augmented systems   (extension: r^3 kernel, polynomial block, local eps)
+ *                       polynomial reproduction of linear/quadratic fields, the
+ *                       interpolation condition at every node, KKT residual and
+ *                       numpy/LAPACK solve of the same system, closed-form h_local
+ *                                                                        (pinned)
  *   orc_sytrf/sytrs     LAPACK dsytrf/dsytrs via scipy + committed vectors (pinned)
  *   orc_potrf/potrs     LAPACK dpotrf via scipy                          (pinned)
  *   orc_locate          brute-force interval search on tiny grids        (pinned)
@@ -38,7 +43,9 @@ extern "C" {
 
 /* KernelType, include/kernels/kernels_rbf.h:34-42 */
 enum { ORC_KERNEL_LINEAR = 0, ORC_KERNEL_WENDLAND_C0, ORC_KERNEL_WENDLAND_C2,
-       ORC_KERNEL_BUMP, ORC_KERNEL_MQ, ORC_KERNEL_IMQ, ORC_KERNEL_GAUSSIAN };
+       ORC_KERNEL_BUMP, ORC_KERNEL_MQ, ORC_KERNEL_IMQ, ORC_KERNEL_GAUSSIAN,
+       /* extension, no reference counterpart: polyharmonic spline phi = (eps r)^3 */
+       ORC_KERNEL_PHS3 };
 /* ASolveMethod, include/lagrangian_rbf.h:91-95 */
 enum { ORC_A_FACTOR = 0, ORC_A_KSP, ORC_A_KSP_ASMB };
 /* AdvectionScheme, include/advection/advection_ctx.h:34-39 */
@@ -63,6 +70,12 @@ typedef struct {
   int    nthreads;        /* OpenMP threads (0 = runtime default) */
   int    setup_all;       /* 1: every patch is set up (serving datasets), not only the
                              corners of particle cells */
+  /* Extensions (no reference counterpart; the reference solves pure RBF systems,
+   * python_api/lagrbfpy/interpolation.py:43-45): */
+  int    poly_degree;     /* -lagrbf_poly_degree: -1 none, 0 constant, 1 linear, 2 quadratic
+                             augmentation of every patch system ([A P; P^T 0]) */
+  double local_eps;       /* -lagrbf_local_eps c: shape parameter of patch p is c / h_local(p),
+                             h_local = (patch cells' volume / members)^(1/d); 0: global eps_r */
 } orc_config;
 
 typedef struct orc_ctx orc_ctx;
@@ -72,6 +85,13 @@ void     orc_config_default(orc_config *cfg);
 
 /* ---- pure building blocks ---- */
 double   orc_kernel(int ktype, double r, double eps);
+/* number of polynomial terms of degree <= deg in dim dimensions (0 for deg < 0) */
+int      orc_poly_terms(int dim, int deg);
+/* monomials 1, x, y, (z), xx, xy, (xz), yy, (yz), (zz) of degree <= deg at xi */
+void     orc_poly_eval(int dim, int deg, const double *xi, double *out);
+/* degree a patch of n members takes under -lagrbf_poly_degree D: the largest
+ * k <= D with terms(k) <= max(1, n / 2); -1 for D < 0 or n == 0 */
+int      orc_patch_degree(int dim, int D, int n);
 int      orc_sytrf_lower(int n, double *A, int lda, int *ipiv);
 void     orc_sytrs_lower(int n, int nrhs, const double *A, int lda, const int *ipiv, double *B, int ldb);
 int      orc_potrf_lower(int n, double *A, int lda);
@@ -105,6 +125,9 @@ int      orc_patch_size(const orc_ctx *ctx, int p);
 int      orc_patch_members(const orc_ctx *ctx, int p, int32_t *members);
 int      orc_patch_coeffs(const orc_ctx *ctx, int p, double *X);
 int      orc_patch_flags(const orc_ctx *ctx, int p, int *setup, int *solved);
+/* polynomial part of a patch: returns its term count m and writes beta
+ * column-major [nf][m] (NULL: count only); eps (NULL ok): the patch's shape parameter */
+int      orc_patch_poly(const orc_ctx *ctx, int p, double *beta, double *eps);
 int64_t  orc_removed_count(const orc_ctx *ctx);
 double   orc_get_dt(const orc_ctx *ctx);
 int      orc_get_alive(const orc_ctx *ctx, int32_t *alive);

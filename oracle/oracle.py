@@ -38,6 +38,7 @@ class OrcConfig(ctypes.Structure):
         ("dopri_dt_fac_max", ctypes.c_double), ("dopri_atol", ctypes.c_double),
         ("dopri_rtol", ctypes.c_double),
         ("weights_brute", ctypes.c_int), ("nthreads", ctypes.c_int), ("setup_all", ctypes.c_int),
+        ("poly_degree", ctypes.c_int), ("local_eps", ctypes.c_double),
     ]
 
 
@@ -92,6 +93,10 @@ def lib():
         L.orc_get_dt.restype = ctypes.c_double
         L.orc_get_dt.argtypes = [V]
         L.orc_get_alive.argtypes = [V, I32]
+        L.orc_patch_poly.argtypes = [V, ctypes.c_int, D, D]
+        L.orc_poly_terms.argtypes = [ctypes.c_int, ctypes.c_int]
+        L.orc_poly_eval.argtypes = [ctypes.c_int, ctypes.c_int, D, D]
+        L.orc_patch_degree.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
         _lib = L
     return _lib
 
@@ -152,6 +157,8 @@ def config_from_options(options):
     c.dopri_rtol = float(o.get("dopri_rtol", c.dopri_rtol))
     c.weights_brute = int(_bool(o.get("oracle_weights_brute", False)))
     c.setup_all = int(_bool(o.get("oracle_setup_all", False)))
+    c.poly_degree = int(o.get("lagrbf_poly_degree", -1))
+    c.local_eps = float(o.get("lagrbf_local_eps", 0.0))
     return c
 
 
@@ -285,6 +292,14 @@ class OracleCtx:
         lib().orc_patch_coeffs(self.ctx, p, _dp(out))
         return out[: n * self.nfields].reshape(self.nfields, n).T.copy()   # [n, nf]
 
+    def patch_poly(self, p):
+        """(beta [m, nf], eps) of patch p: polynomial coefficients and shape parameter."""
+        e = ctypes.c_double()
+        m = lib().orc_patch_poly(self.ctx, p, None, ctypes.byref(e))
+        out = np.zeros(max(m * self.nfields, 1))
+        lib().orc_patch_poly(self.ctx, p, _dp(out), None)
+        return out[: m * self.nfields].reshape(self.nfields, m).T.copy(), e.value
+
     def patch_flags(self, p):
         s = ctypes.c_int(); v = ctypes.c_int()
         lib().orc_patch_flags(self.ctx, p, ctypes.byref(s), ctypes.byref(v))
@@ -343,3 +358,18 @@ def cell_sort(cellid, ncells):
     cn = np.zeros(max(len(cellid), 1), dtype=np.int32)
     lib().orc_cell_sort(len(cellid), _ip(cellid), ncells, _ip(cs), _ip(cn))
     return cs, cn[: len(cellid)]
+
+
+def poly_terms(dim, deg):
+    return lib().orc_poly_terms(dim, deg)
+
+
+def poly_eval(dim, deg, xi):
+    xi = np.ascontiguousarray(xi, dtype=np.float64)
+    out = np.zeros(10)
+    lib().orc_poly_eval(dim, deg, _dp(xi), _dp(out))
+    return out[: poly_terms(dim, deg)]
+
+
+def patch_degree(dim, D, n):
+    return lib().orc_patch_degree(dim, D, n)
