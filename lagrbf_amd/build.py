@@ -7,7 +7,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "liblagrbf.so")
-SOURCES = ["context.hip", "kernels.hip", "eval_cell.hip", "eval_blk.hip", "serve.hip", "patch_chol.hip", "adaptive.hip", "options.cpp", "io.cpp"]
+SOURCES = ["eval_blk_wc2.hip", "eval_blk_phs3.hip", "eval_blk_gauss.hip", "eval_blk_other.hip",
+           "context.hip", "kernels.hip", "eval_cell.hip", "eval_blk.hip", "serve.hip", "patch_chol.hip", "adaptive.hip", "aug.hip", "options.cpp", "io.cpp"]
 ARCH = os.environ.get("LAGRBF_OFFLOAD_ARCH", "gfx950")
 
 
@@ -18,16 +19,22 @@ def hipcc():
     raise RuntimeError("hipcc not found")
 
 
-def build(verbose=False, jobs=4):
-    """Compile every source to an object (in parallel), then link the shared library."""
+def build(verbose=False, jobs=8, incremental=False):
+    """Compile every source to an object (in parallel), then link the shared library.
+    incremental: keep objects newer than their source and every header."""
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
+    hdr_time = max(os.path.getmtime(os.path.join(CSRC, f)) for f in os.listdir(CSRC) if f.endswith((".hpp", ".h")))
     common = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
               "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
               "-Wno-unused-result", "-I", os.path.join(HERE, "..", "include")]
     procs, objs = [], []
     for src in SOURCES:
         obj = os.path.join(objdir, src + ".o")
+        if incremental and os.path.exists(obj) and os.path.getmtime(obj) > max(
+                hdr_time, os.path.getmtime(os.path.join(CSRC, src))):
+            objs.append(obj)
+            continue
         lang = ["-x", "hip"] if src.endswith(".hip") else []
         cmd = [hipcc()] + common + lang + ["-c", os.path.join(CSRC, src), "-o", obj]
         if verbose:
@@ -52,4 +59,4 @@ def build(verbose=False, jobs=4):
 
 
 if __name__ == "__main__":
-    print(build(verbose="-v" in sys.argv))
+    print(build(verbose="-v" in sys.argv, incremental="-i" in sys.argv))

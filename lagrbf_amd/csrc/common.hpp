@@ -6,8 +6,41 @@
 
 namespace lrb {
 
-// KernelType, include/kernels/kernels_rbf.h:34-42
-enum KernelType { K_LINEAR = 0, K_WC0 = 1, K_WC2 = 2, K_BUMP = 3, K_MQ = 4, K_IMQ = 5, K_GAUSS = 6 };
+// KernelType, include/kernels/kernels_rbf.h:34-42, plus K_PHS3 (extension, no
+// reference counterpart): the polyharmonic spline phi = (eps r)^3, used with a
+// polynomial block.  K_GAUSS_LOCAL is internal to the evaluation: a Gaussian whose
+// shape parameter belongs to the patch (eps_p = c / h_local), not to the grid.
+enum KernelType { K_LINEAR = 0, K_WC0 = 1, K_WC2 = 2, K_BUMP = 3, K_MQ = 4, K_IMQ = 5, K_GAUSS = 6, K_PHS3 = 7,
+                  K_GAUSS_LOCAL = 8 };
+constexpr int K_MAX_OPTION = 7;   // largest kernel_type accepted as an option
+
+// Polynomial block of a patch system (extension): monomials of degree <= deg in
+// the member coordinates shifted to the patch's PU node and scaled by 1 / (largest
+// cell width), ordered 1, x, y, (z), xx, xy, (xz), yy, (yz), (zz).
+constexpr int POLY_MAX = 10;
+__host__ __device__ inline int poly_terms(int dim, int deg) {
+  return deg < 0 ? 0 : (deg == 0 ? 1 : (deg == 1 ? dim + 1 : (dim + 1) * (dim + 2) / 2));
+}
+// Degree a patch of n members takes under a requested degree D: the largest k <= D
+// whose term count is at most max(1, n / 2); -1 for D < 0 or an empty patch.
+__host__ __device__ inline int patch_degree(int dim, int D, int n) {
+  if (D < 0 || n <= 0) return -1;
+  const int lim = n / 2 > 1 ? n / 2 : 1;
+  for (int k = D; k >= 0; k--)
+    if (poly_terms(dim, k) <= lim) return k;
+  return -1;
+}
+template <typename T>
+__host__ __device__ inline void poly_eval(int dim, int deg, const T* xi, T* out) {
+  if (deg < 0) return;
+  int k = 0;
+  out[k++] = T(1);
+  if (deg >= 1)
+    for (int a = 0; a < dim; a++) out[k++] = xi[a];
+  if (deg >= 2)
+    for (int a = 0; a < dim; a++)
+      for (int b = a; b < dim; b++) out[k++] = xi[a] * xi[b];
+}
 
 // Uniform PU grid: n nodes per axis on [O, L] (L is the max corner), nc = n-1 cells.
 struct Grid {
@@ -16,6 +49,8 @@ struct Grid {
   double O[3], L[3], h[3];
   double eps_w, eps_r;
   int P, C;
+  int wnorm;   // Shepard normalisation: 0 over every PU node (swarm_interp.c:327-372), 1 over the
+               // cell's 2^d corners only (extension: the used weights then sum to 1)
 };
 
 // phi(r, eps) of kernels_rbf.c:31-91 for a kernel fixed at compile time.
@@ -40,6 +75,9 @@ __device__ __forceinline__ T rbf(T r, T eps) {
   } else if constexpr (K == K_IMQ) {
     T u = eps * r;
     return T(1) / sqrt(T(1) + u * u);
+  } else if constexpr (K == K_PHS3) {
+    T u = eps * r;
+    return u * u * u;
   } else {
     T u = eps * r;
     return exp(-u * u);
@@ -56,6 +94,7 @@ __device__ __forceinline__ T rbf_rt(int k, T r, T eps) {
     case K_BUMP: return rbf<K_BUMP, T>(r, eps);
     case K_MQ: return rbf<K_MQ, T>(r, eps);
     case K_IMQ: return rbf<K_IMQ, T>(r, eps);
+    case K_PHS3: return rbf<K_PHS3, T>(r, eps);
     default: return rbf<K_GAUSS, T>(r, eps);
   }
 }

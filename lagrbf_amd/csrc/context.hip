@@ -92,7 +92,25 @@ static int ctx_from_options(Ctx& c, const OptView& o) {
   o.get_real("cfl_fac", c.cfl_fac, bad);
   o.get_bool("advection", c.advection, bad);
   if (bad) return set_err(E_WRONG, fn, "malformed option value");
-  if (c.ktype < 0 || c.ktype > 6) return set_err(E_RANGE, fn, "Unknown kernel type %d", c.ktype);
+  if (c.ktype < 0 || c.ktype > K_MAX_OPTION) return set_err(E_RANGE, fn, "Unknown kernel type %d", c.ktype);
+  // extensions: polynomial block ([A P; P^T 0]) and per-patch shape parameter
+  o.get_int("lagrbf_poly_degree", c.poly_deg, bad);
+  o.get_real("lagrbf_local_eps", c.local_eps, bad);
+  if (bad) return set_err(E_WRONG, fn, "malformed -lagrbf_poly_degree / -lagrbf_local_eps option");
+  if (c.poly_deg < -1 || c.poly_deg > 2)
+    return set_err(E_RANGE, fn, "-lagrbf_poly_degree must be -1 (none), 0, 1 or 2 (found %d)", c.poly_deg);
+  if (!(c.local_eps >= 0.0)) return set_err(E_RANGE, fn, "-lagrbf_local_eps must be >= 0");
+  if (c.local_eps > 0.0 && c.ktype != K_GAUSS)
+    return set_err(E_SUP, fn, "-lagrbf_local_eps applies to the Gaussian kernel (kernel_type 6)");
+  std::string wn;
+  if (o.get_string("lagrbf_weights", wn)) {
+    if (wn == "all") c.g.wnorm = 0;
+    else if (wn == "corners") c.g.wnorm = 1;
+    else return set_err(E_WRONG, fn, "-lagrbf_weights must be all or corners (found %s)", wn.c_str());
+  }
+  if (c.poly_deg >= 0 && c.nf > 8)
+    return set_err(E_SUP, fn, "a polynomial block handles up to 8 interpolated fields (found %d)", c.nf);
+  c.poly_mm = poly_terms(dim, c.poly_deg);
   if (c.solve_method < 0 || c.solve_method > 2) return set_err(E_RANGE, fn, "Unknown method type: %d", c.solve_method);
   if (c.nf < 0) return set_err(E_RANGE, fn, "-n_interpolation_fields must be >= 0");
   if (c.advection) {
@@ -191,6 +209,15 @@ static int ctx_init_device(Ctx& c) {
   HIPCHK(c.p_fac_off.reserve(P)); HIPCHK(c.touched.reserve(P)); HIPCHK(c.list.reserve(P)); HIPCHK(c.info.reserve(P));
   HIPCHK(c.pcount.reserve(P));
   c.h_method.assign(P, 0);
+  c.h_m.assign(P, 0); c.h_deg.assign(P, -1); c.h_eps.assign(P, c.g.eps_r); c.h_aux_off.assign(P, 0);
+  HIPCHK(c.p_m.reserve(P)); HIPCHK(c.p_deg.reserve(P)); HIPCHK(c.p_eps.reserve(P)); HIPCHK(c.p_aux_off.reserve(P));
+  HIPCHK(hipMemset(c.p_m.p, 0, sizeof(int32_t) * P));
+  HIPCHK(hipMemset(c.p_aux_off.p, 0, sizeof(int64_t) * P));
+  HIPCHK(hipMemcpy(c.p_eps.p, c.h_eps.data(), sizeof(double) * P, hipMemcpyHostToDevice));
+  if (c.poly_deg >= 0) {
+    HIPCHK(c.beta_pool.reserve((size_t)P * c.poly_mm * std::max(c.nf, 1)));
+    HIPCHK(hipMemset(c.beta_pool.p, 0, sizeof(double) * (size_t)P * c.poly_mm * std::max(c.nf, 1)));
+  }
   HIPCHK(c.pstatus.reserve(P)); HIPCHK(c.piters.reserve(P));
   HIPCHK(c.list.reserve((size_t)N_METHODS * P));
   HIPCHK(hipMemset(c.p_n.p, 0, sizeof(int32_t) * P));
@@ -338,13 +365,67 @@ static int fill_points(Ctx& c, int n, const double* coords, const double* values
 }
 
 // ---------------------------------------------------------------------------
-// Factor slots: n*n elements in the FP64 pool (Bunch-Kaufman, FP64 Cholesky) or
-// the FP32 pool (FP32 Cholesky of the mixed path).
+AugView aug_view(Ctx& c, bool with_poly) {
+  AugView a{};
+  const bool poly = with_poly && c.poly_deg >= 0;
+  a.m = poly ? c.p_m.p : nullptr;
+  a.deg = poly ? c.p_deg.p : nullptr;
+  a.eps = c.local_eps > 0.0 ? c.p_eps.p : nullptr;
+  a.aux_off = c.p_aux_off.p;
+  a.aux = c.aux_pool.p;
+  a.beta = c.beta_pool.p;
+  a.bstride = c.poly_mm * std::max(c.nf, 1);
+  a.mmax = c.poly_mm;
+  double hm = c.g.h[0];
+  for (int d = 1; d < c.g.dim; d++) hm = std::max(hm, c.g.h[d]);
+  a.ps = 1.0 / hm;
+  return a;
+}
+
+int eval_ktype(const Ctx& c) { return (c.local_eps > 0.0 && c.ktype == K_GAUSS) ? K_GAUSS_LOCAL : c.ktype; }
+bool eval_f32(const Ctx& c) { return c.precision && eval_f32_supported(eval_ktype(c)); }
+
+EvalExt eval_ext(const Ctx& c, int group) {
+  EvalExt e;
+  if (c.poly_deg >= 0) {
+    e.ptab = group < 0 ? c.ptab.p : c.ptabg[group].p;
+    e.mm = c.poly_mm;
+    e.deg = c.poly_deg;
+    double hm = c.g.h[0];
+    for (int d = 1; d < c.g.dim; d++) hm = std::max(hm, c.g.h[d]);
+    e.ps = 1.0 / hm;
+  }
+  if (c.local_eps > 0.0) e.peps = c.p_eps.p;
+  return e;
+}
+
+// Shape parameter of patch p with -lagrbf_local_eps c: c / h_local, h_local =
+// (volume of the cells touching the PU node / members)^(1/d).
+static double patch_eps(const Ctx& c, int p, int n) {
+  if (!(c.local_eps > 0.0) || n <= 0) return c.g.eps_r;
+  const Grid& g = c.g;
+  const int a[3] = {p % g.n[0], (p / g.n[0]) % g.n[1], g.dim == 3 ? p / (g.n[0] * g.n[1]) : 0};
+  double vol = 1.0;
+  for (int d = 0; d < g.dim; d++) vol *= g.h[d] * (double)((a[d] > 0 ? 1 : 0) + (a[d] < g.nc[d] ? 1 : 0));
+  return c.local_eps / std::pow(vol / (double)n, 1.0 / g.dim);
+}
+
+// Elements of a patch's factor slot: Bunch-Kaufman factors the (n + m) square
+// saddle-point matrix, the null-space Cholesky the (n - m) square B22.
+static size_t slot_size(const Ctx& c, int p, int method) {
+  const size_t n = (size_t)c.h_n[p], m = (size_t)c.h_m[p];
+  if (method == M_BK64) return (n + m) * (n + m);
+  if (method_aug(method)) return (n - m) * (n - m);
+  return n * n;
+}
+
+// Factor slots in the FP64 pool (Bunch-Kaufman, FP64 Cholesky) or the FP32 pool
+// (FP32 Cholesky of the mixed path).
 static int alloc_factor_slots(Ctx& c, const std::vector<int32_t>& l, int m) {
   if (l.empty()) return 0;
   size_t& used = method_fp32(m) ? c.fac32_used : c.fac_used;
   size_t need = used;
-  for (int p : l) { c.h_fac_off[p] = (int64_t)need; need += (size_t)c.h_n[p] * c.h_n[p]; }
+  for (int p : l) { c.h_fac_off[p] = (int64_t)need; need += slot_size(c, p, m); }
   if (method_fp32(m)) HIPCHK(c.fac32_pool.grow(need + 1, used, c.s));
   else HIPCHK(c.fac_pool.grow(need + 1, used, c.s));
   used = need;
@@ -360,17 +441,24 @@ static int upload_list(Ctx& c, const std::vector<int32_t>& l, int seg) {
 // refactored with Bunch-Kaufman in FP64 (counted in "fallback").
 static int factor_lists(Ctx& c, std::vector<int32_t>* fl) {
   const int P = c.g.P;
+  const AugView ap = aug_view(c, true), ae = aug_view(c, false);
   for (int m = 0; m < N_METHODS; m++) if (!fl[m].empty()) CHK(upload_list(c, fl[m], m));
   if (!fl[M_BK64].empty())
     launch_factor_bk(c.g, c.ktype, c.list.p, (int)fl[M_BK64].size(), c.p_n.p, c.p_mem_off.p, c.p_fac_off.p, c.mem_pool.p,
-                     c.dx.p, c.fac_pool.p, c.piv_pool.p, c.info.p, c.s);
+                     c.dx.p, c.fac_pool.p, c.piv_pool.p, c.info.p, ap, c.s);
+  // null-space data of the saddle-point patches before their factors
+  for (int m = M_AUG64; m <= M_AUG32B; m++)
+    if (!fl[m].empty())
+      launch_aug_prep(c.g, c.ktype, c.list.p + (size_t)m * P, (int)fl[m].size(), c.p_n.p, c.p_mem_off.p, c.mem_pool.p,
+                      c.dx.p, ap, c.s);
   bool any_chol = false;
   for (int m = M_CHOL64; m < N_METHODS; m++)
     if (!fl[m].empty()) {
       any_chol = true;
       launch_factor_chol(c.g, c.ktype, method_fp32(m), c.list.p + (size_t)m * P, (int)fl[m].size(), c.p_n.p,
                          c.p_mem_off.p, c.p_fac_off.p, c.mem_pool.p, c.dx.p,
-                         method_fp32(m) ? (void*)c.fac32_pool.p : (void*)c.fac_pool.p, c.pstatus.p, method_big(m), c.s);
+                         method_fp32(m) ? (void*)c.fac32_pool.p : (void*)c.fac_pool.p, c.pstatus.p, method_big(m),
+                         method_aug(m) ? ap : ae, c.s);
     }
   if (!any_chol) return 0;
   std::vector<int32_t> st(P);
@@ -378,14 +466,18 @@ static int factor_lists(Ctx& c, std::vector<int32_t>* fl) {
   HIPCHK(hipStreamSynchronize(c.s));
   std::vector<int32_t> redo;
   for (int m = M_CHOL64; m < N_METHODS; m++)
-    for (int p : fl[m]) if (st[p] != 0) { redo.push_back(p); c.h_method[p] = M_BK64; }
+    for (int p : fl[m]) if (st[p] != 0) {
+      redo.push_back(p);
+      if (method_aug(m)) c.aug_fallback_bk += 1;
+      c.h_method[p] = M_BK64;
+    }
   if (redo.empty()) return 0;
   c.fallback += (double)redo.size();
   CHK(alloc_factor_slots(c, redo, M_BK64));
   HIPCHK(hipMemcpyAsync(c.p_fac_off.p, c.h_fac_off.data(), sizeof(int64_t) * P, hipMemcpyHostToDevice, c.s));
   CHK(upload_list(c, redo, 0));
   launch_factor_bk(c.g, c.ktype, c.list.p, (int)redo.size(), c.p_n.p, c.p_mem_off.p, c.p_fac_off.p, c.mem_pool.p,
-                   c.dx.p, c.fac_pool.p, c.piv_pool.p, c.info.p, c.s);
+                   c.dx.p, c.fac_pool.p, c.piv_pool.p, c.info.p, ap, c.s);
   HIPCHK(hipStreamSynchronize(c.s));
   return 0;
 }
@@ -395,40 +487,55 @@ static int factor_lists(Ctx& c, std::vector<int32_t>* fl) {
 // Bunch-Kaufman factor and solved again (counted in "fallback").
 static int solve_list(Ctx& c, const std::vector<int32_t>& sl) {
   const int P = c.g.P;
+  const AugView ap = aug_view(c, true), ae = aug_view(c, false);
   std::vector<int32_t> l[N_METHODS];
   for (int p : sl) l[c.h_method[p]].push_back(p);
   for (int m = 0; m < N_METHODS; m++) if (!l[m].empty()) CHK(upload_list(c, l[m], m));
   if (!l[M_BK64].empty())
     launch_solve_bk(c.list.p, (int)l[M_BK64].size(), c.p_n.p, c.p_mem_off.p, c.p_fac_off.p, c.mem_pool.p, c.fac_pool.p,
-                    c.piv_pool.p, c.dval.p, c.nf, c.coef_pool.p, c.s);
+                    c.piv_pool.p, c.dval.p, c.nf, c.coef_pool.p, ap, c.s);
   bool any_chol = false;
   for (int m = M_CHOL64; m < N_METHODS; m++) any_chol = any_chol || !l[m].empty();
   if (c.nf > 8 && any_chol)
     return set_err(E_SUP, "PartitionSolve", "Cholesky solve handles up to 8 interpolated fields (found %d)", c.nf);
-  if (!l[M_CHOL64B].empty() || !l[M_CHOL32B].empty())
-    HIPCHK(c.solve_ws.reserve((c.mem_used + 1) * 2 * (size_t)std::max(c.nf, 1)));
+  size_t ws_need = 0;
+  if (!l[M_CHOL64B].empty() || !l[M_CHOL32B].empty()) ws_need = (c.mem_used + 1) * 2 * (size_t)std::max(c.nf, 1);
+  if (!l[M_AUG64B].empty() || !l[M_AUG32B].empty())
+    ws_need = std::max(ws_need, (c.mem_used + 1) * aug_solve_ws_per_member(std::max(c.nf, 1)));
+  if (ws_need) HIPCHK(c.solve_ws.reserve(ws_need));
   for (int m = M_CHOL64; m < N_METHODS; m++)
     if (!l[m].empty()) {
       int nmax = 1;
       for (int p : l[m]) nmax = std::max(nmax, (int)c.h_n[p]);
-      launch_solve_chol(c.g, c.ktype, method_fp32(m), c.list.p + (size_t)m * P, (int)l[m].size(), c.p_n.p, c.p_mem_off.p,
-                        c.p_fac_off.p, c.mem_pool.p, c.dx.p, method_fp32(m) ? (const void*)c.fac32_pool.p : (const void*)c.fac_pool.p,
-                        c.dval.p, c.nf, c.coef_pool.p, c.refine_max_iter, c.refine_tol, c.pstatus.p, c.piters.p,
-                        method_big(m) ? c.solve_ws.p : nullptr, nmax, c.s);
+      const void* fac = method_fp32(m) ? (const void*)c.fac32_pool.p : (const void*)c.fac_pool.p;
+      if (method_aug(m))
+        launch_solve_aug(c.g, c.ktype, method_fp32(m), c.list.p + (size_t)m * P, (int)l[m].size(), c.p_n.p,
+                         c.p_mem_off.p, c.p_fac_off.p, c.mem_pool.p, c.dx.p, fac, c.dval.p, c.nf, c.coef_pool.p,
+                         c.refine_max_iter, c.refine_tol, c.pstatus.p, c.piters.p, method_big(m) ? c.solve_ws.p : nullptr,
+                         nmax, ap, c.s);
+      else
+        launch_solve_chol(c.g, c.ktype, method_fp32(m), c.list.p + (size_t)m * P, (int)l[m].size(), c.p_n.p,
+                          c.p_mem_off.p, c.p_fac_off.p, c.mem_pool.p, c.dx.p, fac, c.dval.p, c.nf, c.coef_pool.p,
+                          c.refine_max_iter, c.refine_tol, c.pstatus.p, c.piters.p, method_big(m) ? c.solve_ws.p : nullptr,
+                          nmax, ae, c.s);
     }
-  if (l[M_CHOL32].empty() && l[M_CHOL32B].empty()) return 0;
+  if (l[M_CHOL32].empty() && l[M_CHOL32B].empty() && l[M_AUG32].empty() && l[M_AUG32B].empty()) return 0;
   std::vector<int32_t> st(P), its(P);
   HIPCHK(hipMemcpyAsync(st.data(), c.pstatus.p, sizeof(int32_t) * P, hipMemcpyDeviceToHost, c.s));
   HIPCHK(hipMemcpyAsync(its.data(), c.piters.p, sizeof(int32_t) * P, hipMemcpyDeviceToHost, c.s));
   HIPCHK(hipStreamSynchronize(c.s));
   std::vector<int32_t> redo;
   std::vector<int32_t> refined(l[M_CHOL32]);
-  refined.insert(refined.end(), l[M_CHOL32B].begin(), l[M_CHOL32B].end());
+  for (int m : {M_CHOL32B, M_AUG32, M_AUG32B}) refined.insert(refined.end(), l[m].begin(), l[m].end());
   for (int p : refined) {
     c.refine_iters_max = std::max(c.refine_iters_max, (double)its[p]);
     c.refine_iters_sum += its[p];
     c.refined_patches += 1;
-    if (st[p] != 0) { redo.push_back(p); c.h_method[p] = M_BK64; }
+    if (st[p] != 0) {
+      redo.push_back(p);
+      if (method_aug(c.h_method[p])) c.aug_fallback_bk += 1;
+      c.h_method[p] = M_BK64;
+    }
   }
   if (redo.empty()) return 0;
   c.fallback += (double)redo.size();
@@ -436,9 +543,9 @@ static int solve_list(Ctx& c, const std::vector<int32_t>& sl) {
   HIPCHK(hipMemcpyAsync(c.p_fac_off.p, c.h_fac_off.data(), sizeof(int64_t) * P, hipMemcpyHostToDevice, c.s));
   CHK(upload_list(c, redo, 0));
   launch_factor_bk(c.g, c.ktype, c.list.p, (int)redo.size(), c.p_n.p, c.p_mem_off.p, c.p_fac_off.p, c.mem_pool.p,
-                   c.dx.p, c.fac_pool.p, c.piv_pool.p, c.info.p, c.s);
+                   c.dx.p, c.fac_pool.p, c.piv_pool.p, c.info.p, ap, c.s);
   launch_solve_bk(c.list.p, (int)redo.size(), c.p_n.p, c.p_mem_off.p, c.p_fac_off.p, c.mem_pool.p, c.fac_pool.p,
-                  c.piv_pool.p, c.dval.p, c.nf, c.coef_pool.p, c.s);
+                  c.piv_pool.p, c.dval.p, c.nf, c.coef_pool.p, ap, c.s);
   HIPCHK(hipStreamSynchronize(c.s));
   return 0;
 }
@@ -467,16 +574,32 @@ int partitions_setup(Ctx& c, bool all_patches) {
       const int n = c.data_filled ? c.h_pcount[p] : 0;
       c.h_n[p] = n;
       c.h_mem_off[p] = (int64_t)need_mem;
-      need_mem += n;
+      // with a polynomial block a patch's slots in the member-indexed pools are
+      // padded by POLY_MAX: its Bunch-Kaufman pivots cover n + m rows
+      need_mem += n + (c.poly_deg >= 0 ? POLY_MAX : 0);
       newl.push_back(p);
       c.max_patch = std::max(c.max_patch, n);
+      // extensions: polynomial terms and shape parameter of the patch
+      c.h_deg[p] = patch_degree(c.g.dim, c.poly_deg, n);
+      c.h_m[p] = poly_terms(c.g.dim, c.h_deg[p]);
+      c.h_eps[p] = patch_eps(c, p, n);
+      const int mp = c.h_m[p];
       int m = M_BK64;
-      if (c.factor_mode != 1 && chol_kernel_supported(c.ktype)) {
+      const bool chol = chol_kernel_supported(c.ktype) || (c.ktype == K_PHS3 && mp > 0);
+      if (c.factor_mode != 1 && chol) {
         // the mixed path factors in FP32 and refines in FP64 unless told to factor
         // in FP64 (the FP32 evaluation then reads the FP64 solution all the same)
         const bool f32 = c.precision && !c.mixed_factor64;
-        if (n <= c.chol_big_min) m = f32 ? M_CHOL32 : M_CHOL64;
-        else m = f32 ? M_CHOL32B : M_CHOL64B;
+        if (mp > 0) {
+          if (n - mp <= c.chol_big_min) m = f32 ? M_AUG32 : M_AUG64;
+          else m = f32 ? M_AUG32B : M_AUG64B;
+          c.h_aux_off[p] = (int64_t)c.aux_used;
+          c.aux_used += aug_aux_size(n, mp);
+        } else if (n <= c.chol_big_min) {
+          m = f32 ? M_CHOL32 : M_CHOL64;
+        } else {
+          m = f32 ? M_CHOL32B : M_CHOL64B;
+        }
       }
       c.h_method[p] = (uint8_t)m;
       if (n > 0) fl[m].push_back(p);
@@ -485,6 +608,13 @@ int partitions_setup(Ctx& c, bool all_patches) {
   if (!newl.empty()) {
     HIPCHK(c.mem_pool.grow(need_mem + 1, c.mem_used, c.s));
     HIPCHK(c.piv_pool.grow(need_mem + 1, c.mem_used, c.s));
+    if (c.aux_used > 0) {
+      HIPCHK(c.aux_pool.grow(c.aux_used + 1, c.aux_pool.cap, c.s));
+    }
+    HIPCHK(hipMemcpyAsync(c.p_m.p, c.h_m.data(), sizeof(int32_t) * P, hipMemcpyHostToDevice, c.s));
+    HIPCHK(hipMemcpyAsync(c.p_deg.p, c.h_deg.data(), sizeof(int32_t) * P, hipMemcpyHostToDevice, c.s));
+    HIPCHK(hipMemcpyAsync(c.p_eps.p, c.h_eps.data(), sizeof(double) * P, hipMemcpyHostToDevice, c.s));
+    HIPCHK(hipMemcpyAsync(c.p_aux_off.p, c.h_aux_off.data(), sizeof(int64_t) * P, hipMemcpyHostToDevice, c.s));
     HIPCHK(c.coef_pool.grow((need_mem + 1) * std::max(c.nf, 1), c.mem_used * std::max(c.nf, 1), c.s));
     if (c.precision) HIPCHK(c.coef32_pool.grow((need_mem + 1) * std::max(c.nf, 1), c.mem_used * std::max(c.nf, 1), c.s));
     c.mem_used = need_mem;
@@ -542,7 +672,7 @@ int partitions_setup(Ctx& c, bool all_patches) {
 
 void release_factors(Ctx& c) {
   (void)hipStreamSynchronize(c.s);
-  c.fac_pool.release(); c.fac32_pool.release(); c.piv_pool.release();
+  c.fac_pool.release(); c.fac32_pool.release(); c.piv_pool.release(); c.aux_pool.release(); c.solve_ws.release();
   c.factors_released = true;
 }
 
@@ -555,7 +685,7 @@ static int partitions_reset(Ctx& c) {
     std::fill(c.h_solved.begin(), c.h_solved.end(), 0);
     std::fill(c.h_n.begin(), c.h_n.end(), 0);
     std::fill(c.h_active.begin(), c.h_active.end(), 0);
-    c.mem_used = 0; c.fac_used = 0; c.fac32_used = 0; c.setup_N = -1;
+    c.mem_used = 0; c.fac_used = 0; c.fac32_used = 0; c.aux_used = 0; c.setup_N = -1;
     c.h_cell_start_prev.clear(); c.h_cell_nodes_prev.clear();
     c.memb_current = true;
     HIPCHK(hipMemsetAsync(c.p_active.p, 0, P, c.s));
@@ -576,6 +706,11 @@ int ensure_blk(Ctx& c, bool f32) {
   if (c.N > 0)
     launch_build_blk(c.g, f32, c.dx.p, c.N, c.cell_nodes.p, c.dcell.p, c.cell_start.p, c.p_active.p, c.p_mem_off.p,
                      c.coef_pool.p, c.nf, c.nf, 0, f32 ? (void*)c.blk32.p : (void*)c.blk64.p, c.s);
+  if (c.poly_deg >= 0) {
+    // polynomial part of every active patch, zero for the others
+    HIPCHK(c.ptab.reserve((size_t)c.g.P * c.poly_mm * std::max(c.nf, 1)));
+    launch_build_ptab(c.g, c.p_active.p, aug_view(c, true), c.poly_mm, c.nf, c.nf, 0, c.ptab.p, c.s);
+  }
   c.blk_valid = true;
   c.blk_f32 = f32;
   return 0;
@@ -628,6 +763,10 @@ static int ensure_blk_group(Ctx& c, bool f32, int g) {
   if (c.N > 0)
     launch_build_blk(c.g, f32, c.dx.p, c.N, c.cell_nodes.p, c.dcell.p, c.cell_start.p, c.p_active.p, c.p_mem_off.p,
                      c.coef_pool.p, dim, c.nf, g * dim, f32 ? (void*)c.blkg32[g].p : (void*)c.blkg64[g].p, c.s);
+  if (c.poly_deg >= 0) {
+    HIPCHK(c.ptabg[g].reserve((size_t)c.g.P * c.poly_mm * dim));
+    launch_build_ptab(c.g, c.p_active.p, aug_view(c, true), c.poly_mm, dim, c.nf, g * dim, c.ptabg[g].p, c.s);
+  }
   c.blkg_gen[g] = c.table_gen;
   c.blkg_f32[g] = f32;
   return 0;
@@ -656,7 +795,10 @@ static int rbf_interpolation(Ctx& c, double* out, int nfields, int group = -1) {
     c.sorted_valid = true;
   }
   if (!c.zeros.p) { HIPCHK(c.zeros.reserve(1)); HIPCHK(hipMemset(c.zeros.p, 0, sizeof(double))); }
-  const bool blk32 = c.precision && eval_f32_supported(c.ktype);
+  const bool blk32 = eval_f32(c);
+  // the per-cell fallback kernels know neither the r^3 kernel nor the extensions:
+  // those go through the reference-order kernel
+  const bool ext = c.poly_deg >= 0 || c.local_eps > 0.0 || c.ktype == K_PHS3;
   const bool use_blk = cell_order && eval_blk_supported(group >= 0 ? c.g.dim : c.nf, c.g.dim, blk32);
   if (group >= 0 && !use_blk)
     return set_err(E_SUP, "RBFInterpolation", "snapshot fields need the cell-table evaluation (patches from the current data)");
@@ -685,16 +827,16 @@ static int rbf_interpolation(Ctx& c, double* out, int nfields, int group = -1) {
       bool listed;
       int lanes, pipe, waves;
       eval_variant(v, listed, lanes, pipe, waves);
-      launch_eval_blk(c.g, c.ktype, blk32, c.px.p, c.ld, c.perm.p, c.pstart.p, c.w.p, table, nft, c.cell_start.p,
+      launch_eval_blk(c.g, eval_ktype(c), blk32, c.px.p, c.ld, c.perm.p, c.pstart.p, c.w.p, table, nft, c.cell_start.p,
                       nfields, out, block, c.Np, listed ? c.cell_list.p : nullptr, listed ? c.list_count.p : nullptr,
-                      lanes, pipe, waves, c.s);
+                      lanes, pipe, waves, eval_ext(c, group), c.s);
     };
     set_eval_timing_events(e0, e1);
     c.last_variant =
         c.tuner.run(bucket, nft == c.g.dim && nfields == nft, blk32 && nft == c.g.dim, c.cell_ordered || !c.reorder,
                     !c.use_graph, c.s, launch);
     set_eval_timing_events(nullptr, nullptr);
-  } else if (cell_order) {
+  } else if (cell_order && !ext) {
     const int per_cell = (c.Np + c.g.C - 1) / std::max(c.g.C, 1);
     const int block = std::min(256, std::max(64, (per_cell + 63) / 64 * 64));
     const bool f32 = c.precision && eval_f32_supported(c.ktype) && c.xs32.p && c.coef32_pool.p;
@@ -707,7 +849,7 @@ static int rbf_interpolation(Ctx& c, double* out, int nfields, int group = -1) {
                        nfields, out, block, c.s);
   } else {
     launch_eval_ref(c.g, c.ktype, c.px.p, c.ld, c.Np, c.pcell.p, c.alive.p, c.w.p, pv, c.dx.p ? c.dx.p : c.pux.p,
-                    c.cell_start.p, nfields, out, c.s);
+                    c.cell_start.p, nfields, out, aug_view(c, true), c.s);
   }
   if (e0) {
     if (!use_blk) HIPCHK(hipEventRecord(e1, c.s));
@@ -1021,7 +1163,7 @@ static int fixed_step(Ctx& c, double dt) {
 // particle buffers and two graphs, one per parity, cover the loop).
 static bool graph_eligible(const Ctx& c) {
   if (!c.use_graph || c.analytical || c.scheme > 2 || !c.reorder || c.Np == 0) return false;
-  if (c.eval_mode == 2 || (c.eval_mode == 0 && !c.memb_current) || !eval_blk_supported(c.nf, c.g.dim, c.precision && eval_f32_supported(c.ktype)) || !c.blk_valid) return false;
+  if (c.eval_mode == 2 || (c.eval_mode == 0 && !c.memb_current) || !eval_blk_supported(c.nf, c.g.dim, eval_f32(c)) || !c.blk_valid) return false;
   if (!c.sorted_valid || !c.mem_pool.p) return false;
   if (c.tuner.planned(eval_bucket(c)) < 0) return false;
   return setup_complete(c);
@@ -1110,7 +1252,7 @@ static int adaptive_velocity(Ctx& c, double* dst, double cst) {
   if (!c.blend) return rbf_interpolation(c, dst, dim);
   // -1 (default) and 1: fused where the cell table evaluates 2d fields, which is
   // the FP32 evaluation (the FP64 one would need 96 accumulator registers per lane)
-  const bool f32 = c.precision && eval_f32_supported(c.ktype);
+  const bool f32 = eval_f32(c);
   const bool fused = c.blend_fused != 0 && eval_blk_supported(2 * dim, dim, f32);
   if (fused) {
     // both snapshots in one evaluation (the 2d fields of the data): phi once per
@@ -1657,6 +1799,8 @@ int lagrbf_get_info(void* ctx, const char* key, double* value) {
   else if (k == "max_patch") *value = c->max_patch;
   else if (k == "eps_w") *value = c->g.eps_w;
   else if (k == "eps_r") *value = c->g.eps_r;
+  else if (k == "aug_fallback_bk") *value = c->aug_fallback_bk;
+  else if (k == "poly_terms") *value = c->poly_mm;
   else return set_err(E_WRONG, "lagrbf_get_info", "unknown key '%s'", key);
   return 0;
 }
@@ -1680,6 +1824,17 @@ int64_t lagrbf_debug_get(void* ctx, const char* what, int32_t index, void* out, 
     n = c->h_n[index];
     if (w == "patch_members") src = c->mem_pool.p + off;
     else { src = c->coef_pool.p + off * c->nf; n *= c->nf; esz = 8; }
+  } else if (w == "patch_beta") {
+    // polynomial coefficients of a patch, row-major [m][nf] (extension)
+    if (index < 0 || index >= c->g.P) return -set_err(E_RANGE, "lagrbf_debug_get", "patch %d out of range", index);
+    if (!c->h_setup[index] || c->poly_deg < 0) return 0;
+    src = c->beta_pool.p + (size_t)index * c->poly_mm * std::max(c->nf, 1);
+    n = (int64_t)c->h_m[index] * c->nf;
+    esz = 8;
+  } else if (w == "patch_eps") {
+    if (index < 0 || index >= c->g.P) return -set_err(E_RANGE, "lagrbf_debug_get", "patch %d out of range", index);
+    if (out && cap >= 1) *static_cast<double*>(out) = c->h_eps[index];
+    return 1;
   } else if (w == "patch_flags") {
     if (index < 0 || index >= c->g.P) return -set_err(E_RANGE, "lagrbf_debug_get", "patch %d out of range", index);
     if (out && cap >= 3) {

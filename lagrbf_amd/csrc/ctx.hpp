@@ -176,13 +176,31 @@ struct Ctx {
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
   double graph_dt = 0.0, graph_steps = 0, graph_ms = 0;
   int max_patch = 0;
+  // extensions (no reference counterpart): polynomial block of every patch system
+  // and a shape parameter per patch
+  int poly_deg = -1;                       // -lagrbf_poly_degree (-1: pure RBF systems, as the reference)
+  int poly_mm = 0;                         // terms of poly_deg
+  double local_eps = 0.0;                  // -lagrbf_local_eps c: eps_p = c / h_local(p), 0: eps_r
+  std::vector<int32_t> h_m, h_deg;         // per patch: polynomial terms and degree
+  std::vector<double> h_eps;               // per patch: shape parameter
+  std::vector<int64_t> h_aux_off;
+  size_t aux_used = 0;
+  DBuf<int32_t> p_m, p_deg;
+  DBuf<double> p_eps, aux_pool, beta_pool;
+  DBuf<int64_t> p_aux_off;
+  DBuf<double> ptab, ptabg[2];             // polynomial tables of the evaluation (main, blend groups)
+  int ptab_gen = -1, ptabg_gen[2] = {-1, -1};
+  double aug_fallback_bk = 0;              // saddle-point patches refactored with Bunch-Kaufman
 };
 
 // factor methods: Bunch-Kaufman FP64; Cholesky FP64 / FP32 (n <= chol_max_n());
-// the same for larger patches (kernels without n-sized LDS)
-enum { M_BK64 = 0, M_CHOL64 = 1, M_CHOL32 = 2, M_CHOL64B = 3, M_CHOL32B = 4, N_METHODS = 5 };
-inline bool method_fp32(int m) { return m == M_CHOL32 || m == M_CHOL32B; }
-inline bool method_big(int m) { return m == M_CHOL64B || m == M_CHOL32B; }
+// the same for larger patches (kernels without n-sized LDS); the saddle-point
+// systems' null-space Cholesky (B22, FP64 / FP32, small / large)
+enum { M_BK64 = 0, M_CHOL64 = 1, M_CHOL32 = 2, M_CHOL64B = 3, M_CHOL32B = 4,
+       M_AUG64 = 5, M_AUG32 = 6, M_AUG64B = 7, M_AUG32B = 8, N_METHODS = 9 };
+inline bool method_fp32(int m) { return m == M_CHOL32 || m == M_CHOL32B || m == M_AUG32 || m == M_AUG32B; }
+inline bool method_big(int m) { return m == M_CHOL64B || m == M_CHOL32B || m == M_AUG64B || m == M_AUG32B; }
+inline bool method_aug(int m) { return m >= M_AUG64 && m <= M_AUG32B; }
 
 template <typename T>
 inline void swap_buf(DBuf<T>& a, DBuf<T>& b) {
@@ -197,6 +215,14 @@ int fill_data(Ctx& c, int n, const double* coords, const double* values, const i
 // all_patches: set up every patch (not only the corners of particle cells)
 int partitions_setup(Ctx& c, bool all_patches = false);
 int ensure_blk(Ctx& c, bool f32);
+// kernel of the evaluation (K_GAUSS_LOCAL for the Gaussian with per-patch shape)
+int eval_ktype(const Ctx& c);
+// FP32 evaluation table for this context's kernel
+bool eval_f32(const Ctx& c);
+// polynomial table and per-patch shape of the evaluation (group -1: main table)
+EvalExt eval_ext(const Ctx& c, int group);
+// patch views of the extensions (with_poly false: shape parameter only)
+AugView aug_view(Ctx& c, bool with_poly);
 // free the factor pools once every patch is solved (coefficients stay)
 void release_factors(Ctx& c);
 

@@ -241,7 +241,12 @@ static void eval_cell_dim(const Grid& g, int ktype, const double* px, int64_t ld
 #undef LRB_K
 }
 
-bool eval_f32_supported(int ktype) { return ktype == K_WC0 || ktype == K_WC2 || ktype == K_IMQ || ktype == K_GAUSS; }
+// FP32 evaluation: the positive definite kernels, r^3 (always with a polynomial
+// block) and the Gaussian with per-patch shape (cell-table kernels only)
+bool eval_f32_supported(int ktype) {
+  return ktype == K_WC0 || ktype == K_WC2 || ktype == K_IMQ || ktype == K_GAUSS || ktype == K_PHS3 ||
+         ktype == K_GAUSS_LOCAL;
+}
 
 void launch_eval_cell(const Grid& g, int ktype, bool fp32, const double* px, int64_t ld, const int32_t* perm,
                       const int32_t* pstart, const double* w, const uint8_t* active, const int64_t* mem_off,

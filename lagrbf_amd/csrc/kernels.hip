@@ -228,28 +228,50 @@ __global__ __launch_bounds__(FNT) void k_factor_bk(Grid g, const int32_t* __rest
                                                    const int64_t* __restrict__ mem_off, const int64_t* __restrict__ fac_off,
                                                    const int32_t* __restrict__ mem_pool, const double* __restrict__ xd,
                                                    double* __restrict__ fac, int32_t* __restrict__ piv_pool,
-                                                   int32_t* __restrict__ info_out) {
+                                                   int32_t* __restrict__ info_out, AugView aug) {
   __shared__ ArgMax red[FNT / WAVE];
   __shared__ double redd[FNT / WAVE];
   __shared__ int s_kp, s_kstep;
   const int p = list[blockIdx.x];
-  const int n = pn[p];
-  if (n == 0) return;
+  const int n0 = pn[p];
+  if (n0 == 0) return;
+  // with a polynomial block of m terms the factored matrix is the (n0 + m) square
+  // saddle-point matrix [A P; P^T 0] (extension)
+  const int m = aug.m ? aug.m[p] : 0;
+  const int n = n0 + m;
   double* A = fac + fac_off[p];
   const int32_t* mem = mem_pool + mem_off[p];
-  int32_t* piv = piv_pool + mem_off[p];
+  int32_t* piv = piv_pool + bk_piv_base(aug, p, mem_off[p]);
   const int tid = threadIdx.x, lane = tid & (WAVE - 1), wid = tid / WAVE;
   const int dim = g.dim;
-  const double eps = g.eps_r;
+  const double eps = aug.eps ? aug.eps[p] : g.eps_r;
   // assemble the lower triangle: A_ij = phi(|x_i - x_j|, eps_r)  (partition_solve.c:47-82)
-  for (int j = wid; j < n; j += FNT / WAVE) {
+  for (int j = wid; j < n0; j += FNT / WAVE) {
     const double* xj = xd + (size_t)mem[j] * dim;
     const double xj0 = xj[0], xj1 = xj[1], xj2 = dim == 3 ? xj[2] : 0.0;
-    for (int i = j + lane; i < n; i += WAVE) {
+    for (int i = j + lane; i < n0; i += WAVE) {
       const double* xi = xd + (size_t)mem[i] * dim;
       double d0 = xi[0] - xj0, d1 = xi[1] - xj1, d2 = dim == 3 ? xi[2] - xj2 : 0.0;
       double r = sqrt(d0 * d0 + d1 * d1 + d2 * d2);
       A[(size_t)j * n + i] = rbf<KT, double>(r, eps);
+    }
+  }
+  if (m > 0) {
+    // rows n0.. of the lower triangle: P^T (monomials of the member shifted to the
+    // patch node, scaled by ps), then the zero block
+    const int deg = aug.deg[p];
+    const int pi = p % g.n[0], pj = (p / g.n[0]) % g.n[1], pk = dim == 3 ? p / (g.n[0] * g.n[1]) : 0;
+    const double c[3] = {g.O[0] + pi * g.h[0], g.O[1] + pj * g.h[1], dim == 3 ? g.O[2] + pk * g.h[2] : 0.0};
+    for (int i = tid; i < n0; i += FNT) {
+      const double* x = xd + (size_t)mem[i] * dim;
+      double xi[3], pv[POLY_MAX];
+      for (int d = 0; d < dim; d++) xi[d] = (x[d] - c[d]) * aug.ps;
+      poly_eval(dim, deg, xi, pv);
+      for (int q = 0; q < m; q++) A[(size_t)i * n + n0 + q] = pv[q];
+    }
+    for (int e = tid; e < m * m; e += FNT) {
+      const int r = e / m, q = e % m;
+      if (q >= r) A[(size_t)(n0 + r) * n + n0 + q] = 0.0;
     }
   }
   __syncthreads();
@@ -343,9 +365,9 @@ __global__ __launch_bounds__(FNT) void k_factor_bk(Grid g, const int32_t* __rest
 
 void launch_factor_bk(const Grid& g, int ktype, const int32_t* list, int nlist, const int32_t* pn,
                       const int64_t* mem_off, const int64_t* fac_off, const int32_t* mem_pool, const double* xd,
-                      double* fac, int32_t* piv, int32_t* info, hipStream_t s) {
+                      double* fac, int32_t* piv, int32_t* info, const AugView& a, hipStream_t s) {
   if (nlist <= 0) return;
-#define LRB_FBK(K) hipLaunchKernelGGL(k_factor_bk<K>, dim3(nlist), dim3(FNT), 0, s, g, list, pn, mem_off, fac_off, mem_pool, xd, fac, piv, info)
+#define LRB_FBK(K) hipLaunchKernelGGL(k_factor_bk<K>, dim3(nlist), dim3(FNT), 0, s, g, list, pn, mem_off, fac_off, mem_pool, xd, fac, piv, info, a)
   switch (ktype) {
     case K_LINEAR: LRB_FBK(K_LINEAR); break;
     case K_WC0: LRB_FBK(K_WC0); break;
@@ -353,6 +375,7 @@ void launch_factor_bk(const Grid& g, int ktype, const int32_t* list, int nlist, 
     case K_BUMP: LRB_FBK(K_BUMP); break;
     case K_MQ: LRB_FBK(K_MQ); break;
     case K_IMQ: LRB_FBK(K_IMQ); break;
+    case K_PHS3: LRB_FBK(K_PHS3); break;
     default: LRB_FBK(K_GAUSS); break;
   }
 #undef LRB_FBK
@@ -364,50 +387,55 @@ __global__ __launch_bounds__(FNT) void k_solve_bk(const int32_t* __restrict__ li
                                                   const int64_t* __restrict__ mem_off, const int64_t* __restrict__ fac_off,
                                                   const int32_t* __restrict__ mem_pool, const double* __restrict__ fac,
                                                   const int32_t* __restrict__ piv_pool, const double* __restrict__ values,
-                                                  int nf, double* __restrict__ coef) {
+                                                  int nf, double* __restrict__ coef, AugView aug) {
   __shared__ double redd[FNT / WAVE];
   const int p = list[blockIdx.x];
-  const int n = pn[p];
-  if (n == 0 || nf == 0) return;
+  const int n0 = pn[p];
+  if (n0 == 0 || nf == 0) return;
+  // saddle-point system (extension): right-hand side [f; 0], rows n0.. (beta) in aug.beta
+  const int m = aug.m ? aug.m[p] : 0;
+  const int n = n0 + m;
   const double* A = fac + fac_off[p];
   const int32_t* mem = mem_pool + mem_off[p];
-  const int32_t* piv = piv_pool + mem_off[p];
+  const int32_t* piv = piv_pool + bk_piv_base(aug, p, mem_off[p]);
   double* B = coef + mem_off[p] * nf;
+  double* Bb = m > 0 ? aug.beta + (size_t)p * aug.bstride : nullptr;
+#define BR(i) ((i) < n0 ? B + (size_t)(i) * nf : Bb + (size_t)((i) - n0) * nf)
   const int tid = threadIdx.x;
-  for (int t = tid; t < n * nf; t += FNT) { int i = t / nf, f = t % nf; B[t] = values[(size_t)mem[i] * nf + f]; }
+  for (int t = tid; t < n * nf; t += FNT) { int i = t / nf, f = t % nf; BR(i)[f] = i < n0 ? values[(size_t)mem[i] * nf + f] : 0.0; }
   __syncthreads();
   // forward: L D y = b
   int k = 0;
   while (k < n) {
     if (piv[k] > 0) {
       const int kp = piv[k] - 1;
-      if (kp != k && tid < nf) { double t = B[k * nf + tid]; B[k * nf + tid] = B[kp * nf + tid]; B[kp * nf + tid] = t; }
+      if (kp != k && tid < nf) { double t = BR(k)[tid]; BR(k)[tid] = BR(kp)[tid]; BR(kp)[tid] = t; }
       __syncthreads();
       const double* ck = A + (size_t)k * n;
       for (int f = 0; f < nf; f++) {
-        const double bk = B[k * nf + f];
-        for (int i = k + 1 + tid; i < n; i += FNT) B[i * nf + f] -= ck[i] * bk;
+        const double bk = BR(k)[f];
+        for (int i = k + 1 + tid; i < n; i += FNT) BR(i)[f] -= ck[i] * bk;
       }
       __syncthreads();
-      if (tid < nf) B[k * nf + tid] *= 1.0 / ck[k];
+      if (tid < nf) BR(k)[tid] *= 1.0 / ck[k];
       __syncthreads();
       k += 1;
     } else {
       const int kp = -piv[k] - 1;
-      if (kp != k + 1 && tid < nf) { double t = B[(k + 1) * nf + tid]; B[(k + 1) * nf + tid] = B[kp * nf + tid]; B[kp * nf + tid] = t; }
+      if (kp != k + 1 && tid < nf) { double t = BR(k + 1)[tid]; BR(k + 1)[tid] = BR(kp)[tid]; BR(kp)[tid] = t; }
       __syncthreads();
       const double* c0 = A + (size_t)k * n;
       const double* c1 = A + (size_t)(k + 1) * n;
       for (int f = 0; f < nf; f++) {
-        const double b0 = B[k * nf + f], b1 = B[(k + 1) * nf + f];
-        for (int i = k + 2 + tid; i < n; i += FNT) { double v = B[i * nf + f] - c0[i] * b0; B[i * nf + f] = v - c1[i] * b1; }
+        const double b0 = BR(k)[f], b1 = BR(k + 1)[f];
+        for (int i = k + 2 + tid; i < n; i += FNT) { double v = BR(i)[f] - c0[i] * b0; BR(i)[f] = v - c1[i] * b1; }
       }
       __syncthreads();
       if (tid < nf) {
         const double akm1k = c0[k + 1], akm1 = c0[k] / akm1k, ak = c1[k + 1] / akm1k, denom = akm1 * ak - 1.0;
-        const double bkm1 = B[k * nf + tid] / akm1k, bk = B[(k + 1) * nf + tid] / akm1k;
-        B[k * nf + tid] = (ak * bkm1 - bk) / denom;
-        B[(k + 1) * nf + tid] = (akm1 * bk - bkm1) / denom;
+        const double bkm1 = BR(k)[tid] / akm1k, bk = BR(k + 1)[tid] / akm1k;
+        BR(k)[tid] = (ak * bkm1 - bk) / denom;
+        BR(k + 1)[tid] = (akm1 * bk - bkm1) / denom;
       }
       __syncthreads();
       k += 2;
@@ -420,13 +448,13 @@ __global__ __launch_bounds__(FNT) void k_solve_bk(const int32_t* __restrict__ li
       const double* ck = A + (size_t)k * n;
       for (int f = 0; f < nf; f++) {
         double s = 0.0;
-        for (int i = k + 1 + tid; i < n; i += FNT) s += B[i * nf + f] * ck[i];
+        for (int i = k + 1 + tid; i < n; i += FNT) s += BR(i)[f] * ck[i];
         s = block_sum<FNT>(s, redd);
-        if (tid == 0) B[k * nf + f] -= s;
+        if (tid == 0) BR(k)[f] -= s;
       }
       __syncthreads();
       const int kp = piv[k] - 1;
-      if (kp != k && tid < nf) { double t = B[k * nf + tid]; B[k * nf + tid] = B[kp * nf + tid]; B[kp * nf + tid] = t; }
+      if (kp != k && tid < nf) { double t = BR(k)[tid]; BR(k)[tid] = BR(kp)[tid]; BR(kp)[tid] = t; }
       __syncthreads();
       k -= 1;
     } else {
@@ -434,24 +462,25 @@ __global__ __launch_bounds__(FNT) void k_solve_bk(const int32_t* __restrict__ li
       const double* ckm1 = A + (size_t)(k - 1) * n;
       for (int f = 0; f < nf; f++) {
         double s = 0.0, s1 = 0.0;
-        for (int i = k + 1 + tid; i < n; i += FNT) { s += B[i * nf + f] * ck[i]; s1 += B[i * nf + f] * ckm1[i]; }
+        for (int i = k + 1 + tid; i < n; i += FNT) { s += BR(i)[f] * ck[i]; s1 += BR(i)[f] * ckm1[i]; }
         s = block_sum<FNT>(s, redd);
         s1 = block_sum<FNT>(s1, redd);
-        if (tid == 0) { B[k * nf + f] -= s; B[(k - 1) * nf + f] -= s1; }
+        if (tid == 0) { BR(k)[f] -= s; BR(k - 1)[f] -= s1; }
       }
       __syncthreads();
       const int kp = -piv[k] - 1;
-      if (kp != k && tid < nf) { double t = B[k * nf + tid]; B[k * nf + tid] = B[kp * nf + tid]; B[kp * nf + tid] = t; }
+      if (kp != k && tid < nf) { double t = BR(k)[tid]; BR(k)[tid] = BR(kp)[tid]; BR(kp)[tid] = t; }
       __syncthreads();
       k -= 2;
     }
   }
+#undef BR
 }
 
 void launch_solve_bk(const int32_t* list, int nlist, const int32_t* pn, const int64_t* mem_off, const int64_t* fac_off,
                      const int32_t* mem_pool, const double* fac, const int32_t* piv, const double* values, int nf,
-                     double* coef, hipStream_t s) {
-  if (nlist > 0) hipLaunchKernelGGL(k_solve_bk, dim3(nlist), dim3(FNT), 0, s, list, pn, mem_off, fac_off, mem_pool, fac, piv, values, nf, coef);
+                     double* coef, const AugView& a, hipStream_t s) {
+  if (nlist > 0) hipLaunchKernelGGL(k_solve_bk, dim3(nlist), dim3(FNT), 0, s, list, pn, mem_off, fac_off, mem_pool, fac, piv, values, nf, coef, a);
 }
 
 // --------------------------------------------------------------------------
@@ -514,6 +543,10 @@ __device__ __forceinline__ void weights_point(const Grid& g, const double* __res
       }
     }
   }
+  if (g.wnorm) {
+    sum = 0.0;
+    for (int k = 0; k < g.nen; k++) sum += w[k];
+  }
   const double inv = sum > 0.0 ? 1.0 / sum : 0.0;
   for (int k = 0; k < g.nen; k++) w[k] *= inv;
 }
@@ -544,7 +577,7 @@ __global__ __launch_bounds__(256) void k_eval_ref(Grid g, const double* __restri
                                                   const int32_t* __restrict__ cell, const int32_t* __restrict__ alive,
                                                   const double* __restrict__ w, PatchView pv,
                                                   const double* __restrict__ xd, const int32_t* __restrict__ cs,
-                                                  int nfields, int f0, double* __restrict__ out) {
+                                                  int nfields, int f0, double* __restrict__ out, AugView aug) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= Np || !alive[i]) return;
   const int c = cell[i];
@@ -577,10 +610,24 @@ __global__ __launch_bounds__(256) void k_eval_ref(Grid g, const double* __restri
     for (int nd = 0; nd < n; nd++) {
       const double* xj = xd + (size_t)mem[nd] * dim;
       const double d0 = x - xj[0], d1 = y - xj[1], d2 = dim == 3 ? z - xj[2] : 0.0;
-      const double phi = rbf<KT, double>(sqrt(d0 * d0 + d1 * d1 + d2 * d2), g.eps_r);
+      const double phi = rbf<KT, double>(sqrt(d0 * d0 + d1 * d1 + d2 * d2), aug.eps ? aug.eps[p] : g.eps_r);
       const double* Xr = X + (size_t)nd * pv.nf_total;
 #pragma unroll
       for (int f = 0; f < NF; f++) if (f < nfc) it[f] += phi * Xr[f];
+    }
+    const int m = aug.m ? aug.m[p] : 0;
+    if (m > 0) {
+      // polynomial part of the patch (extension)
+      const int pi = p % g.n[0], pj = (p / g.n[0]) % g.n[1], pk = dim == 3 ? p / (g.n[0] * g.n[1]) : 0;
+      double xi[3], pvv[POLY_MAX];
+      xi[0] = (x - (g.O[0] + pi * g.h[0])) * aug.ps;
+      xi[1] = (y - (g.O[1] + pj * g.h[1])) * aug.ps;
+      xi[2] = dim == 3 ? (z - (g.O[2] + pk * g.h[2])) * aug.ps : 0.0;
+      poly_eval(dim, aug.deg[p], xi, pvv);
+      const double* bt = aug.beta + (size_t)p * aug.bstride + f0;
+      for (int q = 0; q < m; q++)
+#pragma unroll
+        for (int f = 0; f < NF; f++) if (f < nfc) it[f] += pvv[q] * bt[(size_t)q * pv.nf_total + f];
     }
     const double wk = w[(size_t)k * ld + i];
 #pragma unroll
@@ -593,29 +640,30 @@ __global__ __launch_bounds__(256) void k_eval_ref(Grid g, const double* __restri
 template <int KT>
 static void eval_ref_k(const Grid& g, const double* px, int64_t ld, int Np, const int32_t* cell, const int32_t* alive,
                        const double* w, const PatchView& pv, const double* xd, const int32_t* cs, int nfields,
-                       double* out, hipStream_t s) {
+                       double* out, const AugView& a, hipStream_t s) {
   const dim3 grid((Np + 255) / 256), blk(256);
   for (int f0 = 0; f0 < nfields; f0 += 4) {
     const int r = nfields - f0;
-    if (r == 1) hipLaunchKernelGGL((k_eval_ref<KT, 1>), grid, blk, 0, s, g, px, ld, Np, cell, alive, w, pv, xd, cs, nfields, f0, out);
-    else if (r == 2) hipLaunchKernelGGL((k_eval_ref<KT, 2>), grid, blk, 0, s, g, px, ld, Np, cell, alive, w, pv, xd, cs, nfields, f0, out);
-    else if (r == 3) hipLaunchKernelGGL((k_eval_ref<KT, 3>), grid, blk, 0, s, g, px, ld, Np, cell, alive, w, pv, xd, cs, nfields, f0, out);
-    else hipLaunchKernelGGL((k_eval_ref<KT, 4>), grid, blk, 0, s, g, px, ld, Np, cell, alive, w, pv, xd, cs, nfields, f0, out);
+    if (r == 1) hipLaunchKernelGGL((k_eval_ref<KT, 1>), grid, blk, 0, s, g, px, ld, Np, cell, alive, w, pv, xd, cs, nfields, f0, out, a);
+    else if (r == 2) hipLaunchKernelGGL((k_eval_ref<KT, 2>), grid, blk, 0, s, g, px, ld, Np, cell, alive, w, pv, xd, cs, nfields, f0, out, a);
+    else if (r == 3) hipLaunchKernelGGL((k_eval_ref<KT, 3>), grid, blk, 0, s, g, px, ld, Np, cell, alive, w, pv, xd, cs, nfields, f0, out, a);
+    else hipLaunchKernelGGL((k_eval_ref<KT, 4>), grid, blk, 0, s, g, px, ld, Np, cell, alive, w, pv, xd, cs, nfields, f0, out, a);
   }
 }
 
 void launch_eval_ref(const Grid& g, int ktype, const double* px, int64_t ld, int Np, const int32_t* cell,
                      const int32_t* alive, const double* w, const PatchView& pv, const double* xd,
-                     const int32_t* cs, int nfields, double* out, hipStream_t s) {
+                     const int32_t* cs, int nfields, double* out, const AugView& a, hipStream_t s) {
   if (Np <= 0 || nfields <= 0) return;
   switch (ktype) {
-    case K_LINEAR: eval_ref_k<K_LINEAR>(g, px, ld, Np, cell, alive, w, pv, xd, cs, nfields, out, s); break;
-    case K_WC0: eval_ref_k<K_WC0>(g, px, ld, Np, cell, alive, w, pv, xd, cs, nfields, out, s); break;
-    case K_WC2: eval_ref_k<K_WC2>(g, px, ld, Np, cell, alive, w, pv, xd, cs, nfields, out, s); break;
-    case K_BUMP: eval_ref_k<K_BUMP>(g, px, ld, Np, cell, alive, w, pv, xd, cs, nfields, out, s); break;
-    case K_MQ: eval_ref_k<K_MQ>(g, px, ld, Np, cell, alive, w, pv, xd, cs, nfields, out, s); break;
-    case K_IMQ: eval_ref_k<K_IMQ>(g, px, ld, Np, cell, alive, w, pv, xd, cs, nfields, out, s); break;
-    default: eval_ref_k<K_GAUSS>(g, px, ld, Np, cell, alive, w, pv, xd, cs, nfields, out, s); break;
+    case K_LINEAR: eval_ref_k<K_LINEAR>(g, px, ld, Np, cell, alive, w, pv, xd, cs, nfields, out, a, s); break;
+    case K_WC0: eval_ref_k<K_WC0>(g, px, ld, Np, cell, alive, w, pv, xd, cs, nfields, out, a, s); break;
+    case K_WC2: eval_ref_k<K_WC2>(g, px, ld, Np, cell, alive, w, pv, xd, cs, nfields, out, a, s); break;
+    case K_BUMP: eval_ref_k<K_BUMP>(g, px, ld, Np, cell, alive, w, pv, xd, cs, nfields, out, a, s); break;
+    case K_MQ: eval_ref_k<K_MQ>(g, px, ld, Np, cell, alive, w, pv, xd, cs, nfields, out, a, s); break;
+    case K_IMQ: eval_ref_k<K_IMQ>(g, px, ld, Np, cell, alive, w, pv, xd, cs, nfields, out, a, s); break;
+    case K_PHS3: eval_ref_k<K_PHS3>(g, px, ld, Np, cell, alive, w, pv, xd, cs, nfields, out, a, s); break;
+    default: eval_ref_k<K_GAUSS>(g, px, ld, Np, cell, alive, w, pv, xd, cs, nfields, out, a, s); break;
   }
 }
 

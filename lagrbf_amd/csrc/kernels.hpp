@@ -32,6 +32,30 @@ struct PatchView {
   int nf_total;
 };
 
+// Extensions of the patch systems (no reference counterpart): a polynomial block
+// ([A P; P^T 0], monomials of the member coordinates shifted to the patch's PU node
+// and scaled by ps) and a shape parameter per patch.  Every pointer may be null:
+// m == nullptr means no polynomial block, eps == nullptr the grid's eps_r.
+struct AugView {
+  const int32_t* m;        // [P] polynomial terms of each patch
+  const int32_t* deg;      // [P] their degree
+  const double* eps;       // [P] shape parameter of each patch
+  const int64_t* aux_off;  // [P] offset of the patch's null-space data in aux
+  double* aux;             // per patch: xs [3][n] (member - PU node), V [m][n] (Householder
+                           // vectors of P), X [m][n], B1 [m][n] (first m columns of Q^T A Q),
+                           // R [m][m], T [m][m] (Q = I - V T V^T)
+  double* beta;            // polynomial coefficients, patch p at beta + p * bstride, [m][nf]
+  int bstride;
+  int mmax;                // terms of the requested degree (the largest m)
+  double ps;               // polynomial scale: 1 / largest cell width
+};
+__host__ __device__ inline size_t aug_aux_size(int n, int m) {
+  return m > 0 ? (size_t)3 * n + (size_t)3 * m * n + (size_t)2 * m * m : 0;
+}
+// pivots of a Bunch-Kaufman factor (n + m of them: with a polynomial block the
+// member-indexed pools give each patch n + POLY_MAX slots)
+__host__ __device__ inline int64_t bk_piv_base(const AugView&, int, int64_t mem_off) { return mem_off; }
+
 // ---- binning ----
 void launch_locate_aos(const Grid& g, const double* x, int n, int32_t* cell, hipStream_t s);
 void launch_locate_particles(const Grid& g, double* px, int64_t ld, int Np, int32_t* cell, int32_t* alive,
@@ -44,33 +68,55 @@ void launch_patch_gather(const Grid& g, const int32_t* list, int nlist, const in
 void launch_touch(const Grid& g, const int32_t* cell, const int32_t* alive, int Np, uint8_t* touched, hipStream_t s);
 
 // ---- patch factor / solve ----
+// With a polynomial block the Bunch-Kaufman factor covers the (n+m) x (n+m)
+// saddle-point matrix; the solve writes c to coef and beta to a.beta.
 void launch_factor_bk(const Grid& g, int ktype, const int32_t* list, int nlist, const int32_t* pn,
                       const int64_t* mem_off, const int64_t* fac_off, const int32_t* mem_pool,
-                      const double* xdata, double* fac, int32_t* piv, int32_t* info, hipStream_t s);
+                      const double* xdata, double* fac, int32_t* piv, int32_t* info, const AugView& a, hipStream_t s);
 void launch_solve_bk(const int32_t* list, int nlist, const int32_t* pn, const int64_t* mem_off,
                      const int64_t* fac_off, const int32_t* mem_pool, const double* fac, const int32_t* piv,
-                     const double* values, int nf, double* coef, hipStream_t s);
+                     const double* values, int nf, double* coef, const AugView& a, hipStream_t s);
 
 // blocked Cholesky (positive definite kernels), FP64 or FP32 factor; the FP32
 // solve refines FP64 coefficients against the FP64 matrix (patch_chol.hip).
 // big: patches above chol_max_n() (no n-sized LDS); their solve needs a workspace
 // ws of 2 nf doubles per member, indexed like the member pool.
+// With a polynomial block (a.m) the factor is that of B22 = (Q^T A Q)[m:, m:], the
+// saddle-point matrix on the null space of P^T (k_aug_prep computes Q first); the
+// factor dimension is n - m.
 bool chol_kernel_supported(int ktype);
 int chol_max_n();
 void launch_factor_chol(const Grid& g, int ktype, bool fp32, const int32_t* list, int nlist, const int32_t* pn,
                         const int64_t* mem_off, const int64_t* fac_off, const int32_t* mem_pool, const double* xd,
-                        void* fac, int32_t* status, bool big, hipStream_t s);
+                        void* fac, int32_t* status, bool big, const AugView& a, hipStream_t s);
 void launch_solve_chol(const Grid& g, int ktype, bool fp32, const int32_t* list, int nlist, const int32_t* pn,
                        const int64_t* mem_off, const int64_t* fac_off, const int32_t* mem_pool, const double* xd,
                        const void* fac, const double* values, int nf, double* coef, int max_iter, double tol,
-                       int32_t* status, int32_t* iters, double* ws, int nmax /* largest listed patch */, hipStream_t s);
+                       int32_t* status, int32_t* iters, double* ws, int nmax /* largest listed patch */,
+                       const AugView& a, hipStream_t s);
+// (aug.hip) null-space data of the listed patches with a polynomial block
+void launch_aug_prep(const Grid& g, int ktype, const int32_t* list, int nlist, const int32_t* pn,
+                     const int64_t* mem_off, const int32_t* mem_pool, const double* xd, const AugView& a, hipStream_t s);
+// solve of the saddle-point systems: c and beta from the Cholesky factor of B22
+// (FP64 direct, or FP32 factor with FP64 refinement of the full saddle-point
+// residual); ws: (3 + 3 nf) doubles per member for patches larger than the LDS
+// arrays (nmax: largest listed patch)
+void launch_solve_aug(const Grid& g, int ktype, bool fp32, const int32_t* list, int nlist, const int32_t* pn,
+                      const int64_t* mem_off, const int64_t* fac_off, const int32_t* mem_pool, const double* xd,
+                      const void* fac, const double* values, int nf, double* coef, int max_iter, double tol,
+                      int32_t* status, int32_t* iters, double* ws, int nmax, const AugView& a, hipStream_t s);
+size_t aug_solve_ws_per_member(int nf);
+// polynomial table of the evaluation: [P][mm][nft], beta of fields f0.. of the
+// active patches, zero elsewhere (mm = terms of the requested degree)
+void launch_build_ptab(const Grid& g, const uint8_t* active, const AugView& a, int mm, int nft, int nf, int f0,
+                       double* ptab, hipStream_t s);
 
 // ---- weights / evaluation ----
 void launch_weights(const Grid& g, const double* pux, const double* puy, const double* puz, const double* px,
                     int64_t ld, int Np, const int32_t* cell, const int32_t* alive, double* w, hipStream_t s);
 void launch_eval_ref(const Grid& g, int ktype, const double* px, int64_t ld, int Np, const int32_t* cell,
                      const int32_t* alive, const double* w, const PatchView& pv, const double* xdata,
-                     const int32_t* cell_start, int nfields, double* out, hipStream_t s);
+                     const int32_t* cell_start, int nfields, double* out, const AugView& a, hipStream_t s);
 
 // cell-ordered evaluation: particles grouped by cell (pstart[c]..pstart[c+1]), dead
 // particles in the tail bucket C, so perm covers the whole swarm
@@ -114,10 +160,20 @@ void launch_build_blk(const Grid& g, bool fp32, const double* x, int N, const in
 // variant performs the same arithmetic per particle.
 // Events the next launch_eval_blk dispatch records its start and end in (nullptr: none).
 void set_eval_timing_events(hipEvent_t start, hipEvent_t stop);
+// Extensions: ev.ptab (polynomial table, launch_build_ptab) adds each corner
+// patch's polynomial part; ev.peps with ktype K_GAUSS_LOCAL gives the Gaussian
+// the shape parameter of each corner patch.
+struct EvalExt {
+  const double* ptab = nullptr;
+  int mm = 0;              // polynomial terms per patch in ptab
+  int deg = -1;            // their degree
+  double ps = 1.0;         // polynomial scale
+  const double* peps = nullptr;
+};
 void launch_eval_blk(const Grid& g, int ktype, bool fp32, const double* px, int64_t ld, const int32_t* perm,
                      const int32_t* pstart, const double* w, const void* blk, int nft, const int32_t* cs, int nfields,
                      double* out, int block, int np, const int32_t* cell_list, const int32_t* list_count, int group,
-                     int pipe, int waves, hipStream_t s);
+                     int pipe, int waves, const EvalExt& ev, hipStream_t s);
 // ascending non-empty cells; list needs nonempty_list_capacity(C) entries (the tail is scratch)
 void launch_nonempty_cells(const int32_t* pstart, int C, int32_t* list, int32_t* count, hipStream_t s);
 size_t nonempty_list_capacity(int C);

@@ -14,6 +14,7 @@
 #include "kernels.hpp"
 
 #include <algorithm>
+#include <type_traits>
 
 #include <cmath>
 
@@ -33,51 +34,89 @@ __device__ __forceinline__ double patch_centre(const Grid& g, int p, int d) {
   return g.O[d] + idx * g.h[d];
 }
 
-template <typename T, int KT>
+// Saddle-point systems (MM > 0, extension): the factored matrix is B22 of
+// Q^T A Q = A - X V^T - V X^T (V, X from k_aug_prep), restricted to members m..n-1;
+// its entries are assembled in FP64 (phi minus the rank-2m correction) before any
+// rounding to T.  AT is the assembly type.
+template <int MM, typename T>
+using assembly_t = typename std::conditional<(MM > 0), double, T>::type;
+template <int MM, typename T>
+__host__ __device__ constexpr int chol_kchunk() { return (MM > 4 && sizeof(T) == 8) ? 32 : CTT; }
+
+template <typename T, int KT, int MM>
 __global__ __launch_bounds__(CNT) void k_factor_chol(Grid g, const int32_t* __restrict__ list,
                                                      const int32_t* __restrict__ pn, const int64_t* __restrict__ mem_off,
                                                      const int64_t* __restrict__ fac_off,
                                                      const int32_t* __restrict__ mem_pool,
                                                      const double* __restrict__ xd, T* __restrict__ fac,
-                                                     int32_t* __restrict__ status) {
+                                                     int32_t* __restrict__ status, AugView aug) {
+  using AT = assembly_t<MM, T>;
+  constexpr int TT = chol_kchunk<MM, T>();
+  constexpr int MR = MM > 0 ? MM : 1;
   extern __shared__ __align__(16) unsigned char smem[];
   const int p = list[blockIdx.x];
-  const int n = pn[p];
-  if (n == 0) { if (threadIdx.x == 0) status[p] = 0; return; }
-  T* xr = reinterpret_cast<T*>(smem);                 // [3][CNMAX] member coords relative to the patch node
-  T* P = xr + 3 * CNMAX;                              // panel [rows][CNB] column-major, ld = rows
-  T* Lr = P + (size_t)CNMAX * CNB;                    // [CNB][CTT] rows J..J+w of L, columns t0..t0+CTT
+  const int n0 = pn[p];
+  const int m = MM > 0 ? aug.m[p] : 0;
+  const int n = n0 - m;                               // factor dimension
+  if (n <= 0) { if (threadIdx.x == 0) status[p] = 0; return; }
+  AT* xr = reinterpret_cast<AT*>(smem);               // [3][CNMAX] member coords relative to the patch node
+  T* P = reinterpret_cast<T*>(xr + 3 * CNMAX);        // panel [rows][CNB] column-major, ld = rows
+  T* Lr = P + (size_t)CNMAX * CNB;                    // [CNB][TT] rows J..J+w of L, columns t0..t0+TT
+  double* cv = reinterpret_cast<double*>(Lr + CNB * TT);   // [2][MM][CNB] V and X rows of the panel columns
   __shared__ int s_fail;
   const int tid = threadIdx.x;
   const int dim = g.dim;
-  const int32_t* mem = mem_pool + mem_off[p];
+  const int32_t* mem = mem_pool + mem_off[p] + m;     // factor row i = member m + i
   T* L = fac + fac_off[p];
+  const double* V = MM > 0 ? aug.aux + aug.aux_off[p] + 3 * (size_t)n0 + m : nullptr;   // V[q * n0 + i]
+  const double* X = MM > 0 ? V + (size_t)m * n0 : nullptr;
   if (tid == 0) s_fail = 0;
   for (int i = tid; i < n; i += CNT) {
     const double* x = xd + (size_t)mem[i] * dim;
-    for (int d = 0; d < 3; d++) xr[d * CNMAX + i] = d < dim ? (T)(x[d] - patch_centre(g, p, d)) : T(0);
+    for (int d = 0; d < 3; d++) xr[d * CNMAX + i] = d < dim ? (AT)(x[d] - patch_centre(g, p, d)) : AT(0);
   }
   __syncthreads();
-  const T eps = (T)g.eps_r;
+  const AT eps = (AT)(aug.eps ? aug.eps[p] : g.eps_r);
   for (int J = 0; J < n; J += CNB) {
     const int w = min(CNB, n - J), rows = n - J;
+    if constexpr (MM > 0) {
+      __syncthreads();
+      for (int e = tid; e < 2 * MM * CNB; e += CNT) {
+        const int which = e / (MM * CNB), q = (e / CNB) % MM, c = e % CNB;
+        cv[e] = (q < m && c < w) ? (which ? X : V)[(size_t)q * n0 + J + c] : 0.0;
+      }
+      __syncthreads();
+    }
     for (int r0 = 0; r0 < rows; r0 += CNT) {
       const int r = r0 + tid;
       const bool live = r < rows;
       const int gi = J + (live ? r : 0);
       T acc[CNB];
-      const T xi0 = xr[gi], xi1 = xr[CNMAX + gi], xi2 = xr[2 * CNMAX + gi];
+      const AT xi0 = xr[gi], xi1 = xr[CNMAX + gi], xi2 = xr[2 * CNMAX + gi];
+      double rv[MR], rx[MR];
+      if constexpr (MM > 0) {
+#pragma unroll
+        for (int q = 0; q < MM; q++) {
+          rv[q] = q < m ? V[(size_t)q * n0 + gi] : 0.0;
+          rx[q] = q < m ? X[(size_t)q * n0 + gi] : 0.0;
+        }
+      }
 #pragma unroll
       for (int c = 0; c < CNB; c++) {
         const int gj = J + (c < w ? c : 0);
-        const T d0 = xi0 - xr[gj], d1 = xi1 - xr[CNMAX + gj], d2 = xi2 - xr[2 * CNMAX + gj];
-        acc[c] = rbf<KT, T>(sqrt_fast(d0 * d0 + d1 * d1 + d2 * d2), eps);
+        const AT d0 = xi0 - xr[gj], d1 = xi1 - xr[CNMAX + gj], d2 = xi2 - xr[2 * CNMAX + gj];
+        AT a = rbf<KT, AT>(sqrt_fast(d0 * d0 + d1 * d1 + d2 * d2), eps);
+        if constexpr (MM > 0) {
+#pragma unroll
+          for (int q = 0; q < MM; q++) a -= rx[q] * cv[q * CNB + c] + rv[q] * cv[(MM + q) * CNB + c];
+        }
+        acc[c] = (T)a;
       }
-      for (int t0 = 0; t0 < J; t0 += CTT) {
-        const int tw = min(CTT, J - t0);
+      for (int t0 = 0; t0 < J; t0 += TT) {
+        const int tw = min(TT, J - t0);
         __syncthreads();
-        for (int e = tid; e < CNB * CTT; e += CNT) {
-          const int c = e / CTT, tt = e % CTT;
+        for (int e = tid; e < CNB * TT; e += CNT) {
+          const int c = e / TT, tt = e % TT;
           Lr[e] = (c < w && tt < tw) ? L[(size_t)(t0 + tt) * n + J + c] : T(0);
         }
         __syncthreads();
@@ -85,7 +124,7 @@ __global__ __launch_bounds__(CNT) void k_factor_chol(Grid g, const int32_t* __re
           for (int tt = 0; tt < tw; tt++) {
             const T lit = L[(size_t)(t0 + tt) * n + gi];
 #pragma unroll
-            for (int c = 0; c < CNB; c++) acc[c] -= lit * Lr[c * CTT + tt];
+            for (int c = 0; c < CNB; c++) acc[c] -= lit * Lr[c * TT + tt];
           }
         }
       }
@@ -127,7 +166,8 @@ __global__ __launch_bounds__(NT) void k_solve_chol(Grid g, const int32_t* __rest
                                                     const int32_t* __restrict__ mem_pool, const double* __restrict__ xd,
                                                     const TL* __restrict__ fac, const double* __restrict__ values, int nf,
                                                     double* __restrict__ coef, int max_iter, double tol,
-                                                    int32_t* __restrict__ status, int32_t* __restrict__ iters, int S) {
+                                                    int32_t* __restrict__ status, int32_t* __restrict__ iters, int S,
+                                                    const double* __restrict__ peps) {
   // S: row stride of the LDS arrays (the largest patch of the launch)
   extern __shared__ __align__(16) unsigned char smem[];
   double* xr = reinterpret_cast<double*>(smem);      // [3][S] relative coords (FP64)
@@ -149,6 +189,7 @@ __global__ __launch_bounds__(NT) void k_solve_chol(Grid g, const int32_t* __rest
   const int dim = g.dim;
   const int32_t* mem = mem_pool + mem_off[p];
   const TL* L = fac + fac_off[p];
+  const double eps_p = peps ? peps[p] : g.eps_r;
   double* X = coef + mem_off[p] * nf;                // row-major [n][nf]
   for (int i = tid; i < n; i += NT) {
     const double* x = xd + (size_t)mem[i] * dim;
@@ -169,7 +210,7 @@ __global__ __launch_bounds__(NT) void k_solve_chol(Grid g, const int32_t* __rest
         const double xi0 = xr[i], xi1 = xr[S + i], xi2 = xr[2 * S + i];
         for (int j = 0; j < n; j++) {
           const double d0 = xi0 - xr[j], d1 = xi1 - xr[S + j], d2 = xi2 - xr[2 * S + j];
-          const double a = rbf<KT, double>(sqrt_fast(d0 * d0 + d1 * d1 + d2 * d2), g.eps_r);
+          const double a = rbf<KT, double>(sqrt_fast(d0 * d0 + d1 * d1 + d2 * d2), eps_p);
           FOR_F s[f] -= a * xl[f * S + j];
         }
         FOR_F b[f * S + i] = s[f];
@@ -291,34 +332,47 @@ __global__ __launch_bounds__(NT) void k_solve_chol(Grid g, const int32_t* __rest
 // one row per thread) is assembled, updated and solved against the diagonal block
 // (l_r = a_r L_JJ^-T, forward substitution).  Member coordinates are gathered
 // from the node array and shifted to the patch node in FP64.
-template <typename T, int KT>
+template <typename T, int KT, int MM>
 __global__ __launch_bounds__(CNT) void k_factor_chol_big(Grid g, const int32_t* __restrict__ list,
                                                          const int32_t* __restrict__ pn,
                                                          const int64_t* __restrict__ mem_off,
                                                          const int64_t* __restrict__ fac_off,
                                                          const int32_t* __restrict__ mem_pool,
                                                          const double* __restrict__ xd, T* __restrict__ fac,
-                                                         int32_t* __restrict__ status) {
+                                                         int32_t* __restrict__ status, AugView aug) {
+  using AT = assembly_t<MM, T>;
+  constexpr int MR = MM > 0 ? MM : 1;
   __shared__ T Lr[CNB * CTT];
   __shared__ T D[CNB * CNB];
-  __shared__ T colx[3][CNB];
+  __shared__ AT colx[3][CNB];
+  __shared__ double cv[2 * MR * CNB];
   __shared__ int s_fail;
   const int p = list[blockIdx.x];
-  const int n = pn[p];
+  const int n0 = pn[p];
+  const int m = MM > 0 ? aug.m[p] : 0;
+  const int n = n0 - m;                               // factor dimension
   const int tid = threadIdx.x, dim = g.dim;
-  if (n == 0) { if (tid == 0) status[p] = 0; return; }
-  const int32_t* mem = mem_pool + mem_off[p];
+  if (n <= 0) { if (tid == 0) status[p] = 0; return; }
+  const int32_t* mem = mem_pool + mem_off[p] + m;     // factor row i = member m + i
   T* L = fac + fac_off[p];
+  const double* V = MM > 0 ? aug.aux + aug.aux_off[p] + 3 * (size_t)n0 + m : nullptr;   // V[q * n0 + i]
+  const double* X = MM > 0 ? V + (size_t)m * n0 : nullptr;
   const double cx = patch_centre(g, p, 0), cy = patch_centre(g, p, 1), cz = dim == 3 ? patch_centre(g, p, 2) : 0.0;
   if (tid == 0) s_fail = 0;
-  const T eps = (T)g.eps_r;
+  const AT eps = (AT)(aug.eps ? aug.eps[p] : g.eps_r);
   for (int J = 0; J < n; J += CNB) {
     const int w = min(CNB, n - J), rows = n - J;
     __syncthreads();
     if (tid < CNB) {
       const int gj = J + (tid < w ? tid : 0);
       const double* x = xd + (size_t)mem[gj] * dim;
-      colx[0][tid] = (T)(x[0] - cx); colx[1][tid] = (T)(x[1] - cy); colx[2][tid] = dim == 3 ? (T)(x[2] - cz) : T(0);
+      colx[0][tid] = (AT)(x[0] - cx); colx[1][tid] = (AT)(x[1] - cy); colx[2][tid] = dim == 3 ? (AT)(x[2] - cz) : AT(0);
+    }
+    if constexpr (MM > 0) {
+      for (int e = tid; e < 2 * MM * CNB; e += CNT) {
+        const int which = e / (MM * CNB), q = (e / CNB) % MM, c = e % CNB;
+        cv[e] = (q < m && c < w) ? (which ? X : V)[(size_t)q * n0 + J + c] : 0.0;
+      }
     }
     __syncthreads();
     for (int r0 = 0; r0 < rows; r0 += CNT) {
@@ -326,12 +380,25 @@ __global__ __launch_bounds__(CNT) void k_factor_chol_big(Grid g, const int32_t* 
       const bool live = r < rows;
       const int gi = J + (live ? r : 0);
       const double* x = xd + (size_t)mem[gi] * dim;
-      const T xi0 = (T)(x[0] - cx), xi1 = (T)(x[1] - cy), xi2 = dim == 3 ? (T)(x[2] - cz) : T(0);
+      const AT xi0 = (AT)(x[0] - cx), xi1 = (AT)(x[1] - cy), xi2 = dim == 3 ? (AT)(x[2] - cz) : AT(0);
+      double rv[MR], rx[MR];
+      if constexpr (MM > 0) {
+#pragma unroll
+        for (int q = 0; q < MM; q++) {
+          rv[q] = q < m ? V[(size_t)q * n0 + gi] : 0.0;
+          rx[q] = q < m ? X[(size_t)q * n0 + gi] : 0.0;
+        }
+      }
       T acc[CNB];
 #pragma unroll
       for (int c = 0; c < CNB; c++) {
-        const T d0 = xi0 - colx[0][c], d1 = xi1 - colx[1][c], d2 = xi2 - colx[2][c];
-        acc[c] = rbf<KT, T>(sqrt_fast(d0 * d0 + d1 * d1 + d2 * d2), eps);
+        const AT d0 = xi0 - colx[0][c], d1 = xi1 - colx[1][c], d2 = xi2 - colx[2][c];
+        AT a = rbf<KT, AT>(sqrt_fast(d0 * d0 + d1 * d1 + d2 * d2), eps);
+        if constexpr (MM > 0) {
+#pragma unroll
+          for (int q = 0; q < MM; q++) a -= rx[q] * cv[q * CNB + c] + rv[q] * cv[(MM + q) * CNB + c];
+        }
+        acc[c] = (T)a;
       }
       for (int t0 = 0; t0 < J; t0 += CTT) {
         const int tw = min(CTT, J - t0);
@@ -405,7 +472,8 @@ __global__ __launch_bounds__(CNT) void k_solve_chol_big(Grid g, const int32_t* _
                                                         const double* __restrict__ xd, const TL* __restrict__ fac,
                                                         const double* __restrict__ values, int nf, double* __restrict__ coef,
                                                         int max_iter, double tol, double* __restrict__ ws,
-                                                        int32_t* __restrict__ status, int32_t* __restrict__ iters) {
+                                                        int32_t* __restrict__ status, int32_t* __restrict__ iters,
+                                                        const double* __restrict__ peps) {
   __shared__ double red[CNT / CW][8];
   __shared__ double zk_s[8];
   const int p = list[blockIdx.x];
@@ -435,7 +503,7 @@ __global__ __launch_bounds__(CNT) void k_solve_chol_big(Grid g, const int32_t* _
         for (int j = 0; j < n; j++) {
           const double* xj = xd + (size_t)mem[j] * dim;
           const double d0 = xi0 - (xj[0] - cx), d1 = xi1 - (xj[1] - cy), d2 = dim == 3 ? xi2 - (xj[2] - cz) : 0.0;
-          const double a = rbf<KT, double>(sqrt_fast(d0 * d0 + d1 * d1 + d2 * d2), g.eps_r);
+          const double a = rbf<KT, double>(sqrt_fast(d0 * d0 + d1 * d1 + d2 * d2), peps ? peps[p] : g.eps_r);
           FOR_F sres[f] -= a * X[(size_t)j * nf + f];
         }
         FOR_F b[(size_t)f * n + i] = sres[f];
@@ -505,7 +573,9 @@ __global__ __launch_bounds__(CNT) void k_solve_chol_big(Grid g, const int32_t* _
   }
 }
 
-static size_t factor_smem(size_t esz) { return esz * (3 * CNMAX + (size_t)CNMAX * CNB + CNB * CTT); }
+static size_t factor_smem(size_t esz, size_t asz, int mm, int tt) {
+  return asz * 3 * CNMAX + esz * ((size_t)CNMAX * CNB + (size_t)CNB * tt) + sizeof(double) * 2 * (size_t)mm * CNB;
+}
 // LDS of the solve: coordinates [3][S]; right-hand side, work vector, solution [nf][S]
 static size_t solve_smem(int S, int nf) { return sizeof(double) * (size_t)S * (3 + 3 * (size_t)nf); }
 static constexpr int SNT = 256;  // threads per patch in the solve (measured against one wave:
@@ -513,29 +583,33 @@ static constexpr int SNT = 256;  // threads per patch in the solve (measured aga
 
 int chol_max_n() { return CNMAX; }
 
-template <typename T, int KT>
+template <typename T, int KT, int MM>
 static void factor_chol_k(const Grid& g, const int32_t* list, int nlist, const int32_t* pn, const int64_t* mem_off,
                           const int64_t* fac_off, const int32_t* mem_pool, const double* xd, T* fac, int32_t* status,
-                          bool big, hipStream_t s) {
+                          bool big, const AugView& a, hipStream_t s) {
   if (big) {
-    hipLaunchKernelGGL((k_factor_chol_big<T, KT>), dim3(nlist), dim3(CNT), 0, s, g, list, pn, mem_off, fac_off, mem_pool,
-                       xd, fac, status);
+    hipLaunchKernelGGL((k_factor_chol_big<T, KT, MM>), dim3(nlist), dim3(CNT), 0, s, g, list, pn, mem_off, fac_off,
+                       mem_pool, xd, fac, status, a);
     return;
   }
   static bool attr = false;
-  const size_t sm = factor_smem(sizeof(T));
-  if (!attr) { (void)hipFuncSetAttribute((const void*)k_factor_chol<T, KT>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm); attr = true; }
-  hipLaunchKernelGGL((k_factor_chol<T, KT>), dim3(nlist), dim3(CNT), sm, s, g, list, pn, mem_off, fac_off, mem_pool, xd, fac, status);
+  const size_t sm = factor_smem(sizeof(T), sizeof(assembly_t<MM, T>), MM, chol_kchunk<MM, T>());
+  if (!attr) {
+    (void)hipFuncSetAttribute((const void*)k_factor_chol<T, KT, MM>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)sm);
+    attr = true;
+  }
+  hipLaunchKernelGGL((k_factor_chol<T, KT, MM>), dim3(nlist), dim3(CNT), sm, s, g, list, pn, mem_off, fac_off, mem_pool,
+                     xd, fac, status, a);
 }
 
 template <typename TL, int KT>
 static void solve_chol_k(const Grid& g, const int32_t* list, int nlist, const int32_t* pn, const int64_t* mem_off,
                          const int64_t* fac_off, const int32_t* mem_pool, const double* xd, const TL* fac,
                          const double* values, int nf, double* coef, int max_iter, double tol, int32_t* status,
-                         int32_t* iters, double* ws, int nmax, hipStream_t s) {
+                         int32_t* iters, double* ws, int nmax, const double* peps, hipStream_t s) {
   if (ws) {
     hipLaunchKernelGGL((k_solve_chol_big<TL, KT>), dim3(nlist), dim3(CNT), 0, s, g, list, pn, mem_off, fac_off, mem_pool,
-                       xd, fac, values, nf, coef, max_iter, tol, ws, status, iters);
+                       xd, fac, values, nf, coef, max_iter, tol, ws, status, iters, peps);
     return;
   }
   static bool attr = false;
@@ -547,51 +621,68 @@ static void solve_chol_k(const Grid& g, const int32_t* list, int nlist, const in
     attr = true;
   }
   hipLaunchKernelGGL((k_solve_chol<TL, KT, SNT>), dim3(nlist), dim3(SNT), sm, s, g, list, pn, mem_off, fac_off, mem_pool,
-                     xd, fac, values, nf, coef, max_iter, tol, status, iters, S);
+                     xd, fac, values, nf, coef, max_iter, tol, status, iters, S, peps);
 }
 
+// positive definite kernels; r^3 only on the null space of a polynomial block
 bool chol_kernel_supported(int ktype) { return ktype == K_WC0 || ktype == K_WC2 || ktype == K_GAUSS || ktype == K_IMQ; }
+
+template <typename T, int MM>
+static void factor_chol_mm(const Grid& g, int ktype, const int32_t* list, int nlist, const int32_t* pn,
+                           const int64_t* mem_off, const int64_t* fac_off, const int32_t* mem_pool, const double* xd,
+                           T* fac, int32_t* status, bool big, const AugView& a, hipStream_t s) {
+#define LRB_FC(K) factor_chol_k<T, K, MM>(g, list, nlist, pn, mem_off, fac_off, mem_pool, xd, fac, status, big, a, s)
+  switch (ktype) {
+    case K_WC0: LRB_FC(K_WC0); break;
+    case K_WC2: LRB_FC(K_WC2); break;
+    case K_IMQ: LRB_FC(K_IMQ); break;
+    case K_PHS3:
+      if constexpr (MM > 0) LRB_FC(K_PHS3);
+      break;
+    default: LRB_FC(K_GAUSS); break;
+  }
+#undef LRB_FC
+}
 
 template <typename T>
 static void factor_chol_dispatch(const Grid& g, int ktype, const int32_t* list, int nlist, const int32_t* pn,
                                  const int64_t* mem_off, const int64_t* fac_off, const int32_t* mem_pool, const double* xd,
-                                 T* fac, int32_t* status, bool big, hipStream_t s) {
-  switch (ktype) {
-    case K_WC0: factor_chol_k<T, K_WC0>(g, list, nlist, pn, mem_off, fac_off, mem_pool, xd, fac, status, big, s); break;
-    case K_WC2: factor_chol_k<T, K_WC2>(g, list, nlist, pn, mem_off, fac_off, mem_pool, xd, fac, status, big, s); break;
-    case K_IMQ: factor_chol_k<T, K_IMQ>(g, list, nlist, pn, mem_off, fac_off, mem_pool, xd, fac, status, big, s); break;
-    default: factor_chol_k<T, K_GAUSS>(g, list, nlist, pn, mem_off, fac_off, mem_pool, xd, fac, status, big, s); break;
-  }
+                                 T* fac, int32_t* status, bool big, const AugView& a, hipStream_t s) {
+  if (!a.m) factor_chol_mm<T, 0>(g, ktype, list, nlist, pn, mem_off, fac_off, mem_pool, xd, fac, status, big, a, s);
+  else if (a.mmax <= 4) factor_chol_mm<T, 4>(g, ktype, list, nlist, pn, mem_off, fac_off, mem_pool, xd, fac, status, big, a, s);
+  else factor_chol_mm<T, POLY_MAX>(g, ktype, list, nlist, pn, mem_off, fac_off, mem_pool, xd, fac, status, big, a, s);
 }
 
 void launch_factor_chol(const Grid& g, int ktype, bool fp32, const int32_t* list, int nlist, const int32_t* pn,
                         const int64_t* mem_off, const int64_t* fac_off, const int32_t* mem_pool, const double* xd,
-                        void* fac, int32_t* status, bool big, hipStream_t s) {
+                        void* fac, int32_t* status, bool big, const AugView& a, hipStream_t s) {
   if (nlist <= 0) return;
-  if (fp32) factor_chol_dispatch<float>(g, ktype, list, nlist, pn, mem_off, fac_off, mem_pool, xd, (float*)fac, status, big, s);
-  else factor_chol_dispatch<double>(g, ktype, list, nlist, pn, mem_off, fac_off, mem_pool, xd, (double*)fac, status, big, s);
+  if (fp32) factor_chol_dispatch<float>(g, ktype, list, nlist, pn, mem_off, fac_off, mem_pool, xd, (float*)fac, status, big, a, s);
+  else factor_chol_dispatch<double>(g, ktype, list, nlist, pn, mem_off, fac_off, mem_pool, xd, (double*)fac, status, big, a, s);
 }
 
 template <typename TL>
 static void solve_chol_dispatch(const Grid& g, int ktype, const int32_t* list, int nlist, const int32_t* pn,
                                 const int64_t* mem_off, const int64_t* fac_off, const int32_t* mem_pool, const double* xd,
                                 const TL* fac, const double* values, int nf, double* coef, int max_iter, double tol,
-                                int32_t* status, int32_t* iters, double* ws, int nmax, hipStream_t s) {
+                                int32_t* status, int32_t* iters, double* ws, int nmax, const double* peps, hipStream_t s) {
+#define LRB_SC(K) solve_chol_k<TL, K>(g, list, nlist, pn, mem_off, fac_off, mem_pool, xd, fac, values, nf, coef, max_iter, tol, status, iters, ws, nmax, peps, s)
   switch (ktype) {
-    case K_WC0: solve_chol_k<TL, K_WC0>(g, list, nlist, pn, mem_off, fac_off, mem_pool, xd, fac, values, nf, coef, max_iter, tol, status, iters, ws, nmax, s); break;
-    case K_WC2: solve_chol_k<TL, K_WC2>(g, list, nlist, pn, mem_off, fac_off, mem_pool, xd, fac, values, nf, coef, max_iter, tol, status, iters, ws, nmax, s); break;
-    case K_IMQ: solve_chol_k<TL, K_IMQ>(g, list, nlist, pn, mem_off, fac_off, mem_pool, xd, fac, values, nf, coef, max_iter, tol, status, iters, ws, nmax, s); break;
-    default: solve_chol_k<TL, K_GAUSS>(g, list, nlist, pn, mem_off, fac_off, mem_pool, xd, fac, values, nf, coef, max_iter, tol, status, iters, ws, nmax, s); break;
+    case K_WC0: LRB_SC(K_WC0); break;
+    case K_WC2: LRB_SC(K_WC2); break;
+    case K_IMQ: LRB_SC(K_IMQ); break;
+    default: LRB_SC(K_GAUSS); break;
   }
+#undef LRB_SC
 }
 
 void launch_solve_chol(const Grid& g, int ktype, bool fp32, const int32_t* list, int nlist, const int32_t* pn,
                        const int64_t* mem_off, const int64_t* fac_off, const int32_t* mem_pool, const double* xd,
                        const void* fac, const double* values, int nf, double* coef, int max_iter, double tol,
-                       int32_t* status, int32_t* iters, double* ws, int nmax, hipStream_t s) {
+                       int32_t* status, int32_t* iters, double* ws, int nmax, const AugView& a, hipStream_t s) {
   if (nlist <= 0) return;
-  if (fp32) solve_chol_dispatch<float>(g, ktype, list, nlist, pn, mem_off, fac_off, mem_pool, xd, (const float*)fac, values, nf, coef, max_iter, tol, status, iters, ws, nmax, s);
-  else solve_chol_dispatch<double>(g, ktype, list, nlist, pn, mem_off, fac_off, mem_pool, xd, (const double*)fac, values, nf, coef, max_iter, tol, status, iters, ws, nmax, s);
+  if (fp32) solve_chol_dispatch<float>(g, ktype, list, nlist, pn, mem_off, fac_off, mem_pool, xd, (const float*)fac, values, nf, coef, max_iter, tol, status, iters, ws, nmax, a.eps, s);
+  else solve_chol_dispatch<double>(g, ktype, list, nlist, pn, mem_off, fac_off, mem_pool, xd, (const double*)fac, values, nf, coef, max_iter, tol, status, iters, ws, nmax, a.eps, s);
 }
 
 }  // namespace lrb

@@ -202,16 +202,17 @@ static constexpr int SERVE_TUNE_MIN_DEFAULT = 1 << 22;
 // particle density (the first batch of a density times every shape).
 void eval_batch(Batch& b, double* out) {
   Ctx& d = *b.ds;
-  const bool f32 = d.precision && eval_f32_supported(d.ktype);
+  const bool f32 = eval_f32(d);
+  const EvalExt ev = eval_ext(d, -1);
   const int per_cell = (b.Np + d.g.C - 1) / std::max(d.g.C, 1);
   const int block = std::min(256, std::max(64, (per_cell + 63) / 64 * 64));
   auto launch = [&](int v) {
     bool listed;
     int grp, pipe, waves;
     eval_variant(v, listed, grp, pipe, waves);
-    launch_eval_blk(d.g, d.ktype, f32, b.px.p, b.ld, b.perm.p, b.pstart.p, b.w.p,
+    launch_eval_blk(d.g, eval_ktype(d), f32, b.px.p, b.ld, b.perm.p, b.pstart.p, b.w.p,
                     f32 ? (const void*)d.blk32.p : (const void*)d.blk64.p, d.nf, d.cell_start.p, d.g.dim, out, block,
-                    b.Np, listed ? b.cell_list.p : nullptr, listed ? b.list_count.p : nullptr, grp, pipe, waves, b.s);
+                    b.Np, listed ? b.cell_list.p : nullptr, listed ? b.list_count.p : nullptr, grp, pipe, waves, ev, b.s);
   };
   // the batch is stored in cell order at every tick
   const int bucket = density_bucket(b.Np, d.g.C);
@@ -482,7 +483,7 @@ int32_t lagrbf_serve_add_dataset(void* srv, const char* options, int32_t n, doub
   if (hipSetDevice(c->device) != hipSuccess) return -set_err(E_LIB, fn, "hipSetDevice failed");
   int e = fill_data(*c, n, coords, values, nullptr);
   if (!e) e = partitions_setup(*c, true);
-  if (!e) e = ensure_blk(*c, c->precision && eval_f32_supported(c->ktype));
+  if (!e) e = ensure_blk(*c, eval_f32(*c));
   if (e) return -e;
   release_factors(*c);
   sv->ds.push_back(std::move(own));

@@ -491,6 +491,10 @@ static void weights_point(const orc_ctx *ctx, const double *x, int32_t cell, dou
           for (k = 0; k < nen; k++) if (cn[k] == p) { w[k] = wt; break; }
         }
   }
+  if (ctx->cfg.weights_corners) {   /* extension: normalised over the used corners */
+    sum = 0.0;
+    for (k = 0; k < nen; k++) sum += w[k];
+  }
   for (k = 0; k < nen; k++) w[k] = sum > 0.0 ? w[k] / sum : 0.0;
 }
 

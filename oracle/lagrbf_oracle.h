@@ -76,6 +76,8 @@ typedef struct {
                              augmentation of every patch system ([A P; P^T 0]) */
   double local_eps;       /* -lagrbf_local_eps c: shape parameter of patch p is c / h_local(p),
                              h_local = (patch cells' volume / members)^(1/d); 0: global eps_r */
+  int    weights_corners; /* -lagrbf_weights corners: Shepard weights normalised over the cell's
+                             2^d corners instead of every PU node (swarm_interp.c:327-372) */
 } orc_config;
 
 typedef struct orc_ctx orc_ctx;

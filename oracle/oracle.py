@@ -38,7 +38,7 @@ class OrcConfig(ctypes.Structure):
         ("dopri_dt_fac_max", ctypes.c_double), ("dopri_atol", ctypes.c_double),
         ("dopri_rtol", ctypes.c_double),
         ("weights_brute", ctypes.c_int), ("nthreads", ctypes.c_int), ("setup_all", ctypes.c_int),
-        ("poly_degree", ctypes.c_int), ("local_eps", ctypes.c_double),
+        ("poly_degree", ctypes.c_int), ("local_eps", ctypes.c_double), ("weights_corners", ctypes.c_int),
     ]
 
 
@@ -159,6 +159,7 @@ def config_from_options(options):
     c.setup_all = int(_bool(o.get("oracle_setup_all", False)))
     c.poly_degree = int(o.get("lagrbf_poly_degree", -1))
     c.local_eps = float(o.get("lagrbf_local_eps", 0.0))
+    c.weights_corners = int(str(o.get("lagrbf_weights", "all")) == "corners")
     return c
 
 

@@ -140,3 +140,21 @@ def test_rotation_field_reproduced_c1(oracle_mod):
     assert np.abs(w - 1).max() < 1e-14
     ex = np.stack([-(xs[:, 1] - 0.5), xs[:, 0] - 0.5], axis=1)
     assert np.abs(vals - ex).max() < 1e-13
+
+
+def test_corner_normalised_weights(oracle_mod):
+    """-lagrbf_weights corners: the used weights sum to 1 at C1's radius (1.5 cell
+    diagonals, where the reference's all-node normalisation leaves them below 1), so
+    the rotation field is reproduced exactly by r^3 + linear."""
+    opts, X, V, P = cases.rotation_2d(N=4096, Np=2000, n=17, kernel=7, advection=False)
+    opts.update(lagrbf_poly_degree=1)
+    o_all, _, _ = _ctx(oracle_mod, opts, X, V, P)
+    assert o_all.weights().sum(axis=1).max() < 1 - 1e-3
+    opts["lagrbf_weights"] = "corners"
+    o, vals, xs = _ctx(oracle_mod, opts, X, V, P)
+    w = o.weights()
+    assert np.abs(w.sum(axis=1) - 1).max() < 1e-15
+    assert np.allclose(w / w.sum(axis=1, keepdims=True), o_all.weights() / o_all.weights().sum(axis=1, keepdims=True),
+                       rtol=1e-13, atol=0)
+    ex = np.stack([-(xs[:, 1] - 0.5), xs[:, 0] - 0.5], axis=1)
+    assert np.abs(vals - ex).max() < 1e-13
