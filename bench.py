@@ -2,10 +2,13 @@
 
 Workload (the 3D plain-vs-mixed configuration): 1,048,576 data nodes (first 2^20
 of a seeded permutation of a 102^3 lattice jittered by +-0.3 spacing) sampling a
-divergence-free 16-mode Fourier velocity; PU grid 32^3 nodes on [0,1]^3, Wendland
-C2 kernel, overlap 1, shape factor 1; 4,194,304 particles uniform in the box;
-RK4 with dt = 1e-3.  A step is one RK4 step of the whole swarm (four PU
-evaluations, Shepard weights, re-location, lazy patch set-up check).
+divergence-free 16-mode Fourier velocity; PU grid 32^3 nodes on [0,1]^3, r^3
+kernel with a linear polynomial block (saddle-point patch systems), overlap 1,
+shape factor 1; 4,194,304 particles uniform in the box; RK4 with dt = 1e-3.  A
+step is one RK4 step of the whole swarm (four PU evaluations, Shepard weights,
+re-location, lazy patch set-up check).  The same workload with the reference's
+Wendland C2 kernel (no polynomial) is reported as a secondary key, and the 2D
+rotation configuration (r^3 + linear, 628 RK4 steps) with its closure errors.
 
 Both precision paths run on the same inputs, one after the other: FP32 factor +
 FP64 refinement with FP32 evaluation ("mixed", the headline `value`) and plain
@@ -15,8 +18,11 @@ first interpolation) and --warmup steps run before it.  The line also carries
 the mixed path's error against the FP64 path (patch coefficients, positions).
 
 Roofline of the dominant kernel (PU evaluation): algorithmic FLOPs per launch
-(per particle n_nb * 16 + sum_k 2 nf n_pk + 2^d 2 nf) over the kernel's average
-duration, timed with HIP events on the library's stream.  HBM traffic per
+(per particle n_nb * F_k + sum_k 2 nf n_pk + 2^d 2 nf + polynomial terms 2^d (2 nf m
++ m d); F_k = 10 for r^3 in 3D) over the kernel's average duration, timed with HIP
+events on the library's stream.  r^3 has no compact support, so every counted
+(particle, node) pair is computed; the Wendland key's count is nominal (the
+kernel skips node rows outside the support).  HBM traffic per
 launch comes from rocprofv3 FETCH_SIZE / WRITE_SIZE runs stored beside this
 file in profiles/pu_evaluation_c2_traffic.json (null when absent).
 """
@@ -32,19 +38,21 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK = {"fp64": 78.6e12, "mixed": 157.3e12}     # MI355X dense FP64 / FP32, FLOP/s
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pu_evaluation_c2_traffic.json")
-WORKLOAD = ("3D plain-vs-mixed: 2^20 jittered-lattice nodes, 16-mode Fourier field, 32^3 PU, Wendland C2, "
-            "4,194,304 particles, RK4 dt=1e-3")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pu_evaluation_c2_phs3_traffic.json")
+WORKLOAD = ("3D plain-vs-mixed: 2^20 jittered-lattice nodes, 16-mode Fourier field, 32^3 PU, r^3 kernel + linear "
+            "polynomial, 4,194,304 particles, RK4 dt=1e-3")
+FK = {7: 10, 2: 16}      # flops per (particle, node) pair, distance included: r^3, Wendland C2 (3D)
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def eval_flops(cell_start, nc, n_p, cellid, alive, nf=3, Fk=16):
-    """Algorithmic FLOPs of one PU evaluation over the swarm (Wendland C2, 3D):
-    per particle n_nb * F_k + sum_k 2 nf n_pk + 2^d 2 nf, with n_nb the population of
-    the 3x3x3 cell neighbourhood and n_pk the sizes of the 8 corner patches."""
+def eval_flops(cell_start, nc, n_p, cellid, alive, nf=3, Fk=10, m=4, d=3):
+    """Algorithmic FLOPs of one PU evaluation over the swarm (3D):
+    per particle n_nb * F_k + sum_k 2 nf n_pk + 2^d 2 nf + 2^d (2 nf m + m d), with
+    n_nb the population of the 3x3x3 cell neighbourhood, n_pk the sizes of the 8
+    corner patches and m the polynomial terms per patch (0 without a block)."""
     cnt = np.diff(cell_start).reshape(nc, nc, nc)          # [k][j][i]
     pad = np.pad(cnt, 1)
     nb = sum(pad[1 + dk:1 + dk + nc, 1 + dj:1 + dj + nc, 1 + di:1 + di + nc]
@@ -53,12 +61,15 @@ def eval_flops(cell_start, nc, n_p, cellid, alive, nf=3, Fk=16):
     npk = n_p.reshape(n, n, n)
     corner_sum = sum(npk[dk:dk + nc, dj:dj + nc, di:di + nc] for dk in (0, 1) for dj in (0, 1) for di in (0, 1)).ravel()
     c = cellid[alive.astype(bool)]
-    return float(np.sum(nb[c]) * Fk + np.sum(corner_sum[c]) * 2 * nf + c.size * 8 * 2 * nf)
+    return float(np.sum(nb[c]) * Fk + np.sum(corner_sum[c]) * 2 * nf + c.size * 8 * (2 * nf + 2 * nf * m + m * d))
 
 
-def build_workload(Np, nlat=102, N=1 << 20, n=32):
+def build_workload(Np, nlat=102, N=1 << 20, n=32, kernel=7):
+    """C2: r^3 + linear polynomial (kernel 7); kernel 2 gives the Wendland C2 variant."""
     from tests import cases
-    opts, X, V, P = cases.fourier_3d(nlat=nlat, N=N, Np=Np, n=n, kernel=2, scheme=2, solve_method=0, overlap=1.0)
+    opts, X, V, P = cases.fourier_3d(nlat=nlat, N=N, Np=Np, n=n, kernel=kernel, scheme=2, solve_method=0, overlap=1.0)
+    if kernel == 7:
+        opts["lagrbf_poly_degree"] = 1
     return opts, X, V, P
 
 
@@ -75,9 +86,9 @@ def patch_sizes(cs, nc):
     return pc.ravel()
 
 
-def load_traffic(precision):
+def load_traffic(precision, path=TRAFFIC_FILE):
     try:
-        with open(TRAFFIC_FILE) as f:
+        with open(path) as f:
             t = json.load(f)[precision]
         return float(t["fetch_bytes"]) + float(t["write_bytes"])
     except (OSError, KeyError, ValueError):
@@ -98,7 +109,7 @@ def copy_calibration(mib):
     del a, b
 
 
-def run_path(L, precision, opts, X, V, P, steps, warmup, dt, sample):
+def run_path(L, precision, opts, X, V, P, steps, warmup, dt, sample, traffic_file=TRAFFIC_FILE):
     """Set up one context, run warmup + timed steps; returns the result record,
     a sample of patch coefficients and the final positions (rows by pid)."""
     ctx = L.LagRBFCtx(petsc_options=dict(opts), precision=precision)
@@ -110,6 +121,7 @@ def run_path(L, precision, opts, X, V, P, steps, warmup, dt, sample):
     log(f"[{precision}] set-up + first interpolation: {t_setup:.2f}s, max patch {ctx.info('max_patch'):.0f}, "
         f"fallback {ctx.info('fallback'):.0f}, refine iters max {ctx.info('refine_iters'):.0f}")
     ctx.advect_steps_c(warmup, dt)
+    removed0 = ctx.info("removed")
     ev0, n_ev0 = ctx.info("eval_ms"), ctx.info("eval_count")
     t1 = time.perf_counter()
     ctx.advect_steps_c(steps, dt)
@@ -117,13 +129,18 @@ def run_path(L, precision, opts, X, V, P, steps, warmup, dt, sample):
     ev1, n_ev1 = ctx.info("eval_ms"), ctx.info("eval_count")
     wall = t2 - t1
     removed = ctx.info("removed")
+    # particle-steps of the live swarm: particles alive at the end count every step,
+    # the ones removed inside the timed region none (a lower bound)
+    live_steps = (P.shape[0] - removed) * steps
     log(f"[{precision}] timed {steps} steps: {wall:.3f}s ({wall / max(steps, 1) * 1e3:.2f} ms/step), removed {removed:.0f}")
 
     cs = ctx.debug_get("cell_start")
     alive = ctx.debug_get("alive")
     cellid = ctx.debug_get("cellid_points")
     nc = opts["pu_mesh_n"][0] - 1
-    flops = eval_flops(cs, nc, patch_sizes(cs, nc), cellid, alive)
+    kt = int(opts["kernel_type"])
+    m = 4 if int(opts.get("lagrbf_poly_degree", -1)) == 1 else 0
+    flops = eval_flops(cs, nc, patch_sizes(cs, nc), cellid, alive, Fk=FK[kt], m=m)
     # evaluations are timed once their launch shape is settled (the autotuner's
     # trials and run-off are not): with too few warmup steps none may be
     n_launch = int(n_ev1 - n_ev0)
@@ -133,23 +150,27 @@ def run_path(L, precision, opts, X, V, P, steps, warmup, dt, sample):
         achieved = flops / (avg_ms * 1e-3)
     else:
         log(f"[{precision}] no settled evaluation in the timed steps: raise --warmup for the roofline figures")
-    traffic = load_traffic(precision)
+    traffic = load_traffic(precision, traffic_file)
     coefs = {}
     for p in sample:
         c = ctx.debug_get("patch_coeffs", int(p), dtype=np.float64)
         if c.size:
-            coefs[int(p)] = c
+            # c and, with a polynomial block, beta
+            coefs[int(p)] = np.concatenate([c, ctx.debug_get("patch_beta", int(p), dtype=np.float64)])
     pos = np.zeros_like(P)
     ctx.output(coords=pos)
     rec = {
-        "value": P.shape[0] * steps / wall, "ms_per_step": wall / max(steps, 1) * 1e3,
-        "end_to_end": P.shape[0] * steps / (wall + t_setup), "setup_s": t_setup, "removed_particles": removed,
+        "value": live_steps / wall, "ms_per_step": wall / max(steps, 1) * 1e3,
+        "end_to_end": live_steps / (wall + t_setup), "setup_s": t_setup, "removed_particles": removed,
+        "removed_in_timed_steps": removed - removed0,
         "fallback_patches": ctx.info("fallback"), "refine_iters_max": ctx.info("refine_iters"),
         "roofline": {"bound": "valu", "kernel": "pu_evaluation (k_eval_blk)",
                      "achieved": achieved / 1e12 if achieved else None, "peak": PEAK[precision] / 1e12,
                      "unit": "TFLOP/s", "frac": achieved / PEAK[precision] if achieved else None, "traffic": traffic,
                      "flops_per_launch": flops, "avg_launch_ms": avg_ms, "launches_timed": n_launch,
-                     "launch_shape": int(ctx.info("eval_variant"))},
+                     "launch_shape": int(ctx.info("eval_variant")),
+                     "flops_note": "exact: every (particle, node) pair is computed" if kt == 7 else
+                     "nominal: node rows outside the compact support are skipped"},
     }
     ctx.destroy()
     return rec, coefs, pos
@@ -173,6 +194,43 @@ def run_graph(L, precision, opts, X, V, P, steps, warmup, dt):
         f"{rec['graph_steps']} replayed")
     ctx.destroy()
     return rec
+
+
+def c1_bench(L, precision, weights):
+    """2D rotation configuration as written: 4,096 uniform nodes, 16x16 PU cells with
+    weight radius 1.5 cell diagonals, r^3 kernel + linear polynomial, 10,000 particles
+    uniform in the disk r < 0.4 about (1/2, 1/2), RK4 dt = 0.01, 628 steps.
+    weights "all": the reference's Shepard normalisation over every PU node (the used
+    corner weights sum to less than 1 at this radius, swarm_interp.c:327-372);
+    "corners": normalised over the used corners.  Reports the return-to-start error
+    and the error against the exact rotation by 6.28 rad (628 x 0.01 < 2 pi)."""
+    from tests import cases
+    opts, X, V, P = cases.rotation_2d(N=4096, Np=10000, n=17, kernel=7, scheme=2)
+    opts.update(lagrbf_poly_degree=1, lagrbf_weights=weights)
+    ctx = L.LagRBFCtx(petsc_options=dict(opts), precision=precision)
+    ctx.fill_data(X, V)
+    ctx.fill_points(P)
+    ctx.interpolate_c()
+    t = time.perf_counter()
+    ctx.advect_steps_c(628, 0.01)
+    wall = time.perf_counter() - t
+    xs = np.zeros_like(P)
+    ctx.output(coords=xs)
+    removed = ctx.info("removed")
+    ctx.destroy()
+    th = 6.28
+    R = np.array([[np.cos(th), -np.sin(th)], [np.sin(th), np.cos(th)]])
+    exact = 0.5 + (P - 0.5) @ R.T
+    ret = np.linalg.norm(xs - P, axis=1)
+    rot = np.linalg.norm(xs - exact, axis=1)
+    res = {"precision": precision, "weights": weights, "wall_s": wall, "steps_per_s": 628 / wall,
+           "particle_rk4_steps_per_s": P.shape[0] * 628 / wall, "removed": removed,
+           "return_to_start_max": float(ret.max()), "return_to_start_mean": float(ret.mean()),
+           "vs_exact_rotation_max": float(rot.max()), "vs_exact_rotation_mean": float(rot.mean()),
+           "vs_exact_rotation_median": float(np.median(rot))}
+    log(f"c1 [{precision}, weights {weights}]: {wall:.2f}s, return-to-start max {ret.max():.4e} mean {ret.mean():.4e}, "
+        f"vs exact 6.28 rad max {rot.max():.4e} mean {rot.mean():.4e}")
+    return res
 
 
 def cpu_baseline(opts, X, V, P, steps, dt, nsample, threads):
@@ -375,6 +433,8 @@ def main():
     ap.add_argument("--c3-only", action="store_true", help="the uneven-batching configuration only")
     ap.add_argument("--no-c3", action="store_true", help="skip the uneven-batching measurement")
     ap.add_argument("--no-graph", action="store_true", help="skip the graph-replay measurement of the fixed-step loop")
+    ap.add_argument("--no-wendland", action="store_true", help="skip the Wendland C2 variant of the workload")
+    ap.add_argument("--no-c1", action="store_true", help="skip the 2D rotation configuration")
     ap.add_argument("--c4-only", action="store_true", help="the adaptive DP45 configuration only")
     ap.add_argument("--no-c4", action="store_true", help="skip the adaptive DP45 measurement")
     ap.add_argument("--c4-t-end", type=float, default=0.1, help="end time of the adaptive DP45 run")
@@ -423,6 +483,21 @@ def main():
             res[prec]["graph"] = run_graph(L, prec, opts, X, V, P, args.steps, args.warmup, dt)
     main_path = paths[0]
     r = res[main_path]
+    # the same workload with the reference's Wendland C2 kernel (no polynomial block)
+    wendland = None
+    if not args.no_wendland:
+        opts_w, _, _, _ = build_workload(1, kernel=2)
+        if args.eval_variant >= 0:
+            opts_w["lagrbf_eval_variant"] = args.eval_variant
+        wendland = {}
+        for prec in paths:
+            rw, _, _ = run_path(L, prec, opts_w, X, V, P, args.steps, args.warmup, dt, [],
+                                traffic_file=os.path.join(ROOT, "profiles", "pu_evaluation_c2_traffic.json"))
+            wendland[prec] = {k: rw[k] for k in ("value", "ms_per_step", "setup_s", "roofline", "removed_particles")}
+        wendland["config"] = WORKLOAD.replace("r^3 kernel + linear polynomial", "Wendland C2 (no polynomial)")
+    c1 = None
+    if not args.no_c1:
+        c1 = [c1_bench(L, prec, wn) for prec in paths for wn in ("all", "corners")]
     cmp = None
     if len(paths) == 2:
         common = [p for p in coefs["mixed"] if p in coefs["fp64"]]
@@ -483,12 +558,14 @@ def main():
         "warmup": args.warmup, "ms_per_step": r["ms_per_step"], "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32+f64" if main_path == "mixed" else "f64", "data": "synthetic",
         "config": {"workload": WORKLOAD, "nodes": n_nodes, "particles": n_particles, "pu_n": 32,
-                   "kernel": "wendland_c2", "precision": main_path},
+                   "kernel": "phs3+linear", "precision": main_path},
         "roofline": r["roofline"],
         "cpu_baseline": cpu,
         "setup_s": r["setup_s"], "end_to_end": r["end_to_end"], "removed_particles": r["removed_particles"],
         "paths": {p: {k: v for k, v in res[p].items()} for p in paths},
         "mixed_vs_fp64": cmp,
+        "wendland_c2": wendland,
+        "rotation_2d": c1,
         "serve": serve,
         "uneven_batching": c3,
         "adaptive_dp45": c4,
