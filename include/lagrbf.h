@@ -128,6 +128,23 @@ int64_t lagrbf_debug_get(void *ctx, const char *what, int32_t index, void *out, 
 /* Copy the last error message (NUL-terminated, truncated to len). */
 int lagrbf_last_error(char *buf, int32_t len);
 
+/* No counterpart in the reference: an entry point that allocates nothing, for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc).  The variant of
+ * lagrbf_fill_swarm(is_data = false) + lagrbf_interpolation + lagrbf_output
+ * ("fields") for points already in device memory: interpolates the context's
+ * fields at n points (coords: device [n * dim] AoS) into values (device
+ * [n * n_interpolation_fields] AoS, rows in point order; points outside the PU box
+ * and points with an empty 3^d neighbourhood give zero rows), with the same
+ * kernels and the same values bit for bit.  Every work array lives in a static
+ * device array of 40 MiB per GPU (held until the process ends): at most 262,144
+ * points per call, 262,144 PU cells and 4 fields (error 60 beyond).  The context
+ * must have every patch set up and solved and its evaluation table built
+ * (-lagrbf_setup_all, then one lagrbf_interpolation; error 73 otherwise); its own
+ * particle swarm is untouched.  Returns after the values are written.  Calls that
+ * overlap on one GPU share its scratch and race. */
+int lagrbf_interpolate_device_noalloc(void *ctx, int32_t n, const double *coords, double *values);
+
 /* ---------------- serving (no reference counterpart) ----------------
  * Continuous batching of advection requests against device-resident datasets.
  * A dataset is filled once and all its patches are set up and solved (factors

@@ -178,6 +178,15 @@ def register_functions(clib):
     return f
 
 
+def noalloc_fn():
+    """lagrbf_interpolate_device_noalloc (device pointers as integers)."""
+    cfuncs()
+    c = _clib.lagrbf_interpolate_device_noalloc
+    c.restype = ctypes.c_int
+    c.argtypes = [ctypes.c_void_p, PetscInt, ctypes.c_void_p, ctypes.c_void_p]
+    return c
+
+
 def cfuncs():
     global _clib, _cfuncs
     if _cfuncs is None:
@@ -205,4 +214,5 @@ EXPORTED_SYMBOLS = [
     "lagrbf_io_stag_load", "lagrbf_io_petsc_load", "lagrbf_io_load_data", "lagrbf_io_dataset_info",
     "lagrbf_io_dataset_copy", "lagrbf_io_dataset_free", "lagrbf_swarm_output", "lagrbf_swarm_load",
     "lagrbf_load_data", "lagrbf_fill_swarm_from_dataset", "lagrbf_advect_adaptive",
+    "lagrbf_interpolate_device_noalloc",
 ]

@@ -1637,6 +1637,9 @@ Ctx* create_from_view(const OptView& o) {
     delete c;
     return nullptr;
   }
+  // the no-allocation entry point's scratch on this device (its own error is kept
+  // for that entry point, which refuses to run without it)
+  (void)noalloc_init_device(c->device);
   return c;
 }
 

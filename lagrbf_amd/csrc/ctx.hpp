@@ -221,6 +221,9 @@ int eval_ktype(const Ctx& c);
 bool eval_f32(const Ctx& c);
 // polynomial table and per-patch shape of the evaluation (group -1: main table)
 EvalExt eval_ext(const Ctx& c, int group);
+// (noalloc.hip) device address of the no-allocation entry point's static scratch on
+// `device`, resolved once per device (called at context creation)
+int noalloc_init_device(int device);
 // patch views of the extensions (with_poly false: shape parameter only)
 AugView aug_view(Ctx& c, bool with_poly);
 // free the factor pools once every patch is solved (coefficients stay)
