@@ -394,6 +394,8 @@ EvalExt eval_ext(const Ctx& c, int group) {
     double hm = c.g.h[0];
     for (int d = 1; d < c.g.dim; d++) hm = std::max(hm, c.g.h[d]);
     e.ps = 1.0 / hm;
+    e.pcell = c.pcell.p;
+    e.palive = c.alive.p;
   }
   if (c.local_eps > 0.0) e.peps = c.p_eps.p;
   return e;

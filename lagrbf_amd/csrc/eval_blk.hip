@@ -172,6 +172,63 @@ void launch_build_blk(const Grid& g, bool fp32, const double* x, int N, const in
                        mem_off, coef, nft, cstride, f0, (double*)blk);
 }
 
+// Polynomial parts of the corner patches: out += sum_k w_k p_k(x) . beta_k for
+// every live particle (one thread each).  A particle whose 3^d neighbourhood holds
+// no node has only empty corner patches, whose table rows are zero: its stale
+// values stay as they are.
+template <int DIM, int NFT>
+__global__ __launch_bounds__(256) void k_add_poly(Grid g, const double* __restrict__ px, int64_t ld, int np,
+                                                  const double* __restrict__ w, EvalExt ev, int nfields,
+                                                  double* __restrict__ out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np || !ev.palive[i]) return;
+  const int c = ev.pcell[i];
+  if (c < 0) return;
+  const int ci = c % g.nc[0], cj = (c / g.nc[0]) % g.nc[1], ck = DIM == 3 ? c / (g.nc[0] * g.nc[1]) : 0;
+  const double x[3] = {px[i], px[ld + i], DIM == 3 ? px[2 * ld + i] : 0.0};
+  double add[NFT];
+#pragma unroll
+  for (int f = 0; f < NFT; f++) add[f] = 0.0;
+#pragma unroll
+  for (int k = 0; k < (1 << DIM); k++) {
+    int a, b, e;
+    corner_offset(k, a, b, e);
+    const int pi = ci + a, pj = cj + b, pk = DIM == 3 ? ck + e : 0;
+    const int p = pi + g.n[0] * (pj + g.n[1] * pk);
+    const int idx[3] = {pi, pj, pk};
+    double xi[3] = {0.0, 0.0, 0.0}, pv[POLY_MAX];
+#pragma unroll
+    for (int d = 0; d < DIM; d++) xi[d] = (x[d] - (g.O[d] + idx[d] * g.h[d])) * ev.ps;
+    poly_eval(DIM, ev.deg, xi, pv);
+    const double* pt = ev.ptab + (size_t)p * ev.mm * NFT;
+    const double wk = w[(size_t)k * ld + i];
+#pragma unroll
+    for (int f = 0; f < NFT; f++) {
+      double sp = 0.0;
+#pragma unroll
+      for (int q = 0; q < POLY_MAX; q++)
+        if (q < ev.mm) sp += pv[q] * pt[q * NFT + f];
+      add[f] += wk * sp;
+    }
+  }
+#pragma unroll
+  for (int f = 0; f < NFT; f++)
+    if (f < nfields) out[(size_t)f * ld + i] += add[f];
+}
+
+template <int DIM>
+static void add_poly_dim(const Grid& g, const double* px, int64_t ld, int np, const double* w, const EvalExt& ev,
+                         int nft, int nfields, double* out, hipStream_t s) {
+  const dim3 grid((np + 255) / 256), blk(256);
+  switch (nft) {
+    case 1: hipLaunchKernelGGL((k_add_poly<DIM, 1>), grid, blk, 0, s, g, px, ld, np, w, ev, nfields, out); break;
+    case 2: hipLaunchKernelGGL((k_add_poly<DIM, 2>), grid, blk, 0, s, g, px, ld, np, w, ev, nfields, out); break;
+    case 3: hipLaunchKernelGGL((k_add_poly<DIM, 3>), grid, blk, 0, s, g, px, ld, np, w, ev, nfields, out); break;
+    case 4: hipLaunchKernelGGL((k_add_poly<DIM, 4>), grid, blk, 0, s, g, px, ld, np, w, ev, nfields, out); break;
+    default: hipLaunchKernelGGL((k_add_poly<DIM, 6>), grid, blk, 0, s, g, px, ld, np, w, ev, nfields, out); break;
+  }
+}
+
 void launch_eval_blk(const Grid& g, int ktype, bool fp32, const double* px, int64_t ld, const int32_t* perm,
                      const int32_t* pstart, const double* w, const void* blk, int nft, const int32_t* cs, int nfields,
                      double* out, int block, int np, const int32_t* cell_list, const int32_t* list_count,
@@ -190,6 +247,10 @@ void launch_eval_blk(const Grid& g, int ktype, bool fp32, const double* px, int6
     default: LRB_K(K_GAUSS); break;
   }
 #undef LRB_K
+  if (ev.ptab && ev.pcell && ev.palive) {
+    if (g.dim == 3) add_poly_dim<3>(g, px, ld, np, w, ev, nft, nfields, out, s);
+    else add_poly_dim<2>(g, px, ld, np, w, ev, nft, nfields, out, s);
+  }
 }
 
 }  // namespace lrb

@@ -327,58 +327,19 @@ __device__ __forceinline__ void corner_eps2(const Grid& g, int ci, int cj, int c
   }
 }
 
-// Monomial q of the polynomial basis 1, x, y, (z), xx, xy, (xz), yy, (yz), (zz)
-// at xi: the product of coordinates a and b (-1: factor 1), fixed at compile time.
-template <int DIM>
-__host__ __device__ constexpr int mono_factor(int q, int which) {
-  if (q == 0) return -1;
-  if (q <= DIM) return which == 0 ? q - 1 : -1;
-  int k = DIM + 1;
-  for (int i = 0; i < DIM; i++)
-    for (int j = i; j < DIM; j++) {
-      if (k == q) return which == 0 ? i : j;
-      k++;
-    }
-  return -1;
-}
-
-// Shepard combination of the corner sums, val = sum_k w_k (acc_k + polynomial part
-// of corner patch k); without a polynomial table this is sum_k w_k acc_k.
+// Shepard combination of the corner sums, val = sum_k w_k acc_k (the polynomial
+// parts of the corner patches, if any, are added by a separate pass: in this
+// epilogue they cost the node loops their registers).
 template <int DIM, int NFT, typename T>
-__device__ __forceinline__ void combine_corners(const Grid& g, int ci, int cj, int ck, const double* __restrict__ px,
-                                                int64_t ld, int i, const double* __restrict__ w,
-                                                const T (&acc)[1 << DIM][NFT], const EvalExt& ev, double (&val)[NFT]) {
-  constexpr int NEN = 1 << DIM;
+__device__ __forceinline__ void combine_corners(const double* __restrict__ w, int64_t ld, int i,
+                                                const T (&acc)[1 << DIM][NFT], double (&val)[NFT]) {
 #pragma unroll
   for (int f = 0; f < NFT; f++) val[f] = 0.0;
 #pragma unroll
-  for (int k = 0; k < NEN; k++) {
+  for (int k = 0; k < (1 << DIM); k++) {
     const double wk = w[(size_t)k * ld + i];
-    double s[NFT];
 #pragma unroll
-    for (int f = 0; f < NFT; f++) s[f] = (double)acc[k][f];
-    if (ev.ptab) {
-      int a, b, c;
-      corner_offset(k, a, b, c);
-      const int pi = ci + a, pj = cj + b, pk = DIM == 3 ? ck + c : 0;
-      const int p = pi + g.n[0] * (pj + g.n[1] * pk);
-      double xi[3];
-      xi[0] = (px[i] - (g.O[0] + pi * g.h[0])) * ev.ps;
-      xi[1] = (px[ld + i] - (g.O[1] + pj * g.h[1])) * ev.ps;
-      xi[2] = DIM == 3 ? (px[2 * ld + i] - (g.O[2] + pk * g.h[2])) * ev.ps : 0.0;
-      const double* pt = ev.ptab + (size_t)p * ev.mm * NFT;
-#pragma unroll
-      for (int q = 0; q < POLY_MAX; q++) {
-        if (q < ev.mm) {
-          const int fa = mono_factor<DIM>(q, 0), fb = mono_factor<DIM>(q, 1);
-          const double mq = (fa < 0 ? 1.0 : xi[fa]) * (fb < 0 ? 1.0 : xi[fb]);
-#pragma unroll
-          for (int f = 0; f < NFT; f++) s[f] += mq * pt[q * NFT + f];
-        }
-      }
-    }
-#pragma unroll
-    for (int f = 0; f < NFT; f++) val[f] += s[f] * wk;
+    for (int f = 0; f < NFT; f++) val[f] += (double)acc[k][f] * wk;
   }
 }
 
@@ -392,8 +353,10 @@ __device__ __forceinline__ void combine_corners(const Grid& g, int ci, int cj, i
 // PIPE (FP32 only): table rows per wait in the node loops of the non-centre
 // neighbours, 1 or 2; WAVES: occupancy floor in waves per SIMD (the autotuner
 // chooses among the combinations; each suits other launch shapes)
-template <int DIM, int KT, int NFT, typename T, int PIPE,
-          int WAVES = (sizeof(T) == 8 ? LRB_EVAL_WAVES_F64 : LRB_EVAL_WAVES_F32)>
+template <typename T>
+__host__ __device__ constexpr int default_waves() { return sizeof(T) == 8 ? LRB_EVAL_WAVES_F64 : LRB_EVAL_WAVES_F32; }
+
+template <int DIM, int KT, int NFT, typename T, int PIPE, int WAVES>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WAVES)))
 void k_eval_blk(Grid g, const double* __restrict__ px, int64_t ld,
                                                   const int32_t* __restrict__ perm, const int32_t* __restrict__ pstart,
@@ -514,7 +477,7 @@ void k_eval_blk(Grid g, const double* __restrict__ px, int64_t ld,
     }
     if (!live) continue;
     double val[NFT];
-    combine_corners<DIM, NFT, T>(g, ci, cj, ck, px, ld, i, w, acc, ev, val);
+    combine_corners<DIM, NFT, T>(w, ld, i, acc, val);
 #pragma unroll
     for (int f = 0; f < NFT; f++) if (f < nfields) out[(size_t)f * ld + i] = val[f];
   }
@@ -649,7 +612,7 @@ __global__ __launch_bounds__(256) void k_eval_sparse(Grid g, const double* __res
     }
     if (!live) continue;
     double val[NFT];
-    combine_corners<DIM, NFT, T>(g, ci, cj, ck, px, ld, i, w, acc, ev, val);
+    combine_corners<DIM, NFT, T>(w, ld, i, acc, val);
 #pragma unroll
     for (int f = 0; f < NFT; f++) if (f < nfields) out[(size_t)f * ld + i] = val[f];
   }
@@ -687,7 +650,7 @@ static void eval_blk_k(const Grid& g, const double* px, int64_t ld, const int32_
   const dim3 grid(nblk), blkd(block);
   // group 0: contiguous XCD runs; group -K: round-robin in runs of K listed cells
   const int deal = group >= 0 ? 0 : -group;
-#define LRB_EB(NFT, P) LRB_LAUNCH_TIMED((k_eval_blk<DIM, KT, NFT, T, P>), grid, blkd, s, g, px, ld, perm, pstart, w, blk, cs, nfields, out, cell_list, list_count, deal, ev)
+#define LRB_EB(NFT, P) LRB_LAUNCH_TIMED((k_eval_blk<DIM, KT, NFT, T, P, default_waves<T>()>), grid, blkd, s, g, px, ld, perm, pstart, w, blk, cs, nfields, out, cell_list, list_count, deal, ev)
   if constexpr (sizeof(T) == 4) {
     // two rows per wait: the velocity evaluations (nft = dim) only
     if (pipe == 2 && nft == DIM) {

@@ -523,8 +523,11 @@ __device__ __forceinline__ void weights_point(const Grid& g, const double* __res
       hi[d] = min(hi[d], (int)floor(t + rh + 1e-6));
     }
   }
+  // w is indexed with compile-time indices only (unrolled selects), so it stays in
+  // registers: a runtime index lets the compiler move the array to LDS
   double sum = 0.0;
-  for (int k = 0; k < g.nen; k++) w[k] = 0.0;
+#pragma unroll
+  for (int k = 0; k < 8; k++) w[k] = 0.0;
   for (int kk = lo[2]; kk <= hi[2]; kk++) {
     const double dz = g.dim == 3 ? z - pu_coord(g, 2, kk) : 0.0;
     const int ek = kk - ck;
@@ -538,17 +541,22 @@ __device__ __forceinline__ void weights_point(const Grid& g, const double* __res
         const int ei = i - ci;
         if ((unsigned)ei <= 1u && (unsigned)ej <= 1u && (unsigned)ek <= 1u) {
           const int q = (ej ? (ei ? 2 : 3) : (ei ? 1 : 0)) + 4 * ek;
-          w[q] = wt;
+#pragma unroll
+          for (int k = 0; k < 8; k++)
+            if (k == q) w[k] = wt;
         }
       }
     }
   }
   if (g.wnorm) {
     sum = 0.0;
-    for (int k = 0; k < g.nen; k++) sum += w[k];
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+      if (k < g.nen) sum += w[k];
   }
   const double inv = sum > 0.0 ? 1.0 / sum : 0.0;
-  for (int k = 0; k < g.nen; k++) w[k] *= inv;
+#pragma unroll
+  for (int k = 0; k < 8; k++) w[k] *= inv;
 }
 
 __global__ void k_weights(Grid g, const double* __restrict__ pux, const double* __restrict__ puy, const double* __restrict__ puz,
@@ -558,7 +566,9 @@ __global__ void k_weights(Grid g, const double* __restrict__ pux, const double* 
   if (i >= Np || !alive[i]) return;
   double wl[8];
   weights_point(g, pux, puy, puz, px[i], px[ld + i], g.dim == 3 ? px[2 * ld + i] : 0.0, cell[i], wl);
-  for (int k = 0; k < g.nen; k++) w[(size_t)k * ld + i] = wl[k];
+#pragma unroll
+  for (int k = 0; k < 8; k++)
+    if (k < g.nen) w[(size_t)k * ld + i] = wl[k];
 }
 
 void launch_weights(const Grid& g, const double* pux, const double* puy, const double* puz, const double* px, int64_t ld,
@@ -869,7 +879,9 @@ __global__ void k_stage(Grid g, const double* __restrict__ pux, const double* __
   if (c < 0) { alive[i] = 0; atomicAdd(removed, 1ull); return; }
   double wl[8];
   weights_point(g, pux, puy, puz, x[0], x[1], x[2], c, wl);
-  for (int k = 0; k < g.nen; k++) w[(size_t)k * ld + i] = wl[k];
+#pragma unroll
+  for (int k = 0; k < 8; k++)
+    if (k < g.nen) w[(size_t)k * ld + i] = wl[k];
 }
 
 void launch_stage(const Grid& g, const double* pux, const double* puy, const double* puz, double* px, const double* x0,

@@ -161,14 +161,17 @@ void launch_build_blk(const Grid& g, bool fp32, const double* x, int N, const in
 // Events the next launch_eval_blk dispatch records its start and end in (nullptr: none).
 void set_eval_timing_events(hipEvent_t start, hipEvent_t stop);
 // Extensions: ev.ptab (polynomial table, launch_build_ptab) adds each corner
-// patch's polynomial part; ev.peps with ktype K_GAUSS_LOCAL gives the Gaussian
-// the shape parameter of each corner patch.
+// patch's polynomial part, in a pass over the particles (cell ids and alive flags
+// in pcell / palive, np of them) after the evaluation kernel; ev.peps with ktype
+// K_GAUSS_LOCAL gives the Gaussian the shape parameter of each corner patch.
 struct EvalExt {
   const double* ptab = nullptr;
   int mm = 0;              // polynomial terms per patch in ptab
   int deg = -1;            // their degree
   double ps = 1.0;         // polynomial scale
   const double* peps = nullptr;
+  const int32_t* pcell = nullptr;
+  const int32_t* palive = nullptr;
 };
 void launch_eval_blk(const Grid& g, int ktype, bool fp32, const double* px, int64_t ld, const int32_t* perm,
                      const int32_t* pstart, const double* w, const void* blk, int nft, const int32_t* cs, int nfields,

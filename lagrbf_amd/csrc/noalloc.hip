@@ -149,9 +149,12 @@ extern "C" int lagrbf_interpolate_device_noalloc(void* ctx, int32_t n, const dou
   const int per_cell = (n + c->g.C - 1) / std::max(c->g.C, 1);
   const int block = std::min(256, std::max(64, (per_cell + 63) / 64 * 64));
   set_eval_timing_events(nullptr, nullptr);
+  EvalExt ev = eval_ext(*c, -1);
+  ev.pcell = cell;
+  ev.palive = alive;
   launch_eval_blk(c->g, eval_ktype(*c), f32, px, ld, perm, pstart, w, f32 ? (const void*)c->blk32.p : (const void*)c->blk64.p,
                   nf, c->cell_start.p, nf, out, block, n, listed ? list : nullptr, listed ? count : nullptr, group, pipe,
-                  waves, eval_ext(*c, -1), s);
+                  waves, ev, s);
   // lagrbf_output("fields"): AoS rows in point order
   launch_soa_to_aos(out, ld, n, nf, values, s);
   const hipError_t e = hipStreamSynchronize(s);

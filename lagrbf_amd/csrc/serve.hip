@@ -203,7 +203,9 @@ static constexpr int SERVE_TUNE_MIN_DEFAULT = 1 << 22;
 void eval_batch(Batch& b, double* out) {
   Ctx& d = *b.ds;
   const bool f32 = eval_f32(d);
-  const EvalExt ev = eval_ext(d, -1);
+  EvalExt ev = eval_ext(d, -1);
+  ev.pcell = b.cell.p;     // the batch's particles, for the polynomial pass
+  ev.palive = b.alive.p;
   const int per_cell = (b.Np + d.g.C - 1) / std::max(d.g.C, 1);
   const int block = std::min(256, std::max(64, (per_cell + 63) / 64 * 64));
   auto launch = [&](int v) {
