@@ -19,6 +19,7 @@
 // residual of the full saddle-point system (f - A c - P beta, -P^T c), A recomputed
 // from the member coordinates, until the correction is below tol.
 #include "kernels.hpp"
+#include "blocked_solve.hpp"
 
 #include <algorithm>
 
@@ -312,7 +313,7 @@ __device__ void col_dots(const double* __restrict__ W, int n, int i0, int m, con
 // Solve of the saddle-point systems (see the file comment).  Vectors live in LDS
 // (stride S) when the patch fits, else in ws at the patch's member offset.
 template <typename TL, int KT, int MM>
-__global__ __launch_bounds__(ANT) void k_solve_aug(Grid g, const int32_t* __restrict__ list,
+__global__ __launch_bounds__(ANT, 3) void k_solve_aug(Grid g, const int32_t* __restrict__ list,
                                                    const int32_t* __restrict__ pn, const int64_t* __restrict__ mem_off,
                                                    const int64_t* __restrict__ fac_off,
                                                    const int32_t* __restrict__ mem_pool, const double* __restrict__ xd,
@@ -325,6 +326,7 @@ __global__ __launch_bounds__(ANT) void k_solve_aug(Grid g, const int32_t* __rest
   __shared__ double red[(ANT / AW) * MF];
   __shared__ double sv[MF], sb[MF], su1[MF], sh1[MF], sdb[MF], st2[MF], zk[AMAXF];
   __shared__ double sR[MM * MM], sT[MM * MM], sB11[MM * MM];
+  __shared__ double zb[SOLVE_NB * 8], lb[SOLVE_LB];
   const int p = list[blockIdx.x];
   const int n = pn[p];
   const int m = a.m[p];
@@ -443,7 +445,9 @@ __global__ __launch_bounds__(ANT) void k_solve_aug(Grid g, const int32_t* __rest
     }
     __syncthreads();
     // u2 = B22^-1 (h2 - B21 u1)
-    llt_solve<TL>(L, nn, y + m, ldv, nf, red, zk);
+    const int nbp = a.fnb ? a.fnb[p] : 0;
+    if (nbp == SOLVE_NB) llt_solve_blocked<TL, ANT>(L, nn, y + m, ldv, nf, zb, lb);
+    else llt_solve<TL>(L, nn, y + m, ldv, nf, red, zk);
     // beta correction: R^-1 (h1 - B11 u1 - B12 u2)
     col_dots(B1, n, m, m, y, ldv, nf, red, sv);
     if (tid < nf) {

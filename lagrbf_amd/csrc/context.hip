@@ -12,7 +12,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/lagrbf.h"
@@ -157,6 +160,9 @@ static int ctx_from_options(Ctx& c, const OptView& o) {
     else if (ev == "ref") c.eval_mode = 2;
     else return set_err(E_WRONG, fn, "-lagrbf_eval must be auto, cell or ref (found %s)", ev.c_str());
   }
+  o.get_int("lagrbf_factor_nb", c.factor_nb, bad);
+  if (bad || !(c.factor_nb == -1 || c.factor_nb == 0 || c.factor_nb == 16))
+    return set_err(E_WRONG, fn, "-lagrbf_factor_nb must be -1 (tune), 0 (vector ALU) or 16 (matrix cores)");
   c.chol_big_min = chol_max_n();
   o.get_int("lagrbf_chol_big_min", c.chol_big_min, bad);
   c.chol_big_min = std::max(0, std::min(c.chol_big_min, chol_max_n()));
@@ -210,6 +216,9 @@ static int ctx_init_device(Ctx& c) {
   HIPCHK(c.pcount.reserve(P));
   c.h_method.assign(P, 0);
   c.h_m.assign(P, 0); c.h_deg.assign(P, -1); c.h_eps.assign(P, c.g.eps_r); c.h_aux_off.assign(P, 0);
+  c.h_fnb.assign(P, 0);
+  HIPCHK(c.p_fnb.reserve(P));
+  HIPCHK(hipMemset(c.p_fnb.p, 0, P));
   HIPCHK(c.p_m.reserve(P)); HIPCHK(c.p_deg.reserve(P)); HIPCHK(c.p_eps.reserve(P)); HIPCHK(c.p_aux_off.reserve(P));
   HIPCHK(hipMemset(c.p_m.p, 0, sizeof(int32_t) * P));
   HIPCHK(hipMemset(c.p_aux_off.p, 0, sizeof(int64_t) * P));
@@ -376,6 +385,7 @@ AugView aug_view(Ctx& c, bool with_poly) {
   a.beta = c.beta_pool.p;
   a.bstride = c.poly_mm * std::max(c.nf, 1);
   a.mmax = c.poly_mm;
+  a.fnb = c.p_fnb.p;
   double hm = c.g.h[0];
   for (int d = 1; d < c.g.dim; d++) hm = std::max(hm, c.g.h[d]);
   a.ps = 1.0 / hm;
@@ -439,11 +449,74 @@ static int upload_list(Ctx& c, const std::vector<int32_t>& l, int seg) {
   return 0;
 }
 
+// ---------------------------------------------------------------------------
+// Factor kernel choice for the small-patch Cholesky (the matrix-core kernel with
+// panels of 16 or 32 columns; the vector-ALU kernel by option only), per precision, kernel,
+// polynomial block and size bucket of the factor dimension (<= 128, <= 256, <=
+// 512).  The first factorisation in a bucket times every candidate on a sample of
+// its patches (up to 256; factoring is idempotent, the sample is factored again
+// with the winner); the choice is kept for the process.
+struct FactorTuneKey {
+  int ktype, mmax, fp32, bucket;
+  bool operator<(const FactorTuneKey& o) const {
+    return std::tie(ktype, mmax, fp32, bucket) < std::tie(o.ktype, o.mmax, o.fp32, o.bucket);
+  }
+};
+static std::mutex g_ftune_mu;
+static std::map<FactorTuneKey, int> g_ftune;
+static constexpr int FACTOR_NB[] = {16, 0};
+static constexpr int N_FACTOR_NB = (int)(sizeof(FACTOR_NB) / sizeof(FACTOR_NB[0]));
+
+static int factor_bucket(int nn) { return nn <= 128 ? 0 : (nn <= 256 ? 1 : 2); }
+
+static int factor_nb_for(Ctx& c, bool fp32, int bucket, const int32_t* dlist, int nsample, void* fac,
+                         const AugView& a) {
+  if (c.factor_nb >= 0) return c.factor_nb;
+  const FactorTuneKey key{c.ktype, a.m ? a.mmax : 0, fp32 ? 1 : 0, bucket};
+  {
+    std::lock_guard<std::mutex> lk(g_ftune_mu);
+    auto it = g_ftune.find(key);
+    if (it != g_ftune.end()) return it->second;
+  }
+  hipEvent_t e[N_FACTOR_NB + 1];
+  for (auto& x : e) if (hipEventCreate(&x) != hipSuccess) return 16;
+  float best = 1e30f, tot = 0.f;
+  int nb_best = 16;
+  for (int v = 0; v < N_FACTOR_NB; v++) {
+    (void)hipEventRecord(e[v], c.s);
+    launch_factor_chol(c.g, c.ktype, fp32, dlist, nsample, c.p_n.p, c.p_mem_off.p, c.p_fac_off.p, c.mem_pool.p, c.dx.p,
+                       fac, c.pstatus.p, false, a, FACTOR_NB[v], 128 << bucket, c.s);
+  }
+  (void)hipEventRecord(e[N_FACTOR_NB], c.s);
+  (void)hipEventSynchronize(e[N_FACTOR_NB]);
+  for (int v = 0; v < N_FACTOR_NB; v++) {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e[v], e[v + 1]);
+    tot += ms;
+    if (ms < best) { best = ms; nb_best = FACTOR_NB[v]; }
+  }
+  for (auto& x : e) (void)hipEventDestroy(x);
+  c.factor_trial_ms += tot;
+  std::lock_guard<std::mutex> lk(g_ftune_mu);
+  g_ftune[key] = nb_best;
+  return nb_best;
+}
+
 // Factor the new patches of each method; FP32/FP64 Cholesky breakdowns are
 // refactored with Bunch-Kaufman in FP64 (counted in "fallback").
 static int factor_lists(Ctx& c, std::vector<int32_t>* fl) {
   const int P = c.g.P;
   const AugView ap = aug_view(c, true), ae = aug_view(c, false);
+  // small-patch methods: patches ordered by size bucket, each bucket a launch of its own
+  std::vector<int> bstart[N_METHODS];
+  for (int m = M_CHOL64; m < N_METHODS; m++) {
+    if (fl[m].empty() || method_big(m)) continue;
+    auto nn = [&](int p) { return c.h_n[p] - (method_aug(m) ? c.h_m[p] : 0); };
+    std::stable_sort(fl[m].begin(), fl[m].end(), [&](int a, int b) { return factor_bucket(nn(a)) < factor_bucket(nn(b)); });
+    bstart[m].assign(4, (int)fl[m].size());
+    for (int i = (int)fl[m].size() - 1; i >= 0; i--) bstart[m][factor_bucket(nn(fl[m][i]))] = i;
+    for (int b = 2; b >= 0; b--) bstart[m][b] = std::min(bstart[m][b], bstart[m][b + 1]);
+  }
   for (int m = 0; m < N_METHODS; m++) if (!fl[m].empty()) CHK(upload_list(c, fl[m], m));
   if (!fl[M_BK64].empty())
     launch_factor_bk(c.g, c.ktype, c.list.p, (int)fl[M_BK64].size(), c.p_n.p, c.p_mem_off.p, c.p_fac_off.p, c.mem_pool.p,
@@ -457,11 +530,25 @@ static int factor_lists(Ctx& c, std::vector<int32_t>* fl) {
   for (int m = M_CHOL64; m < N_METHODS; m++)
     if (!fl[m].empty()) {
       any_chol = true;
-      launch_factor_chol(c.g, c.ktype, method_fp32(m), c.list.p + (size_t)m * P, (int)fl[m].size(), c.p_n.p,
-                         c.p_mem_off.p, c.p_fac_off.p, c.mem_pool.p, c.dx.p,
-                         method_fp32(m) ? (void*)c.fac32_pool.p : (void*)c.fac_pool.p, c.pstatus.p, method_big(m),
-                         method_aug(m) ? ap : ae, c.s);
+      void* fac = method_fp32(m) ? (void*)c.fac32_pool.p : (void*)c.fac_pool.p;
+      const AugView& a = method_aug(m) ? ap : ae;
+      const int32_t* dl = c.list.p + (size_t)m * P;
+      if (method_big(m)) {
+        for (int p : fl[m]) c.h_fnb[p] = 0;
+        launch_factor_chol(c.g, c.ktype, method_fp32(m), dl, (int)fl[m].size(), c.p_n.p, c.p_mem_off.p, c.p_fac_off.p,
+                           c.mem_pool.p, c.dx.p, fac, c.pstatus.p, true, a, 0, 0, c.s);
+        continue;
+      }
+      for (int b = 0; b < 3; b++) {
+        const int b0 = bstart[m][b], b1 = bstart[m][b + 1];
+        if (b1 <= b0) continue;
+        const int nb = factor_nb_for(c, method_fp32(m), b, dl + b0, std::min(b1 - b0, 256), fac, a);
+        launch_factor_chol(c.g, c.ktype, method_fp32(m), dl + b0, b1 - b0, c.p_n.p, c.p_mem_off.p, c.p_fac_off.p,
+                           c.mem_pool.p, c.dx.p, fac, c.pstatus.p, false, a, nb, 128 << b, c.s);
+        for (int i = b0; i < b1; i++) c.h_fnb[fl[m][i]] = (uint8_t)nb;
+      }
     }
+  HIPCHK(hipMemcpyAsync(c.p_fnb.p, c.h_fnb.data(), P, hipMemcpyHostToDevice, c.s));
   if (!any_chol) return 0;
   std::vector<int32_t> st(P);
   HIPCHK(hipMemcpyAsync(st.data(), c.pstatus.p, sizeof(int32_t) * P, hipMemcpyDeviceToHost, c.s));
@@ -698,6 +785,10 @@ static int partitions_reset(Ctx& c) {
   return 0;
 }
 
+// r^3 tables carry the coefficients times eps_r^3: the evaluation then forms phi as
+// r^2 sqrt(r^2) (two multiplications fewer per node than (eps r)^3).
+static double table_coef_scale(const Ctx& c) { return c.ktype == K_PHS3 ? c.g.eps_r * c.g.eps_r * c.g.eps_r : 1.0; }
+
 // RBFInterpolation(ctx, swi, swd, name), interpolation.c:178-272
 // Cell-major node table for the evaluation, rebuilt after the data or any patch changed.
 int ensure_blk(Ctx& c, bool f32) {
@@ -707,7 +798,7 @@ int ensure_blk(Ctx& c, bool f32) {
   else HIPCHK(c.blk64.reserve(n));
   if (c.N > 0)
     launch_build_blk(c.g, f32, c.dx.p, c.N, c.cell_nodes.p, c.dcell.p, c.cell_start.p, c.p_active.p, c.p_mem_off.p,
-                     c.coef_pool.p, c.nf, c.nf, 0, f32 ? (void*)c.blk32.p : (void*)c.blk64.p, c.s);
+                     c.coef_pool.p, c.nf, c.nf, 0, table_coef_scale(c), f32 ? (void*)c.blk32.p : (void*)c.blk64.p, c.s);
   if (c.poly_deg >= 0) {
     // polynomial part of every active patch, zero for the others
     HIPCHK(c.ptab.reserve((size_t)c.g.P * c.poly_mm * std::max(c.nf, 1)));
@@ -764,7 +855,8 @@ static int ensure_blk_group(Ctx& c, bool f32, int g) {
   else HIPCHK(c.blkg64[g].reserve(n));
   if (c.N > 0)
     launch_build_blk(c.g, f32, c.dx.p, c.N, c.cell_nodes.p, c.dcell.p, c.cell_start.p, c.p_active.p, c.p_mem_off.p,
-                     c.coef_pool.p, dim, c.nf, g * dim, f32 ? (void*)c.blkg32[g].p : (void*)c.blkg64[g].p, c.s);
+                     c.coef_pool.p, dim, c.nf, g * dim, table_coef_scale(c), f32 ? (void*)c.blkg32[g].p : (void*)c.blkg64[g].p,
+                     c.s);
   if (c.poly_deg >= 0) {
     HIPCHK(c.ptabg[g].reserve((size_t)c.g.P * c.poly_mm * dim));
     launch_build_ptab(c.g, c.p_active.p, aug_view(c, true), c.poly_mm, dim, c.nf, g * dim, c.ptabg[g].p, c.s);
@@ -1806,6 +1898,7 @@ int lagrbf_get_info(void* ctx, const char* key, double* value) {
   else if (k == "eps_r") *value = c->g.eps_r;
   else if (k == "aug_fallback_bk") *value = c->aug_fallback_bk;
   else if (k == "poly_terms") *value = c->poly_mm;
+  else if (k == "factor_trial_ms") *value = c->factor_trial_ms;
   else return set_err(E_WRONG, "lagrbf_get_info", "unknown key '%s'", key);
   return 0;
 }

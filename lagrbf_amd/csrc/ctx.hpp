@@ -191,6 +191,12 @@ struct Ctx {
   DBuf<double> ptab, ptabg[2];             // polynomial tables of the evaluation (main, blend groups)
   int ptab_gen = -1, ptabg_gen[2] = {-1, -1};
   double aug_fallback_bk = 0;              // saddle-point patches refactored with Bunch-Kaufman
+  // factor kernel of the small-patch Cholesky per size bucket: -1 tune (default),
+  // 0 vector ALU, 16 / 32 matrix cores with panels of that width (-lagrbf_factor_nb)
+  int factor_nb = -1;
+  std::vector<uint8_t> h_fnb;              // per patch: panel width of its matrix-core factor, 0 otherwise
+  DBuf<uint8_t> p_fnb;
+  double factor_trial_ms = 0;              // time spent in the factor tuner's trials
 };
 
 // factor methods: Bunch-Kaufman FP64; Cholesky FP64 / FP32 (n <= chol_max_n());

@@ -46,7 +46,8 @@ template <typename T>
 __global__ void k_build_blk(Grid g, const double* __restrict__ x, int N, const int32_t* __restrict__ cn,
                             const int32_t* __restrict__ dcell, const int32_t* __restrict__ cs,
                             const uint8_t* __restrict__ active, const int64_t* __restrict__ mem_off,
-                            const double* __restrict__ coef, int nft, int cstride, int f0, T* __restrict__ blk) {
+                            const double* __restrict__ coef, int nft, int cstride, int f0, double cscale,
+                            T* __restrict__ blk) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= N + BLK_PAD_ROWS) return;
   const int dim = g.dim, BS = 4 + g.nen * nft;
@@ -83,7 +84,7 @@ __global__ void k_build_blk(Grid g, const double* __restrict__ x, int N, const i
           if (c2 < q) off += cs[c2 + 1] - cs[c2];
         }
     const double* src = coef + (size_t)(mem_off[p] + off + r) * cstride + f0;
-    for (int f = 0; f < nft; f++) dst[f] = (T)src[f];
+    for (int f = 0; f < nft; f++) dst[f] = (T)(src[f] * cscale);
   }
 }
 
@@ -160,16 +161,31 @@ size_t blk_row(int dim, int nft) { return 4 + ((size_t)1 << dim) * (size_t)nft; 
 
 void launch_build_blk(const Grid& g, bool fp32, const double* x, int N, const int32_t* cn, const int32_t* dcell,
                       const int32_t* cs, const uint8_t* active, const int64_t* mem_off, const double* coef, int nft,
-                      int cstride, int f0, void* blk, hipStream_t s) {
+                      int cstride, int f0, double cscale, void* blk, hipStream_t s) {
   if (N <= 0) return;
   // N + BLK_PAD_ROWS threads: rows 0..N-1 and the zero padding rows
   const int nt = N + BLK_PAD_ROWS;
   if (fp32)
     hipLaunchKernelGGL(k_build_blk<float>, dim3((nt + 255) / 256), dim3(256), 0, s, g, x, N, cn, dcell, cs, active,
-                       mem_off, coef, nft, cstride, f0, (float*)blk);
+                       mem_off, coef, nft, cstride, f0, cscale, (float*)blk);
   else
     hipLaunchKernelGGL(k_build_blk<double>, dim3((nt + 255) / 256), dim3(256), 0, s, g, x, N, cn, dcell, cs, active,
-                       mem_off, coef, nft, cstride, f0, (double*)blk);
+                       mem_off, coef, nft, cstride, f0, cscale, (double*)blk);
+}
+
+// Monomial q of the basis 1, x, y, (z), xx, xy, (xz), yy, (yz), (zz) is the product
+// of coordinates mono_factor(q, 0) and mono_factor(q, 1) (-1: factor 1).
+template <int DIM>
+__host__ __device__ constexpr int mono_factor(int q, int which) {
+  if (q == 0) return -1;
+  if (q <= DIM) return which == 0 ? q - 1 : -1;
+  int k = DIM + 1;
+  for (int i = 0; i < DIM; i++)
+    for (int j = i; j < DIM; j++) {
+      if (k == q) return which == 0 ? i : j;
+      k++;
+    }
+  return -1;
 }
 
 // Polynomial parts of the corner patches: out += sum_k w_k p_k(x) . beta_k for
@@ -196,20 +212,27 @@ __global__ __launch_bounds__(256) void k_add_poly(Grid g, const double* __restri
     const int pi = ci + a, pj = cj + b, pk = DIM == 3 ? ck + e : 0;
     const int p = pi + g.n[0] * (pj + g.n[1] * pk);
     const int idx[3] = {pi, pj, pk};
-    double xi[3] = {0.0, 0.0, 0.0}, pv[POLY_MAX];
+    double xi[3] = {0.0, 0.0, 0.0};
 #pragma unroll
     for (int d = 0; d < DIM; d++) xi[d] = (x[d] - (g.O[d] + idx[d] * g.h[d])) * ev.ps;
-    poly_eval(DIM, ev.deg, xi, pv);
     const double* pt = ev.ptab + (size_t)p * ev.mm * NFT;
     const double wk = w[(size_t)k * ld + i];
+    double sp[NFT];
 #pragma unroll
-    for (int f = 0; f < NFT; f++) {
-      double sp = 0.0;
+    for (int f = 0; f < NFT; f++) sp[f] = 0.0;
+    // monomial q = xi[fa] xi[fb] (factor 1 for -1), indices fixed at compile time so
+    // that nothing here is indexed at run time (that would put arrays in scratch)
 #pragma unroll
-      for (int q = 0; q < POLY_MAX; q++)
-        if (q < ev.mm) sp += pv[q] * pt[q * NFT + f];
-      add[f] += wk * sp;
+    for (int q = 0; q < POLY_MAX; q++) {
+      if (q < ev.mm) {
+        const int fa = mono_factor<DIM>(q, 0), fb = mono_factor<DIM>(q, 1);
+        const double mq = (fa < 0 ? 1.0 : xi[fa]) * (fb < 0 ? 1.0 : xi[fb]);
+#pragma unroll
+        for (int f = 0; f < NFT; f++) sp[f] += mq * pt[q * NFT + f];
+      }
     }
+#pragma unroll
+    for (int f = 0; f < NFT; f++) add[f] += wk * sp[f];
   }
 #pragma unroll
   for (int f = 0; f < NFT; f++)

@@ -76,6 +76,14 @@ __host__ __device__ constexpr bool two_rows_fit(int nbi) {
   return 2 * (4 + n_corners_in_nbr<DIM>(nbi) * NFT) * (int)(sizeof(T) / 4) <= 72;
 }
 
+// phi from the squared distance.  r^3: eps^3 is folded into the table's
+// coefficients (k_build_blk), so phi = r^2 sqrt(r^2).
+template <int KT, typename T>
+__device__ __forceinline__ T phi_r2(T r2, T eps) {
+  if constexpr (KT == K_PHS3) return r2 * sqrt_fast(r2);
+  else return rbf<KT, T>(sqrt_fast(r2), eps);
+}
+
 template <int KT>
 __host__ __device__ constexpr bool compact_support_blk() { return KT == K_WC0 || KT == K_WC2 || KT == K_BUMP; }
 
@@ -109,7 +117,7 @@ __device__ __forceinline__ void accum_blk(T xp, T yp, T zp, const T* __restrict_
       if (!__any(r2 < rcut2)) continue;
     }
     T phi = T(0);
-    if constexpr (KT != K_GAUSS_LOCAL) phi = rbf<KT, T>(sqrt_fast(r2), eps);
+    if constexpr (KT != K_GAUSS_LOCAL) phi = phi_r2<KT, T>(r2, eps);
 #pragma unroll
     for (int k = 0; k < NEN; k++) {
       const int kp = corner_in_nbr<DIM>(NBI, k);
@@ -172,7 +180,7 @@ __device__ __forceinline__ void node_blk(const BlkRow<DIM, NFT, T>& row, T xp, T
     }
     return;
   }
-  const T phi = rbf<KT, T>(sqrt_fast(r2), eps);
+  const T phi = phi_r2<KT, T>(r2, eps);
 #pragma unroll
   for (int k = 0; k < NEN; k++) {
     const int kp = corner_in_nbr<DIM>(NBI, k);
@@ -508,7 +516,7 @@ __device__ __forceinline__ void accum_sparse(T xp, T yp, T zp, const T* __restri
     }
     if (act) {
       T phi = T(0);
-      if constexpr (KT != K_GAUSS_LOCAL) phi = rbf<KT, T>(sqrt_fast(r2), eps);
+      if constexpr (KT != K_GAUSS_LOCAL) phi = phi_r2<KT, T>(r2, eps);
 #pragma unroll
       for (int k = 0; k < NEN; k++) {
         const int kp = corner_in_nbr<DIM>(NBI, k);

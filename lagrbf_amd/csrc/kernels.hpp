@@ -48,6 +48,9 @@ struct AugView {
   int bstride;
   int mmax;                // terms of the requested degree (the largest m)
   double ps;               // polynomial scale: 1 / largest cell width
+  // [P] panel width of the matrix-core factor of each patch (its solve is blocked,
+  // blocked_solve.hpp), 0: a factor of the vector-ALU kernels (column solve)
+  const uint8_t* fnb;
 };
 __host__ __device__ inline size_t aug_aux_size(int n, int m) {
   return m > 0 ? (size_t)3 * n + (size_t)3 * m * n + (size_t)2 * m * m : 0;
@@ -86,14 +89,22 @@ void launch_solve_bk(const int32_t* list, int nlist, const int32_t* pn, const in
 // factor dimension is n - m.
 bool chol_kernel_supported(int ktype);
 int chol_max_n();
+// nb: 0 the vector-ALU kernels below, 16 the matrix-core kernel of chol_mfma.hip
+// (16-column panels, factor dimensions up to nmax <= chol_mfma_max_n())
 void launch_factor_chol(const Grid& g, int ktype, bool fp32, const int32_t* list, int nlist, const int32_t* pn,
                         const int64_t* mem_off, const int64_t* fac_off, const int32_t* mem_pool, const double* xd,
-                        void* fac, int32_t* status, bool big, const AugView& a, hipStream_t s);
+                        void* fac, int32_t* status, bool big, const AugView& a, int nb, int nmax, hipStream_t s);
 void launch_solve_chol(const Grid& g, int ktype, bool fp32, const int32_t* list, int nlist, const int32_t* pn,
                        const int64_t* mem_off, const int64_t* fac_off, const int32_t* mem_pool, const double* xd,
                        const void* fac, const double* values, int nf, double* coef, int max_iter, double tol,
                        int32_t* status, int32_t* iters, double* ws, int nmax /* largest listed patch */,
                        const AugView& a, hipStream_t s);
+// (chol_mfma.hip) the same factor with the panel products on the matrix cores,
+// panels of 16 columns, factor dimension up to nmax <= chol_mfma_max_n()
+int chol_mfma_max_n();
+void launch_factor_chol_mfma(const Grid& g, int ktype, bool fp32, int nmax, const int32_t* list, int nlist,
+                             const int32_t* pn, const int64_t* mem_off, const int64_t* fac_off, const int32_t* mem_pool,
+                             const double* xd, void* fac, int32_t* status, const AugView& a, hipStream_t s);
 // (aug.hip) null-space data of the listed patches with a polynomial block
 void launch_aug_prep(const Grid& g, int ktype, const int32_t* list, int nlist, const int32_t* pn,
                      const int64_t* mem_off, const int32_t* mem_pool, const double* xd, const AugView& a, hipStream_t s);
@@ -152,7 +163,8 @@ size_t blk_row(int dim, int nft);
 constexpr int BLK_PAD_ROWS = 2;
 void launch_build_blk(const Grid& g, bool fp32, const double* x, int N, const int32_t* cn, const int32_t* dcell,
                       const int32_t* cs, const uint8_t* active, const int64_t* mem_off, const double* coef, int nft,
-                      int cstride, int f0, void* blk, hipStream_t s);   // fields f0 .. f0+nft-1 of cstride
+                      int cstride, int f0, double cscale, void* blk, hipStream_t s);   // fields f0 .. f0+nft-1 of
+                                                                                       // cstride, times cscale
 // cell_list / list_count (device): evaluate only the listed non-empty cells, one
 // block per cell (group 0: contiguous runs of cells per XCD; group -K: runs of K
 // cells dealt round-robin over the XCDs) or 64/group cells per wave in lane groups

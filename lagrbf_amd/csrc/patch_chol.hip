@@ -12,6 +12,7 @@
 // Coordinates are shifted to the patch's PU node in FP64 before any rounding to
 // T, so the FP32 matrix carries relative, not absolute, rounding.
 #include "kernels.hpp"
+#include "blocked_solve.hpp"
 
 #include <algorithm>
 #include <type_traits>
@@ -161,13 +162,13 @@ __global__ __launch_bounds__(CNT) void k_factor_chol(Grid g, const int32_t* __re
 // FP64 solution is refined: r = B - A X with A recomputed in FP64, X += solve(r),
 // until the correction is below tol * |X| (or max_iter); status 1 = not converged.
 template <typename TL, int KT, int NT>
-__global__ __launch_bounds__(NT) void k_solve_chol(Grid g, const int32_t* __restrict__ list, const int32_t* __restrict__ pn,
+__global__ __launch_bounds__(NT, 3) void k_solve_chol(Grid g, const int32_t* __restrict__ list, const int32_t* __restrict__ pn,
                                                     const int64_t* __restrict__ mem_off, const int64_t* __restrict__ fac_off,
                                                     const int32_t* __restrict__ mem_pool, const double* __restrict__ xd,
                                                     const TL* __restrict__ fac, const double* __restrict__ values, int nf,
                                                     double* __restrict__ coef, int max_iter, double tol,
                                                     int32_t* __restrict__ status, int32_t* __restrict__ iters, int S,
-                                                    const double* __restrict__ peps) {
+                                                    const double* __restrict__ peps, const uint8_t* __restrict__ fnb) {
   // S: row stride of the LDS arrays (the largest patch of the launch)
   extern __shared__ __align__(16) unsigned char smem[];
   double* xr = reinterpret_cast<double*>(smem);      // [3][S] relative coords (FP64)
@@ -175,10 +176,12 @@ __global__ __launch_bounds__(NT) void k_solve_chol(Grid g, const int32_t* __rest
   double* y = b + (size_t)nf * S;                    // [nf][S] work vector
   double* xl = y + (size_t)nf * S;                   // [nf][S] the solution, kept here until the end
   __shared__ double red[NT / CW][8];
+  __shared__ double zb[SOLVE_NB * 8], lb[SOLVE_LB];
   const int p = list[blockIdx.x];
   const int n = pn[p];
   const int tid = threadIdx.x, lane = tid & (CW - 1), wid = tid / CW;
   if (n == 0 || nf == 0) return;
+  const int nbp = fnb ? fnb[p] : 0;   // matrix-core factor (16-column blocks): blocked solves
   if (n > S) {   // not sized for this patch: reported as not converged (refactored in FP64)
     if (tid == 0) {
       if (status) status[p] = 1;
@@ -220,6 +223,9 @@ __global__ __launch_bounds__(NT) void k_solve_chol(Grid g, const int32_t* __rest
     for (int f = 0; f < nf; f++)
       for (int i = tid; i < n; i += NT) y[f * S + i] = b[f * S + i];
     __syncthreads();
+    if (nbp == SOLVE_NB) {
+      llt_solve_blocked<TL, NT>(L, n, y, S, nf, zb, lb);
+    } else {
     // forward: L z = b (column oriented).  A thread holds rows k+1+tid and
     // k+1+tid+NT of column k (n <= 2 NT); the next column's entries are loaded one
     // step ahead, so the global loads overlap the current step instead of
@@ -293,6 +299,7 @@ __global__ __launch_bounds__(NT) void k_solve_chol(Grid g, const int32_t* __rest
       }
       dkb = pd;
       __syncthreads();
+    }
     }
     // X += d; convergence test on max |d| / max |X|
     double dmax = 0.0, xmax = 0.0;
@@ -606,7 +613,7 @@ template <typename TL, int KT>
 static void solve_chol_k(const Grid& g, const int32_t* list, int nlist, const int32_t* pn, const int64_t* mem_off,
                          const int64_t* fac_off, const int32_t* mem_pool, const double* xd, const TL* fac,
                          const double* values, int nf, double* coef, int max_iter, double tol, int32_t* status,
-                         int32_t* iters, double* ws, int nmax, const double* peps, hipStream_t s) {
+                         int32_t* iters, double* ws, int nmax, const double* peps, const uint8_t* fnb, hipStream_t s) {
   if (ws) {
     hipLaunchKernelGGL((k_solve_chol_big<TL, KT>), dim3(nlist), dim3(CNT), 0, s, g, list, pn, mem_off, fac_off, mem_pool,
                        xd, fac, values, nf, coef, max_iter, tol, ws, status, iters, peps);
@@ -621,7 +628,7 @@ static void solve_chol_k(const Grid& g, const int32_t* list, int nlist, const in
     attr = true;
   }
   hipLaunchKernelGGL((k_solve_chol<TL, KT, SNT>), dim3(nlist), dim3(SNT), sm, s, g, list, pn, mem_off, fac_off, mem_pool,
-                     xd, fac, values, nf, coef, max_iter, tol, status, iters, S, peps);
+                     xd, fac, values, nf, coef, max_iter, tol, status, iters, S, peps, fnb);
 }
 
 // positive definite kernels; r^3 only on the null space of a polynomial block
@@ -655,8 +662,12 @@ static void factor_chol_dispatch(const Grid& g, int ktype, const int32_t* list, 
 
 void launch_factor_chol(const Grid& g, int ktype, bool fp32, const int32_t* list, int nlist, const int32_t* pn,
                         const int64_t* mem_off, const int64_t* fac_off, const int32_t* mem_pool, const double* xd,
-                        void* fac, int32_t* status, bool big, const AugView& a, hipStream_t s) {
+                        void* fac, int32_t* status, bool big, const AugView& a, int nb, int nmax, hipStream_t s) {
   if (nlist <= 0) return;
+  if (!big && nb > 0) {
+    launch_factor_chol_mfma(g, ktype, fp32, nmax, list, nlist, pn, mem_off, fac_off, mem_pool, xd, fac, status, a, s);
+    return;
+  }
   if (fp32) factor_chol_dispatch<float>(g, ktype, list, nlist, pn, mem_off, fac_off, mem_pool, xd, (float*)fac, status, big, a, s);
   else factor_chol_dispatch<double>(g, ktype, list, nlist, pn, mem_off, fac_off, mem_pool, xd, (double*)fac, status, big, a, s);
 }
@@ -665,8 +676,9 @@ template <typename TL>
 static void solve_chol_dispatch(const Grid& g, int ktype, const int32_t* list, int nlist, const int32_t* pn,
                                 const int64_t* mem_off, const int64_t* fac_off, const int32_t* mem_pool, const double* xd,
                                 const TL* fac, const double* values, int nf, double* coef, int max_iter, double tol,
-                                int32_t* status, int32_t* iters, double* ws, int nmax, const double* peps, hipStream_t s) {
-#define LRB_SC(K) solve_chol_k<TL, K>(g, list, nlist, pn, mem_off, fac_off, mem_pool, xd, fac, values, nf, coef, max_iter, tol, status, iters, ws, nmax, peps, s)
+                                int32_t* status, int32_t* iters, double* ws, int nmax, const double* peps,
+                                const uint8_t* fnb, hipStream_t s) {
+#define LRB_SC(K) solve_chol_k<TL, K>(g, list, nlist, pn, mem_off, fac_off, mem_pool, xd, fac, values, nf, coef, max_iter, tol, status, iters, ws, nmax, peps, fnb, s)
   switch (ktype) {
     case K_WC0: LRB_SC(K_WC0); break;
     case K_WC2: LRB_SC(K_WC2); break;
@@ -681,8 +693,8 @@ void launch_solve_chol(const Grid& g, int ktype, bool fp32, const int32_t* list,
                        const void* fac, const double* values, int nf, double* coef, int max_iter, double tol,
                        int32_t* status, int32_t* iters, double* ws, int nmax, const AugView& a, hipStream_t s) {
   if (nlist <= 0) return;
-  if (fp32) solve_chol_dispatch<float>(g, ktype, list, nlist, pn, mem_off, fac_off, mem_pool, xd, (const float*)fac, values, nf, coef, max_iter, tol, status, iters, ws, nmax, a.eps, s);
-  else solve_chol_dispatch<double>(g, ktype, list, nlist, pn, mem_off, fac_off, mem_pool, xd, (const double*)fac, values, nf, coef, max_iter, tol, status, iters, ws, nmax, a.eps, s);
+  if (fp32) solve_chol_dispatch<float>(g, ktype, list, nlist, pn, mem_off, fac_off, mem_pool, xd, (const float*)fac, values, nf, coef, max_iter, tol, status, iters, ws, nmax, a.eps, a.fnb, s);
+  else solve_chol_dispatch<double>(g, ktype, list, nlist, pn, mem_off, fac_off, mem_pool, xd, (const double*)fac, values, nf, coef, max_iter, tol, status, iters, ws, nmax, a.eps, a.fnb, s);
 }
 
 }  // namespace lrb
