@@ -7,9 +7,12 @@ between t = 0 and 1 moves every particle along its streamline by the parameter
 s(t) = t - t^2, which returns to 0 at t = 1, so particles come back to their
 start; without a blend the rotation field's flow is a rotation; particles are
 independent (alone or together, bit for bit); a tighter tolerance converges.
-GPU tests compare the library with the oracle: positions 1e-8 relative (a step
-decision that flips near the acceptance threshold moves a position by at most
-the local tolerance), accepted and rejected counts within 1%."""
+GPU tests compare the library with the oracle per particle: at least 99% of the
+trajectories agree to 1e-11 relative (rounding of the velocities only); a step
+decision that flips near the acceptance threshold (the velocities differ in the
+last bits, so an error estimate at 1.0 can fall either side) moves that
+particle by at most the local tolerance, so every position is within rtol
+relative; accepted and rejected counts within 1%."""
 import numpy as np
 import pytest
 
@@ -119,12 +122,15 @@ def test_adaptive_gpu_matches_oracle():
     xg = np.zeros_like(P)
     g.output(coords=xg)
     xo, co = _oracle_run(o, X, F, P, 0.0, 0.05)
-    rel = np.max(np.abs(xg - xo)) / np.max(np.abs(xo))
-    print(f"adaptive gpu vs oracle: rel {rel:.3e}, counts gpu {cg} oracle {co}", flush=True)
+    per = np.max(np.abs(xg - xo), axis=1) / np.max(np.abs(xo))
+    same = float(np.mean(per < 1e-11))
+    print(f"adaptive gpu vs oracle: rel max {per.max():.3e}, {same * 100:.2f}% of particles within 1e-11, "
+          f"counts gpu {cg} oracle {co}", flush=True)
     assert cg["active"] == 0 and co["active"] == 0
     assert abs(cg["accepted"] - co["accepted"]) <= 0.01 * co["accepted"]
     assert abs(cg["rejected"] - co["rejected"]) <= 0.01 * max(co["rejected"], 100)
-    assert rel < 1e-8
+    assert same >= 0.99
+    assert per.max() < o["dopri_rtol"]
 
 
 @pytest.mark.gpu
