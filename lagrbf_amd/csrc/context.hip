@@ -166,6 +166,12 @@ static int ctx_from_options(Ctx& c, const OptView& o) {
   c.chol_big_min = chol_max_n();
   o.get_int("lagrbf_chol_big_min", c.chol_big_min, bad);
   c.chol_big_min = std::max(0, std::min(c.chol_big_min, chol_max_n()));
+  std::string bf;
+  if (o.get_string("lagrbf_big_factor", bf)) {
+    if (bf == "tiled") c.big_factor = 1;
+    else if (bf == "vector") c.big_factor = 0;
+    else return set_err(E_WRONG, fn, "-lagrbf_big_factor must be tiled or vector (found %s)", bf.c_str());
+  }
   o.get_bool("lagrbf_reorder", c.reorder, bad);
   if (bad) return set_err(E_WRONG, fn, "malformed -lagrbf_reorder option");
   o.get_bool("lagrbf_graph", c.use_graph, bad);
@@ -502,8 +508,114 @@ static int factor_nb_for(Ctx& c, bool fp32, int bucket, const int32_t* dlist, in
   return nb_best;
 }
 
-// Factor the new patches of each method; FP32/FP64 Cholesky breakdowns are
-// refactored with Bunch-Kaufman in FP64 (counted in "fallback").
+// Large patches of Cholesky method m (host list l, the same list on the device at
+// dl): the tiled factor over many workgroups, or one workgroup per patch with
+// -lagrbf_big_factor vector.  Records each patch's panel width for its solve.
+static int factor_big(Ctx& c, int m, const std::vector<int32_t>& l, const int32_t* dl) {
+  if (l.empty()) return 0;
+  const AugView a = aug_view(c, method_aug(m));
+  void* fac = method_fp32(m) ? (void*)c.fac32_pool.p : (void*)c.fac_pool.p;
+  if (!c.big_factor) {
+    for (int p : l) c.h_fnb[p] = 0;
+    launch_factor_chol(c.g, c.ktype, method_fp32(m), dl, (int)l.size(), c.p_n.p, c.p_mem_off.p, c.p_fac_off.p,
+                       c.mem_pool.p, c.dx.p, fac, c.pstatus.p, true, a, 0, 0, c.s);
+    return 0;
+  }
+  std::vector<int32_t> nn(l.size());
+  for (size_t i = 0; i < l.size(); i++) nn[i] = c.h_n[l[i]] - (method_aug(m) ? c.h_m[l[i]] : 0);
+  // an earlier tiled launch may still read the task buffer
+  HIPCHK(hipStreamSynchronize(c.s));
+  tile_tasks(l, nn, c.h_tile_tasks, c.h_tile_off);
+  HIPCHK(c.tile_tasks.reserve(c.h_tile_tasks.size()));
+  HIPCHK(hipMemcpyAsync(c.tile_tasks.p, c.h_tile_tasks.data(), sizeof(int2) * c.h_tile_tasks.size(),
+                        hipMemcpyHostToDevice, c.s));
+  launch_factor_tiled(c.g, c.ktype, method_fp32(m), c.tile_tasks.p, c.h_tile_off, c.p_n.p, c.p_mem_off.p,
+                      c.p_fac_off.p, c.mem_pool.p, c.dx.p, fac, c.pstatus.p, a, c.s);
+  for (int p : l) c.h_fnb[p] = 16;   // 16-column blocks with stored inverses (blocked_solve.hpp)
+  return 0;
+}
+
+// Refactor the listed patches in FP64 after an FP32 breakdown or a refinement
+// that did not converge (counted in "fallback"): large patches of an FP32
+// Cholesky method with the FP64 tiled Cholesky (a Bunch-Kaufman factor of
+// thousands of members in one workgroup would take minutes), the others, and any
+// whose FP64 Cholesky breaks down, with Bunch-Kaufman.  solve: also solve them.
+static int refactor_fp64(Ctx& c, const std::vector<int32_t>& ps, bool solve) {
+  if (ps.empty()) return 0;
+  const int P = c.g.P;
+  const AugView ap = aug_view(c, true), ae = aug_view(c, false);
+  c.fallback += (double)ps.size();
+  std::vector<int32_t> big[2], bk;   // big[1]: with a polynomial block
+  for (int p : ps) {
+    const int mc = c.h_method[p];
+    if (c.big_factor && method_big(mc) && method_fp32(mc)) {
+      const int a = method_aug(mc) ? 1 : 0;
+      big[a].push_back(p);
+      c.h_method[p] = (uint8_t)(a ? M_AUG64B : M_CHOL64B);
+      c.big_refactor64 += 1;
+    } else {
+      bk.push_back(p);
+      if (method_aug(mc)) c.aug_fallback_bk += 1;
+      c.h_method[p] = M_BK64;
+    }
+  }
+  for (int a = 0; a < 2; a++) CHK(alloc_factor_slots(c, big[a], a ? M_AUG64B : M_CHOL64B));
+  HIPCHK(hipMemcpyAsync(c.p_fac_off.p, c.h_fac_off.data(), sizeof(int64_t) * P, hipMemcpyHostToDevice, c.s));
+  for (int a = 0; a < 2; a++) {
+    if (big[a].empty()) continue;
+    const int m = a ? M_AUG64B : M_CHOL64B;
+    CHK(upload_list(c, big[a], m));
+    CHK(factor_big(c, m, big[a], c.list.p + (size_t)m * P));
+  }
+  HIPCHK(hipMemcpyAsync(c.p_fnb.p, c.h_fnb.data(), P, hipMemcpyHostToDevice, c.s));
+  if (!big[0].empty() || !big[1].empty()) {
+    std::vector<int32_t> st(P);
+    HIPCHK(hipMemcpyAsync(st.data(), c.pstatus.p, sizeof(int32_t) * P, hipMemcpyDeviceToHost, c.s));
+    HIPCHK(hipStreamSynchronize(c.s));
+    for (int a = 0; a < 2; a++) {
+      std::vector<int32_t> keep;
+      for (int p : big[a]) {
+        if (st[p] == 0) { keep.push_back(p); continue; }
+        bk.push_back(p);
+        if (a) c.aug_fallback_bk += 1;
+        c.h_method[p] = M_BK64;
+      }
+      big[a].swap(keep);
+    }
+  }
+  if (!bk.empty()) {
+    CHK(alloc_factor_slots(c, bk, M_BK64));
+    HIPCHK(hipMemcpyAsync(c.p_fac_off.p, c.h_fac_off.data(), sizeof(int64_t) * P, hipMemcpyHostToDevice, c.s));
+    CHK(upload_list(c, bk, 0));
+    launch_factor_bk(c.g, c.ktype, c.list.p, (int)bk.size(), c.p_n.p, c.p_mem_off.p, c.p_fac_off.p, c.mem_pool.p,
+                     c.dx.p, c.fac_pool.p, c.piv_pool.p, c.info.p, ap, c.s);
+    if (solve)
+      launch_solve_bk(c.list.p, (int)bk.size(), c.p_n.p, c.p_mem_off.p, c.p_fac_off.p, c.mem_pool.p, c.fac_pool.p,
+                      c.piv_pool.p, c.dval.p, c.nf, c.coef_pool.p, ap, c.s);
+  }
+  if (solve) {
+    for (int a = 0; a < 2; a++) {
+      if (big[a].empty()) continue;
+      const int m = a ? M_AUG64B : M_CHOL64B;
+      int nmax = 1;
+      for (int p : big[a]) nmax = std::max(nmax, (int)c.h_n[p]);
+      CHK(upload_list(c, big[a], m));
+      if (a)
+        launch_solve_aug(c.g, c.ktype, false, c.list.p + (size_t)m * P, (int)big[a].size(), c.p_n.p, c.p_mem_off.p,
+                         c.p_fac_off.p, c.mem_pool.p, c.dx.p, c.fac_pool.p, c.dval.p, c.nf, c.coef_pool.p,
+                         c.refine_max_iter, c.refine_tol, c.pstatus.p, c.piters.p, c.solve_ws.p, nmax, ap, c.s);
+      else
+        launch_solve_chol(c.g, c.ktype, false, c.list.p + (size_t)m * P, (int)big[a].size(), c.p_n.p, c.p_mem_off.p,
+                          c.p_fac_off.p, c.mem_pool.p, c.dx.p, c.fac_pool.p, c.dval.p, c.nf, c.coef_pool.p,
+                          c.refine_max_iter, c.refine_tol, c.pstatus.p, c.piters.p, c.solve_ws.p, nmax, ae, c.s);
+    }
+  }
+  HIPCHK(hipStreamSynchronize(c.s));
+  return 0;
+}
+
+// Factor the new patches of each method; Cholesky breakdowns are refactored in
+// FP64 (refactor_fp64, counted in "fallback").
 static int factor_lists(Ctx& c, std::vector<int32_t>* fl) {
   const int P = c.g.P;
   const AugView ap = aug_view(c, true), ae = aug_view(c, false);
@@ -534,9 +646,7 @@ static int factor_lists(Ctx& c, std::vector<int32_t>* fl) {
       const AugView& a = method_aug(m) ? ap : ae;
       const int32_t* dl = c.list.p + (size_t)m * P;
       if (method_big(m)) {
-        for (int p : fl[m]) c.h_fnb[p] = 0;
-        launch_factor_chol(c.g, c.ktype, method_fp32(m), dl, (int)fl[m].size(), c.p_n.p, c.p_mem_off.p, c.p_fac_off.p,
-                           c.mem_pool.p, c.dx.p, fac, c.pstatus.p, true, a, 0, 0, c.s);
+        CHK(factor_big(c, m, fl[m], dl));
         continue;
       }
       for (int b = 0; b < 3; b++) {
@@ -555,20 +665,8 @@ static int factor_lists(Ctx& c, std::vector<int32_t>* fl) {
   HIPCHK(hipStreamSynchronize(c.s));
   std::vector<int32_t> redo;
   for (int m = M_CHOL64; m < N_METHODS; m++)
-    for (int p : fl[m]) if (st[p] != 0) {
-      redo.push_back(p);
-      if (method_aug(m)) c.aug_fallback_bk += 1;
-      c.h_method[p] = M_BK64;
-    }
-  if (redo.empty()) return 0;
-  c.fallback += (double)redo.size();
-  CHK(alloc_factor_slots(c, redo, M_BK64));
-  HIPCHK(hipMemcpyAsync(c.p_fac_off.p, c.h_fac_off.data(), sizeof(int64_t) * P, hipMemcpyHostToDevice, c.s));
-  CHK(upload_list(c, redo, 0));
-  launch_factor_bk(c.g, c.ktype, c.list.p, (int)redo.size(), c.p_n.p, c.p_mem_off.p, c.p_fac_off.p, c.mem_pool.p,
-                   c.dx.p, c.fac_pool.p, c.piv_pool.p, c.info.p, ap, c.s);
-  HIPCHK(hipStreamSynchronize(c.s));
-  return 0;
+    for (int p : fl[m]) if (st[p] != 0) redo.push_back(p);
+  return refactor_fp64(c, redo, false);
 }
 
 // Solve the coefficients of the listed patches with each patch's factor; an FP32
@@ -620,23 +718,10 @@ static int solve_list(Ctx& c, const std::vector<int32_t>& sl) {
     c.refine_iters_max = std::max(c.refine_iters_max, (double)its[p]);
     c.refine_iters_sum += its[p];
     c.refined_patches += 1;
-    if (st[p] != 0) {
-      redo.push_back(p);
-      if (method_aug(c.h_method[p])) c.aug_fallback_bk += 1;
-      c.h_method[p] = M_BK64;
-    }
+    if (its[p] >= 0 && its[p] < (int)c.refine_hist.size()) c.refine_hist[its[p]] += 1;
+    if (st[p] != 0) redo.push_back(p);
   }
-  if (redo.empty()) return 0;
-  c.fallback += (double)redo.size();
-  CHK(alloc_factor_slots(c, redo, M_BK64));
-  HIPCHK(hipMemcpyAsync(c.p_fac_off.p, c.h_fac_off.data(), sizeof(int64_t) * P, hipMemcpyHostToDevice, c.s));
-  CHK(upload_list(c, redo, 0));
-  launch_factor_bk(c.g, c.ktype, c.list.p, (int)redo.size(), c.p_n.p, c.p_mem_off.p, c.p_fac_off.p, c.mem_pool.p,
-                   c.dx.p, c.fac_pool.p, c.piv_pool.p, c.info.p, ap, c.s);
-  launch_solve_bk(c.list.p, (int)redo.size(), c.p_n.p, c.p_mem_off.p, c.p_fac_off.p, c.mem_pool.p, c.fac_pool.p,
-                  c.piv_pool.p, c.dval.p, c.nf, c.coef_pool.p, ap, c.s);
-  HIPCHK(hipStreamSynchronize(c.s));
-  return 0;
+  return refactor_fp64(c, redo, true);
 }
 
 // LagRBF_PartitionsSetUp (partition.c:417-478): touch the corner patches of every
@@ -1897,6 +1982,12 @@ int lagrbf_get_info(void* ctx, const char* key, double* value) {
   else if (k == "eps_w") *value = c->g.eps_w;
   else if (k == "eps_r") *value = c->g.eps_r;
   else if (k == "aug_fallback_bk") *value = c->aug_fallback_bk;
+  else if (k == "big_refactor64") *value = c->big_refactor64;
+  else if (k.rfind("refine_hist_", 0) == 0) {
+    // refined patches that took i iterations (FP32 factors)
+    const int i = std::atoi(k.c_str() + 12);
+    *value = (i >= 0 && i < (int)c->refine_hist.size()) ? c->refine_hist[i] : 0.0;
+  }
   else if (k == "poly_terms") *value = c->poly_mm;
   else if (k == "factor_trial_ms") *value = c->factor_trial_ms;
   else return set_err(E_WRONG, "lagrbf_get_info", "unknown key '%s'", key);

@@ -156,6 +156,11 @@ struct Ctx {
   int factor_mode = 0;                     // 0 auto (Cholesky for PD kernels), 1 Bunch-Kaufman always
   bool factors_released = false;           // serving datasets keep coefficients only
   int chol_big_min = 512;                  // patches above this size use the large-patch Cholesky kernels
+  int big_factor = 1;                      // large patches: 1 tiled over many workgroups (matrix cores), 0 one
+                                           // workgroup per patch (vector ALU)
+  DBuf<int2> tile_tasks;                   // (patch, row tile) pairs of the tiled factor, step by step
+  std::vector<int2> h_tile_tasks;
+  std::vector<int64_t> h_tile_off;         // steps: tasks h_tile_off[k] .. h_tile_off[k + 1]
   int refine_max_iter = 10;
   double refine_tol = 1e-12;
   double fallback = 0, refine_iters_max = 0, refine_iters_sum = 0, refined_patches = 0;
@@ -191,6 +196,8 @@ struct Ctx {
   DBuf<double> ptab, ptabg[2];             // polynomial tables of the evaluation (main, blend groups)
   int ptab_gen = -1, ptabg_gen[2] = {-1, -1};
   double aug_fallback_bk = 0;              // saddle-point patches refactored with Bunch-Kaufman
+  double big_refactor64 = 0;               // large FP32-factored patches refactored with the FP64 tiled Cholesky
+  std::vector<double> refine_hist = std::vector<double>(64, 0.0);   // refined patches by iteration count
   // factor kernel of the small-patch Cholesky per size bucket: -1 tune (default),
   // 0 vector ALU, 16 / 32 matrix cores with panels of that width (-lagrbf_factor_nb)
   int factor_nb = -1;

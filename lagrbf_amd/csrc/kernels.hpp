@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <vector>
 #include "common.hpp"
 
 namespace lrb {
@@ -105,6 +106,16 @@ int chol_mfma_max_n();
 void launch_factor_chol_mfma(const Grid& g, int ktype, bool fp32, int nmax, const int32_t* list, int nlist,
                              const int32_t* pn, const int64_t* mem_off, const int64_t* fac_off, const int32_t* mem_pool,
                              const double* xd, void* fac, int32_t* status, const AugView& a, hipStream_t s);
+// (chol_tiled.hip) Cholesky of large patches over many workgroups, tiles of
+// TILE_NB columns: tile_tasks lists, for each tile column k, the (patch, row tile
+// s >= k) pairs of every patch with more than k tiles (off[k] .. off[k + 1]); the
+// factor slot is laid out as by launch_factor_chol_mfma (blocked solves apply)
+constexpr int TILE_NB = 64;
+void tile_tasks(const std::vector<int32_t>& list, const std::vector<int32_t>& nn, std::vector<int2>& tasks,
+                std::vector<int64_t>& off);
+void launch_factor_tiled(const Grid& g, int ktype, bool fp32, const int2* tasks, const std::vector<int64_t>& off,
+                         const int32_t* pn, const int64_t* mem_off, const int64_t* fac_off, const int32_t* mem_pool,
+                         const double* xd, void* fac, int32_t* status, const AugView& a, hipStream_t s);
 // (aug.hip) null-space data of the listed patches with a polynomial block
 void launch_aug_prep(const Grid& g, int ktype, const int32_t* list, int nlist, const int32_t* pn,
                      const int64_t* mem_off, const int32_t* mem_pool, const double* xd, const AugView& a, hipStream_t s);

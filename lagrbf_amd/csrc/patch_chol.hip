@@ -480,9 +480,11 @@ __global__ __launch_bounds__(CNT) void k_solve_chol_big(Grid g, const int32_t* _
                                                         const double* __restrict__ values, int nf, double* __restrict__ coef,
                                                         int max_iter, double tol, double* __restrict__ ws,
                                                         int32_t* __restrict__ status, int32_t* __restrict__ iters,
-                                                        const double* __restrict__ peps) {
+                                                        const double* __restrict__ peps,
+                                                        const uint8_t* __restrict__ fnb) {
   __shared__ double red[CNT / CW][8];
   __shared__ double zk_s[8];
+  __shared__ double zb[SOLVE_NB * 8], lb[SOLVE_LB];
   const int p = list[blockIdx.x];
   const int n = pn[p];
   const int tid = threadIdx.x, lane = tid & (CW - 1), wid = tid / CW;
@@ -520,6 +522,10 @@ __global__ __launch_bounds__(CNT) void k_solve_chol_big(Grid g, const int32_t* _
     for (int f = 0; f < nf; f++)
       for (int i = tid; i < n; i += CNT) y[(size_t)f * n + i] = b[(size_t)f * n + i];
     __syncthreads();
+    if (fnb && fnb[p] == SOLVE_NB) {
+      // factor of the tiled matrix-core kernel: blocked solves on its inverted diagonal blocks
+      llt_solve_blocked<TL, CNT>(L, n, y, n, nf, zb, lb);
+    } else {
     for (int k = 0; k < n; k++) {
       if (tid == 0) {
         const double lkk = (double)L[(size_t)k * n + k];
@@ -553,6 +559,7 @@ __global__ __launch_bounds__(CNT) void k_solve_chol_big(Grid g, const int32_t* _
         y[(size_t)tid * n + k] = (y[(size_t)tid * n + k] - t) / (double)L[(size_t)k * n + k];
       }
       __syncthreads();
+    }
     }
     double dmax = 0.0, xmax = 0.0;
     for (int i = tid; i < n; i += CNT)
@@ -616,7 +623,7 @@ static void solve_chol_k(const Grid& g, const int32_t* list, int nlist, const in
                          int32_t* iters, double* ws, int nmax, const double* peps, const uint8_t* fnb, hipStream_t s) {
   if (ws) {
     hipLaunchKernelGGL((k_solve_chol_big<TL, KT>), dim3(nlist), dim3(CNT), 0, s, g, list, pn, mem_off, fac_off, mem_pool,
-                       xd, fac, values, nf, coef, max_iter, tol, ws, status, iters, peps);
+                       xd, fac, values, nf, coef, max_iter, tol, ws, status, iters, peps, fnb);
     return;
   }
   static bool attr = false;
