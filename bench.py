@@ -327,8 +327,11 @@ def serve_bench(L, precision, dsets, nreq, rate, seed=20261203 + 5, budget=200e9
 
 
 def c3_bench(L, precision, steps, warmup):
-    """Uneven batching: 1,000,000 uniform nodes + 64 Gaussian clusters of 15,625
-    (sigma in [0.03, 0.05]), PU 49^3, Wendland C2, 8,388,608 particles, RK2 dt=1e-3."""
+    """Uneven batching as written: 1,000,000 uniform nodes + 64 Gaussian clusters of
+    15,625 (sigma in [0.01, 0.05]), PU 49^3, Gaussian kernel with eps = 3 / h_local
+    per patch and a quadratic polynomial block, 8,388,608 particles, RK2 dt=1e-3.
+    Reports the patch-size histogram, the fallbacks and the distribution of
+    refinement iterations."""
     from tests import cases
     opts, X, V, P = cases.clustered_c3()
     ctx = L.LagRBFCtx(petsc_options=dict(opts), precision=precision)
@@ -343,21 +346,35 @@ def c3_bench(L, precision, steps, warmup):
     ctx.advect_steps_c(steps, 1e-3)
     wall = time.perf_counter() - t1
     ev1, n1 = ctx.info("eval_ms"), ctx.info("eval_count")
+    removed = ctx.info("removed")
     cs = ctx.debug_get("cell_start")
     nc = opts["pu_mesh_n"][0] - 1
     sizes = patch_sizes(cs, nc)
+    edges = [0, 16, 64, 128, 256, 512, 1024, 2048, 4096, 8192, 1 << 30]
+    hist = {f"{a}-{b - 1}" if b < (1 << 30) else f">={a}": int(((sizes >= a) & (sizes < b)).sum())
+            for a, b in zip(edges[:-1], edges[1:])}
+    iters = {}
+    if precision == "mixed":
+        for i in range(64):
+            v = ctx.info(f"refine_hist_{i}")
+            if v:
+                iters[str(i)] = int(v)
     res = {"precision": precision, "nodes": int(X.shape[0]), "particles": int(P.shape[0]), "setup_s": t_setup,
            "setup_ms_patches": ctx.info("setup_ms"), "patches_setup": ctx.info("patches_setup"),
            "patch_size": {"min": int(sizes.min()), "p50": float(np.median(sizes)), "p99": float(np.percentile(sizes, 99)),
-                          "max": int(sizes.max()), "above_512": int((sizes > 512).sum())},
-           "fallback_patches": ctx.info("fallback"), "refine_iters_max": ctx.info("refine_iters"),
-           "particle_rk2_steps_per_s": P.shape[0] * steps / wall, "ms_per_step": wall / steps * 1e3,
-           "eval_ms": (ev1 - ev0) / max(n1 - n0, 1), "removed": ctx.info("removed")}
+                          "max": int(sizes.max()), "above_512": int((sizes > 512).sum()), "histogram": hist},
+           "fallback_patches": ctx.info("fallback"), "refactored_fp64_tiled": ctx.info("big_refactor64"),
+           "refactored_bunch_kaufman": ctx.info("aug_fallback_bk"), "refine_iters_max": ctx.info("refine_iters"),
+           "refine_iters_histogram": iters,
+           "particle_rk2_steps_per_s": (P.shape[0] - removed) * steps / wall, "ms_per_step": wall / steps * 1e3,
+           "eval_ms": (ev1 - ev0) / max(n1 - n0, 1), "removed": removed}
     ctx.destroy()
     if precision == "mixed":
         res["setup_s_fp64_factor"] = setup_with_fp64_factor(L, opts, X, V, P)
-    log(f"c3 [{precision}]: set-up {t_setup:.2f}s, patch sizes {res['patch_size']}, fallback {res['fallback_patches']:.0f}, "
-        f"{res['particle_rk2_steps_per_s']:.3e} particle-RK2-steps/s ({res['ms_per_step']:.1f} ms/step)")
+    log(f"c3 [{precision}]: set-up {t_setup:.2f}s, patch sizes {res['patch_size']}, fallback {res['fallback_patches']:.0f} "
+        f"(FP64 tiled {res['refactored_fp64_tiled']:.0f}, Bunch-Kaufman {res['refactored_bunch_kaufman']:.0f}), "
+        f"refinement iterations {iters}, {res['particle_rk2_steps_per_s']:.3e} particle-RK2-steps/s "
+        f"({res['ms_per_step']:.1f} ms/step)")
     return res
 
 
@@ -431,6 +448,7 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-only", action="store_true")
     ap.add_argument("--c3-only", action="store_true", help="the uneven-batching configuration only")
+    ap.add_argument("--c3-precision", default="both", choices=["both", "fp64", "mixed"])
     ap.add_argument("--no-c3", action="store_true", help="skip the uneven-batching measurement")
     ap.add_argument("--no-graph", action="store_true", help="skip the graph-replay measurement of the fixed-step loop")
     ap.add_argument("--no-wendland", action="store_true", help="skip the Wendland C2 variant of the workload")
@@ -454,7 +472,8 @@ def main():
     if args.copy_calibration > 0:
         copy_calibration(args.copy_calibration)
     if args.c3_only:
-        out = {p: c3_bench(L, p, args.steps, args.warmup) for p in ("mixed", "fp64")}
+        precs = ("mixed", "fp64") if args.c3_precision == "both" else (args.c3_precision,)
+        out = {p: c3_bench(L, p, args.steps, args.warmup) for p in precs}
         print(json.dumps({"c3": out}), flush=True)
         return
     if args.c4_only:
@@ -537,8 +556,9 @@ def main():
     if not args.no_c3:
         try:
             c3 = {p: c3_bench(L, p, 5, 1) for p in ("mixed", "fp64")}
-            c3["config"] = ("uneven batching: 1,000,000 uniform + 64 Gaussian clusters x 15,625 nodes (sigma 0.03-0.05), "
-                            "PU 49^3, Wendland C2, 8,388,608 particles, RK2 dt=1e-3, 5 steps")
+            c3["config"] = ("uneven batching: 1,000,000 uniform + 64 Gaussian clusters x 15,625 nodes (sigma 0.01-0.05), "
+                            "PU 49^3, Gaussian eps = 3/h_local + quadratic polynomial, 8,388,608 particles, RK2 dt=1e-3, "
+                            "5 steps")
         except Exception as e:
             c3 = {"error": str(e)}
             log(f"c3 failed: {e}")

@@ -540,6 +540,7 @@ static int factor_big(Ctx& c, int m, const std::vector<int32_t>& l, const int32_
 // Cholesky method with the FP64 tiled Cholesky (a Bunch-Kaufman factor of
 // thousands of members in one workgroup would take minutes), the others, and any
 // whose FP64 Cholesky breaks down, with Bunch-Kaufman.  solve: also solve them.
+static constexpr int BK_FALLBACK_MAX = 4096;
 static int refactor_fp64(Ctx& c, const std::vector<int32_t>& ps, bool solve) {
   if (ps.empty()) return 0;
   const int P = c.g.P;
@@ -584,6 +585,12 @@ static int refactor_fp64(Ctx& c, const std::vector<int32_t>& ps, bool solve) {
     }
   }
   if (!bk.empty()) {
+    // one workgroup factors a Bunch-Kaufman patch: beyond a few thousand rows that
+    // takes minutes, so such a patch is reported instead
+    for (int p : bk)
+      if (c.h_n[p] + c.h_m[p] > BK_FALLBACK_MAX)
+        return set_err(E_SUP, "PartitionFactor", "patch %d (%d members) is not positive definite in FP64 and exceeds "
+                       "the %d rows of the Bunch-Kaufman fallback", p, c.h_n[p], BK_FALLBACK_MAX);
     CHK(alloc_factor_slots(c, bk, M_BK64));
     HIPCHK(hipMemcpyAsync(c.p_fac_off.p, c.h_fac_off.data(), sizeof(int64_t) * P, hipMemcpyHostToDevice, c.s));
     CHK(upload_list(c, bk, 0));

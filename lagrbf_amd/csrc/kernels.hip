@@ -119,13 +119,23 @@ __global__ void k_scatter(const int32_t* __restrict__ cell, int n, int32_t* __re
   if (c >= 0) nodes[atomicAdd(&cursor[c], 1)] = i;
 }
 
-// Restore ascending node index inside each cell (the atomic scatter is unordered):
-// one wave per cell, every element's rank = number of smaller elements.
+// Restore ascending node index inside each cell (the atomic scatter is unordered).
+// Cells up to SEG_SMALL nodes: one wave per cell, every element's rank = number of
+// smaller elements.  Larger cells (clusters put thousands of nodes into one cell,
+// where the rank count is quadratic): listed, then one block per listed cell sorts
+// the cell in LDS (bitonic, padded to a power of two) up to SEG_LDS nodes, and
+// ranks in global memory with the whole block beyond.  Node ids are distinct, so
+// every variant gives the same order.
+static constexpr int SEG_SMALL = 256;
+static constexpr int SEG_LDS = 16384;
+static constexpr int SEG_NT = 1024;
+
 __global__ __launch_bounds__(256) void k_segsort(const int32_t* __restrict__ cell_start, int C,
                                                  const int32_t* __restrict__ unsorted, int32_t* __restrict__ sorted) {
   const int wid = (blockIdx.x * blockDim.x + threadIdx.x) / WAVE, lane = threadIdx.x & (WAVE - 1);
   if (wid >= C) return;
   const int b = cell_start[wid], e = cell_start[wid + 1], m = e - b;
+  if (m > SEG_SMALL) return;
   for (int t = lane; t < m; t += WAVE) {
     int v = unsorted[b + t], r = 0;
     for (int u = 0; u < m; u++) r += unsorted[b + u] < v;
@@ -133,9 +143,56 @@ __global__ __launch_bounds__(256) void k_segsort(const int32_t* __restrict__ cel
   }
 }
 
+// big[0]: number of cells above SEG_SMALL nodes (zeroed by the caller), big[1..]: the cells
+__global__ void k_big_cells(const int32_t* __restrict__ cell_start, int C, int32_t* __restrict__ big) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= C) return;
+  if (cell_start[c + 1] - cell_start[c] > SEG_SMALL) big[1 + atomicAdd(&big[0], 1)] = c;
+}
+
+__global__ __launch_bounds__(SEG_NT) void k_segsort_big(const int32_t* __restrict__ cell_start,
+                                                        const int32_t* __restrict__ big,
+                                                        const int32_t* __restrict__ unsorted,
+                                                        int32_t* __restrict__ sorted) {
+  __shared__ int32_t key[SEG_LDS];
+  const int tid = threadIdx.x;
+  const int nbig = big[0];
+  for (int bi = blockIdx.x; bi < nbig; bi += gridDim.x) {
+    const int cell = big[1 + bi];
+    const int beg = cell_start[cell], m = cell_start[cell + 1] - beg;
+    if (m <= SEG_LDS) {
+      int np2 = 1;
+      while (np2 < m) np2 <<= 1;
+      for (int i = tid; i < np2; i += SEG_NT) key[i] = i < m ? unsorted[beg + i] : INT32_MAX;
+      __syncthreads();
+      for (int k = 2; k <= np2; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+          for (int i = tid; i < np2; i += SEG_NT) {
+            const int ij = i ^ j;
+            if (ij > i) {
+              const int a = key[i], b2 = key[ij];
+              if ((a > b2) == ((i & k) == 0)) { key[i] = b2; key[ij] = a; }
+            }
+          }
+          __syncthreads();
+        }
+      for (int i = tid; i < m; i += SEG_NT) sorted[beg + i] = key[i];
+    } else {
+      for (int t = tid; t < m; t += SEG_NT) {
+        const int v = unsorted[beg + t];
+        int r = 0;
+        for (int u = 0; u < m; u++) r += unsorted[beg + u] < v;
+        sorted[beg + r] = v;
+      }
+    }
+    __syncthreads();
+  }
+}
+
 void launch_cell_lists(const int32_t* cell, int n, int C, int32_t* cell_start, int32_t* cell_nodes,
                        int32_t* scratch, hipStream_t s) {
-  // scratch layout: [cursor C+1][unsorted n][scan tmp]
+  // scratch layout: [cursor C+1][unsorted n][scan tmp]; after the scatter the cursor
+  // region holds the list of large cells
   int32_t* cursor = scratch;
   int32_t* unsorted = scratch + (C + 1);
   int32_t* tmp = unsorted + (n > 0 ? n : 1);
@@ -146,6 +203,10 @@ void launch_cell_lists(const int32_t* cell, int n, int C, int32_t* cell_start, i
   if (n > 0) {
     hipLaunchKernelGGL(k_scatter, dim3((n + 255) / 256), dim3(256), 0, s, cell, n, cursor, unsorted);
     hipLaunchKernelGGL(k_segsort, dim3((C + 3) / 4), dim3(256), 0, s, cell_start, C, unsorted, cell_nodes);
+    int32_t* big = cursor;
+    hipMemsetAsync(big, 0, sizeof(int32_t), s);
+    hipLaunchKernelGGL(k_big_cells, dim3((C + 255) / 256), dim3(256), 0, s, cell_start, C, big);
+    hipLaunchKernelGGL(k_segsort_big, dim3(1024), dim3(SEG_NT), 0, s, cell_start, big, unsorted, cell_nodes);
   }
 }
 

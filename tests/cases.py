@@ -83,11 +83,14 @@ def fourier_3d(nlat=102, N=None, Np=4096, n=32, kernel=2, scheme=2, solve_method
     return opts, np.ascontiguousarray(X), np.ascontiguousarray(V), P
 
 
-def clustered_c3(n_uniform=1000000, n_clusters=64, per_cluster=15625, sigma=(0.03, 0.05), Np=8388608, n=49,
-                 seed_off=0):
+def clustered_c3(n_uniform=1000000, n_clusters=64, per_cluster=15625, sigma=(0.01, 0.05), Np=8388608, n=49,
+                 kernel=6, deg=2, local_eps=3.0, seed_off=0):
     """Uneven-batching configuration: uniform nodes in [0,1]^3 plus Gaussian clusters
     (sigma ~ U[sigma], centres ~ U[0.1,0.9]^3), the 16-mode Fourier velocity, PU
-    n^3 nodes ((n-1)^3 cells), Wendland C2, Np uniform particles, RK2."""
+    n^3 nodes ((n-1)^3 cells), Np uniform particles, RK2.  As written: the Gaussian
+    kernel with eps = local_eps / h_local per patch (h_local = (volume of the
+    patch's cells / members)^(1/3)) and a quadratic polynomial block; kernel=2,
+    deg=-1, local_eps=0 gives the Wendland C2 variant."""
     rng = np.random.default_rng(SEED + 2 + seed_off)
     parts = [rng.random((n_uniform, 3))]
     for _ in range(n_clusters):
@@ -98,8 +101,12 @@ def clustered_c3(n_uniform=1000000, n_clusters=64, per_cluster=15625, sigma=(0.0
     V = fourier_velocity(X, fourier_modes())
     P = rng.random((Np, 3))
     opts = dict(pu_mesh_dim=3, pu_mesh_O=[0.0] * 3, pu_mesh_L=[1.0] * 3, pu_mesh_n=[n] * 3, n_interpolation_fields=3,
-                pu_mesh_overlap=1.0, pu_mesh_shape_factor=1.0, kernel_type=2, solve_method=0,
+                pu_mesh_overlap=1.0, pu_mesh_shape_factor=1.0, kernel_type=kernel, solve_method=0,
                 advection=True, advection_scheme=1, cfl_fac=1e6, dt=1e-3)
+    if deg >= 0:
+        opts["lagrbf_poly_degree"] = deg
+    if local_eps > 0:
+        opts["lagrbf_local_eps"] = local_eps
     return opts, X, np.ascontiguousarray(V), P
 
 

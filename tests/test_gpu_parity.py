@@ -229,6 +229,28 @@ def test_binning_bit_exact():
     assert rel_err(wg, wo) < 1e-13
 
 
+@pytest.mark.parametrize("per_cell", [300, 5000, 20000])
+def test_cell_lists_large_cells_bit_exact(per_cell):
+    """Cells far above 256 nodes (a cluster inside one cell of a 4^3 PU: the LDS
+    bitonic sort up to 16,384 nodes per cell, the block-wide rank count beyond)
+    next to ordinary cells: per-cell node lists in ascending node order (patch
+    membership concatenates them), bit-exact against the oracle."""
+    rng = np.random.default_rng(cases.SEED + 401)
+    X = np.concatenate([rng.random((3000, 3)), 0.4 + 0.2 * rng.random((per_cell, 3))])
+    X = np.ascontiguousarray(X[rng.permutation(X.shape[0])])
+    V = np.zeros_like(X)
+    opts = dict(pu_mesh_dim=3, pu_mesh_O=[0.0] * 3, pu_mesh_L=[1.0] * 3, pu_mesh_n=[4] * 3, n_interpolation_fields=3,
+                pu_mesh_overlap=1.0, kernel_type=2, solve_method=2)
+    print(f"case: cell lists, {per_cell} nodes in one cell", flush=True)
+    g, o = pair(opts)
+    for c in (g, o):
+        c.fill_data(X, V)
+    cs_o, cn_o = o.cell_lists(X.shape[0])
+    assert np.diff(cs_o).max() >= per_cell
+    assert np.array_equal(g.debug_get("cell_start"), cs_o)
+    assert np.array_equal(g.debug_get("cell_nodes"), cn_o)
+
+
 @pytest.mark.parametrize("dim,overlap", [(3, 0.9), (3, 1.0), (3, 1.7), (2, 1.0), (2, 3.0)])
 def test_weights_vs_brute_force(dim, overlap):
     """Shepard weights from the support window against the oracle's sum over every
