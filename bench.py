@@ -24,7 +24,8 @@ events on the library's stream.  r^3 has no compact support, so every counted
 (particle, node) pair is computed; the Wendland key's count is nominal (the
 kernel skips node rows outside the support).  HBM traffic per
 launch comes from rocprofv3 FETCH_SIZE / WRITE_SIZE runs stored beside this
-file in profiles/pu_evaluation_c2_traffic.json (null when absent).
+file in profiles/pu_evaluation_c2_phs3_traffic.json (null when absent, and for the
+Wendland key, which has no traffic profile).
 """
 import argparse
 import json
@@ -87,6 +88,8 @@ def patch_sizes(cs, nc):
 
 
 def load_traffic(precision, path=TRAFFIC_FILE):
+    if path is None:
+        return None
     try:
         with open(path) as f:
             t = json.load(f)[precision]
@@ -511,7 +514,7 @@ def main():
         wendland = {}
         for prec in paths:
             rw, _, _ = run_path(L, prec, opts_w, X, V, P, args.steps, args.warmup, dt, [],
-                                traffic_file=os.path.join(ROOT, "profiles", "pu_evaluation_c2_traffic.json"))
+                                traffic_file=None)
             wendland[prec] = {k: rw[k] for k in ("value", "ms_per_step", "setup_s", "roofline", "removed_particles")}
         wendland["config"] = WORKLOAD.replace("r^3 kernel + linear polynomial", "Wendland C2 (no polynomial)")
     c1 = None
