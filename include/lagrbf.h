@@ -113,7 +113,7 @@ int lagrbf_advect_adaptive(void *ctx, double t_start, double t_end, int32_t max_
  * (patches refactored in FP64 after FP32 refinement stalled), "refine_iters",
  * "patches_setup", "dt" (current advection dt), "eval_ms" and "eval_count"
  * (evaluation kernel time and launches timed, outside graphs), "eval_variant"
- * (launch shape 0..13 of the last cell-table evaluation, -1 before one),
+ * (launch shape 0..17 of the last cell-table evaluation, -1 before one),
  * "graph_ms" and "graph_steps" (replayed steps and their time), "setup_ms", "n_particles",
  * "n_nodes", "max_patch", "eps_w", "eps_r". */
 int lagrbf_get_info(void *ctx, const char *key, double *value);

@@ -282,6 +282,108 @@ __device__ __forceinline__ void accum_blk_pipe2(T xp, T yp, T zp, const T* __res
 #define LRB_EVAL_PIPE 1
 #endif
 
+// Matrix-core accumulation (FP32 tables, up to 4 fields): the node loop computes phi
+// on the vector ALUs as above (one lane per particle, node coordinates as scalar
+// loads), and each corner patch's accumulation phi * coefficient becomes one
+// v_mfma_f32_4x4x1_16b_f32 instead of NFT vector FMAs.  That instruction holds 16
+// independent 4 x 4 blocks, D[v] of lane 4b + j += A(lane 4b + v) * B(lane 4b + j):
+// lane 4b + i supplies A = the coefficient of field i (a vector load of the node row
+// at column i; column 3 reads the next entry and lands in an unused register) and
+// B = phi of its own particle, so its four result registers hold its own
+// particle's sums for fields 0 .. 3.  K = 1: each accumulation is one product added
+// with one rounding, as the vector FMA does it, so the values are the same bit for
+// bit as the vector-ALU loops'.
+typedef float ev_f4 __attribute__((ext_vector_type(4)));
+#ifndef LRB_MFMA_WAVES
+#define LRB_MFMA_WAVES 4
+#endif
+
+template <int DIM, int KT, int NFT, int NBI, int R>
+__device__ __forceinline__ void mfma_rows(float xp, float yp, float zp, const float* __restrict__ B,
+                                          ev_f4 (&acc)[1 << DIM], float eps, float rcut2,
+                                          const float (&e2)[1 << DIM], int lq) {
+  constexpr int NEN = 1 << DIM, BS = blk_stride<DIM, NFT>();
+  // every load of the R rows is issued before any of their arithmetic, so the
+  // rows share one memory latency
+  float c[R][3], cf[R][NEN];
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    c[r][0] = B[r * BS]; c[r][1] = B[r * BS + 1]; c[r][2] = DIM == 3 ? B[r * BS + 2] : 0.f;
+#pragma unroll
+    for (int k = 0; k < NEN; k++) {
+      const int kp = corner_in_nbr<DIM>(NBI, k);
+      cf[r][k] = kp >= 0 ? B[r * BS + 4 + kp * NFT + lq] : 0.f;
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < R; r++) {
+    const float d0 = xp - c[r][0], d1 = yp - c[r][1], d2 = DIM == 3 ? zp - c[r][2] : 0.f;
+    const float r2 = dist2(d0, d1, d2);
+    bool use = true;
+    if constexpr (compact_support_blk<KT>()) use = __any(r2 < rcut2);
+    if (use) {
+      float phi = 0.f;
+      if constexpr (KT != K_GAUSS_LOCAL) phi = phi_r2<KT, float>(r2, eps);
+#pragma unroll
+      for (int k = 0; k < NEN; k++) {
+        const int kp = corner_in_nbr<DIM>(NBI, k);
+        if (kp >= 0) {
+          if constexpr (KT == K_GAUSS_LOCAL) phi = exp(-e2[k] * r2);
+          acc[k] = __builtin_amdgcn_mfma_f32_4x4x1f32(cf[r][k], phi, acc[k], 0, 0, 0);
+        }
+      }
+    }
+  }
+}
+
+// The m rows of neighbour NBI in batches of MR (fewer corners: larger batches fit
+// the registers), the tail one row at a time; same node order as accum_blk.
+template <int DIM, int KT, int NFT, int NBI>
+__device__ __forceinline__ void accum_mfma(float xp, float yp, float zp, const float* __restrict__ B, int m,
+                                           ev_f4 (&acc)[1 << DIM], float eps, float rcut2,
+                                           const float (&e2)[1 << DIM], int lq) {
+  constexpr int BS = blk_stride<DIM, NFT>();
+  constexpr int NK = n_corners_in_nbr<DIM>(NBI);
+  constexpr int MR = NK >= 8 ? 2 : (NK >= 4 ? 4 : 8);
+  int r = 0;
+  for (; r + MR <= m; r += MR) mfma_rows<DIM, KT, NFT, NBI, MR>(xp, yp, zp, B + (size_t)r * BS, acc, eps, rcut2, e2, lq);
+  for (; r < m; r++) mfma_rows<DIM, KT, NFT, NBI, 1>(xp, yp, zp, B + (size_t)r * BS, acc, eps, rcut2, e2, lq);
+}
+
+// The particles s0 .. s0 + 63 of cell (ci, cj, ck) with matrix-core accumulation:
+// register f of acc[k] ends as field f of corner k's sum for the lane's particle.
+template <int DIM, int KT, int NFT>
+__device__ __forceinline__ void eval_wave_mfma(const Grid& g, int ci, int cj, int ck, float xp, float yp, float zp,
+                                               const float* __restrict__ blk, const int32_t* __restrict__ cs,
+                                               float eps, float rcut2, const float (&e2)[1 << DIM], int lane,
+                                               ev_f4 (&acc)[1 << DIM]) {
+  constexpr int NNB = DIM == 3 ? 27 : 9;
+  constexpr int BS = blk_stride<DIM, NFT>();
+  const int lq = lane & 3;
+#pragma unroll 1
+  for (int nbi = 0; nbi < NNB; nbi++) {
+    const int di = nbi % 3 - 1, dj = (nbi / 3) % 3 - 1, dk = DIM == 3 ? nbi / 9 - 1 : 0;
+    const int a = ci + di, bb = cj + dj, e = ck + dk;
+    if (a < 0 || a >= g.nc[0] || bb < 0 || bb >= g.nc[1] || (DIM == 3 && (e < 0 || e >= g.nc[2]))) continue;
+    const int q = a + g.nc[0] * (bb + g.nc[1] * e);
+    const int nb = cs[q], ne = cs[q + 1];
+    if (nb == ne) continue;
+    const float qx = xp - (float)(di * g.h[0]), qy = yp - (float)(dj * g.h[1]);
+    const float qz = DIM == 3 ? zp - (float)(dk * g.h[2]) : 0.f;
+    const float* B = blk + (size_t)nb * BS;
+    switch (nbi) {
+#define LRB_MCASE(V) case V: if constexpr (V < NNB) accum_mfma<DIM, KT, NFT, V>(qx, qy, qz, B, ne - nb, acc, eps, rcut2, e2, lq); break;
+      LRB_MCASE(0) LRB_MCASE(1) LRB_MCASE(2) LRB_MCASE(3) LRB_MCASE(4) LRB_MCASE(5) LRB_MCASE(6) LRB_MCASE(7)
+      LRB_MCASE(8) LRB_MCASE(9) LRB_MCASE(10) LRB_MCASE(11) LRB_MCASE(12) LRB_MCASE(13) LRB_MCASE(14)
+      LRB_MCASE(15) LRB_MCASE(16) LRB_MCASE(17) LRB_MCASE(18) LRB_MCASE(19) LRB_MCASE(20) LRB_MCASE(21)
+      LRB_MCASE(22) LRB_MCASE(23) LRB_MCASE(24) LRB_MCASE(25) LRB_MCASE(26)
+#undef LRB_MCASE
+      default: break;
+    }
+  }
+}
+
+
 // The 3^d neighbour cells in ascending cell index, as one straight sequence of
 // loops specialised per neighbour (no dispatch between them, so the accumulators
 // keep their registers across the whole sequence).
@@ -437,6 +539,24 @@ void k_eval_blk(Grid g, const double* __restrict__ px, int64_t ld,
       xp = (T)(px[i] - ox); yp = (T)(px[ld + i] - oy); zp = DIM == 3 ? (T)(px[2 * ld + i] - oz) : T(0);
     } else {
       xp = px[i]; yp = px[ld + i]; zp = DIM == 3 ? px[2 * ld + i] : T(0);
+    }
+    if constexpr (PIPE == 3 && sizeof(T) == 4 && NFT <= 4) {
+      // matrix-core accumulation: register f of acc4[k] holds field f of the lane's particle
+      ev_f4 acc4[NEN];
+#pragma unroll
+      for (int k = 0; k < NEN; k++) acc4[k] = ev_f4{0.f, 0.f, 0.f, 0.f};
+      eval_wave_mfma<DIM, KT, NFT>(g, ci, cj, ck, xp, yp, zp, blk, cs, eps, rcut2, e2, lane, acc4);
+      if (!live) continue;
+      T acc[NEN][NFT];
+#pragma unroll
+      for (int k = 0; k < NEN; k++)
+#pragma unroll
+        for (int f = 0; f < NFT; f++) acc[k][f] = acc4[k][f];
+      double val[NFT];
+      combine_corners<DIM, NFT, T>(w, ld, i, acc, val);
+#pragma unroll
+      for (int f = 0; f < NFT; f++) if (f < nfields) out[(size_t)f * ld + i] = val[f];
+      continue;
     }
     T acc[NEN][NFT];
 #pragma unroll
@@ -660,6 +780,16 @@ static void eval_blk_k(const Grid& g, const double* px, int64_t ld, const int32_
   const int deal = group >= 0 ? 0 : -group;
 #define LRB_EB(NFT, P) LRB_LAUNCH_TIMED((k_eval_blk<DIM, KT, NFT, T, P, default_waves<T>()>), grid, blkd, s, g, px, ld, perm, pstart, w, blk, cs, nfields, out, cell_list, list_count, deal, ev)
   if constexpr (sizeof(T) == 4) {
+    // matrix-core accumulation: up to 4 fields
+    if (pipe == 3 && nft <= 4) {
+      switch (nft) {
+        case 1: LRB_LAUNCH_TIMED((k_eval_blk<DIM, KT, 1, T, 3, LRB_MFMA_WAVES>), grid, blkd, s, g, px, ld, perm, pstart, w, blk, cs, nfields, out, cell_list, list_count, deal, ev); break;
+        case 2: LRB_LAUNCH_TIMED((k_eval_blk<DIM, KT, 2, T, 3, LRB_MFMA_WAVES>), grid, blkd, s, g, px, ld, perm, pstart, w, blk, cs, nfields, out, cell_list, list_count, deal, ev); break;
+        case 3: LRB_LAUNCH_TIMED((k_eval_blk<DIM, KT, 3, T, 3, LRB_MFMA_WAVES>), grid, blkd, s, g, px, ld, perm, pstart, w, blk, cs, nfields, out, cell_list, list_count, deal, ev); break;
+        default: LRB_LAUNCH_TIMED((k_eval_blk<DIM, KT, 4, T, 3, LRB_MFMA_WAVES>), grid, blkd, s, g, px, ld, perm, pstart, w, blk, cs, nfields, out, cell_list, list_count, deal, ev); break;
+      }
+      return;
+    }
     // two rows per wait: the velocity evaluations (nft = dim) only
     if (pipe == 2 && nft == DIM) {
       if (waves == 6)

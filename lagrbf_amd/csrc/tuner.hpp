@@ -16,7 +16,7 @@
 
 namespace lrb {
 
-constexpr int N_EVAL_VARIANTS = 14;
+constexpr int N_EVAL_VARIANTS = 18;
 
 // variant -> (evaluate listed cells only, lane group size; 0 = one block per cell,
 // -K = one block per listed cell with runs of K consecutive listed cells dealt
@@ -25,12 +25,14 @@ constexpr int N_EVAL_VARIANTS = 14;
 // an L2; rows per wait of the FP32 node loop, 1 or 2: variants 6-8 are 0, 1 and 5
 // with two; FP32 occupancy floor in waves per SIMD, 0 = the default: variant 9 is
 // 8 with 6 waves, which leaves the loop its registers without spilling; variants
-// 10-11 are 5 and 12-13 are 9 in runs of 4 and 16 cells)
+// 10-11 are 5 and 12-13 are 9 in runs of 4 and 16 cells; pipe 3: the accumulation
+// on the matrix cores (FP32, up to 4 fields), variants 14-17 are 1, 5, 10 and 11
+// with it)
 inline void eval_variant(int v, bool& listed, int& group, int& pipe, int& waves) {
-  static const int G[N_EVAL_VARIANTS] = {0, 0, 32, 16, 8, -1, 0, 0, -1, -1, -4, -16, -4, -16};
-  static const int L[N_EVAL_VARIANTS] = {0, 1, 1, 1, 1, 1, 0, 1, 1, 1, 1, 1, 1, 1};
-  static const int PIPE[N_EVAL_VARIANTS] = {1, 1, 1, 1, 1, 1, 2, 2, 2, 2, 1, 1, 2, 2};
-  static const int WAVES[N_EVAL_VARIANTS] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 6, 0, 0, 6, 6};
+  static const int G[N_EVAL_VARIANTS] = {0, 0, 32, 16, 8, -1, 0, 0, -1, -1, -4, -16, -4, -16, 0, -1, -4, -16};
+  static const int L[N_EVAL_VARIANTS] = {0, 1, 1, 1, 1, 1, 0, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1, 1};
+  static const int PIPE[N_EVAL_VARIANTS] = {1, 1, 1, 1, 1, 1, 2, 2, 2, 2, 1, 1, 2, 2, 3, 3, 3, 3};
+  static const int WAVES[N_EVAL_VARIANTS] = {0, 0, 0, 0, 0, 0, 0, 0, 0, 6, 0, 0, 6, 6, 0, 0, 0, 0};
   listed = L[v] != 0;
   group = G[v];
   pipe = PIPE[v];
@@ -39,7 +41,7 @@ inline void eval_variant(int v, bool& listed, int& group, int& pipe, int& waves)
 
 // the same shape with one row per wait (the FP64 evaluation has no two-row loop)
 inline int eval_variant_pipe1(int v) {
-  static const int P1[N_EVAL_VARIANTS] = {0, 1, 2, 3, 4, 5, 0, 1, 5, 5, 10, 11, 10, 11};
+  static const int P1[N_EVAL_VARIANTS] = {0, 1, 2, 3, 4, 5, 0, 1, 5, 5, 10, 11, 10, 11, 1, 5, 10, 11};
   return P1[v];
 }
 
@@ -121,7 +123,7 @@ struct EvalTuner {
     bool listed;
     int group, pipe, waves;
     eval_variant(v, listed, group, pipe, waves);
-    return !(group > 0 && !grouped_ok) && !(pipe == 2 && !pipe2_ok);
+    return !(group > 0 && !grouped_ok) && !(pipe >= 2 && !pipe2_ok);
   }
 
   // the variant a forced choice stands for where it does not apply

@@ -165,18 +165,18 @@ def test_linear_reproduction_gpu():
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_phs3_launch_shapes_identical(precision):
-    """Every evaluation launch shape gives the same values bit for bit with the
-    polynomial table in the epilogue."""
+    """Every evaluation launch shape (the matrix-core accumulation included) gives
+    the same values bit for bit with the polynomial pass."""
     opts, X, V, P = small_c2(Np=20000, precision=precision)
     opts["advection"] = False
     vals = []
-    for v in range(14):
+    for v in range(18):
         g = L.LagRBFCtx(petsc_options=dict(opts, lagrbf_eval_variant=v))
         g.fill_data(X, V); g.fill_points(P); g.interpolate_c()
         out = np.zeros((P.shape[0], 3)); g.output(fvals=out)
         g.destroy()
         vals.append(out)
-    for v in range(1, 14):
+    for v in range(1, 18):
         assert np.array_equal(vals[0], vals[v]), f"variant {v} differs"
 
 

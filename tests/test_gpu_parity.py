@@ -518,11 +518,13 @@ def test_eval_launch_shapes_identical(precision, Np, case):
     """Every evaluation launch shape the autotuner chooses between (one block per
     cell, per listed non-empty cell in XCD runs or round-robin, lane groups of
     32 / 16 / 8 over the ascending cell list, one or two table rows per wait and
-    two occupancy floors in FP32) gives the same values bit for bit, on a sparse
-    and a dense swarm, in 3D and 2D, and with 0-9 nodes per cell (the node loops'
-    short and empty tails); FP64 against the oracle at 1e-12.  FP64 has no
-    two-row loop: there the variants 6-9 run as 0, 1, 5 and 5, and 12-13 (runs of
-    4 and 16 listed cells dealt round-robin) as 10-11."""
+    two occupancy floors in FP32, and in FP32 the accumulation on the matrix cores)
+    gives the same values bit for bit, on a sparse and a dense swarm, in 3D and 2D,
+    and with 0-9 nodes per cell (the node loops' short and empty tails); FP64
+    against the oracle at 1e-12.  FP64 has neither the two-row loop nor the
+    matrix-core accumulation: there the variants 6-9 run as 0, 1, 5 and 5, 12-13
+    (runs of 4 and 16 listed cells dealt round-robin) as 10-11, and 14-17 as 1, 5,
+    10 and 11."""
     if case == "3d":
         opts, X, V, P = cases.fourier_3d(nlat=20, Np=Np, n=9)
     elif case == "3d-few-nodes":
@@ -533,7 +535,7 @@ def test_eval_launch_shapes_identical(precision, Np, case):
     opts["advection"] = False
     print(f"case: launch shapes {case} Np={Np} {precision}", flush=True)
     vals = []
-    nvar = 14
+    nvar = 18
     for v in range(nvar):
         g = L.LagRBFCtx(petsc_options=dict(opts, lagrbf_eval_variant=v), precision=precision)
         g.fill_data(X, V)
@@ -544,7 +546,8 @@ def test_eval_launch_shapes_identical(precision, Np, case):
         used = g.info("eval_variant")
         g.destroy()
         vals.append(out)
-        want = v if precision == "mixed" else {6: 0, 7: 1, 8: 5, 9: 5, 12: 10, 13: 11}.get(v, v)
+        want = v if precision == "mixed" else {6: 0, 7: 1, 8: 5, 9: 5, 12: 10, 13: 11, 14: 1, 15: 5, 16: 10,
+                                               17: 11}.get(v, v)
         assert used == want, (v, used)
     for v in range(1, nvar):
         assert np.array_equal(vals[0], vals[v]), f"variant {v} differs"
