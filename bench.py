@@ -264,9 +264,10 @@ def cpu_baseline(opts, X, V, P, steps, dt, nsample, threads):
 
 def serve_datasets(seed=20261203 + 4):
     """The serving configuration's four datasets: 2D 65,536 uniform nodes under the
-    solid-body rotation (PU 45^2); 3D jittered lattices with the 16-mode Fourier
-    field of 2^20 nodes (PU 32^3), 2,000,000 nodes (PU 40^3) and 4,000,000 nodes
-    (PU 50^3); Wendland C2, about 34 nodes per PU cell."""
+    solid-body rotation (PU 45^2, Wendland C2); 3D jittered lattices with the 16-mode
+    Fourier field of 2^20 nodes (PU 32^3), 2,000,000 nodes (PU 40^3) and 4,000,000
+    nodes (PU 50^3), built as the 3D configuration is: r^3 kernel with a linear
+    polynomial block; about 34 nodes per PU cell."""
     from tests import cases
     rng = np.random.default_rng(seed)
     X = rng.random((65536, 2))
@@ -274,10 +275,10 @@ def serve_datasets(seed=20261203 + 4):
     out = [(dict(pu_mesh_dim=2, pu_mesh_O=[0.0, 0.0], pu_mesh_L=[1.0, 1.0], pu_mesh_n=[45, 45], pu_mesh_overlap=1.0,
                  pu_mesh_shape_factor=1.0, kernel_type=2), X, V, 1e-2)]
     for nlat, N, n in ((102, 1 << 20, 32), (128, 2000000, 40), (160, 4000000, 50)):
-        o, X, V, _ = cases.fourier_3d(nlat=nlat, N=N, Np=1, n=n, overlap=1.0)
+        o, X, V, _ = cases.fourier_3d(nlat=nlat, N=N, Np=1, n=n, overlap=1.0, kernel=7)
         keep = ("pu_mesh_dim", "pu_mesh_O", "pu_mesh_L", "pu_mesh_n", "pu_mesh_overlap", "pu_mesh_shape_factor",
                 "kernel_type")
-        out.append(({k: o[k] for k in keep}, X, V, 1e-3))
+        out.append((dict({k: o[k] for k in keep}, lagrbf_poly_degree=1), X, V, 1e-3))
     return out
 
 
@@ -301,14 +302,16 @@ def serve_bench(L, precision, dsets, nreq, rate, seed=20261203 + 5, budget=200e9
         nsteps = int(rng.integers(10, 201))
         P = 0.05 + 0.9 * rng.random((n, dim))
         reqs.append((ds, scheme, nsteps, dsets[ds][3], P))
+    offered = sum(r[4].shape[0] * r[2] for r in reqs) / arrivals[-1]
     t0 = srv.info("now")
     w0 = time.perf_counter()
-    i, pending, lat, work = 0, {}, [], 0.0
+    i, pending, lat, work = 0, {}, np.zeros(nreq), 0.0
+    ndone = 0
     while i < nreq or pending:
         now = time.perf_counter() - w0
         while i < nreq and arrivals[i] <= now:
             ds, scheme, nsteps, dt, P = reqs[i]
-            pending[srv.submit(ds, scheme, nsteps, dt, P)] = (t0 + arrivals[i], P.shape[0] * nsteps)
+            pending[srv.submit(ds, scheme, nsteps, dt, P)] = (t0 + arrivals[i], P.shape[0] * nsteps, i)
             i += 1
         if not pending:
             time.sleep(max(0.0, arrivals[i] - (time.perf_counter() - w0)))
@@ -317,13 +320,20 @@ def serve_bench(L, precision, dsets, nreq, rate, seed=20261203 + 5, budget=200e9
         for rid in list(pending):
             done, _, t = srv.poll(rid, want_result=False)
             if done:
-                arr, w = pending.pop(rid)
-                lat.append(t[2] - arr)
+                arr, w, k = pending.pop(rid)
+                lat[k] = t[2] - arr
                 work += w
+                ndone += 1
+                if nreq >= 500 and ndone % 100 == 0:
+                    log(f"serve [{precision}] {ndone}/{nreq} requests done at {time.perf_counter() - w0:.0f} s")
                 srv.release(rid)
     wall = time.perf_counter() - w0
+    h = nreq // 2
     res = {"precision": precision, "requests": nreq, "rate_hz": rate, "wall_s": wall, "setup_s": t_setup,
            "p50_ms": float(np.percentile(lat, 50) * 1e3), "p99_ms": float(np.percentile(lat, 99) * 1e3),
+           "p50_ms_first_half": float(np.percentile(lat[:h], 50) * 1e3),
+           "p50_ms_second_half": float(np.percentile(lat[h:], 50) * 1e3),
+           "offered_particle_steps_per_s": float(offered),
            "particle_steps_per_s": work / wall, "ticks": srv.info("ticks"), "max_active": srv.info("max_active")}
     srv.close()
     return res
@@ -450,6 +460,7 @@ def main():
     ap.add_argument("--serve-requests", type=int, default=64)
     ap.add_argument("--serve-rate", type=float, default=200.0)
     ap.add_argument("--serve-only", action="store_true")
+    ap.add_argument("--serve-precision", default="both", choices=["both", "fp64", "mixed"])
     ap.add_argument("--c3-only", action="store_true", help="the uneven-batching configuration only")
     ap.add_argument("--c3-precision", default="both", choices=["both", "fp64", "mixed"])
     ap.add_argument("--no-c3", action="store_true", help="skip the uneven-batching measurement")
@@ -485,7 +496,8 @@ def main():
         return
     if args.serve_only:
         dsets = serve_datasets()
-        out = {p: serve_bench(L, p, dsets, args.serve_requests, args.serve_rate) for p in ("mixed", "fp64")}
+        precs = ("mixed", "fp64") if args.serve_precision == "both" else (args.serve_precision,)
+        out = {p: serve_bench(L, p, dsets, args.serve_requests, args.serve_rate) for p in precs}
         print(json.dumps({"serve": out}), flush=True)
         return
     dt = 1e-3
@@ -551,7 +563,8 @@ def main():
                     f"wall {serve[p]['wall_s']:.1f} s, set-up {serve[p]['setup_s']:.1f} s")
             serve["config"] = ("4 device-resident datasets (2D 65,536 nodes; 3D 2^20, 2,000,000, 4,000,000 nodes), "
                                f"{args.serve_requests} requests, Poisson arrivals at {args.serve_rate:.0f}/s, particles "
-                               "log-uniform in [1e3, 1e6], RK4 or DOPRI, 10-200 steps, HBM budget 200 GB")
+                               "log-uniform in [1e3, 1e6], RK4 or DOPRI, 10-200 steps, HBM budget 200 GB; 3D datasets r^3 + linear "
+                               "polynomial, 2D Wendland C2")
             del dsets
         except Exception as e:       # the headline measurement above stands on its own
             serve = {"error": str(e)}

@@ -63,7 +63,9 @@ template <int KT, int MM>
 __global__ __launch_bounds__(ANT) void k_aug_prep(Grid g, const int32_t* __restrict__ list, const int32_t* __restrict__ pn,
                                                   const int64_t* __restrict__ mem_off,
                                                   const int32_t* __restrict__ mem_pool, const double* __restrict__ xd,
-                                                  AugView a) {
+                                                  AugView a, int stage) {
+  // stage 0: everything; 1: steps 1-4 (QR, T); 2: steps 6-8, after U = A V came from
+  // k_aug_av_big (large patches: the O(n^2) product over many workgroups)
   __shared__ double red[(ANT / AW) * MM];
   __shared__ double sv[MM];
   __shared__ double sG[MM * MM], sT[MM * MM], sM[MM * MM], sS[MM * MM];
@@ -82,6 +84,13 @@ __global__ __launch_bounds__(ANT) void k_aug_prep(Grid g, const int32_t* __restr
   const int32_t* mem = mem_pool + mem_off[p];
   const double eps = a.eps ? a.eps[p] : g.eps_r;
   const double c0 = centre_of(g, p, 0), c1 = centre_of(g, p, 1), c2 = dim == 3 ? centre_of(g, p, 2) : 0.0;
+  if (stage == 2) {
+    for (int e = tid; e < MM * MM; e += ANT) {
+      const int q = e / MM, r = e % MM;
+      sT[e] = (q < m && r < m) ? Tm[(size_t)q * m + r] : 0.0;
+    }
+    __syncthreads();
+  } else {
   // 1. coordinates and the polynomial block
   for (int i = tid; i < n; i += ANT) {
     const double* x = xd + (size_t)mem[i] * dim;
@@ -165,6 +174,7 @@ __global__ __launch_bounds__(ANT) void k_aug_prep(Grid g, const int32_t* __restr
   }
   __syncthreads();
   for (int e = tid; e < m * m; e += ANT) Tm[(size_t)(e / m) * m + e % m] = sT[(e / m) * MM + e % m];
+  if (stage == 1) return;
   // 5. U = A V (into X)
   for (int i = tid; i < n; i += ANT) {
     double u[MM];
@@ -179,6 +189,7 @@ __global__ __launch_bounds__(ANT) void k_aug_prep(Grid g, const int32_t* __restr
         if (q < m) u[q] += phi * V[(size_t)q * n + j];
     }
     for (int q = 0; q < m; q++) X[(size_t)q * n + i] = u[q];
+  }
   }
   __syncthreads();
   // 6. M = V^T U, S = T^T M T (symmetrised)
@@ -240,6 +251,53 @@ __global__ __launch_bounds__(ANT) void k_aug_prep(Grid g, const int32_t* __restr
       b -= X[(size_t)q * n + i] * V[(size_t)q * n + k] + V[(size_t)q * n + i] * X[(size_t)q * n + k];
     B1[(size_t)k * n + i] = b;
   }
+}
+
+// U = A V of the large patches (step 5 of k_aug_prep) over many workgroups: one
+// block of ANT rows per task (patch, row block), the columns through LDS in chunks;
+// each row sums in ascending column order with the same operations, so the values
+// are those of the one-workgroup loop.
+template <int KT, int MM>
+__global__ __launch_bounds__(ANT) void k_aug_av_big(Grid g, const int2* __restrict__ tasks,
+                                                    const int32_t* __restrict__ pn, AugView a) {
+  __shared__ double sx[3][ANT];
+  __shared__ double sv[MM][ANT];
+  const int2 tk = tasks[blockIdx.x];
+  const int p = tk.x;
+  const int n = pn[p], m = a.m[p];
+  if (n == 0 || m == 0) return;
+  const int tid = threadIdx.x;
+  const int i = tk.y * ANT + tid;
+  const bool live = i < n;
+  const int ii = live ? i : n - 1;
+  const double* xs = a.aux + a.aux_off[p];
+  const double* V = xs + 3 * (size_t)n;
+  double* X = const_cast<double*>(V) + (size_t)m * n;
+  const double eps = a.eps ? a.eps[p] : g.eps_r;
+  double u[MM];
+#pragma unroll
+  for (int q = 0; q < MM; q++) u[q] = 0.0;
+  const double xi0 = xs[ii], xi1 = xs[n + ii], xi2 = xs[2 * n + ii];
+  for (int j0 = 0; j0 < n; j0 += ANT) {
+    const int jn = min(ANT, n - j0);
+    __syncthreads();
+    if (tid < jn) {
+      sx[0][tid] = xs[j0 + tid]; sx[1][tid] = xs[n + j0 + tid]; sx[2][tid] = xs[2 * n + j0 + tid];
+#pragma unroll
+      for (int q = 0; q < MM; q++)
+        if (q < m) sv[q][tid] = V[(size_t)q * n + j0 + tid];
+    }
+    __syncthreads();
+    for (int jj = 0; jj < jn; jj++) {
+      const double d0 = xi0 - sx[0][jj], d1 = xi1 - sx[1][jj], d2 = xi2 - sx[2][jj];
+      const double phi = rbf<KT, double>(sqrt(d0 * d0 + d1 * d1 + d2 * d2), eps);
+#pragma unroll
+      for (int q = 0; q < MM; q++)
+        if (q < m) u[q] += phi * sv[q][jj];
+    }
+  }
+  if (live)
+    for (int q = 0; q < m; q++) X[(size_t)q * n + i] = u[q];
 }
 
 // ---------------------------------------------------------------------------
@@ -320,7 +378,8 @@ __global__ __launch_bounds__(ANT, 3) void k_solve_aug(Grid g, const int32_t* __r
                                                    const TL* __restrict__ fac, const double* __restrict__ values, int nf,
                                                    double* __restrict__ coef, int max_iter, double tol,
                                                    int32_t* __restrict__ status, int32_t* __restrict__ iters,
-                                                   double* __restrict__ ws, int S, AugView a) {
+                                                   double* __restrict__ ws, int S, AugView a, int step_it,
+                                                   double* __restrict__ dstate) {
   extern __shared__ __align__(16) double sm[];
   constexpr int MF = MM * AMAXF;
   __shared__ double red[(ANT / AW) * MF];
@@ -341,7 +400,14 @@ __global__ __launch_bounds__(ANT, 3) void k_solve_aug(Grid g, const int32_t* __r
   const int32_t* mem = mem_pool + mem_off[p];
   const TL* L = fac + fac_off[p];
   const double eps = a.eps ? a.eps[p] : g.eps_r;
-  const bool lds = n <= S;
+  // step_it >= 0: one refinement iteration per launch (the large patches of the FP32
+  // factor), the state kept in the workspace (c), in the polynomial coefficients
+  // (beta), in status (2 while running), iters and dstate (the last correction);
+  // the residual of iterations >= 1 comes from k_aug_resid_big.  step_it < 0: the
+  // whole solve in this launch.
+  const bool stepping = step_it >= 0;
+  if (stepping && step_it > 0 && status[p] != 2) return;
+  const bool lds = n <= S && !stepping;
   if (!lds && !ws) {   // not sized for this patch: reported as not converged (refactored in FP64)
     if (tid == 0) {
       if (status) status[p] = 1;
@@ -356,7 +422,8 @@ __global__ __launch_bounds__(ANT, 3) void k_solve_aug(Grid g, const int32_t* __r
   double* r = y + (size_t)nf * ldv;     // [nf][ldv] residual
   for (int i = tid; i < n; i += ANT) {
     for (int d = 0; d < 3; d++) xs[(size_t)d * ldv + i] = xsg[(size_t)d * n + i];
-    for (int f = 0; f < nf; f++) c[(size_t)f * ldv + i] = 0.0;
+    if (!(stepping && step_it > 0))
+      for (int f = 0; f < nf; f++) c[(size_t)f * ldv + i] = 0.0;
   }
   for (int e = tid; e < m * m; e += ANT) {
     const int k = e % m, j = e / m;
@@ -364,14 +431,21 @@ __global__ __launch_bounds__(ANT, 3) void k_solve_aug(Grid g, const int32_t* __r
     sT[j * MM + k] = Tm[(size_t)j * m + k];
     sB11[j * MM + k] = B1[(size_t)j * n + k];   // B(k, j), j < m
   }
-  for (int e = tid; e < MF; e += ANT) sb[e] = 0.0;
+  double* bt = a.beta + (size_t)p * a.bstride;
+  for (int e = tid; e < MF; e += ANT) {
+    const int q = e / AMAXF, f = e % AMAXF;
+    sb[e] = (stepping && step_it > 0 && q < m && f < nf) ? bt[q * nf + f] : 0.0;
+  }
   __syncthreads();
   const bool refine = sizeof(TL) == sizeof(float);
-  int it = 0, converged = 0;
-  double dprev = 1e300;
-  for (; it <= (refine ? max_iter : 0); it++) {
+  int it = stepping ? step_it : 0, converged = 0;
+  const int it_end = stepping ? step_it : (refine ? max_iter : 0);
+  double dprev = (stepping && step_it > 0) ? dstate[p] : 1e300;
+  for (; it <= it_end; it++) {
     // residual of the saddle-point system: r = f - A c - P beta, g = -P^T c  (g in su1)
     {
+      // (iterations >= 1 of a stepped solve: r is already in the workspace)
+      if (!(stepping && it > 0))
       for (int i = tid; i < n; i += ANT) {
         double s[AMAXF];
 #pragma unroll
@@ -516,11 +590,83 @@ __global__ __launch_bounds__(ANT, 3) void k_solve_aug(Grid g, const int32_t* __r
   double* X = coef + mem_off[p] * nf;
   for (int i = tid; i < n; i += ANT)
     for (int f = 0; f < nf; f++) X[(size_t)i * nf + f] = c[(size_t)f * ldv + i];
-  double* bt = a.beta + (size_t)p * a.bstride;
   for (int e = tid; e < m * nf; e += ANT) bt[e] = sb[(e / nf) * AMAXF + e % nf];
   if (tid == 0) {
-    if (status) status[p] = converged ? 0 : 1;
-    if (iters) iters[p] = it;
+    if (stepping && !converged && step_it < max_iter) {
+      status[p] = 2;   // another iteration follows
+      dstate[p] = dprev;
+    } else {
+      if (status) status[p] = converged ? 0 : 1;
+      if (iters) iters[p] = it;
+    }
+  }
+}
+
+// Residual r = f - A c - P beta of the saddle-point systems of the large patches
+// during a stepped FP32 refinement (k_solve_aug, step_it >= 1): one block of ANT
+// rows per task (patch, row block), so the O(n^2) matrix-free product of a patch of
+// thousands of members spreads over many compute units.  The columns come through
+// LDS in chunks of ANT; each row sums them in ascending column order with the same
+// operations as the in-kernel residual, so the values are the same.
+template <int KT, int MM>
+__global__ __launch_bounds__(ANT) void k_aug_resid_big(Grid g, const int2* __restrict__ tasks,
+                                                       const int32_t* __restrict__ pn,
+                                                       const int64_t* __restrict__ mem_off,
+                                                       const int32_t* __restrict__ mem_pool,
+                                                       const double* __restrict__ values, int nf,
+                                                       const int32_t* __restrict__ status, double* __restrict__ ws,
+                                                       AugView a) {
+  __shared__ double sx[3][ANT];
+  __shared__ double sc[AMAXF][ANT];
+  const int2 tk = tasks[blockIdx.x];
+  const int p = tk.x;
+  if (status[p] != 2) return;
+  const int n = pn[p], m = a.m[p], deg = a.deg[p], dim = g.dim;
+  const int tid = threadIdx.x;
+  const int i = tk.y * ANT + tid;
+  const bool live = i < n;
+  const int ii = live ? i : n - 1;
+  const size_t ldv = (size_t)n;
+  double* xs = ws + (size_t)mem_off[p] * (3 + 3 * (size_t)nf);
+  const double* c = xs + 3 * ldv;
+  double* r = xs + (3 + 2 * (size_t)nf) * ldv;
+  const int32_t* mem = mem_pool + mem_off[p];
+  const double eps = a.eps ? a.eps[p] : g.eps_r;
+  const double* bt = a.beta + (size_t)p * a.bstride;
+  double s[AMAXF];
+#pragma unroll
+  for (int f = 0; f < AMAXF; f++) s[f] = f < nf ? values[(size_t)mem[ii] * nf + f] : 0.0;
+  const double xi0 = xs[ii], xi1 = xs[ldv + ii], xi2 = xs[2 * ldv + ii];
+  for (int j0 = 0; j0 < n; j0 += ANT) {
+    const int jn = min(ANT, n - j0);
+    __syncthreads();
+    if (tid < jn) {
+      sx[0][tid] = xs[j0 + tid]; sx[1][tid] = xs[ldv + j0 + tid]; sx[2][tid] = xs[2 * ldv + j0 + tid];
+#pragma unroll
+      for (int f = 0; f < AMAXF; f++)
+        if (f < nf) sc[f][tid] = c[(size_t)f * ldv + j0 + tid];
+    }
+    __syncthreads();
+    for (int jj = 0; jj < jn; jj++) {
+      const double d0 = xi0 - sx[0][jj], d1 = xi1 - sx[1][jj], d2 = xi2 - sx[2][jj];
+      const double phi = rbf<KT, double>(sqrt(d0 * d0 + d1 * d1 + d2 * d2), eps);
+#pragma unroll
+      for (int f = 0; f < AMAXF; f++)
+        if (f < nf) s[f] -= phi * sc[f][jj];
+    }
+  }
+  double xi[3] = {xi0 * a.ps, xi1 * a.ps, xi2 * a.ps}, pv[POLY_MAX];
+  poly_eval(dim, deg, xi, pv);
+#pragma unroll
+  for (int q = 0; q < MM; q++)
+    if (q < m)
+#pragma unroll
+      for (int f = 0; f < AMAXF; f++)
+        if (f < nf) s[f] -= pv[q] * bt[q * nf + f];
+  if (live) {
+#pragma unroll
+    for (int f = 0; f < AMAXF; f++)
+      if (f < nf) r[(size_t)f * ldv + i] = s[f];
   }
 }
 
@@ -536,21 +682,37 @@ __global__ void k_build_ptab(int P, const uint8_t* __restrict__ active, AugView 
 // ---------------------------------------------------------------------------
 template <int KT>
 static void prep_k(const Grid& g, const int32_t* list, int nlist, const int32_t* pn, const int64_t* mem_off,
-                   const int32_t* mem_pool, const double* xd, const AugView& a, hipStream_t s) {
-  if (a.mmax <= 4) hipLaunchKernelGGL((k_aug_prep<KT, 4>), dim3(nlist), dim3(ANT), 0, s, g, list, pn, mem_off, mem_pool, xd, a);
-  else hipLaunchKernelGGL((k_aug_prep<KT, POLY_MAX>), dim3(nlist), dim3(ANT), 0, s, g, list, pn, mem_off, mem_pool, xd, a);
+                   const int32_t* mem_pool, const double* xd, const AugView& a, const int2* tasks, int ntasks,
+                   int stage, hipStream_t s) {
+  if (a.mmax <= 4) hipLaunchKernelGGL((k_aug_prep<KT, 4>), dim3(nlist), dim3(ANT), 0, s, g, list, pn, mem_off, mem_pool, xd, a, stage);
+  else hipLaunchKernelGGL((k_aug_prep<KT, POLY_MAX>), dim3(nlist), dim3(ANT), 0, s, g, list, pn, mem_off, mem_pool, xd, a, stage);
+  if (stage == 1 && ntasks > 0) {
+    if (a.mmax <= 4) hipLaunchKernelGGL((k_aug_av_big<KT, 4>), dim3(ntasks), dim3(ANT), 0, s, g, tasks, pn, a);
+    else hipLaunchKernelGGL((k_aug_av_big<KT, POLY_MAX>), dim3(ntasks), dim3(ANT), 0, s, g, tasks, pn, a);
+    if (a.mmax <= 4) hipLaunchKernelGGL((k_aug_prep<KT, 4>), dim3(nlist), dim3(ANT), 0, s, g, list, pn, mem_off, mem_pool, xd, a, 2);
+    else hipLaunchKernelGGL((k_aug_prep<KT, POLY_MAX>), dim3(nlist), dim3(ANT), 0, s, g, list, pn, mem_off, mem_pool, xd, a, 2);
+  }
 }
 
 void launch_aug_prep(const Grid& g, int ktype, const int32_t* list, int nlist, const int32_t* pn,
                      const int64_t* mem_off, const int32_t* mem_pool, const double* xd, const AugView& a, hipStream_t s) {
+  launch_aug_prep_tasks(g, ktype, list, nlist, pn, mem_off, mem_pool, xd, a, nullptr, 0, s);
+}
+
+void launch_aug_prep_tasks(const Grid& g, int ktype, const int32_t* list, int nlist, const int32_t* pn,
+                           const int64_t* mem_off, const int32_t* mem_pool, const double* xd, const AugView& a,
+                           const int2* tasks, int ntasks, hipStream_t s) {
   if (nlist <= 0) return;
+  const int stage = tasks ? 1 : 0;
+#define LRB_PK(K) prep_k<K>(g, list, nlist, pn, mem_off, mem_pool, xd, a, tasks, ntasks, stage, s)
   switch (ktype) {
-    case K_WC0: prep_k<K_WC0>(g, list, nlist, pn, mem_off, mem_pool, xd, a, s); break;
-    case K_WC2: prep_k<K_WC2>(g, list, nlist, pn, mem_off, mem_pool, xd, a, s); break;
-    case K_IMQ: prep_k<K_IMQ>(g, list, nlist, pn, mem_off, mem_pool, xd, a, s); break;
-    case K_PHS3: prep_k<K_PHS3>(g, list, nlist, pn, mem_off, mem_pool, xd, a, s); break;
-    default: prep_k<K_GAUSS>(g, list, nlist, pn, mem_off, mem_pool, xd, a, s); break;
+    case K_WC0: LRB_PK(K_WC0); break;
+    case K_WC2: LRB_PK(K_WC2); break;
+    case K_IMQ: LRB_PK(K_IMQ); break;
+    case K_PHS3: LRB_PK(K_PHS3); break;
+    default: LRB_PK(K_GAUSS); break;
   }
+#undef LRB_PK
 }
 
 // largest patch whose vectors sit in LDS: every patch of the small-patch methods
@@ -559,11 +721,15 @@ static constexpr int AUG_LDS_MAX = 512 + POLY_MAX;
 static size_t aug_solve_smem(int S, int nf) { return sizeof(double) * (size_t)S * (3 + 3 * (size_t)nf); }
 size_t aug_solve_ws_per_member(int nf) { return 3 + 3 * (size_t)nf; }
 
+// a stepped solve keeps every vector in the workspace
+static size_t stepping_smem(int S, int nf, int step_it) { return step_it >= 0 ? 0 : aug_solve_smem(S, nf); }
+
 template <typename TL, int KT, int MM>
 static void solve_aug_k(const Grid& g, const int32_t* list, int nlist, const int32_t* pn, const int64_t* mem_off,
                         const int64_t* fac_off, const int32_t* mem_pool, const double* xd, const TL* fac,
                         const double* values, int nf, double* coef, int max_iter, double tol, int32_t* status,
-                        int32_t* iters, double* ws, int nmax, const AugView& a, hipStream_t s) {
+                        int32_t* iters, double* ws, int nmax, const AugView& a, int step_it, double* dstate,
+                        hipStream_t s) {
   static bool attr = false;
   if (!attr) {
     (void)hipFuncSetAttribute((const void*)k_solve_aug<TL, KT, MM>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -571,16 +737,18 @@ static void solve_aug_k(const Grid& g, const int32_t* list, int nlist, const int
     attr = true;
   }
   const int S = std::max(1, std::min(nmax, AUG_LDS_MAX));
-  hipLaunchKernelGGL((k_solve_aug<TL, KT, MM>), dim3(nlist), dim3(ANT), aug_solve_smem(S, nf), s, g, list, pn, mem_off,
-                     fac_off, mem_pool, xd, fac, values, nf, coef, max_iter, tol, status, iters, ws, S, a);
+  hipLaunchKernelGGL((k_solve_aug<TL, KT, MM>), dim3(nlist), dim3(ANT), stepping_smem(S, nf, step_it), s, g, list, pn,
+                     mem_off, fac_off, mem_pool, xd, fac, values, nf, coef, max_iter, tol, status, iters, ws, S, a,
+                     step_it, dstate);
 }
 
 template <typename TL, int MM>
 static void solve_aug_mm(const Grid& g, int ktype, const int32_t* list, int nlist, const int32_t* pn,
                          const int64_t* mem_off, const int64_t* fac_off, const int32_t* mem_pool, const double* xd,
                          const TL* fac, const double* values, int nf, double* coef, int max_iter, double tol,
-                         int32_t* status, int32_t* iters, double* ws, int nmax, const AugView& a, hipStream_t s) {
-#define LRB_SA(K) solve_aug_k<TL, K, MM>(g, list, nlist, pn, mem_off, fac_off, mem_pool, xd, fac, values, nf, coef, max_iter, tol, status, iters, ws, nmax, a, s)
+                         int32_t* status, int32_t* iters, double* ws, int nmax, const AugView& a, int step_it,
+                         double* dstate, hipStream_t s) {
+#define LRB_SA(K) solve_aug_k<TL, K, MM>(g, list, nlist, pn, mem_off, fac_off, mem_pool, xd, fac, values, nf, coef, max_iter, tol, status, iters, ws, nmax, a, step_it, dstate, s)
   switch (ktype) {
     case K_WC0: LRB_SA(K_WC0); break;
     case K_WC2: LRB_SA(K_WC2); break;
@@ -596,13 +764,56 @@ void launch_solve_aug(const Grid& g, int ktype, bool fp32, const int32_t* list, 
                       const void* fac, const double* values, int nf, double* coef, int max_iter, double tol,
                       int32_t* status, int32_t* iters, double* ws, int nmax, const AugView& a, hipStream_t s) {
   if (nlist <= 0 || nf <= 0) return;
+  launch_solve_aug_step(g, ktype, fp32, list, nlist, pn, mem_off, fac_off, mem_pool, xd, fac, values, nf, coef, max_iter,
+                        tol, status, iters, ws, nmax, a, -1, nullptr, s);
+}
+
+void launch_solve_aug_step(const Grid& g, int ktype, bool fp32, const int32_t* list, int nlist, const int32_t* pn,
+                           const int64_t* mem_off, const int64_t* fac_off, const int32_t* mem_pool, const double* xd,
+                           const void* fac, const double* values, int nf, double* coef, int max_iter, double tol,
+                           int32_t* status, int32_t* iters, double* ws, int nmax, const AugView& a, int step_it,
+                           double* dstate, hipStream_t s) {
+  if (nlist <= 0 || nf <= 0) return;
   if (fp32) {
-    if (a.mmax <= 4) solve_aug_mm<float, 4>(g, ktype, list, nlist, pn, mem_off, fac_off, mem_pool, xd, (const float*)fac, values, nf, coef, max_iter, tol, status, iters, ws, nmax, a, s);
-    else solve_aug_mm<float, POLY_MAX>(g, ktype, list, nlist, pn, mem_off, fac_off, mem_pool, xd, (const float*)fac, values, nf, coef, max_iter, tol, status, iters, ws, nmax, a, s);
+    if (a.mmax <= 4) solve_aug_mm<float, 4>(g, ktype, list, nlist, pn, mem_off, fac_off, mem_pool, xd, (const float*)fac, values, nf, coef, max_iter, tol, status, iters, ws, nmax, a, step_it, dstate, s);
+    else solve_aug_mm<float, POLY_MAX>(g, ktype, list, nlist, pn, mem_off, fac_off, mem_pool, xd, (const float*)fac, values, nf, coef, max_iter, tol, status, iters, ws, nmax, a, step_it, dstate, s);
   } else {
-    if (a.mmax <= 4) solve_aug_mm<double, 4>(g, ktype, list, nlist, pn, mem_off, fac_off, mem_pool, xd, (const double*)fac, values, nf, coef, max_iter, tol, status, iters, ws, nmax, a, s);
-    else solve_aug_mm<double, POLY_MAX>(g, ktype, list, nlist, pn, mem_off, fac_off, mem_pool, xd, (const double*)fac, values, nf, coef, max_iter, tol, status, iters, ws, nmax, a, s);
+    if (a.mmax <= 4) solve_aug_mm<double, 4>(g, ktype, list, nlist, pn, mem_off, fac_off, mem_pool, xd, (const double*)fac, values, nf, coef, max_iter, tol, status, iters, ws, nmax, a, step_it, dstate, s);
+    else solve_aug_mm<double, POLY_MAX>(g, ktype, list, nlist, pn, mem_off, fac_off, mem_pool, xd, (const double*)fac, values, nf, coef, max_iter, tol, status, iters, ws, nmax, a, step_it, dstate, s);
   }
+}
+
+template <int KT, int MM>
+static void resid_big_k(const Grid& g, const int2* tasks, int ntasks, const int32_t* pn, const int64_t* mem_off,
+                        const int32_t* mem_pool, const double* values, int nf, const int32_t* status, double* ws,
+                        const AugView& a, hipStream_t s) {
+  hipLaunchKernelGGL((k_aug_resid_big<KT, MM>), dim3(ntasks), dim3(ANT), 0, s, g, tasks, pn, mem_off, mem_pool, values,
+                     nf, status, ws, a);
+}
+
+template <int MM>
+static void resid_big_mm(const Grid& g, int ktype, const int2* tasks, int ntasks, const int32_t* pn,
+                         const int64_t* mem_off, const int32_t* mem_pool, const double* values, int nf,
+                         const int32_t* status, double* ws, const AugView& a, hipStream_t s) {
+#define LRB_RB(K) resid_big_k<K, MM>(g, tasks, ntasks, pn, mem_off, mem_pool, values, nf, status, ws, a, s)
+  switch (ktype) {
+    case K_WC0: LRB_RB(K_WC0); break;
+    case K_WC2: LRB_RB(K_WC2); break;
+    case K_IMQ: LRB_RB(K_IMQ); break;
+    case K_PHS3: LRB_RB(K_PHS3); break;
+    default: LRB_RB(K_GAUSS); break;
+  }
+#undef LRB_RB
+}
+
+int aug_resid_rows() { return ANT; }
+
+void launch_aug_resid_big(const Grid& g, int ktype, const int2* tasks, int ntasks, const int32_t* pn,
+                          const int64_t* mem_off, const int32_t* mem_pool, const double* values, int nf,
+                          const int32_t* status, double* ws, const AugView& a, hipStream_t s) {
+  if (ntasks <= 0 || nf <= 0) return;
+  if (a.mmax <= 4) resid_big_mm<4>(g, ktype, tasks, ntasks, pn, mem_off, mem_pool, values, nf, status, ws, a, s);
+  else resid_big_mm<POLY_MAX>(g, ktype, tasks, ntasks, pn, mem_off, mem_pool, values, nf, status, ws, a, s);
 }
 
 void launch_build_ptab(const Grid& g, const uint8_t* active, const AugView& a, int mm, int nft, int nf, int f0,

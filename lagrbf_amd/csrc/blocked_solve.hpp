@@ -54,13 +54,40 @@ __device__ void llt_solve_blocked(const TL* __restrict__ L, int n, double* y, in
       const int r = tid % NB, f = tid / NB;
       if (r < w) y[(size_t)f * ldy + J + r] = zb[r * 8 + f];
     }
-    // rows below exist only under a full block (w = 16)
-    for (int i = J + w + tid; i < n; i += NTH) {
-#pragma unroll 1
-      for (int f = 0; f < nf; f++) {
+    // rows below exist only under a full block (w = 16); a row's 16 factor entries are
+    // loaded once for all fields, two rows per iteration, every load issued before
+    // the arithmetic (each field's sum keeps its order)
+    int i = J + w + tid;
+    for (; i + NTH < n; i += 2 * NTH) {
+      TL la[NB], lb2[NB];
+#pragma unroll
+      for (int c = 0; c < NB; c++) {
+        la[c] = L[(size_t)(J + c) * n + i];
+        lb2[c] = L[(size_t)(J + c) * n + i + NTH];
+      }
+#pragma unroll
+      for (int f = 0; f < 8; f++) {
+        if (f >= nf) break;
+        double a = 0.0, b = 0.0;
+#pragma unroll
+        for (int c = 0; c < NB; c++) {
+          a += (double)la[c] * zb[c * 8 + f];
+          b += (double)lb2[c] * zb[c * 8 + f];
+        }
+        y[(size_t)f * ldy + i] -= a;
+        y[(size_t)f * ldy + i + NTH] -= b;
+      }
+    }
+    if (i < n) {
+      TL la[NB];
+#pragma unroll
+      for (int c = 0; c < NB; c++) la[c] = L[(size_t)(J + c) * n + i];
+#pragma unroll
+      for (int f = 0; f < 8; f++) {
+        if (f >= nf) break;
         double a = 0.0;
 #pragma unroll
-        for (int c = 0; c < NB; c++) a += (double)L[(size_t)(J + c) * n + i] * zb[c * 8 + f];
+        for (int c = 0; c < NB; c++) a += (double)la[c] * zb[c * 8 + f];
         y[(size_t)f * ldy + i] -= a;
       }
     }
@@ -76,18 +103,37 @@ __device__ void llt_solve_blocked(const TL* __restrict__ L, int n, double* y, in
     stage_block_inverse<TL, NTH>(L, n, J, w, lb);
     if (c < NB) {
       const TL* Lc = L + (size_t)(J + c) * n;
-#pragma unroll 1
-      for (int f = 0; f < nf; f++) {
-        const double* yf = y + (size_t)f * ldy;
-        double acc = 0.0;
-        if (c < w) {
-          int i = J + w + g;
-          for (; i + NB < n; i += 2 * NB) acc += (double)Lc[i] * yf[i] + (double)Lc[i + NB] * yf[i + NB];
-          if (i < n) acc += (double)Lc[i] * yf[i];
-        }
+      // one pass over the column for all fields (each field's sum keeps its order)
+      double acc[8];
 #pragma unroll
-        for (int o = NB / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
-        if (g == 0) zb[c * 8 + f] = c < w ? yf[J + c] - acc : 0.0;
+      for (int f = 0; f < 8; f++) acc[f] = 0.0;
+      if (c < w) {
+        int i = J + w + g;
+        for (; i + NB < n; i += 2 * NB) {
+          const double l0 = (double)Lc[i], l1 = (double)Lc[i + NB];
+#pragma unroll
+          for (int f = 0; f < 8; f++) {
+            if (f >= nf) break;
+            const double* yf = y + (size_t)f * ldy;
+            acc[f] += l0 * yf[i] + l1 * yf[i + NB];
+          }
+        }
+        if (i < n) {
+          const double l0 = (double)Lc[i];
+#pragma unroll
+          for (int f = 0; f < 8; f++) {
+            if (f >= nf) break;
+            acc[f] += l0 * y[(size_t)f * ldy + i];
+          }
+        }
+      }
+#pragma unroll
+      for (int f = 0; f < 8; f++) {
+        if (f >= nf) break;
+        double a = acc[f];
+#pragma unroll
+        for (int o = NB / 2; o > 0; o >>= 1) a += __shfl_xor(a, o);
+        if (g == 0) zb[c * 8 + f] = c < w ? y[(size_t)f * ldy + J + c] - a : 0.0;
       }
     }
     __syncthreads();

@@ -166,6 +166,8 @@ static int ctx_from_options(Ctx& c, const OptView& o) {
   c.chol_big_min = chol_max_n();
   o.get_int("lagrbf_chol_big_min", c.chol_big_min, bad);
   c.chol_big_min = std::max(0, std::min(c.chol_big_min, chol_max_n()));
+  o.get_bool("lagrbf_stepped_refine", c.stepped_refine, bad);
+  if (bad) return set_err(E_WRONG, fn, "malformed -lagrbf_stepped_refine option");
   std::string bf;
   if (o.get_string("lagrbf_big_factor", bf)) {
     if (bf == "tiled") c.big_factor = 1;
@@ -640,11 +642,26 @@ static int factor_lists(Ctx& c, std::vector<int32_t>* fl) {
   if (!fl[M_BK64].empty())
     launch_factor_bk(c.g, c.ktype, c.list.p, (int)fl[M_BK64].size(), c.p_n.p, c.p_mem_off.p, c.p_fac_off.p, c.mem_pool.p,
                      c.dx.p, c.fac_pool.p, c.piv_pool.p, c.info.p, ap, c.s);
-  // null-space data of the saddle-point patches before their factors
-  for (int m = M_AUG64; m <= M_AUG32B; m++)
-    if (!fl[m].empty())
+  // null-space data of the saddle-point patches before their factors (large
+  // patches: the O(n^2) product over many workgroups, by row blocks)
+  for (int m = M_AUG64; m <= M_AUG32B; m++) {
+    if (fl[m].empty()) continue;
+    if (method_big(m) && c.big_factor) {
+      std::vector<int2> tasks;
+      const int R = aug_resid_rows();
+      for (int p : fl[m])
+        for (int b = 0; (int64_t)b * R < c.h_n[p]; b++) tasks.push_back(make_int2(p, b));
+      HIPCHK(hipStreamSynchronize(c.s));   // an earlier use of the task buffer
+      HIPCHK(c.resid_tasks.reserve(tasks.size()));
+      HIPCHK(hipMemcpyAsync(c.resid_tasks.p, tasks.data(), sizeof(int2) * tasks.size(), hipMemcpyHostToDevice, c.s));
+      launch_aug_prep_tasks(c.g, c.ktype, c.list.p + (size_t)m * P, (int)fl[m].size(), c.p_n.p, c.p_mem_off.p,
+                            c.mem_pool.p, c.dx.p, ap, c.resid_tasks.p, (int)tasks.size(), c.s);
+      HIPCHK(hipStreamSynchronize(c.s));   // the host task list goes out of scope
+    } else {
       launch_aug_prep(c.g, c.ktype, c.list.p + (size_t)m * P, (int)fl[m].size(), c.p_n.p, c.p_mem_off.p, c.mem_pool.p,
                       c.dx.p, ap, c.s);
+    }
+  }
   bool any_chol = false;
   for (int m = M_CHOL64; m < N_METHODS; m++)
     if (!fl[m].empty()) {
@@ -676,6 +693,39 @@ static int factor_lists(Ctx& c, std::vector<int32_t>* fl) {
   return refactor_fp64(c, redo, false);
 }
 
+// FP32 factors of large saddle-point patches: one refinement iteration per launch,
+// the O(n^2) matrix-free residual spread over many workgroups (launch_aug_resid_big)
+// instead of one workgroup per patch, which took most of the set-up of patches with
+// thousands of members.  The same operations in the same order as the one-launch
+// solve, so the same coefficients.
+static int solve_aug_stepped(Ctx& c, const std::vector<int32_t>& l, int nmax) {
+  const int P = c.g.P;
+  const AugView ap = aug_view(c, true);
+  const int32_t* dl = c.list.p + (size_t)M_AUG32B * P;
+  HIPCHK(c.refine_dstate.reserve(P));
+  std::vector<int2> tasks;
+  const int R = aug_resid_rows();
+  for (int p : l)
+    for (int b = 0; (int64_t)b * R < c.h_n[p]; b++) tasks.push_back(make_int2(p, b));
+  HIPCHK(c.resid_tasks.reserve(tasks.size()));
+  HIPCHK(hipMemcpyAsync(c.resid_tasks.p, tasks.data(), sizeof(int2) * tasks.size(), hipMemcpyHostToDevice, c.s));
+  std::vector<int32_t> st(P);
+  for (int it = 0; it <= c.refine_max_iter; it++) {
+    if (it > 0)
+      launch_aug_resid_big(c.g, c.ktype, c.resid_tasks.p, (int)tasks.size(), c.p_n.p, c.p_mem_off.p, c.mem_pool.p,
+                           c.dval.p, c.nf, c.pstatus.p, c.solve_ws.p, ap, c.s);
+    launch_solve_aug_step(c.g, c.ktype, true, dl, (int)l.size(), c.p_n.p, c.p_mem_off.p, c.p_fac_off.p, c.mem_pool.p,
+                          c.dx.p, c.fac32_pool.p, c.dval.p, c.nf, c.coef_pool.p, c.refine_max_iter, c.refine_tol,
+                          c.pstatus.p, c.piters.p, c.solve_ws.p, nmax, ap, it, c.refine_dstate.p, c.s);
+    HIPCHK(hipMemcpyAsync(st.data(), c.pstatus.p, sizeof(int32_t) * P, hipMemcpyDeviceToHost, c.s));
+    HIPCHK(hipStreamSynchronize(c.s));
+    bool running = false;
+    for (int p : l) running = running || st[p] == 2;
+    if (!running) break;
+  }
+  return 0;
+}
+
 // Solve the coefficients of the listed patches with each patch's factor; an FP32
 // factor whose refinement does not reach the tolerance is replaced by an FP64
 // Bunch-Kaufman factor and solved again (counted in "fallback").
@@ -702,7 +752,9 @@ static int solve_list(Ctx& c, const std::vector<int32_t>& sl) {
       int nmax = 1;
       for (int p : l[m]) nmax = std::max(nmax, (int)c.h_n[p]);
       const void* fac = method_fp32(m) ? (const void*)c.fac32_pool.p : (const void*)c.fac_pool.p;
-      if (method_aug(m))
+      if (m == M_AUG32B && c.big_factor && c.stepped_refine)
+        CHK(solve_aug_stepped(c, l[m], nmax));
+      else if (method_aug(m))
         launch_solve_aug(c.g, c.ktype, method_fp32(m), c.list.p + (size_t)m * P, (int)l[m].size(), c.p_n.p,
                          c.p_mem_off.p, c.p_fac_off.p, c.mem_pool.p, c.dx.p, fac, c.dval.p, c.nf, c.coef_pool.p,
                          c.refine_max_iter, c.refine_tol, c.pstatus.p, c.piters.p, method_big(m) ? c.solve_ws.p : nullptr,

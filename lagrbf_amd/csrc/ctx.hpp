@@ -161,6 +161,9 @@ struct Ctx {
   DBuf<int2> tile_tasks;                   // (patch, row tile) pairs of the tiled factor, step by step
   std::vector<int2> h_tile_tasks;
   std::vector<int64_t> h_tile_off;         // steps: tasks h_tile_off[k] .. h_tile_off[k + 1]
+  DBuf<int2> resid_tasks;                  // (patch, row block) pairs of the stepped refinement's residual
+  DBuf<double> refine_dstate;              // [P] last correction of a stepped refinement
+  bool stepped_refine = true;              // FP32 factors of large saddle-point patches: refine step by step
   int refine_max_iter = 10;
   double refine_tol = 1e-12;
   double fallback = 0, refine_iters_max = 0, refine_iters_sum = 0, refined_patches = 0;

@@ -119,6 +119,11 @@ void launch_factor_tiled(const Grid& g, int ktype, bool fp32, const int2* tasks,
 // (aug.hip) null-space data of the listed patches with a polynomial block
 void launch_aug_prep(const Grid& g, int ktype, const int32_t* list, int nlist, const int32_t* pn,
                      const int64_t* mem_off, const int32_t* mem_pool, const double* xd, const AugView& a, hipStream_t s);
+// the same for large patches: the O(n^2) product U = A V spread over many
+// workgroups, one block of aug_resid_rows() rows per task (patch, row block)
+void launch_aug_prep_tasks(const Grid& g, int ktype, const int32_t* list, int nlist, const int32_t* pn,
+                           const int64_t* mem_off, const int32_t* mem_pool, const double* xd, const AugView& a,
+                           const int2* tasks, int ntasks, hipStream_t s);
 // solve of the saddle-point systems: c and beta from the Cholesky factor of B22
 // (FP64 direct, or FP32 factor with FP64 refinement of the full saddle-point
 // residual); ws: (3 + 3 nf) doubles per member for patches larger than the LDS
@@ -128,6 +133,20 @@ void launch_solve_aug(const Grid& g, int ktype, bool fp32, const int32_t* list, 
                       const void* fac, const double* values, int nf, double* coef, int max_iter, double tol,
                       int32_t* status, int32_t* iters, double* ws, int nmax, const AugView& a, hipStream_t s);
 size_t aug_solve_ws_per_member(int nf);
+// The same solve one refinement iteration per launch (step_it = 0, 1, ...; FP32
+// factors of large patches, vectors in the workspace ws): status[p] is 2 while the
+// patch iterates, dstate[p] its last correction; iterations >= 1 read the residual
+// that launch_aug_resid_big wrote, one block of aug_resid_rows() rows per task
+// (patch, row block).  step_it = -1: the whole solve in one launch.
+void launch_solve_aug_step(const Grid& g, int ktype, bool fp32, const int32_t* list, int nlist, const int32_t* pn,
+                           const int64_t* mem_off, const int64_t* fac_off, const int32_t* mem_pool, const double* xd,
+                           const void* fac, const double* values, int nf, double* coef, int max_iter, double tol,
+                           int32_t* status, int32_t* iters, double* ws, int nmax, const AugView& a, int step_it,
+                           double* dstate, hipStream_t s);
+int aug_resid_rows();
+void launch_aug_resid_big(const Grid& g, int ktype, const int2* tasks, int ntasks, const int32_t* pn,
+                          const int64_t* mem_off, const int32_t* mem_pool, const double* values, int nf,
+                          const int32_t* status, double* ws, const AugView& a, hipStream_t s);
 // polynomial table of the evaluation: [P][mm][nft], beta of fields f0.. of the
 // active patches, zero elsewhere (mm = terms of the requested degree)
 void launch_build_ptab(const Grid& g, const uint8_t* active, const AugView& a, int mm, int nft, int nf, int f0,
