@@ -8,6 +8,7 @@
 namespace lrb {
 
 struct VPtrs7 { const double* v[7]; };
+static constexpr int DP45_OUT_RETRIES = 2;   // forced rejections of one particle before it is removed
 static constexpr int WAVE = 64;   // wavefront width of gfx950
 
 // out = (1 - tau) vA + tau vB at the particle's stage time t + c dt,
@@ -40,11 +41,19 @@ void launch_blend(double* out, const double* va, const double* vb, int64_t ld, i
 //   e = y1 - (x0 + dt sum b*_j k_j),
 //   dt <- dt min(fac_max', max(fac_min, fac E^(-1/5))), fac_max' = fac_max if E <= 1 else 1.
 // Accepted (E <= 1): x0 <- y1, t += dt, k1 <- k7 (first same as last); a particle
-// whose time reaches t_end finishes (alive 0, fin 1).  Rejected: px <- x0.
+// whose time reaches t_end finishes (alive 0, fin 1).  Rejected: px <- x0.  A
+// particle suspended because a trial point of stages 2-6 left the domain is
+// rejected the same way with dt <- dt fac_min, twice per integration at most: the
+// third (or a step size below 1e-10 max(1, |t_end|)) removes it, its path leaving
+// the domain; a step's end point (stage 7) outside the domain removes a particle
+// (k_stage).  susp: bit 0 suspended in this attempt, the rest the count.
 // The next step size is clipped to t_end - t.  counts: accepted, rejected, still active.
-__global__ __launch_bounds__(256) void k_dp45_control(double* __restrict__ px, double* __restrict__ x0, VPtrs7 kv,
+__global__ __launch_bounds__(256) void k_dp45_control(Grid g, int32_t* __restrict__ cell, double* __restrict__ px,
+                                                      double* __restrict__ x0, VPtrs7 kv,
                                                       double* __restrict__ k1, int64_t ld, int Np, int dim,
                                                       int32_t* __restrict__ alive, int32_t* __restrict__ fin,
+                                                      int32_t* __restrict__ susp,
+                                                      unsigned long long* __restrict__ removed,
                                                       double* __restrict__ tp, double* __restrict__ dtp, double t_end,
                                                       double atol, double rtol, double fac, double fac_min,
                                                       double fac_max, unsigned long long* __restrict__ counts) {
@@ -53,7 +62,34 @@ __global__ __launch_bounds__(256) void k_dp45_control(double* __restrict__ px, d
   const double Bs[7] = {5179.0 / 57600.0, 0, 7571.0 / 16695.0, 393.0 / 640.0, -92097.0 / 339200.0, 187.0 / 2100.0, 1.0 / 40.0};
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   unsigned int acc = 0, rej = 0, act = 0;
-  if (i < Np && alive[i]) {
+  if (i < Np && !alive[i] && susp && (susp[i] & 1)) {
+    // a trial point left the domain: rejected, back to x0, the step size shrunk by
+    // fac_min (the controller's factor for a non-finite error).  The third such
+    // rejection of the particle's integration, or a step size below
+    // 1e-10 max(1, |t_end|), means its path leaves the domain: removed (a path that
+    // leaves does so at its stage points c = 1 and at the step's end point alike,
+    // so without a bound it would creep toward the wall by ever shorter steps)
+    for (int d = 0; d < dim; d++) {
+      const size_t k = (size_t)d * ld + i;
+      px[k] = x0[k];
+    }
+    const int nrej = (susp[i] >> 1) + 1;
+    susp[i] = nrej << 1;
+    rej = 1;
+    double dn = dtp[i] * ((1.0 < fac_min) ? 1.0 : fac_min);
+    if (nrej > DP45_OUT_RETRIES || dn <= 1e-10 * fmax(1.0, fabs(t_end))) {
+      susp[i] = 0;
+      atomicAdd(removed, 1ull);
+    } else {
+      // its cell id is the out-of-domain point's (-1): x0's again, for the grouping
+      cell[i] = locate(g, x0[i], x0[ld + i], dim == 3 ? x0[2 * ld + i] : 0.0);
+      alive[i] = 1;
+      const double rest = t_end - tp[i];
+      if (dn > rest) dn = rest;
+      dtp[i] = dn;
+      act = 1;
+    }
+  } else if (i < Np && alive[i]) {
     const double dt = dtp[i];
     double E = 0.0;
     for (int d = 0; d < dim; d++) {
@@ -116,14 +152,17 @@ __global__ __launch_bounds__(256) void k_dp45_control(double* __restrict__ px, d
   }
 }
 
-void launch_dp45_control(double* px, double* x0, const double* const* k, double* k1, int64_t ld, int Np, int dim,
-                         int32_t* alive, int32_t* fin, double* tp, double* dtp, double t_end, double atol, double rtol,
-                         double fac, double fac_min, double fac_max, unsigned long long* counts, hipStream_t s) {
+void launch_dp45_control(const Grid& g, int32_t* cell, double* px, double* x0, const double* const* k, double* k1,
+                         int64_t ld, int Np, int dim,
+                         int32_t* alive, int32_t* fin, int32_t* susp, unsigned long long* removed, double* tp,
+                         double* dtp, double t_end, double atol, double rtol, double fac, double fac_min,
+                         double fac_max, unsigned long long* counts, hipStream_t s) {
   VPtrs7 kv;
   for (int j = 0; j < 7; j++) kv.v[j] = k[j];
   if (Np > 0)
-    hipLaunchKernelGGL(k_dp45_control, dim3((Np + 255) / 256), dim3(256), 0, s, px, x0, kv, k1, ld, Np, dim, alive, fin,
-                       tp, dtp, t_end, atol, rtol, fac, fac_min, fac_max, counts);
+    hipLaunchKernelGGL(k_dp45_control, dim3((Np + 255) / 256), dim3(256), 0, s, g, cell, px, x0, kv, k1, ld, Np, dim,
+                       alive, fin,
+                       susp, removed, tp, dtp, t_end, atol, rtol, fac, fac_min, fac_max, counts);
 }
 
 // Per-particle state at the start: t = t0, dt = min(dt0, t_end - t0), not finished.

@@ -1219,7 +1219,7 @@ static int reorder_particles(Ctx& c) {
   r.isrc[1] = c.alive.p; r.idst[1] = c.alive_alt.p;
   r.isrc[2] = c.porig.p; r.idst[2] = c.porig_alt.p;
   r.ni = 3;
-  launch_reorder(r, c.perm.p, c.Np, c.ld, c.s);
+  HIPCHK(launch_reorder(r, c.perm.p, c.Np, c.ld, c.s));
   swap_particle_buffers(c);
   c.cell_ordered = true;
   // the cell groups now sit at pstart[c] .. pstart[c+1] in slot order
@@ -1530,6 +1530,7 @@ static int reorder_adaptive(Ctx& c) {
   HIPCHK(c.ptime_alt.reserve(ld));
   HIPCHK(c.pdt_alt.reserve(ld));
   HIPCHK(c.pfin_alt.reserve(ld));
+  HIPCHK(c.psusp_alt.reserve(ld));
   // vadv[1] and vadv[2] are recomputed by every attempt: they take k1 and x0
   ReorderSet r{};
   r.src[0] = c.x0.p; r.dst[0] = c.vadv[2].p; r.rows[0] = dim;
@@ -1538,13 +1539,15 @@ static int reorder_adaptive(Ctx& c) {
   r.src[3] = c.pdt.p; r.dst[3] = c.pdt_alt.p; r.rows[3] = 1;
   r.nd = 4;
   r.isrc[0] = c.pfin.p; r.idst[0] = c.pfin_alt.p;
-  r.ni = 1;
-  launch_reorder(r, c.perm.p, c.Np, c.ld, c.s);
+  r.isrc[1] = c.psusp.p; r.idst[1] = c.psusp_alt.p;
+  r.ni = 2;
+  HIPCHK(launch_reorder(r, c.perm.p, c.Np, c.ld, c.s));
   swap_buf(c.x0, c.vadv[2]);
   swap_buf(c.vadv[0], c.vadv[1]);
   swap_buf(c.ptime, c.ptime_alt);
   swap_buf(c.pdt, c.pdt_alt);
   swap_buf(c.pfin, c.pfin_alt);
+  swap_buf(c.psusp, c.psusp_alt);
   return reorder_particles(c);
 }
 
@@ -1567,6 +1570,8 @@ static int advect_adaptive(Ctx& c, double t0, double t1, int max_attempts) {
   HIPCHK(c.pdt.reserve(ld));
   HIPCHK(c.pfin.reserve(ld));
   HIPCHK(c.pself.reserve(ld));
+  HIPCHK(c.psusp.reserve(ld));
+  launch_fill_i32(c.psusp.p, (int)ld, 0, c.s);
   CHK(partitions_setup(c, c.setup_all));
   const double dt0 = std::min(c.adv_dt, t1 - t0);
   launch_dp45_init(Np, c.alive.p, c.ptime.p, c.pdt.p, c.pfin.p, c.pself.p, t0, dt0, c.s);
@@ -1611,13 +1616,16 @@ static int advect_adaptive(Ctx& c, double t0, double t1, int max_attempts) {
       sa.req = c.pself.p;
       sa.mul = 1.0;
       sa.div = 1.0;
+      // trial points of stages 2-6 outside the domain reject the attempt; the
+      // step's end point (stage 7) outside removes the particle
+      sa.suspend = s < 5 ? c.psusp.p : nullptr;
       stage(c, c.x0.p, sa);
       CHK(adaptive_velocity(c, c.vadv[s + 1].p, Cst[s]));
     }
     HIPCHK(hipMemsetAsync(cnt.p, 0, 3 * sizeof(unsigned long long), c.s));
     const double* k[7];
     for (int j = 0; j < 7; j++) k[j] = c.vadv[j].p;
-    launch_dp45_control(c.px.p, c.x0.p, k, c.vadv[0].p, c.ld, n, dim, c.alive.p, c.pfin.p, c.ptime.p, c.pdt.p, t1,
+    launch_dp45_control(c.g, c.pcell.p, c.px.p, c.x0.p, k, c.vadv[0].p, c.ld, n, dim, c.alive.p, c.pfin.p, c.psusp.p, c.counters.p, c.ptime.p, c.pdt.p, t1,
                         c.dopri_atol, c.dopri_rtol, c.dopri_fac, c.dopri_fac_min, c.dopri_fac_max, cnt.p, c.s);
     c.Np = Np;
     c.bucket_np = -1;

@@ -83,7 +83,7 @@ struct Ctx {
   double blend_t[2] = {0.0, 1.0};
   // per-particle adaptive DP45 state and counters
   DBuf<double> ptime, pdt, vtmp;
-  DBuf<int32_t> pfin, pself;
+  DBuf<int32_t> pfin, pself, psusp, psusp_alt;   // psusp: out-of-domain trial points (adaptive DP45)
   DBuf<double> ptime_alt, pdt_alt;     // reorder targets of the adaptive state
   bool cell_ordered = false;              // swarm stored in cell order since the last fill
   int bucket_np = -1;                     // swarm size for the tuner's bucket while Np is a prefix

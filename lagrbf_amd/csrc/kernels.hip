@@ -877,8 +877,15 @@ __global__ void k_iota(int32_t* __restrict__ a, int n) {
   if (i < n) a[i] = i;
 }
 
-void launch_reorder(const ReorderSet& r, const int32_t* perm, int n, int64_t ld, hipStream_t s) {
-  if (n > 0) hipLaunchKernelGGL(k_reorder, dim3((n + 255) / 256), dim3(256), 0, s, r, perm, n, ld);
+hipError_t launch_reorder(const ReorderSet& r, const int32_t* perm, int n, int64_t ld, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  if (!perm) return hipErrorInvalidValue;
+  for (int a = 0; a < r.nd; a++)
+    if (r.rows[a] > 0 && (!r.src[a] || !r.dst[a])) return hipErrorInvalidValue;
+  for (int a = 0; a < r.ni; a++)
+    if (!r.isrc[a] || !r.idst[a]) return hipErrorInvalidValue;
+  hipLaunchKernelGGL(k_reorder, dim3((n + 255) / 256), dim3(256), 0, s, r, perm, n, ld);
+  return hipSuccess;
 }
 
 void launch_iota(int32_t* a, int n, hipStream_t s) {
@@ -937,7 +944,12 @@ __global__ void k_stage(Grid g, const double* __restrict__ pux, const double* __
   for (int d = 0; d < dim; d++) px[d * ld + i] = x[d];
   const int c = locate(g, x[0], x[1], x[2]);
   cell[i] = c;
-  if (c < 0) { alive[i] = 0; atomicAdd(removed, 1ull); return; }
+  if (c < 0) {
+    alive[i] = 0;
+    if (a.suspend) a.suspend[i] |= 1;
+    else atomicAdd(removed, 1ull);
+    return;
+  }
   double wl[8];
   weights_point(g, pux, puy, puz, x[0], x[1], x[2], c, wl);
 #pragma unroll

@@ -21,6 +21,10 @@ struct StageArgs {
   const double* dt_req;
   const int32_t* req;
   double mul, div;
+  // per-particle adaptive steps: a stage point outside the domain sets suspend[i]
+  // and takes the particle out of the attempt's remaining stages (the controller
+  // rejects the attempt and restores it) instead of removing it; null: removal
+  int32_t* suspend;
 };
 
 // Read-only patch tables used by evaluation.
@@ -175,7 +179,8 @@ struct ReorderSet {
   int32_t* idst[4];
   int ni;
 };
-void launch_reorder(const ReorderSet& r, const int32_t* perm, int n, int64_t ld, hipStream_t s);
+// hipErrorInvalidValue (nothing launched) for a null source or target
+hipError_t launch_reorder(const ReorderSet& r, const int32_t* perm, int n, int64_t ld, hipStream_t s);
 void launch_iota(int32_t* a, int n, hipStream_t s);
 void launch_sorted_coords(const double* x, int n, int dim, const int32_t* cell_nodes, double* xs, hipStream_t s);
 // (eval_cell.hip) fp32: coef/zeros/xs are float (cell-relative coordinates)
@@ -248,9 +253,14 @@ void launch_dopri_err_req(const double* px, const double* x0, const double* cons
 // per-particle adaptive DP45 with time-blended velocity (adaptive.hip)
 void launch_blend(double* out, const double* va, const double* vb, int64_t ld, int Np, int dim, const int32_t* alive,
                   const double* tp, const double* dtp, double cst, double tA, double tB, hipStream_t s);
-void launch_dp45_control(double* px, double* x0, const double* const* k, double* k1, int64_t ld, int Np, int dim,
-                         int32_t* alive, int32_t* fin, double* tp, double* dtp, double t_end, double atol, double rtol,
-                         double fac, double fac_min, double fac_max, unsigned long long* counts, hipStream_t s);
+// susp: particles suspended by an out-of-domain trial point (StageArgs::suspend):
+// rejected, restored to x0 and alive again with step size times fac_min, or
+// removed (counted in removed) once that step size falls below 1e-10 max(1, |t_end|)
+void launch_dp45_control(const Grid& g, int32_t* cell, double* px, double* x0, const double* const* k, double* k1,
+                         int64_t ld, int Np, int dim,
+                         int32_t* alive, int32_t* fin, int32_t* susp, unsigned long long* removed, double* tp,
+                         double* dtp, double t_end, double atol, double rtol, double fac, double fac_min,
+                         double fac_max, unsigned long long* counts, hipStream_t s);
 void launch_dp45_init(int Np, const int32_t* alive, double* tp, double* dtp, int32_t* fin, int32_t* self, double t0,
                       double dt0, hipStream_t s);
 void launch_dp45_finish(int Np, int32_t* alive, const int32_t* fin, hipStream_t s);

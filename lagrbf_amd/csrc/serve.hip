@@ -271,7 +271,7 @@ int batch_tick(Batch& b) {
   r.isrc[2] = b.req.p; r.idst[2] = b.req_b.p;
   r.isrc[3] = b.porig.p; r.idst[3] = b.porig_b.p;
   r.ni = 4;
-  launch_reorder(r, b.perm.p, b.Np, b.ld, b.s);
+  HIPCHK(launch_reorder(r, b.perm.p, b.Np, b.ld, b.s));
   swap_buf(b.px, b.px_b); swap_buf(b.vel, b.vel_b); swap_buf(b.w, b.w_b);
   swap_buf(b.cell, b.cell_b); swap_buf(b.alive, b.alive_b); swap_buf(b.req, b.req_b); swap_buf(b.porig, b.porig_b);
   launch_iota(b.perm.p, b.Np, b.s);

@@ -1019,7 +1019,7 @@ int orc_advect_adaptive(orc_ctx *ctx, double t0, double t1, int max_attempts, in
   const int dim = ctx->dim, Np = ctx->Np, nf = ctx->cfg.n_ifields;
   const size_t n = (size_t)Np * dim + 1;
   double *k[7], *x0, *tp, *dtp, *vals, dt0;
-  int32_t *fin;
+  int32_t *fin, *susp;
   int a, s, j, p, d;
   if (!ctx->data_filled || !ctx->points_filled) return ERR_ARG_NULL;
   if (!(t1 > t0) || max_attempts < 0 || !(ctx->cfg.dt > 0.0)) return ERR_ARG_RANGE;
@@ -1029,6 +1029,7 @@ int orc_advect_adaptive(orc_ctx *ctx, double t0, double t1, int max_attempts, in
   tp = (double *)calloc((size_t)Np + 1, sizeof(double));
   dtp = (double *)calloc((size_t)Np + 1, sizeof(double));
   fin = (int32_t *)calloc((size_t)Np + 1, sizeof(int32_t));
+  susp = (int32_t *)calloc((size_t)Np + 1, sizeof(int32_t));
   vals = (double *)calloc((size_t)Np * (blend ? 2 * dim : dim) + 1, sizeof(double));
   for (j = 0; j < 4; j++) counts[j] = 0.0;
   partitions_setup(ctx);
@@ -1051,12 +1052,37 @@ int orc_advect_adaptive(orc_ctx *ctx, double t0, double t1, int max_attempts, in
           x[d] = x0[(size_t)p * dim + d] + dtp[p] * v;
         }
         project_point(x, dim, ctx->cfg.rmin, ctx->cfg.rmax);
+        /* a trial point of stages 2-6 outside the domain suspends the particle for
+         * the rest of the attempt, which is then rejected (below); only the step's
+         * end point (stage 7) leaving the domain removes it */
+        if (s < 5 && locate_point(dim, ctx->cfg.n, ctx->cfg.O, ctx->cfg.L, ctx->h, x) < 0) {
+          susp[p] |= 1;
+          ctx->alive[p] = 0;
+        }
       }
       after_move(ctx);
       adaptive_eval(ctx, k[s + 1], vals, blend, tp, dtp, Cst[s], tA, tB);
     }
     for (p = 0; p < Np; p++) {
       double E = 0.0, t, g, inner, f, fmx, dn, rest; int ok;
+      if (susp[p] & 1) {
+        /* forced rejection: back to x0, the step size shrunk by fac_min; the third
+         * of the particle's integration (or a step size below 1e-10 max(1, |t1|))
+         * means the path leaves the domain: removed.  susp: bit 0 this attempt, the
+         * rest the count */
+        const int nrej = (susp[p] >> 1) + 1;
+        for (d = 0; d < dim; d++) ctx->px[(size_t)p * dim + d] = x0[(size_t)p * dim + d];
+        susp[p] = nrej << 1;
+        rej += 1;
+        dn = dtp[p] * fmin(1.0, ctx->cfg.dopri_dt_fac_min);
+        if (nrej > 2 || dn <= 1e-10 * fmax(1.0, fabs(t1))) { susp[p] = 0; ctx->removed += 1; continue; }
+        ctx->alive[p] = 1;
+        rest = t1 - tp[p];
+        if (dn > rest) dn = rest;
+        dtp[p] = dn;
+        act += 1;
+        continue;
+      }
       if (!ctx->alive[p]) continue;
       for (d = 0; d < dim; d++) {
         const size_t i = (size_t)p * dim + d;
@@ -1105,7 +1131,7 @@ int orc_advect_adaptive(orc_ctx *ctx, double t0, double t1, int max_attempts, in
   }
   for (p = 0; p < Np; p++) if (fin[p]) ctx->alive[p] = 1;
   for (j = 0; j < 7; j++) free(k[j]);
-  free(x0); free(tp); free(dtp); free(fin); free(vals);
+  free(x0); free(tp); free(dtp); free(fin); free(susp); free(vals);
   return 0;
 }
 

@@ -99,6 +99,70 @@ def test_oracle_particles_independent():
         assert np.array_equal(one[0], xs[i])
 
 
+def _wall_case(dt):
+    """The rotation field with four particles: three on circles about the centre that
+    stay inside the unit square (radii 0.49, 0.485, 0.3, two of them next to a
+    wall) and one whose circle crosses the top wall (radius 0.59); first step
+    size dt."""
+    opts, X, F, _ = _rotation_case(blend=False, rtol=1e-9, n=4)
+    P = np.array([[0.99, 0.5], [0.5, 0.015], [0.97, 0.9], [0.2, 0.5]])
+    return dict(opts, dt=dt), X, F, P
+
+
+def test_oracle_wall_overshoot_rejected_not_removed():
+    """A first step of 0.5 rad sends the trial points of the near-wall particles
+    outside the square: those attempts are rejected (the step size shrinks) instead
+    of removing the particles, so they end where a run started with steps of 0.01
+    ends, on the rotated positions up to the interpolant's error; the particle whose
+    path leaves through the wall is removed, and the loop ends (a forced rejection
+    below a step size of 1e-10 removes a particle sitting at the wall)."""
+    from oracle.oracle import OracleCtx
+    out = {}
+    for dt in (0.5, 0.01):
+        o, X, F, P = _wall_case(dt)
+        c = OracleCtx(o)
+        c.fill_data(X, F); c.fill_points(P); c.interpolate_c()
+        counts = c.advect_adaptive_c(0.0, 1.0)
+        xs = np.zeros_like(P); c.output(coords=xs)
+        out[dt] = (xs, counts, c.removed_count())
+        c.destroy()
+        assert counts["active"] == 0
+        assert out[dt][2] == 1
+    keep = [0, 1, 3]
+    assert out[0.5][1]["rejected"] > 0
+    assert np.max(np.abs(out[0.5][0][keep] - out[0.01][0][keep])) < 1e-7
+    th = 1.0
+    R = np.array([[np.cos(th), -np.sin(th)], [np.sin(th), np.cos(th)]])
+    ex = 0.5 + (P - 0.5) @ R.T
+    assert np.max(np.abs(out[0.5][0][keep] - ex[keep])) < 0.02
+
+
+@pytest.mark.gpu
+def test_adaptive_wall_overshoot_matches_oracle():
+    """The library applies the same rule: out-of-domain trial points reject the
+    attempt, the particle crossing the wall is removed.  The step sequences are the
+    oracle's up to accept/reject decisions that the interpolated velocity's rounding
+    flips (an error norm within rounding of 1): counts within 5%, and positions to
+    1e-7, the bar two step sequences of this integration (first steps 0.5 and 0.01)
+    meet in the oracle (rtol 1e-9)."""
+    import lagrbf_amd as L
+    o, X, F, P = _wall_case(0.5)
+    print("case: adaptive DP45, trial points beyond the wall, first step 0.5", flush=True)
+    g = L.LagRBFCtx(petsc_options=dict(o))
+    g.fill_data(X, F); g.fill_points(P); g.interpolate_c()
+    cg = g.advect_adaptive_c(0.0, 1.0)
+    xg = np.zeros_like(P); g.output(coords=xg)
+    xo, co = _oracle_run(o, X, F, P, 0.0, 1.0)
+    print(f"wall case: gpu {cg}, removed {g.info('removed'):.0f}; oracle {co}", flush=True)
+    assert cg["active"] == 0 and g.info("removed") == 1
+    assert cg["rejected"] > 0
+    assert abs(cg["accepted"] - co["accepted"]) <= 0.05 * co["accepted"]
+    assert abs(cg["rejected"] - co["rejected"]) <= 0.05 * co["rejected"]
+    keep = [0, 1, 3]
+    assert np.all(xg[2] == 0)
+    assert np.max(np.abs(xg[keep] - xo[keep])) < 1e-7
+
+
 def _fourier_blend_case(Np=3000):
     opts, X, V, P = cases.fourier_3d(nlat=18, Np=Np, n=6, scheme=3)
     rng = np.random.default_rng(9)
