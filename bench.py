@@ -164,7 +164,8 @@ def run_path(L, precision, opts, X, V, P, steps, warmup, dt, sample, traffic_fil
     ctx.output(coords=pos)
     rec = {
         "value": live_steps / wall, "ms_per_step": wall / max(steps, 1) * 1e3,
-        "end_to_end": live_steps / (wall + t_setup), "setup_s": t_setup, "removed_particles": removed,
+        "end_to_end": live_steps / (wall + t_setup), "setup_s": t_setup, "setup_phases_ms": setup_phases(ctx),
+        "removed_particles": removed,
         "removed_in_timed_steps": removed - removed0,
         "fallback_patches": ctx.info("fallback"), "refine_iters_max": ctx.info("refine_iters"),
         "roofline": {"bound": "valu", "kernel": "pu_evaluation (k_eval_blk)",
@@ -373,7 +374,7 @@ def c3_bench(L, precision, steps, warmup):
             if v:
                 iters[str(i)] = int(v)
     res = {"precision": precision, "nodes": int(X.shape[0]), "particles": int(P.shape[0]), "setup_s": t_setup,
-           "setup_ms_patches": ctx.info("setup_ms"), "patches_setup": ctx.info("patches_setup"),
+           "setup_phases_ms": setup_phases(ctx), "patches_setup": ctx.info("patches_setup"),
            "patch_size": {"min": int(sizes.min()), "p50": float(np.median(sizes)), "p99": float(np.percentile(sizes, 99)),
                           "max": int(sizes.max()), "above_512": int((sizes > 512).sum()), "histogram": hist},
            "fallback_patches": ctx.info("fallback"), "refactored_fp64_tiled": ctx.info("big_refactor64"),
@@ -383,7 +384,7 @@ def c3_bench(L, precision, steps, warmup):
            "eval_ms": (ev1 - ev0) / max(n1 - n0, 1), "removed": removed}
     ctx.destroy()
     if precision == "mixed":
-        res["setup_s_fp64_factor"] = setup_with_fp64_factor(L, opts, X, V, P)
+        res["with_fp32_factor"] = setup_with_fp32_factor(L, opts, X, V, P)
     log(f"c3 [{precision}]: set-up {t_setup:.2f}s, patch sizes {res['patch_size']}, fallback {res['fallback_patches']:.0f} "
         f"(FP64 tiled {res['refactored_fp64_tiled']:.0f}, Bunch-Kaufman {res['refactored_bunch_kaufman']:.0f}), "
         f"refinement iterations {iters}, {res['particle_rk2_steps_per_s']:.3e} particle-RK2-steps/s "
@@ -391,17 +392,28 @@ def c3_bench(L, precision, steps, warmup):
     return res
 
 
-def setup_with_fp64_factor(L, opts, X, V, P):
-    """Set-up time of the mixed path with lagrbf_mixed_factor fp64 (FP64 factors, no
-    refinement, FP32 evaluation) on the same nodes and particles."""
-    ctx = L.LagRBFCtx(petsc_options=dict(opts, lagrbf_mixed_factor="fp64"), precision="mixed")
+def setup_phases(ctx):
+    """Patch set-up by phase, ms: pool growth (host; after another context freed tens
+    of GB this waits on the driver), factors, solves, and their total on the stream."""
+    return {k: round(ctx.info(f"setup_{k}_ms"), 2) for k in ("alloc", "factor", "solve")} | {
+        "total": round(ctx.info("setup_ms"), 2)}
+
+
+def setup_with_fp32_factor(L, opts, X, V, P):
+    """Set-up of the mixed path with lagrbf_mixed_factor fp32 (FP32 factors refined in
+    FP64) on the same nodes and particles: patch set-up phases in ms (pool growth,
+    factor, solve) and wall seconds."""
+    ctx = L.LagRBFCtx(petsc_options=dict(opts, lagrbf_mixed_factor="fp32"), precision="mixed")
     ctx.fill_data(X, V)
     ctx.fill_points(P)
     ts = time.perf_counter()
     ctx.interpolate_c()
     t = time.perf_counter() - ts
+    iters = {str(i): int(ctx.info(f"refine_hist_{i}")) for i in range(64) if ctx.info(f"refine_hist_{i}")}
+    res = {"setup_s": t, "setup_phases_ms": setup_phases(ctx), "refine_iters_histogram": iters,
+           "fallback_patches": ctx.info("fallback")}
     ctx.destroy()
-    return t
+    return res
 
 
 def c4_sizes(scale):
@@ -433,13 +445,14 @@ def c4_bench(L, precision, t_end, m=126, N=8000000, Np=16777216, pu=38):
     wall = time.perf_counter() - t1
     ev1, n1 = ctx.info("eval_ms"), ctx.info("eval_count")
     res = {"precision": precision, "nodes": int(X.shape[0]), "particles": int(P.shape[0]), "t_end": t_end,
-           "setup_s": t_setup, "wall_s": wall, "attempts": c["attempts"], "accepted_steps": c["accepted"],
+           "setup_s": t_setup, "setup_phases_ms": setup_phases(ctx), "wall_s": wall, "attempts": c["attempts"],
+           "accepted_steps": c["accepted"],
            "rejected_steps": c["rejected"], "unfinished": c["active"], "removed": ctx.info("removed"),
            "particle_steps_per_s": c["accepted"] / wall, "eval_ms": (ev1 - ev0) / max(n1 - n0, 1),
            "evals": int(n1 - n0)}
     ctx.destroy()
     if precision == "mixed":
-        res["setup_s_fp64_factor"] = setup_with_fp64_factor(L, opts, X, V, P)
+        res["with_fp32_factor"] = setup_with_fp32_factor(L, opts, X, V, P)
     log(f"c4 [{precision}]: set-up {t_setup:.2f}s, {c['attempts']:.0f} attempts, {c['accepted']:.3e} accepted / "
         f"{c['rejected']:.3e} rejected steps in {wall:.2f}s = {res['particle_steps_per_s']:.3e} particle-steps/s, "
         f"eval {res['eval_ms']:.2f} ms, removed {res['removed']:.0f}")

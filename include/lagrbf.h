@@ -106,7 +106,10 @@ int lagrbf_advect_steps(void *ctx, int32_t nsteps, double dt);
  * -lagrbf_time_blend (n_interpolation_fields = 2 x dim) the velocity at stage
  * time t is (1 - tau) v_A + tau v_B, tau = (t - tA) / (tB - tA), v_A the first dim
  * fields and v_B the next (-lagrbf_blend_times tA,tB, default 0,1).  Counters:
- * "dp_attempts", "dp_accepted", "dp_rejected", "dp_active" (unfinished). */
+ * "dp_attempts", "dp_accepted", "dp_rejected", "dp_active" (unfinished).  A trial
+ * point of stages 2-6 outside the domain rejects the particle's attempt (back to
+ * its step's start, step size times dopri_dt_fac_min), twice per particle at most;
+ * the third, or a step's end point outside the domain, removes it ("removed"). */
 int lagrbf_advect_adaptive(void *ctx, double t_start, double t_end, int32_t max_attempts);
 
 /* Runtime counters: "removed" (particles that left the domain), "fallback"
@@ -114,8 +117,9 @@ int lagrbf_advect_adaptive(void *ctx, double t_start, double t_end, int32_t max_
  * "patches_setup", "dt" (current advection dt), "eval_ms" and "eval_count"
  * (evaluation kernel time and launches timed, outside graphs), "eval_variant"
  * (launch shape 0..17 of the last cell-table evaluation, -1 before one),
- * "graph_ms" and "graph_steps" (replayed steps and their time), "setup_ms", "n_particles",
- * "n_nodes", "max_patch", "eps_w", "eps_r". */
+ * "graph_ms" and "graph_steps" (replayed steps and their time), "setup_ms" and its
+ * phases "setup_alloc_ms" (pool growth), "setup_factor_ms", "setup_solve_ms",
+ * "n_particles", "n_nodes", "max_patch", "eps_w", "eps_r". */
 int lagrbf_get_info(void *ctx, const char *key, double *value);
 
 /* Introspection for tests: copies device state into `out` (capacity `cap`

@@ -5,6 +5,7 @@
 #include <sys/stat.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cerrno>
 #include <cfloat>
 #include <cmath>
@@ -168,6 +169,8 @@ static int ctx_from_options(Ctx& c, const OptView& o) {
   c.chol_big_min = std::max(0, std::min(c.chol_big_min, chol_max_n()));
   o.get_bool("lagrbf_stepped_refine", c.stepped_refine, bad);
   if (bad) return set_err(E_WRONG, fn, "malformed -lagrbf_stepped_refine option");
+  o.get_bool("lagrbf_lean_solve", c.lean_solve, bad);
+  if (bad) return set_err(E_WRONG, fn, "malformed -lagrbf_lean_solve option");
   std::string bf;
   if (o.get_string("lagrbf_big_factor", bf)) {
     if (bf == "tiled") c.big_factor = 1;
@@ -734,6 +737,32 @@ static int solve_list(Ctx& c, const std::vector<int32_t>& sl) {
   const AugView ap = aug_view(c, true), ae = aug_view(c, false);
   std::vector<int32_t> l[N_METHODS];
   for (int p : sl) l[c.h_method[p]].push_back(p);
+  // one-workgroup solves size their LDS rows by the launch's largest patch: the
+  // patches go by size, in bands launched apart, so that the many small patches
+  // are not held to the occupancy of the few large ones.  Cholesky patches with a
+  // matrix-core factor (inverse diagonal blocks) take the lean solve.
+  static constexpr int SOLVE_BANDS[] = {96, 128, 192, 256, 384, 1 << 30};
+  struct Band { int b0, b1; bool lean; };
+  std::vector<Band> bands[N_METHODS];
+  for (int m = M_CHOL64; m < N_METHODS; m++) {
+    if (l[m].empty()) continue;
+    if (method_big(m)) { bands[m].push_back({0, (int)l[m].size(), false}); continue; }
+    auto lean = [&](int p) {
+      return c.lean_solve && !method_aug(m) && c.h_fnb[p] == 16 && c.h_n[p] <= solve_llt_max_n();
+    };
+    std::sort(l[m].begin(), l[m].end(), [&](int a, int b) {
+      return std::make_pair(!lean(a), c.h_n[a]) < std::make_pair(!lean(b), c.h_n[b]);
+    });
+    size_t i = 0;
+    while (i < l[m].size()) {
+      const bool ln = lean(l[m][i]);
+      for (int lim : SOLVE_BANDS) {
+        const size_t b0 = i;
+        while (i < l[m].size() && lean(l[m][i]) == ln && c.h_n[l[m][i]] <= lim) i++;
+        if (i > b0) bands[m].push_back({(int)b0, (int)i, ln});
+      }
+    }
+  }
   for (int m = 0; m < N_METHODS; m++) if (!l[m].empty()) CHK(upload_list(c, l[m], m));
   if (!l[M_BK64].empty())
     launch_solve_bk(c.list.p, (int)l[M_BK64].size(), c.p_n.p, c.p_mem_off.p, c.p_fac_off.p, c.mem_pool.p, c.fac_pool.p,
@@ -749,21 +778,28 @@ static int solve_list(Ctx& c, const std::vector<int32_t>& sl) {
   if (ws_need) HIPCHK(c.solve_ws.reserve(ws_need));
   for (int m = M_CHOL64; m < N_METHODS; m++)
     if (!l[m].empty()) {
-      int nmax = 1;
-      for (int p : l[m]) nmax = std::max(nmax, (int)c.h_n[p]);
       const void* fac = method_fp32(m) ? (const void*)c.fac32_pool.p : (const void*)c.fac_pool.p;
-      if (m == M_AUG32B && c.big_factor && c.stepped_refine)
-        CHK(solve_aug_stepped(c, l[m], nmax));
-      else if (method_aug(m))
-        launch_solve_aug(c.g, c.ktype, method_fp32(m), c.list.p + (size_t)m * P, (int)l[m].size(), c.p_n.p,
-                         c.p_mem_off.p, c.p_fac_off.p, c.mem_pool.p, c.dx.p, fac, c.dval.p, c.nf, c.coef_pool.p,
-                         c.refine_max_iter, c.refine_tol, c.pstatus.p, c.piters.p, method_big(m) ? c.solve_ws.p : nullptr,
-                         nmax, ap, c.s);
-      else
-        launch_solve_chol(c.g, c.ktype, method_fp32(m), c.list.p + (size_t)m * P, (int)l[m].size(), c.p_n.p,
-                          c.p_mem_off.p, c.p_fac_off.p, c.mem_pool.p, c.dx.p, fac, c.dval.p, c.nf, c.coef_pool.p,
-                          c.refine_max_iter, c.refine_tol, c.pstatus.p, c.piters.p, method_big(m) ? c.solve_ws.p : nullptr,
-                          nmax, ae, c.s);
+      // one launch per band (large-patch methods: one launch)
+      for (const Band& bd : bands[m]) {
+        const int b0 = bd.b0, b1 = bd.b1;
+        int nmax = 1;
+        for (int i = b0; i < b1; i++) nmax = std::max(nmax, (int)c.h_n[l[m][i]]);
+        const int32_t* dl = c.list.p + (size_t)m * P + b0;
+        if (bd.lean)
+          launch_solve_llt(c.g, c.ktype, method_fp32(m), dl, b1 - b0, c.p_n.p, c.p_mem_off.p, c.p_fac_off.p,
+                           c.mem_pool.p, c.dx.p, fac, c.dval.p, c.nf, c.coef_pool.p, c.refine_max_iter, c.refine_tol,
+                           c.pstatus.p, c.piters.p, nmax, ae.eps, c.s);
+        else if (m == M_AUG32B && c.big_factor && c.stepped_refine)
+          CHK(solve_aug_stepped(c, l[m], nmax));
+        else if (method_aug(m))
+          launch_solve_aug(c.g, c.ktype, method_fp32(m), dl, b1 - b0, c.p_n.p, c.p_mem_off.p, c.p_fac_off.p,
+                           c.mem_pool.p, c.dx.p, fac, c.dval.p, c.nf, c.coef_pool.p, c.refine_max_iter, c.refine_tol,
+                           c.pstatus.p, c.piters.p, method_big(m) ? c.solve_ws.p : nullptr, nmax, ap, c.s);
+        else
+          launch_solve_chol(c.g, c.ktype, method_fp32(m), dl, b1 - b0, c.p_n.p, c.p_mem_off.p, c.p_fac_off.p,
+                            c.mem_pool.p, c.dx.p, fac, c.dval.p, c.nf, c.coef_pool.p, c.refine_max_iter, c.refine_tol,
+                            c.pstatus.p, c.piters.p, method_big(m) ? c.solve_ws.p : nullptr, nmax, ae, c.s);
+      }
     }
   if (l[M_CHOL32].empty() && l[M_CHOL32B].empty() && l[M_AUG32].empty() && l[M_AUG32B].empty()) return 0;
   std::vector<int32_t> st(P), its(P);
@@ -839,6 +875,7 @@ int partitions_setup(Ctx& c, bool all_patches) {
     }
   }
   if (!newl.empty()) {
+    const auto ta = std::chrono::steady_clock::now();
     HIPCHK(c.mem_pool.grow(need_mem + 1, c.mem_used, c.s));
     HIPCHK(c.piv_pool.grow(need_mem + 1, c.mem_used, c.s));
     if (c.aux_used > 0) {
@@ -852,12 +889,22 @@ int partitions_setup(Ctx& c, bool all_patches) {
     if (c.precision) HIPCHK(c.coef32_pool.grow((need_mem + 1) * std::max(c.nf, 1), c.mem_used * std::max(c.nf, 1), c.s));
     c.mem_used = need_mem;
     for (int m = 0; m < N_METHODS; m++) CHK(alloc_factor_slots(c, fl[m], m));
+    c.setup_alloc_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - ta).count();
     HIPCHK(hipMemcpyAsync(c.p_n.p, c.h_n.data(), sizeof(int32_t) * P, hipMemcpyHostToDevice, c.s));
     HIPCHK(hipMemcpyAsync(c.p_mem_off.p, c.h_mem_off.data(), sizeof(int64_t) * P, hipMemcpyHostToDevice, c.s));
     HIPCHK(hipMemcpyAsync(c.p_fac_off.p, c.h_fac_off.data(), sizeof(int64_t) * P, hipMemcpyHostToDevice, c.s));
     HIPCHK(hipMemcpyAsync(c.list.p, newl.data(), sizeof(int32_t) * newl.size(), hipMemcpyHostToDevice, c.s));
     launch_patch_gather(c.g, c.list.p, (int)newl.size(), c.cell_start.p, c.cell_nodes.p, c.p_mem_off.p, c.mem_pool.p, c.s);
+    hipEvent_t f0, f1;
+    HIPCHK(hipEventCreate(&f0)); HIPCHK(hipEventCreate(&f1));
+    HIPCHK(hipEventRecord(f0, c.s));
     CHK(factor_lists(c, fl));
+    HIPCHK(hipEventRecord(f1, c.s));
+    HIPCHK(hipEventSynchronize(f1));
+    float fms = 0.f;
+    HIPCHK(hipEventElapsedTime(&fms, f0, f1));
+    c.setup_factor_ms += fms;
+    HIPCHK(hipEventDestroy(f0)); HIPCHK(hipEventDestroy(f1));
     for (int p : newl) c.h_setup[p] = 1;
     if (c.h_cell_start_prev.empty() && c.data_filled) {
       // first patches of this generation: remember the cell lists they were built from
@@ -873,7 +920,16 @@ int partitions_setup(Ctx& c, bool all_patches) {
   for (int p = 0; p < P; p++)
     if (t[p] && c.h_setup[p] && !c.h_solved[p] && c.h_n[p] > 0) solvel.push_back(p);
   if (!solvel.empty()) {
+    hipEvent_t s0, s1;
+    HIPCHK(hipEventCreate(&s0)); HIPCHK(hipEventCreate(&s1));
+    HIPCHK(hipEventRecord(s0, c.s));
     CHK(solve_list(c, solvel));
+    HIPCHK(hipEventRecord(s1, c.s));
+    HIPCHK(hipEventSynchronize(s1));
+    float sms = 0.f;
+    HIPCHK(hipEventElapsedTime(&sms, s0, s1));
+    c.setup_solve_ms += sms;
+    HIPCHK(hipEventDestroy(s0)); HIPCHK(hipEventDestroy(s1));
     if (c.precision) {
       // FP32 copy of the refined coefficients for the FP32 evaluation
       CHK(upload_list(c, solvel, 0));
@@ -2057,6 +2113,10 @@ int lagrbf_get_info(void* ctx, const char* key, double* value) {
   }
   else if (k == "poly_terms") *value = c->poly_mm;
   else if (k == "factor_trial_ms") *value = c->factor_trial_ms;
+  else if (k == "setup_alloc_ms") *value = c->setup_alloc_ms;
+  else if (k == "device_cache_bytes") *value = (double)dev_cached_bytes();
+  else if (k == "setup_factor_ms") *value = c->setup_factor_ms;
+  else if (k == "setup_solve_ms") *value = c->setup_solve_ms;
   else return set_err(E_WRONG, "lagrbf_get_info", "unknown key '%s'", key);
   return 0;
 }

@@ -31,18 +31,26 @@ int set_err(int code, const char* fn, const char* fmt, ...);
     if (r_) return r_;         \
   } while (0)
 
+// (devmem.cpp) device allocations through the process-wide cache of large blocks:
+// a released block of 64 MiB or more is kept for the next reservation on its
+// device instead of returned to the driver (LAGRBF_DEVICE_CACHE=0: no cache)
+hipError_t dev_malloc(void** p, size_t bytes);
+void dev_free(void* p);
+size_t dev_cached_bytes();
+void dev_cache_release();
+
 // Device buffer with grow-only capacity.
 template <typename T>
 struct DBuf {
   T* p = nullptr;
   size_t cap = 0;
   ~DBuf() { release(); }
-  void release() { if (p) (void)hipFree(p); p = nullptr; cap = 0; }
+  void release() { if (p) dev_free(p); p = nullptr; cap = 0; }
   hipError_t reserve(size_t n) {
     if (n <= cap && p) return hipSuccess;
     release();
     size_t m = n ? n : 1;
-    hipError_t e = hipMalloc(&p, m * sizeof(T));
+    hipError_t e = dev_malloc((void**)&p, m * sizeof(T));
     if (e == hipSuccess) cap = m;
     return e;
   }
@@ -50,13 +58,13 @@ struct DBuf {
   hipError_t grow(size_t n, size_t keep, hipStream_t s) {
     if (n <= cap && p) return hipSuccess;
     T* q = nullptr;
-    hipError_t e = hipMalloc(&q, (n ? n : 1) * sizeof(T));
+    hipError_t e = dev_malloc((void**)&q, (n ? n : 1) * sizeof(T));
     if (e != hipSuccess) return e;
     if (p && keep) {
       e = hipMemcpyAsync(q, p, keep * sizeof(T), hipMemcpyDeviceToDevice, s);
-      if (e != hipSuccess) { (void)hipFree(q); return e; }
+      if (e != hipSuccess) { dev_free(q); return e; }
       e = hipStreamSynchronize(s);
-      if (e != hipSuccess) { (void)hipFree(q); return e; }
+      if (e != hipSuccess) { dev_free(q); return e; }
     }
     release();
     p = q;
@@ -152,7 +160,10 @@ struct Ctx {
   DBuf<double> solve_ws;                   // solve workspace of the large patches (2 nf per member)
   DBuf<int32_t> pstatus, piters;
   int precision = 0;                       // 0 FP64, 1 FP32 factor + FP64 refinement
-  bool mixed_factor64 = false;             // mixed: factor in FP64 (no refinement), FP32 evaluation kept
+  // mixed: factor in FP64 (no refinement) with the FP32 evaluation, the default; false:
+  // FP32 factor and FP64 refinement (the solves, not the factor, bound the set-up at
+  // the patch sizes measured, and each refinement pass costs about one more solve)
+  bool mixed_factor64 = true;
   int factor_mode = 0;                     // 0 auto (Cholesky for PD kernels), 1 Bunch-Kaufman always
   bool factors_released = false;           // serving datasets keep coefficients only
   int chol_big_min = 512;                  // patches above this size use the large-patch Cholesky kernels
@@ -164,6 +175,7 @@ struct Ctx {
   DBuf<int2> resid_tasks;                  // (patch, row block) pairs of the stepped refinement's residual
   DBuf<double> refine_dstate;              // [P] last correction of a stepped refinement
   bool stepped_refine = true;              // FP32 factors of large saddle-point patches: refine step by step
+  bool lean_solve = true;                  // small Cholesky patches with matrix-core factors: solve_llt.hip
   int refine_max_iter = 10;
   double refine_tol = 1e-12;
   double fallback = 0, refine_iters_max = 0, refine_iters_sum = 0, refined_patches = 0;
@@ -207,6 +219,9 @@ struct Ctx {
   std::vector<uint8_t> h_fnb;              // per patch: panel width of its matrix-core factor, 0 otherwise
   DBuf<uint8_t> p_fnb;
   double factor_trial_ms = 0;              // time spent in the factor tuner's trials
+  // patch set-up by phase: pool growth (host wall time, the stream idle), factors
+  // and solves (stream time between events)
+  double setup_alloc_ms = 0, setup_factor_ms = 0, setup_solve_ms = 0;
 };
 
 // factor methods: Bunch-Kaufman FP64; Cholesky FP64 / FP32 (n <= chol_max_n());

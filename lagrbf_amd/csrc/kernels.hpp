@@ -104,6 +104,13 @@ void launch_solve_chol(const Grid& g, int ktype, bool fp32, const int32_t* list,
                        const void* fac, const double* values, int nf, double* coef, int max_iter, double tol,
                        int32_t* status, int32_t* iters, double* ws, int nmax /* largest listed patch */,
                        const AugView& a, hipStream_t s);
+// (solve_llt.hip) the same solve for patches of up to solve_llt_max_n() members whose
+// factor holds inverse diagonal blocks (matrix-core factor, panel width 16)
+void launch_solve_llt(const Grid& g, int ktype, bool fp32, const int32_t* list, int nlist, const int32_t* pn,
+                      const int64_t* mem_off, const int64_t* fac_off, const int32_t* mem_pool, const double* xd,
+                      const void* fac, const double* values, int nf, double* coef, int max_iter, double tol,
+                      int32_t* status, int32_t* iters, int nmax, const double* peps, hipStream_t s);
+int solve_llt_max_n();
 // (chol_mfma.hip) the same factor with the panel products on the matrix cores,
 // panels of 16 columns, factor dimension up to nmax <= chol_mfma_max_n()
 int chol_mfma_max_n();

@@ -12,7 +12,9 @@ Tolerances (stated per result):
                                                 quadratic block (cond ~3e7): two backward
                                                 stable FP64 solvers agree to ~cond * 1e-16
   FP32 factor + FP64 refinement coefficients    the same bounds (refinement against the
-                                                FP64 residual reaches the FP64 floor)
+                                                FP64 residual reaches the FP64 floor; the
+                                                mixed cases here factor in FP32,
+                                                lagrbf_mixed_factor fp32)
   FP32-evaluation values (mixed path)           1e-4 relative on these coarse cases: the
                                                 r^3 sums cancel by factors of 1e3-1e4 where
                                                 a patch spans a sizeable part of a
@@ -77,7 +79,7 @@ def small_c2(nlat=16, Np=2000, n=6, deg=1, kernel=7, precision="fp64", **kw):
     gives patches of ~200-300 members, nlat 20 with 5^3 up to ~1,000 (the large-patch
     kernels)."""
     opts, X, V, P = cases.fourier_3d(nlat=nlat, Np=Np, n=n, kernel=kernel)
-    opts.update(lagrbf_poly_degree=deg, lagrbf_precision=precision, **kw)
+    opts.update(lagrbf_poly_degree=deg, lagrbf_precision=precision, lagrbf_mixed_factor="fp32", **kw)
     return opts, X, V, P
 
 
@@ -93,7 +95,7 @@ def clustered_small(local=3.0, deg=2, kernel=6, precision="fp64"):
     opts = dict(pu_mesh_dim=3, pu_mesh_O=[0.0] * 3, pu_mesh_L=[1.0] * 3, pu_mesh_n=[7] * 3, n_interpolation_fields=3,
                 pu_mesh_overlap=1.0, pu_mesh_shape_factor=1.0, kernel_type=kernel, solve_method=0, advection=True,
                 advection_scheme=1, cfl_fac=1e6, dt=1e-3, lagrbf_poly_degree=deg, lagrbf_local_eps=local,
-                lagrbf_precision=precision)
+                lagrbf_precision=precision, lagrbf_mixed_factor="fp32")
     return opts, np.ascontiguousarray(X), np.ascontiguousarray(V), P
 
 

@@ -115,6 +115,7 @@ def test_patch_coefficients_vs_oracle(precision, factor, tol):
     FP32 factor + FP64 refinement within 1e-10 of the oracle's FP64 dsytrf solve."""
     opts, X, V, P = cases.fourier_3d(nlat=24, Np=4000, n=6)
     opts["lagrbf_precision"] = precision
+    opts["lagrbf_mixed_factor"] = "fp32"
     opts["lagrbf_factor"] = factor
     g, o = pair(opts)
     for c in (g, o):
@@ -131,15 +132,16 @@ def test_patch_coefficients_vs_oracle(precision, factor, tol):
         assert err < 1e-5      # FP32 evaluation (values), stated tolerance
 
 
-def test_mixed_with_fp64_factor():
-    """lagrbf_mixed_factor fp64: the mixed path factors in FP64 (no refinement) and
-    keeps the FP32 evaluation: coefficients within 1e-12 of the oracle, values within
-    the FP32 evaluation's 1e-5."""
+@pytest.mark.parametrize("factor", ["fp64", None])
+def test_mixed_with_fp64_factor(factor):
+    """lagrbf_mixed_factor fp64, the default: the mixed path factors in FP64 (no
+    refinement) and keeps the FP32 evaluation: coefficients within 1e-12 of the
+    oracle, values within the FP32 evaluation's 1e-5."""
     opts, X, V, P = cases.fourier_3d(nlat=24, Np=4000, n=6)
     print("case: mixed precision with FP64 factors", flush=True)
     with pytest.raises(RuntimeError, match="lagrbf_mixed_factor"):
         L.LagRBFCtx(petsc_options=dict(opts, lagrbf_mixed_factor="fp16"), precision="mixed")
-    opts = dict(opts, lagrbf_precision="mixed", lagrbf_mixed_factor="fp64")
+    opts = dict(opts, lagrbf_precision="mixed", **({"lagrbf_mixed_factor": factor} if factor else {}))
     g, o = pair(opts)
     for c in (g, o):
         c.fill_data(X, V); c.fill_points(P); c.interpolate_c()
@@ -438,6 +440,7 @@ def test_large_patch_cholesky(precision, big_min, tol):
     oracle is reported with cond(A)."""
     opts, X, V, P = clustered_3d()
     opts["lagrbf_precision"] = precision
+    opts["lagrbf_mixed_factor"] = "fp32"
     opts["lagrbf_chol_big_min"] = big_min
     print(f"case: large patches {precision} big_min={big_min}", flush=True)
     g, o = pair(opts)

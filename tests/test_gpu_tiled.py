@@ -6,7 +6,8 @@ cluster of 3,000 nodes gives patches of thousands of members as the clustered
 workload has them.
 
 Tolerances: FP64 coefficients 1e-12 (Wendland C2) or 1e-10 (saddle-point
-systems, cond ~1e4-1e6); FP32 factor + FP64 refinement 1e-10."""
+systems, cond ~1e4-1e6); FP32 factor + FP64 refinement 1e-10 (the mixed cases
+set lagrbf_mixed_factor fp32)."""
 import numpy as np
 import pytest
 
@@ -22,7 +23,7 @@ pytestmark = pytest.mark.gpu
 def test_tiled_factor_vs_oracle(case, precision):
     if case == "wendland":
         opts, X, V, P = cases.fourier_3d(nlat=24, Np=2000, n=6)
-        opts.update(lagrbf_precision=precision)
+        opts.update(lagrbf_precision=precision, lagrbf_mixed_factor="fp32")
         nP, tol = 6 ** 3, (1e-12 if precision == "fp64" else 1e-10)
     elif case == "phs3-linear":
         opts, X, V, P = small_c2(precision=precision)
@@ -84,7 +85,7 @@ def _cluster_case(kernel, precision, deg=-1, local=0.0):
     P = rng.random((1500, 3))
     opts = dict(pu_mesh_dim=3, pu_mesh_O=[0.0] * 3, pu_mesh_L=[1.0] * 3, pu_mesh_n=[5] * 3, n_interpolation_fields=3,
                 pu_mesh_overlap=1.0, pu_mesh_shape_factor=1.0, kernel_type=kernel, solve_method=0,
-                lagrbf_precision=precision, lagrbf_setup_all=True, oracle_setup_all=True)
+                lagrbf_precision=precision, lagrbf_mixed_factor="fp32", lagrbf_setup_all=True, oracle_setup_all=True)
     if deg >= 0:
         opts["lagrbf_poly_degree"] = deg
     if local > 0:
