@@ -384,7 +384,7 @@ def c3_bench(L, precision, steps, warmup):
            "eval_ms": (ev1 - ev0) / max(n1 - n0, 1), "removed": removed}
     ctx.destroy()
     if precision == "mixed":
-        res["with_fp32_factor"] = setup_with_fp32_factor(L, opts, X, V, P)
+        res["with_fp64_factor"] = setup_with_fp64_factor(L, opts, X, V, P)
     log(f"c3 [{precision}]: set-up {t_setup:.2f}s, patch sizes {res['patch_size']}, fallback {res['fallback_patches']:.0f} "
         f"(FP64 tiled {res['refactored_fp64_tiled']:.0f}, Bunch-Kaufman {res['refactored_bunch_kaufman']:.0f}), "
         f"refinement iterations {iters}, {res['particle_rk2_steps_per_s']:.3e} particle-RK2-steps/s "
@@ -399,11 +399,11 @@ def setup_phases(ctx):
         "total": round(ctx.info("setup_ms"), 2)}
 
 
-def setup_with_fp32_factor(L, opts, X, V, P):
-    """Set-up of the mixed path with lagrbf_mixed_factor fp32 (FP32 factors refined in
-    FP64) on the same nodes and particles: patch set-up phases in ms (pool growth,
-    factor, solve) and wall seconds."""
-    ctx = L.LagRBFCtx(petsc_options=dict(opts, lagrbf_mixed_factor="fp32"), precision="mixed")
+def setup_with_fp64_factor(L, opts, X, V, P):
+    """Set-up of the mixed path with lagrbf_mixed_factor fp64 (FP64 factors, no
+    refinement, FP32 evaluation) on the same nodes and particles: patch set-up phases
+    in ms (pool growth, factor, solve) and wall seconds."""
+    ctx = L.LagRBFCtx(petsc_options=dict(opts, lagrbf_mixed_factor="fp64"), precision="mixed")
     ctx.fill_data(X, V)
     ctx.fill_points(P)
     ts = time.perf_counter()
@@ -452,7 +452,7 @@ def c4_bench(L, precision, t_end, m=126, N=8000000, Np=16777216, pu=38):
            "evals": int(n1 - n0)}
     ctx.destroy()
     if precision == "mixed":
-        res["with_fp32_factor"] = setup_with_fp32_factor(L, opts, X, V, P)
+        res["with_fp64_factor"] = setup_with_fp64_factor(L, opts, X, V, P)
     log(f"c4 [{precision}]: set-up {t_setup:.2f}s, {c['attempts']:.0f} attempts, {c['accepted']:.3e} accepted / "
         f"{c['rejected']:.3e} rejected steps in {wall:.2f}s = {res['particle_steps_per_s']:.3e} particle-steps/s, "
         f"eval {res['eval_ms']:.2f} ms, removed {res['removed']:.0f}")

@@ -160,10 +160,9 @@ struct Ctx {
   DBuf<double> solve_ws;                   // solve workspace of the large patches (2 nf per member)
   DBuf<int32_t> pstatus, piters;
   int precision = 0;                       // 0 FP64, 1 FP32 factor + FP64 refinement
-  // mixed: factor in FP64 (no refinement) with the FP32 evaluation, the default; false:
-  // FP32 factor and FP64 refinement (the solves, not the factor, bound the set-up at
-  // the patch sizes measured, and each refinement pass costs about one more solve)
-  bool mixed_factor64 = true;
+  // mixed: FP32 factor and FP64 refinement (the default); true: factor in FP64 (no
+  // refinement) with the FP32 evaluation kept
+  bool mixed_factor64 = false;
   int factor_mode = 0;                     // 0 auto (Cholesky for PD kernels), 1 Bunch-Kaufman always
   bool factors_released = false;           // serving datasets keep coefficients only
   int chol_big_min = 512;                  // patches above this size use the large-patch Cholesky kernels

@@ -14,7 +14,10 @@
 // refined against the FP64 matrix recomputed from the member coordinates
 // (iterative refinement, PartitionSolve of partition_factor.c:92-110 solves once
 // in FP64; the refinement converges to that solution): the residual is computed
-// straight into y, the refined solution kept in LDS beside the coordinates.
+// in FP64 and rounded into y, and the correction solves run in FP32 arithmetic,
+// as mixed-precision iterative refinement has them (the FP32 factor's rounding,
+// not the solve's, bounds each correction's accuracy); the refined FP64 solution
+// stays in LDS beside the FP64 coordinates.
 #include "kernels.hpp"
 
 #include <algorithm>
@@ -30,31 +33,32 @@ __device__ __forceinline__ double llt_centre(const Grid& g, int p, int d) {
 }
 
 // lb[r * 17 + c] = (L_JJ^-1)[r][c] for the w <= 16 columns at J (zero elsewhere).
-template <typename TL>
-__device__ __forceinline__ void llt_stage_inverse(const TL* __restrict__ L, int n, int J, int w, double* lb) {
+template <typename TL, typename AT>
+__device__ __forceinline__ void llt_stage_inverse(const TL* __restrict__ L, int n, int J, int w, AT* lb) {
   const int t = threadIdx.x;
   const int r = t >> 4, c = t & 15;
-  double v = 0.0;
+  AT v = AT(0);
   if (r < w && c <= r) {
-    const double e = (double)L[(size_t)(J + r) * n + J + c];
-    v = r == c ? 1.0 / e : e;
+    const AT e = (AT)L[(size_t)(J + r) * n + J + c];
+    v = r == c ? AT(1) / e : e;
   }
   lb[r * 17 + c] = v;
 }
 
-// y <- L^-1 y, then y <- L^-T y, for the NF columns y[f * S + i], i < n.
-template <typename TL, int NF>
-__device__ __forceinline__ void llt_solve_lean(const TL* __restrict__ L, int n, double* y, int S, double* zb, double* lb) {
+// y <- L^-1 y, then y <- L^-T y, for the NF columns y[f * S + i], i < n, in the
+// arithmetic of AT.
+template <typename TL, typename AT, int NF>
+__device__ __forceinline__ void llt_solve_lean(const TL* __restrict__ L, int n, AT* y, int S, AT* zb, AT* lb) {
   const int tid = threadIdx.x;
   for (int J = 0; J < n; J += 16) {
     const int w = min(16, n - J);
-    llt_stage_inverse<TL>(L, n, J, w, lb);
+    llt_stage_inverse<TL, AT>(L, n, J, w, lb);
     __syncthreads();
     if (tid < 16 * NF) {
       const int r = tid & 15, f = tid >> 4;
-      double z = 0.0;
+      AT z = AT(0);
 #pragma unroll
-      for (int c = 0; c < 16; c++) z += lb[r * 17 + c] * (c < w ? y[f * S + J + c] : 0.0);
+      for (int c = 0; c < 16; c++) z += lb[r * 17 + c] * (c < w ? y[f * S + J + c] : AT(0));
       zb[r * NF + f] = z;
     }
     __syncthreads();
@@ -77,12 +81,12 @@ __device__ __forceinline__ void llt_solve_lean(const TL* __restrict__ L, int n, 
         }
 #pragma unroll 1
         for (int f = 0; f < NF; f++) {
-          double a0 = 0.0, a1 = 0.0;
+          AT a0 = AT(0), a1 = AT(0);
 #pragma unroll
           for (int c = 0; c < 16; c++) {
-            const double z = zb[c * NF + f];
-            a0 += (double)l0[c] * z;
-            a1 += (double)l1[c] * z;
+            const AT z = zb[c * NF + f];
+            a0 += (AT)l0[c] * z;
+            a1 += (AT)l1[c] * z;
           }
           y[f * S + i0] -= a0;
           if (h1) y[f * S + i1] -= a1;
@@ -96,40 +100,40 @@ __device__ __forceinline__ void llt_solve_lean(const TL* __restrict__ L, int n, 
   const int c = tid >> 4, gl = tid & 15;
   for (int J = ((n - 1) / 16) * 16; J >= 0; J -= 16) {
     const int w = min(16, n - J);
-    llt_stage_inverse<TL>(L, n, J, w, lb);
-    double acc[NF];
+    llt_stage_inverse<TL, AT>(L, n, J, w, lb);
+    AT acc[NF];
 #pragma unroll
-    for (int f = 0; f < NF; f++) acc[f] = 0.0;
+    for (int f = 0; f < NF; f++) acc[f] = AT(0);
     if (c < w) {
       const TL* Lc = L + (size_t)(J + c) * n;
       int i = J + 16 + gl;
 #pragma unroll 1
       for (; i + 48 < n; i += 64) {
-        const double l0 = (double)Lc[i], l1 = (double)Lc[i + 16], l2 = (double)Lc[i + 32], l3 = (double)Lc[i + 48];
+        const AT l0 = (AT)Lc[i], l1 = (AT)Lc[i + 16], l2 = (AT)Lc[i + 32], l3 = (AT)Lc[i + 48];
 #pragma unroll
         for (int f = 0; f < NF; f++) {
-          const double* yf = y + f * S + i;
+          const AT* yf = y + f * S + i;
           acc[f] += l0 * yf[0] + l1 * yf[16] + l2 * yf[32] + l3 * yf[48];
         }
       }
 #pragma unroll 1
       for (; i < n; i += 16) {
-        const double l0 = (double)Lc[i];
+        const AT l0 = (AT)Lc[i];
 #pragma unroll
         for (int f = 0; f < NF; f++) acc[f] += l0 * y[f * S + i];
       }
     }
 #pragma unroll
     for (int f = 0; f < NF; f++) {
-      double a = acc[f];
+      AT a = acc[f];
 #pragma unroll
       for (int o = 8; o > 0; o >>= 1) a += __shfl_xor(a, o);
-      if (gl == 0) zb[c * NF + f] = c < w ? y[f * S + J + c] - a : 0.0;
+      if (gl == 0) zb[c * NF + f] = c < w ? y[f * S + J + c] - a : AT(0);
     }
     __syncthreads();
     if (tid < 16 * NF) {
       const int r = tid & 15, f = tid >> 4;
-      double x = 0.0;
+      AT x = AT(0);
 #pragma unroll
       for (int c2 = 0; c2 < 16; c2++) x += lb[c2 * 17 + r] * zb[c2 * NF + f];   // (L_JJ^-T)[r][c2]
       if (r < w) y[f * S + J + r] = x;
@@ -147,11 +151,12 @@ __global__ __launch_bounds__(LNT) void k_solve_llt(Grid g, const int32_t* __rest
                                                    int32_t* __restrict__ status, int32_t* __restrict__ iters, int S,
                                                    const double* __restrict__ peps) {
   constexpr bool REF = sizeof(TL) == sizeof(float);
+  using AT = TL;   // arithmetic of the triangular solves: FP64, or FP32 for the corrections
   extern __shared__ __align__(16) double lsm[];
-  double* y = lsm;                                  // [NF][S]
-  double* xl = y + (size_t)NF * S;                  // REF: [NF][S] refined solution
-  double* xr = xl + (size_t)NF * S;                 // REF: [3][S] member coordinates
-  __shared__ double zb[16 * NF], lb[16 * 17];
+  double* xl = lsm;                                 // REF: [NF][S] refined solution
+  double* xr = xl + (REF ? (size_t)NF * S : 0);     // REF: [3][S] member coordinates
+  AT* y = reinterpret_cast<AT*>(xr + (REF ? (size_t)3 * S : 0));   // [NF][S]
+  __shared__ AT zb[16 * NF], lb[16 * 17];
   __shared__ double red[LNT / 64][2];
   const int p = list[blockIdx.x];
   const int n = pn[p];
@@ -167,7 +172,7 @@ __global__ __launch_bounds__(LNT) void k_solve_llt(Grid g, const int32_t* __rest
   for (int i = tid; i < n; i += LNT) {
     const int node = mem[i];
 #pragma unroll
-    for (int f = 0; f < NF; f++) y[f * S + i] = f < nf ? values[(size_t)node * nf + f] : 0.0;
+    for (int f = 0; f < NF; f++) y[f * S + i] = (AT)(f < nf ? values[(size_t)node * nf + f] : 0.0);
     if constexpr (REF) {
 #pragma unroll
       for (int f = 0; f < NF; f++) xl[f * S + i] = 0.0;
@@ -177,7 +182,7 @@ __global__ __launch_bounds__(LNT) void k_solve_llt(Grid g, const int32_t* __rest
   }
   __syncthreads();
   if constexpr (!REF) {
-    llt_solve_lean<TL, NF>(L, n, y, S, zb, lb);
+    llt_solve_lean<TL, AT, NF>(L, n, y, S, zb, lb);
     for (int i = tid; i < n; i += LNT)
 #pragma unroll
       for (int f = 0; f < NF; f++)
@@ -205,18 +210,18 @@ __global__ __launch_bounds__(LNT) void k_solve_llt(Grid g, const int32_t* __rest
             for (int f = 0; f < NF; f++) s[f] -= a * xl[f * S + j];
           }
 #pragma unroll
-          for (int f = 0; f < NF; f++) y[f * S + i] = s[f];
+          for (int f = 0; f < NF; f++) y[f * S + i] = (AT)s[f];
         }
         __syncthreads();
       }
-      llt_solve_lean<TL, NF>(L, n, y, S, zb, lb);
+      llt_solve_lean<TL, AT, NF>(L, n, y, S, zb, lb);
       // x += d; converged when max |d| <= tol max |x|, or when the correction has
       // stopped shrinking (FP64 rounding floor of the residual) at or below 100 tol |x|
       double dmax = 0.0, xmax = 0.0;
       for (int i = tid; i < n; i += LNT)
 #pragma unroll
         for (int f = 0; f < NF; f++) {
-          const double d = y[f * S + i];
+          const double d = (double)y[f * S + i];
           const double xn = xl[f * S + i] + d;
           xl[f * S + i] = xn;
           dmax = fmax(dmax, fabs(d));
@@ -241,7 +246,9 @@ __global__ __launch_bounds__(LNT) void k_solve_llt(Grid g, const int32_t* __rest
 }
 
 static size_t llt_smem(bool ref, int nf_pad, int S) {
-  return sizeof(double) * (size_t)S * (ref ? 2 * (size_t)nf_pad + 3 : (size_t)nf_pad);
+  // REF: FP64 solution and coordinates, FP32 right-hand side; else the FP64 one
+  return ref ? (size_t)S * (sizeof(double) * (nf_pad + 3) + sizeof(float) * nf_pad)
+             : sizeof(double) * (size_t)S * nf_pad;
 }
 
 static int llt_nf_pad(int nf) { return nf <= 4 ? nf : (nf <= 6 ? 6 : 8); }

@@ -132,16 +132,15 @@ def test_patch_coefficients_vs_oracle(precision, factor, tol):
         assert err < 1e-5      # FP32 evaluation (values), stated tolerance
 
 
-@pytest.mark.parametrize("factor", ["fp64", None])
-def test_mixed_with_fp64_factor(factor):
-    """lagrbf_mixed_factor fp64, the default: the mixed path factors in FP64 (no
-    refinement) and keeps the FP32 evaluation: coefficients within 1e-12 of the
-    oracle, values within the FP32 evaluation's 1e-5."""
+def test_mixed_with_fp64_factor():
+    """lagrbf_mixed_factor fp64: the mixed path factors in FP64 (no refinement) and
+    keeps the FP32 evaluation: coefficients within 1e-12 of the oracle, values within
+    the FP32 evaluation's 1e-5."""
     opts, X, V, P = cases.fourier_3d(nlat=24, Np=4000, n=6)
     print("case: mixed precision with FP64 factors", flush=True)
     with pytest.raises(RuntimeError, match="lagrbf_mixed_factor"):
         L.LagRBFCtx(petsc_options=dict(opts, lagrbf_mixed_factor="fp16"), precision="mixed")
-    opts = dict(opts, lagrbf_precision="mixed", **({"lagrbf_mixed_factor": factor} if factor else {}))
+    opts = dict(opts, lagrbf_precision="mixed", lagrbf_mixed_factor="fp64")
     g, o = pair(opts)
     for c in (g, o):
         c.fill_data(X, V); c.fill_points(P); c.interpolate_c()
