@@ -466,7 +466,12 @@ static int upload_list(Ctx& c, const std::vector<int32_t>& l, int seg) {
 // polynomial block and size bucket of the factor dimension (<= 128, <= 256, <=
 // 512).  The first factorisation in a bucket times every candidate on a sample of
 // its patches (up to 256; factoring is idempotent, the sample is factored again
-// with the winner); the choice is kept for the process.
+// with the winner); the choice is kept for the process.  Where the lean solves
+// apply (solve_llt.hip, aug_lean.hip) only the matrix-core factor is a candidate:
+// those solves need its inverse diagonal blocks, and the column-oriented solve of
+// the other factor costs more than any factor time it saves (at the clustered
+// workload's 90,780 FP32 saddle-point patches of up to 96 members: 118 ms of
+// solves against 5 ms).
 struct FactorTuneKey {
   int ktype, mmax, fp32, bucket;
   bool operator<(const FactorTuneKey& o) const {
@@ -483,6 +488,7 @@ static int factor_bucket(int nn) { return nn <= 128 ? 0 : (nn <= 256 ? 1 : 2); }
 static int factor_nb_for(Ctx& c, bool fp32, int bucket, const int32_t* dlist, int nsample, void* fac,
                          const AugView& a) {
   if (c.factor_nb >= 0) return c.factor_nb;
+  if (c.lean_solve && (!a.m || solve_aug_lean_supported(c.ktype, fp32))) return 16;
   const FactorTuneKey key{c.ktype, a.m ? a.mmax : 0, fp32 ? 1 : 0, bucket};
   {
     std::lock_guard<std::mutex> lk(g_ftune_mu);
@@ -739,8 +745,8 @@ static int solve_list(Ctx& c, const std::vector<int32_t>& sl) {
   for (int p : sl) l[c.h_method[p]].push_back(p);
   // one-workgroup solves size their LDS rows by the launch's largest patch: the
   // patches go by size, in bands launched apart, so that the many small patches
-  // are not held to the occupancy of the few large ones.  Cholesky patches with a
-  // matrix-core factor (inverse diagonal blocks) take the lean solve.
+  // are not held to the occupancy of the few large ones.  Patches with a
+  // matrix-core factor (inverse diagonal blocks) take the lean solves.
   static constexpr int SOLVE_BANDS[] = {96, 128, 192, 256, 384, 1 << 30};
   struct Band { int b0, b1; bool lean; };
   std::vector<Band> bands[N_METHODS];
@@ -748,7 +754,10 @@ static int solve_list(Ctx& c, const std::vector<int32_t>& sl) {
     if (l[m].empty()) continue;
     if (method_big(m)) { bands[m].push_back({0, (int)l[m].size(), false}); continue; }
     auto lean = [&](int p) {
-      return c.lean_solve && !method_aug(m) && c.h_fnb[p] == 16 && c.h_n[p] <= solve_llt_max_n();
+      if (!c.lean_solve || c.h_fnb[p] != 16) return false;
+      if (method_aug(m))
+        return solve_aug_lean_supported(c.ktype, method_fp32(m)) && c.h_n[p] <= solve_aug_lean_max_n();
+      return c.h_n[p] <= solve_llt_max_n();
     };
     std::sort(l[m].begin(), l[m].end(), [&](int a, int b) {
       return std::make_pair(!lean(a), c.h_n[a]) < std::make_pair(!lean(b), c.h_n[b]);
@@ -785,7 +794,11 @@ static int solve_list(Ctx& c, const std::vector<int32_t>& sl) {
         int nmax = 1;
         for (int i = b0; i < b1; i++) nmax = std::max(nmax, (int)c.h_n[l[m][i]]);
         const int32_t* dl = c.list.p + (size_t)m * P + b0;
-        if (bd.lean)
+        if (bd.lean && method_aug(m))
+          launch_solve_aug_lean(c.g, c.ktype, method_fp32(m), dl, b1 - b0, c.p_n.p, c.p_mem_off.p, c.p_fac_off.p,
+                                c.mem_pool.p, fac, c.dval.p, c.nf, c.coef_pool.p, c.refine_max_iter, c.refine_tol,
+                                c.pstatus.p, c.piters.p, nmax, ap, c.s);
+        else if (bd.lean)
           launch_solve_llt(c.g, c.ktype, method_fp32(m), dl, b1 - b0, c.p_n.p, c.p_mem_off.p, c.p_fac_off.p,
                            c.mem_pool.p, c.dx.p, fac, c.dval.p, c.nf, c.coef_pool.p, c.refine_max_iter, c.refine_tol,
                            c.pstatus.p, c.piters.p, nmax, ae.eps, c.s);

@@ -111,6 +111,15 @@ void launch_solve_llt(const Grid& g, int ktype, bool fp32, const int32_t* list, 
                       const void* fac, const double* values, int nf, double* coef, int max_iter, double tol,
                       int32_t* status, int32_t* iters, int nmax, const double* peps, hipStream_t s);
 int solve_llt_max_n();
+// (aug_lean.hip) the saddle-point solve of k_solve_aug for patches of up to
+// solve_aug_lean_max_n() members with a matrix-core B22 factor; FP32 factors (with
+// refinement) for the kernels solve_aug_lean_supported() names
+void launch_solve_aug_lean(const Grid& g, int ktype, bool fp32, const int32_t* list, int nlist, const int32_t* pn,
+                           const int64_t* mem_off, const int64_t* fac_off, const int32_t* mem_pool, const void* fac,
+                           const double* values, int nf, double* coef, int max_iter, double tol, int32_t* status,
+                           int32_t* iters, int nmax, const AugView& a, hipStream_t s);
+int solve_aug_lean_max_n();
+bool solve_aug_lean_supported(int ktype, bool fp32);
 // (chol_mfma.hip) the same factor with the panel products on the matrix cores,
 // panels of 16 columns, factor dimension up to nmax <= chol_mfma_max_n()
 int chol_mfma_max_n();
