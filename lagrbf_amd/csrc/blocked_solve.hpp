@@ -94,8 +94,10 @@ __device__ void llt_solve_blocked(const TL* __restrict__ L, int n, double* y, in
     __syncthreads();
   }
   // backward: x_J = L_JJ^-T (z_J - L[below, J block]^T x_below).  The dots: thread
-  // (c, g) sums column J + c over the rows J + 16 + g + 16 k, then the 16 lanes of
-  // one c reduce.
+  // (c, g) sums column J + c over the rows J + 16 + g + 16 k, eight rows per
+  // iteration with their loads issued together (a patch of thousands of members
+  // otherwise waits on one load round trip per two rows), then the 16 lanes of one
+  // c reduce.
   const int nblk = (n + NB - 1) / NB;
   const int c = tid / NB, g = tid % NB;
   for (int bi = nblk - 1; bi >= 0; bi--) {
@@ -109,6 +111,20 @@ __device__ void llt_solve_blocked(const TL* __restrict__ L, int n, double* y, in
       for (int f = 0; f < 8; f++) acc[f] = 0.0;
       if (c < w) {
         int i = J + w + g;
+        for (; i + 7 * NB < n; i += 8 * NB) {
+          double l[8];
+#pragma unroll
+          for (int u = 0; u < 8; u++) l[u] = (double)Lc[i + u * NB];
+#pragma unroll
+          for (int f = 0; f < 8; f++) {
+            if (f >= nf) break;
+            const double* yf = y + (size_t)f * ldy + i;
+            double s = 0.0;
+#pragma unroll
+            for (int u = 0; u < 8; u++) s += l[u] * yf[u * NB];
+            acc[f] += s;
+          }
+        }
         for (; i + NB < n; i += 2 * NB) {
           const double l0 = (double)Lc[i], l1 = (double)Lc[i + NB];
 #pragma unroll
