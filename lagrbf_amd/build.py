@@ -8,7 +8,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "liblagrbf.so")
 SOURCES = ["eval_blk_wc2.hip", "eval_blk_phs3.hip", "eval_blk_gauss.hip", "eval_blk_other.hip",
-           "context.hip", "kernels.hip", "eval_cell.hip", "eval_blk.hip", "serve.hip", "patch_chol.hip", "adaptive.hip", "aug.hip", "chol_mfma.hip", "chol_tiled.hip", "solve_llt.hip", "aug_lean.hip", "noalloc.hip", "devmem.cpp", "options.cpp", "io.cpp"]
+           "context.hip", "kernels.hip", "eval_cell.hip", "eval_blk.hip", "serve.hip", "patch_chol.hip", "adaptive.hip", "aug.hip", "chol_mfma.hip", "chol_tiled.hip", "solve_llt.hip", "aug_lean.hip", "trsv_tiles.hip", "noalloc.hip", "devmem.cpp", "options.cpp", "io.cpp"]
 ARCH = os.environ.get("LAGRBF_OFFLOAD_ARCH", "gfx950")
 
 

@@ -379,7 +379,8 @@ __global__ __launch_bounds__(ANT, 3) void k_solve_aug(Grid g, const int32_t* __r
                                                    double* __restrict__ coef, int max_iter, double tol,
                                                    int32_t* __restrict__ status, int32_t* __restrict__ iters,
                                                    double* __restrict__ ws, int S, AugView a, int step_it,
-                                                   double* __restrict__ dstate) {
+                                                   double* __restrict__ dstate, int phase,
+                                                   double* __restrict__ hstate) {
   extern __shared__ __align__(16) double sm[];
   constexpr int MF = MM * AMAXF;
   __shared__ double red[(ANT / AW) * MF];
@@ -407,7 +408,7 @@ __global__ __launch_bounds__(ANT, 3) void k_solve_aug(Grid g, const int32_t* __r
   // whole solve in this launch.
   const bool stepping = step_it >= 0;
   if (stepping && step_it > 0 && status[p] != 2) return;
-  const bool lds = n <= S && !stepping;
+  const bool lds = n <= S && !stepping && phase == 0;   // a two-phase solve keeps its vectors in ws
   if (!lds && !ws) {   // not sized for this patch: reported as not converged (refactored in FP64)
     if (tid == 0) {
       if (status) status[p] = 1;
@@ -441,7 +442,16 @@ __global__ __launch_bounds__(ANT, 3) void k_solve_aug(Grid g, const int32_t* __r
   int it = stepping ? step_it : 0, converged = 0;
   const int it_end = stepping ? step_it : (refine ? max_iter : 0);
   double dprev = (stepping && step_it > 0) ? dstate[p] : 1e300;
+  // phase 1: up to h and u1, which go to hstate (at the block's list position) with
+  // the B22 right-hand side left in the workspace for the multi-workgroup triangular
+  // solves (trsv_tiles.hip); phase 2: from u2 = that solution on.  Phase 0: all.
   for (; it <= it_end; it++) {
+    if (phase == 2) {
+      const double* hs = hstate + (size_t)blockIdx.x * 2 * MF;
+      for (int e = tid; e < MF; e += ANT) { su1[e] = hs[e]; sh1[e] = hs[MF + e]; }
+      __syncthreads();
+      goto after_b22;
+    }
     // residual of the saddle-point system: r = f - A c - P beta, g = -P^T c  (g in su1)
     {
       // (iterations >= 1 of a stepped solve: r is already in the workspace)
@@ -519,9 +529,17 @@ __global__ __launch_bounds__(ANT, 3) void k_solve_aug(Grid g, const int32_t* __r
     }
     __syncthreads();
     // u2 = B22^-1 (h2 - B21 u1)
-    const int nbp = a.fnb ? a.fnb[p] : 0;
-    if (nbp == SOLVE_NB) llt_solve_blocked<TL, ANT>(L, nn, y + m, ldv, nf, zb, lb);
-    else llt_solve<TL>(L, nn, y + m, ldv, nf, red, zk);
+    if (phase == 1) {
+      double* hs = hstate + (size_t)blockIdx.x * 2 * MF;
+      for (int e = tid; e < MF; e += ANT) { hs[e] = su1[e]; hs[MF + e] = sh1[e]; }
+      return;
+    }
+    {
+      const int nbp = a.fnb ? a.fnb[p] : 0;
+      if (nbp == SOLVE_NB) llt_solve_blocked<TL, ANT>(L, nn, y + m, ldv, nf, zb, lb);
+      else llt_solve<TL>(L, nn, y + m, ldv, nf, red, zk);
+    }
+  after_b22:
     // beta correction: R^-1 (h1 - B11 u1 - B12 u2)
     col_dots(B1, n, m, m, y, ldv, nf, red, sv);
     if (tid < nf) {
@@ -729,7 +747,7 @@ static void solve_aug_k(const Grid& g, const int32_t* list, int nlist, const int
                         const int64_t* fac_off, const int32_t* mem_pool, const double* xd, const TL* fac,
                         const double* values, int nf, double* coef, int max_iter, double tol, int32_t* status,
                         int32_t* iters, double* ws, int nmax, const AugView& a, int step_it, double* dstate,
-                        hipStream_t s) {
+                        int phase, double* hstate, hipStream_t s) {
   static bool attr = false;
   if (!attr) {
     (void)hipFuncSetAttribute((const void*)k_solve_aug<TL, KT, MM>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -739,7 +757,7 @@ static void solve_aug_k(const Grid& g, const int32_t* list, int nlist, const int
   const int S = std::max(1, std::min(nmax, AUG_LDS_MAX));
   hipLaunchKernelGGL((k_solve_aug<TL, KT, MM>), dim3(nlist), dim3(ANT), stepping_smem(S, nf, step_it), s, g, list, pn,
                      mem_off, fac_off, mem_pool, xd, fac, values, nf, coef, max_iter, tol, status, iters, ws, S, a,
-                     step_it, dstate);
+                     step_it, dstate, phase, hstate);
 }
 
 template <typename TL, int MM>
@@ -747,8 +765,8 @@ static void solve_aug_mm(const Grid& g, int ktype, const int32_t* list, int nlis
                          const int64_t* mem_off, const int64_t* fac_off, const int32_t* mem_pool, const double* xd,
                          const TL* fac, const double* values, int nf, double* coef, int max_iter, double tol,
                          int32_t* status, int32_t* iters, double* ws, int nmax, const AugView& a, int step_it,
-                         double* dstate, hipStream_t s) {
-#define LRB_SA(K) solve_aug_k<TL, K, MM>(g, list, nlist, pn, mem_off, fac_off, mem_pool, xd, fac, values, nf, coef, max_iter, tol, status, iters, ws, nmax, a, step_it, dstate, s)
+                         double* dstate, int phase, double* hstate, hipStream_t s) {
+#define LRB_SA(K) solve_aug_k<TL, K, MM>(g, list, nlist, pn, mem_off, fac_off, mem_pool, xd, fac, values, nf, coef, max_iter, tol, status, iters, ws, nmax, a, step_it, dstate, phase, hstate, s)
   switch (ktype) {
     case K_WC0: LRB_SA(K_WC0); break;
     case K_WC2: LRB_SA(K_WC2); break;
@@ -765,21 +783,21 @@ void launch_solve_aug(const Grid& g, int ktype, bool fp32, const int32_t* list, 
                       int32_t* status, int32_t* iters, double* ws, int nmax, const AugView& a, hipStream_t s) {
   if (nlist <= 0 || nf <= 0) return;
   launch_solve_aug_step(g, ktype, fp32, list, nlist, pn, mem_off, fac_off, mem_pool, xd, fac, values, nf, coef, max_iter,
-                        tol, status, iters, ws, nmax, a, -1, nullptr, s);
+                        tol, status, iters, ws, nmax, a, -1, nullptr, 0, nullptr, s);
 }
 
 void launch_solve_aug_step(const Grid& g, int ktype, bool fp32, const int32_t* list, int nlist, const int32_t* pn,
                            const int64_t* mem_off, const int64_t* fac_off, const int32_t* mem_pool, const double* xd,
                            const void* fac, const double* values, int nf, double* coef, int max_iter, double tol,
                            int32_t* status, int32_t* iters, double* ws, int nmax, const AugView& a, int step_it,
-                           double* dstate, hipStream_t s) {
+                           double* dstate, int phase, double* hstate, hipStream_t s) {
   if (nlist <= 0 || nf <= 0) return;
   if (fp32) {
-    if (a.mmax <= 4) solve_aug_mm<float, 4>(g, ktype, list, nlist, pn, mem_off, fac_off, mem_pool, xd, (const float*)fac, values, nf, coef, max_iter, tol, status, iters, ws, nmax, a, step_it, dstate, s);
-    else solve_aug_mm<float, POLY_MAX>(g, ktype, list, nlist, pn, mem_off, fac_off, mem_pool, xd, (const float*)fac, values, nf, coef, max_iter, tol, status, iters, ws, nmax, a, step_it, dstate, s);
+    if (a.mmax <= 4) solve_aug_mm<float, 4>(g, ktype, list, nlist, pn, mem_off, fac_off, mem_pool, xd, (const float*)fac, values, nf, coef, max_iter, tol, status, iters, ws, nmax, a, step_it, dstate, phase, hstate, s);
+    else solve_aug_mm<float, POLY_MAX>(g, ktype, list, nlist, pn, mem_off, fac_off, mem_pool, xd, (const float*)fac, values, nf, coef, max_iter, tol, status, iters, ws, nmax, a, step_it, dstate, phase, hstate, s);
   } else {
-    if (a.mmax <= 4) solve_aug_mm<double, 4>(g, ktype, list, nlist, pn, mem_off, fac_off, mem_pool, xd, (const double*)fac, values, nf, coef, max_iter, tol, status, iters, ws, nmax, a, step_it, dstate, s);
-    else solve_aug_mm<double, POLY_MAX>(g, ktype, list, nlist, pn, mem_off, fac_off, mem_pool, xd, (const double*)fac, values, nf, coef, max_iter, tol, status, iters, ws, nmax, a, step_it, dstate, s);
+    if (a.mmax <= 4) solve_aug_mm<double, 4>(g, ktype, list, nlist, pn, mem_off, fac_off, mem_pool, xd, (const double*)fac, values, nf, coef, max_iter, tol, status, iters, ws, nmax, a, step_it, dstate, phase, hstate, s);
+    else solve_aug_mm<double, POLY_MAX>(g, ktype, list, nlist, pn, mem_off, fac_off, mem_pool, xd, (const double*)fac, values, nf, coef, max_iter, tol, status, iters, ws, nmax, a, step_it, dstate, phase, hstate, s);
   }
 }
 

@@ -171,6 +171,8 @@ static int ctx_from_options(Ctx& c, const OptView& o) {
   if (bad) return set_err(E_WRONG, fn, "malformed -lagrbf_stepped_refine option");
   o.get_bool("lagrbf_lean_solve", c.lean_solve, bad);
   if (bad) return set_err(E_WRONG, fn, "malformed -lagrbf_lean_solve option");
+  o.get_bool("lagrbf_tiled_solve", c.tiled_solve, bad);
+  if (bad) return set_err(E_WRONG, fn, "malformed -lagrbf_tiled_solve option");
   std::string bf;
   if (o.get_string("lagrbf_big_factor", bf)) {
     if (bf == "tiled") c.big_factor = 1;
@@ -702,6 +704,44 @@ static int factor_lists(Ctx& c, std::vector<int32_t>* fl) {
   return refactor_fp64(c, redo, false);
 }
 
+// Task lists of the multi-workgroup B22 solves for the large saddle-point patches l
+// (trsv_tiles.hip), uploaded to c.trsv_dtasks; c.aug_hstate sized for n_list list
+// positions of the two-phase solve.
+static int trsv_prepare(Ctx& c, const std::vector<int32_t>& l, size_t n_list, std::vector<int>& off) {
+  std::vector<int> nn(l.size());
+  for (size_t q = 0; q < l.size(); q++) nn[q] = c.h_n[l[q]] - c.h_m[l[q]];
+  std::vector<int2> tasks;
+  trsv_tasks(l, nn, tasks, off);
+  HIPCHK(hipStreamSynchronize(c.s));   // an earlier use of the task buffer
+  HIPCHK(c.trsv_dtasks.reserve(tasks.size() + 1));
+  if (!tasks.empty())
+    HIPCHK(hipMemcpyAsync(c.trsv_dtasks.p, tasks.data(), sizeof(int2) * tasks.size(), hipMemcpyHostToDevice, c.s));
+  HIPCHK(c.aug_hstate.reserve(n_list * 2 * (size_t)POLY_MAX * 8 + 1));
+  HIPCHK(hipStreamSynchronize(c.s));   // tasks is a host temporary
+  return 0;
+}
+
+// One solve (or refinement step step_it >= 0) of the large saddle-point patches l of
+// method m listed at dl: k_solve_aug up to the B22 right-hand side, the B22 solves
+// over many workgroups for the patches run (those still refining), the rest.
+static int solve_aug_big_tiled(Ctx& c, int m, const std::vector<int32_t>& l, const std::vector<int32_t>& run,
+                               const int32_t* dl, int nmax, int step_it, double* dstate) {
+  const AugView ap = aug_view(c, true);
+  const bool f32 = method_fp32(m);
+  const void* fac = f32 ? (const void*)c.fac32_pool.p : (const void*)c.fac_pool.p;
+  std::vector<int> off;
+  CHK(trsv_prepare(c, run, l.size(), off));
+  for (int phase = 1; phase <= 2; phase++) {
+    launch_solve_aug_step(c.g, c.ktype, f32, dl, (int)l.size(), c.p_n.p, c.p_mem_off.p, c.p_fac_off.p, c.mem_pool.p,
+                          c.dx.p, fac, c.dval.p, c.nf, c.coef_pool.p, c.refine_max_iter, c.refine_tol, c.pstatus.p,
+                          c.piters.p, c.solve_ws.p, nmax, ap, step_it, dstate, phase, c.aug_hstate.p, c.s);
+    if (phase == 1)
+      launch_trsv_tiles(f32, c.trsv_dtasks.p, off, c.p_n.p, ap, c.p_mem_off.p, c.p_fac_off.p, fac, c.solve_ws.p, c.nf,
+                        c.s);
+  }
+  return 0;
+}
+
 // FP32 factors of large saddle-point patches: one refinement iteration per launch,
 // the O(n^2) matrix-free residual spread over many workgroups (launch_aug_resid_big)
 // instead of one workgroup per patch, which took most of the set-up of patches with
@@ -719,18 +759,23 @@ static int solve_aug_stepped(Ctx& c, const std::vector<int32_t>& l, int nmax) {
   HIPCHK(c.resid_tasks.reserve(tasks.size()));
   HIPCHK(hipMemcpyAsync(c.resid_tasks.p, tasks.data(), sizeof(int2) * tasks.size(), hipMemcpyHostToDevice, c.s));
   std::vector<int32_t> st(P);
+  std::vector<int32_t> run(l);   // the patches still refining
   for (int it = 0; it <= c.refine_max_iter; it++) {
     if (it > 0)
       launch_aug_resid_big(c.g, c.ktype, c.resid_tasks.p, (int)tasks.size(), c.p_n.p, c.p_mem_off.p, c.mem_pool.p,
                            c.dval.p, c.nf, c.pstatus.p, c.solve_ws.p, ap, c.s);
-    launch_solve_aug_step(c.g, c.ktype, true, dl, (int)l.size(), c.p_n.p, c.p_mem_off.p, c.p_fac_off.p, c.mem_pool.p,
-                          c.dx.p, c.fac32_pool.p, c.dval.p, c.nf, c.coef_pool.p, c.refine_max_iter, c.refine_tol,
-                          c.pstatus.p, c.piters.p, c.solve_ws.p, nmax, ap, it, c.refine_dstate.p, c.s);
+    if (c.tiled_solve)
+      CHK(solve_aug_big_tiled(c, M_AUG32B, l, run, dl, nmax, it, c.refine_dstate.p));
+    else
+      launch_solve_aug_step(c.g, c.ktype, true, dl, (int)l.size(), c.p_n.p, c.p_mem_off.p, c.p_fac_off.p, c.mem_pool.p,
+                            c.dx.p, c.fac32_pool.p, c.dval.p, c.nf, c.coef_pool.p, c.refine_max_iter, c.refine_tol,
+                            c.pstatus.p, c.piters.p, c.solve_ws.p, nmax, ap, it, c.refine_dstate.p, 0, nullptr, c.s);
     HIPCHK(hipMemcpyAsync(st.data(), c.pstatus.p, sizeof(int32_t) * P, hipMemcpyDeviceToHost, c.s));
     HIPCHK(hipStreamSynchronize(c.s));
-    bool running = false;
-    for (int p : l) running = running || st[p] == 2;
-    if (!running) break;
+    run.clear();
+    for (int p : l)
+      if (st[p] == 2) run.push_back(p);
+    if (run.empty()) break;
   }
   return 0;
 }
@@ -804,6 +849,8 @@ static int solve_list(Ctx& c, const std::vector<int32_t>& sl) {
                            c.pstatus.p, c.piters.p, nmax, ae.eps, c.s);
         else if (m == M_AUG32B && c.big_factor && c.stepped_refine)
           CHK(solve_aug_stepped(c, l[m], nmax));
+        else if (m == M_AUG64B && c.big_factor && c.tiled_solve)
+          CHK(solve_aug_big_tiled(c, m, l[m], l[m], dl, nmax, -1, nullptr));
         else if (method_aug(m))
           launch_solve_aug(c.g, c.ktype, method_fp32(m), dl, b1 - b0, c.p_n.p, c.p_mem_off.p, c.p_fac_off.p,
                            c.mem_pool.p, c.dx.p, fac, c.dval.p, c.nf, c.coef_pool.p, c.refine_max_iter, c.refine_tol,

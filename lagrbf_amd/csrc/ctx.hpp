@@ -175,6 +175,9 @@ struct Ctx {
   DBuf<double> refine_dstate;              // [P] last correction of a stepped refinement
   bool stepped_refine = true;              // FP32 factors of large saddle-point patches: refine step by step
   bool lean_solve = true;                  // small Cholesky patches with matrix-core factors: solve_llt.hip
+  bool tiled_solve = true;                 // large saddle-point patches: B22 solves over many workgroups
+  DBuf<int2> trsv_dtasks;                  // their task lists (trsv_tiles.hip)
+  DBuf<double> aug_hstate;                 // h1 and u1 between the two phases of those solves
   int refine_max_iter = 10;
   double refine_tol = 1e-12;
   double fallback = 0, refine_iters_max = 0, refine_iters_sum = 0, refined_patches = 0;

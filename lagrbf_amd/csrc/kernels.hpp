@@ -162,7 +162,16 @@ void launch_solve_aug_step(const Grid& g, int ktype, bool fp32, const int32_t* l
                            const int64_t* mem_off, const int64_t* fac_off, const int32_t* mem_pool, const double* xd,
                            const void* fac, const double* values, int nf, double* coef, int max_iter, double tol,
                            int32_t* status, int32_t* iters, double* ws, int nmax, const AugView& a, int step_it,
-                           double* dstate, hipStream_t s);
+                           double* dstate, int phase, double* hstate, hipStream_t s);
+// (trsv_tiles.hip) multi-workgroup B22 solves of the large saddle-point patches
+// between phase 1 and phase 2 of launch_solve_aug_step: task lists per launch from
+// trsv_tasks (the patches and their factor dimensions), uploaded once
+int trsv_tile();
+void trsv_tasks(const std::vector<int32_t>& l, const std::vector<int>& nn, std::vector<int2>& tasks,
+                std::vector<int>& off);
+void launch_trsv_tiles(bool fp32, const int2* dtasks, const std::vector<int>& off, const int32_t* pn,
+                       const AugView& a, const int64_t* mem_off, const int64_t* fac_off, const void* fac, double* ws,
+                       int nf, hipStream_t s);
 int aug_resid_rows();
 void launch_aug_resid_big(const Grid& g, int ktype, const int2* tasks, int ntasks, const int32_t* pn,
                           const int64_t* mem_off, const int32_t* mem_pool, const double* values, int nf,

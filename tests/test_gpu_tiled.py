@@ -115,3 +115,30 @@ def test_tiled_thousands_of_members(precision):
     assert g.info("fallback") == g.info("big_refactor64")
     if precision == "fp64":
         assert ev < 1e-12
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_tiled_solve_matches_one_workgroup_solve(precision):
+    """The large saddle-point patches' B22 solves over many workgroups (trsv_tiles.hip,
+    tiles of 64, every patch above 64 members here) against the one-workgroup solve
+    of the same factors: coefficients and beta to 1e-12 relative in FP64 (the sums
+    run in another order); the mixed path's refinement reaches the FP64 solution
+    either way (1e-10), with the same refinement-iteration count within one."""
+    opts, X, V, P = clustered_small(precision=precision)
+    opts.update(lagrbf_chol_big_min=64, advection=False)
+    print(f"case: tiled vs one-workgroup B22 solve, {precision}", flush=True)
+    res = {}
+    for tiled in (True, False):
+        g = L.LagRBFCtx(petsc_options=dict(opts, lagrbf_tiled_solve=tiled))
+        g.fill_data(X, V); g.fill_points(P); g.interpolate_c()
+        cs = [g.debug_get("patch_coeffs", p, dtype=np.float64) for p in range(7 ** 3)]
+        bs = [g.debug_get("patch_beta", p, dtype=np.float64) for p in range(7 ** 3)]
+        res[tiled] = (cs, bs, g.info("refine_iters"), g.info("fallback"))
+        g.destroy()
+    ec = max(np.abs(a - b).max() / np.abs(b).max() for a, b in zip(res[True][0], res[False][0]) if b.size)
+    eb = max(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300) for a, b in zip(res[True][1], res[False][1]) if b.size)
+    print(f"{precision}: coefficients {ec:.3e}, beta {eb:.3e}, refine iters {res[True][2]:.0f} / {res[False][2]:.0f}")
+    tol = 1e-12 if precision == "fp64" else 1e-10
+    assert ec < tol and eb < tol
+    assert res[True][3] == res[False][3] == 0
+    assert abs(res[True][2] - res[False][2]) <= 1
