@@ -335,7 +335,8 @@ def serve_bench(L, precision, dsets, nreq, rate, seed=20261203 + 5, budget=200e9
            "p50_ms_first_half": float(np.percentile(lat[:h], 50) * 1e3),
            "p50_ms_second_half": float(np.percentile(lat[h:], 50) * 1e3),
            "offered_particle_steps_per_s": float(offered),
-           "particle_steps_per_s": work / wall, "ticks": srv.info("ticks"), "max_active": srv.info("max_active")}
+           "particle_steps_per_s": work / wall, "ticks": srv.info("ticks"), "max_active": srv.info("max_active"),
+           "host_enqueue_ms": srv.info("enqueue_ms"), "host_tick_ms": srv.info("tick_ms")}
     srv.close()
     return res
 

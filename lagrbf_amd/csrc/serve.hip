@@ -84,6 +84,7 @@ struct Server {
   int64_t next_id = 0;
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
   double completed = 0, particle_steps = 0, ticks = 0, max_active = 0;
+  double enqueue_ms = 0, tick_ms = 0;       // host time: enqueueing the batches' ticks; whole ticks
   double now() const { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); }
 };
 
@@ -407,6 +408,7 @@ int batch_finish(Server& sv, Batch& b) {
 }
 
 int serve_tick(Server& sv) {
+  const auto t0 = std::chrono::steady_clock::now();
   // (1) admission, first come first served, while the HBM budget allows
   while (!sv.queue.empty()) {
     ServeReq* r = sv.queue.front();
@@ -421,6 +423,7 @@ int serve_tick(Server& sv) {
     sv.queue.pop_front();
   }
   // (2) one step of every active batch, each on its own stream
+  const auto t1 = std::chrono::steady_clock::now();
   std::vector<Batch*> act;
   for (auto& kv : sv.batches) {
     Batch& b = *kv.second;
@@ -430,6 +433,7 @@ int serve_tick(Server& sv) {
     CHK(batch_tick(b));
     act.push_back(&b);
   }
+  sv.enqueue_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t1).count();
   // (3) bookkeeping
   double active = 0;
   for (Batch* b : act) {
@@ -438,6 +442,7 @@ int serve_tick(Server& sv) {
   }
   sv.ticks += 1;
   sv.max_active = std::max(sv.max_active, active);
+  sv.tick_ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
   return 0;
 }
 
@@ -562,6 +567,8 @@ int lagrbf_serve_info(void* srv, const char* key, double* value) {
   else if (k == "reserved") *value = sv->reserved;
   else if (k == "queued") *value = (double)sv->queue.size();
   else if (k == "max_active") *value = sv->max_active;
+  else if (k == "enqueue_ms") *value = sv->enqueue_ms;
+  else if (k == "tick_ms") *value = sv->tick_ms;
   else if (k == "now") *value = sv->now();
   else if (k == "datasets") *value = (double)sv->ds.size();
   else return set_err(E_WRONG, "lagrbf_serve_info", "unknown key '%s'", key);
