@@ -221,6 +221,8 @@ static void chol_rp_k(const Grid& g, int nmax, const int32_t* list, int nlist, c
                                        fac_off, mem_pool, xd, fac, status, a)
   if (nmax <= 128) LRB_RP(2);
   else if (nmax <= 256) LRB_RP(4);
+  else if (nmax <= 320) LRB_RP(5);
+  else if (nmax <= 384) LRB_RP(6);
   else LRB_RP(8);
 #undef LRB_RP
 }

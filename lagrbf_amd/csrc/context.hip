@@ -465,7 +465,7 @@ static int upload_list(Ctx& c, const std::vector<int32_t>& l, int seg) {
 // ---------------------------------------------------------------------------
 // Factor kernel choice for the small-patch Cholesky (the matrix-core kernel with
 // panels of 16 or 32 columns; the vector-ALU kernel by option only), per precision, kernel,
-// polynomial block and size bucket of the factor dimension (<= 128, <= 256, <=
+// polynomial block and size bucket of the factor dimension (<= 128, 256, 320, 384,
 // 512).  The first factorisation in a bucket times every candidate on a sample of
 // its patches (up to 256; factoring is idempotent, the sample is factored again
 // with the winner); the choice is kept for the process.  Where the lean solves
@@ -485,7 +485,16 @@ static std::map<FactorTuneKey, int> g_ftune;
 static constexpr int FACTOR_NB[] = {16, 0};
 static constexpr int N_FACTOR_NB = (int)(sizeof(FACTOR_NB) / sizeof(FACTOR_NB[0]));
 
-static int factor_bucket(int nn) { return nn <= 128 ? 0 : (nn <= 256 ? 1 : 2); }
+// size buckets of the factor dimension and the largest dimension of each: the
+// matrix-core kernel is sized by it (16-row strips per wave), and the strips the
+// largest bucket provides for 512 rows cost two waves per SIMD at ~300 rows
+static constexpr int FACTOR_BUCKET_N[] = {128, 256, 320, 384, 512};
+static constexpr int N_FACTOR_BUCKETS = 5;
+static int factor_bucket(int nn) {
+  int b = 0;
+  while (b + 1 < N_FACTOR_BUCKETS && nn > FACTOR_BUCKET_N[b]) b++;
+  return b;
+}
 
 static int factor_nb_for(Ctx& c, bool fp32, int bucket, const int32_t* dlist, int nsample, void* fac,
                          const AugView& a) {
@@ -504,7 +513,7 @@ static int factor_nb_for(Ctx& c, bool fp32, int bucket, const int32_t* dlist, in
   for (int v = 0; v < N_FACTOR_NB; v++) {
     (void)hipEventRecord(e[v], c.s);
     launch_factor_chol(c.g, c.ktype, fp32, dlist, nsample, c.p_n.p, c.p_mem_off.p, c.p_fac_off.p, c.mem_pool.p, c.dx.p,
-                       fac, c.pstatus.p, false, a, FACTOR_NB[v], 128 << bucket, c.s);
+                       fac, c.pstatus.p, false, a, FACTOR_NB[v], FACTOR_BUCKET_N[bucket], c.s);
   }
   (void)hipEventRecord(e[N_FACTOR_NB], c.s);
   (void)hipEventSynchronize(e[N_FACTOR_NB]);
@@ -645,9 +654,9 @@ static int factor_lists(Ctx& c, std::vector<int32_t>* fl) {
     if (fl[m].empty() || method_big(m)) continue;
     auto nn = [&](int p) { return c.h_n[p] - (method_aug(m) ? c.h_m[p] : 0); };
     std::stable_sort(fl[m].begin(), fl[m].end(), [&](int a, int b) { return factor_bucket(nn(a)) < factor_bucket(nn(b)); });
-    bstart[m].assign(4, (int)fl[m].size());
+    bstart[m].assign(N_FACTOR_BUCKETS + 1, (int)fl[m].size());
     for (int i = (int)fl[m].size() - 1; i >= 0; i--) bstart[m][factor_bucket(nn(fl[m][i]))] = i;
-    for (int b = 2; b >= 0; b--) bstart[m][b] = std::min(bstart[m][b], bstart[m][b + 1]);
+    for (int b = N_FACTOR_BUCKETS - 1; b >= 0; b--) bstart[m][b] = std::min(bstart[m][b], bstart[m][b + 1]);
   }
   for (int m = 0; m < N_METHODS; m++) if (!fl[m].empty()) CHK(upload_list(c, fl[m], m));
   if (!fl[M_BK64].empty())
@@ -684,12 +693,12 @@ static int factor_lists(Ctx& c, std::vector<int32_t>* fl) {
         CHK(factor_big(c, m, fl[m], dl));
         continue;
       }
-      for (int b = 0; b < 3; b++) {
+      for (int b = 0; b < N_FACTOR_BUCKETS; b++) {
         const int b0 = bstart[m][b], b1 = bstart[m][b + 1];
         if (b1 <= b0) continue;
         const int nb = factor_nb_for(c, method_fp32(m), b, dl + b0, std::min(b1 - b0, 256), fac, a);
         launch_factor_chol(c.g, c.ktype, method_fp32(m), dl + b0, b1 - b0, c.p_n.p, c.p_mem_off.p, c.p_fac_off.p,
-                           c.mem_pool.p, c.dx.p, fac, c.pstatus.p, false, a, nb, 128 << b, c.s);
+                           c.mem_pool.p, c.dx.p, fac, c.pstatus.p, false, a, nb, FACTOR_BUCKET_N[b], c.s);
         for (int i = b0; i < b1; i++) c.h_fnb[fl[m][i]] = (uint8_t)nb;
       }
     }
