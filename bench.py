@@ -87,14 +87,17 @@ def patch_sizes(cs, nc):
     return pc.ravel()
 
 
-def load_traffic(precision, path=TRAFFIC_FILE):
+def load_traffic(precision, shape, path=TRAFFIC_FILE):
+    """HBM bytes per evaluation launch of `precision` in launch shape `shape`, from
+    the committed counter runs (one per shape); None when that shape was not
+    measured."""
     if path is None:
         return None
     try:
         with open(path) as f:
-            t = json.load(f)[precision]
+            t = json.load(f)[precision]["shapes"][str(int(shape))]
         return float(t["fetch_bytes"]) + float(t["write_bytes"])
-    except (OSError, KeyError, ValueError):
+    except (OSError, KeyError, ValueError, TypeError):
         return None
 
 
@@ -153,7 +156,7 @@ def run_path(L, precision, opts, X, V, P, steps, warmup, dt, sample, traffic_fil
         achieved = flops / (avg_ms * 1e-3)
     else:
         log(f"[{precision}] no settled evaluation in the timed steps: raise --warmup for the roofline figures")
-    traffic = load_traffic(precision, traffic_file)
+    traffic = load_traffic(precision, ctx.info("eval_variant"), traffic_file)
     coefs = {}
     for p in sample:
         c = ctx.debug_get("patch_coeffs", int(p), dtype=np.float64)

@@ -45,6 +45,20 @@ inline int eval_variant_pipe1(int v) {
   return P1[v];
 }
 
+// How far a shape spreads a cell's neighbourhood over the XCDs' L2 caches: 0 keeps
+// neighbouring cells on one XCD (one block per cell in contiguous ranges, listed
+// cells in contiguous runs or in runs of 4), 1 runs of 16 and the lane-group shapes,
+// 2 single cells dealt round-robin.  Measured HBM fetch per FP64 evaluation at the
+// r^3 + linear workload of 2^20 nodes: 1.5, 3.7 and 5.6 GB.
+inline int eval_variant_locality(int v) {
+  bool listed;
+  int group, pipe, waves;
+  eval_variant(v, listed, group, pipe, waves);
+  if (group == 0 || group == -4) return 0;
+  if (group == -1) return 2;
+  return 1;
+}
+
 inline bool eval_variant_listed(int v) {
   bool listed;
   int group, pipe, waves;
@@ -71,6 +85,10 @@ struct EvalTuner {
   //    each, timed with stream events, and the fastest of those stays.
   // All shapes write identical values, so none of this changes results.
   static constexpr float RUNOFF_MARGIN = 1.05f;
+  // run-off times within this factor of the fastest are a tie, which the shape that
+  // moves fewer bytes wins: concurrent work (other serving batches, the set-up's
+  // solves) shares HBM and L2 with the evaluation
+  static constexpr float TIE_MARGIN = 1.02f;
   static constexpr int DEFAULT_VARIANT = 5;
   static constexpr int GROUPED_MAX_BUCKET = 12;   // bucket of 64 particles per cell
   struct Runoff {
@@ -219,22 +237,31 @@ struct EvalTuner {
     r.ev.push_back(b);
     r.next += 1;
     if (r.next == r.cand.size()) {
-      // every candidate ran once in the steady state: the fastest stays
+      // every candidate ran once in the steady state: the fastest stays, ties going
+      // to the shape that keeps neighbourhoods in fewer L2 caches
+      std::vector<float> t(r.cand.size(), 1e30f);
       float best = 1e30f;
-      int bv = choice[bucket];
       for (size_t k = 0; k < r.cand.size(); k++) {
         (void)hipEventSynchronize(r.ev[2 * k + 1]);
         float ms = 0.f;
-        if (hipEventElapsedTime(&ms, r.ev[2 * k], r.ev[2 * k + 1]) == hipSuccess && ms < best) {
-          best = ms;
-          bv = r.cand[k];
-        }
+        if (hipEventElapsedTime(&ms, r.ev[2 * k], r.ev[2 * k + 1]) == hipSuccess) t[k] = ms;
+        best = t[k] < best ? t[k] : best;
       }
+      int bk = -1;
+      for (size_t k = 0; k < r.cand.size(); k++) {
+        if (!(t[k] <= best * TIE_MARGIN)) continue;
+        if (bk < 0) { bk = (int)k; continue; }
+        const int lk = eval_variant_locality(r.cand[k]), lb = eval_variant_locality(r.cand[bk]);
+        if (lk < lb || (lk == lb && t[k] < t[bk])) bk = (int)k;
+      }
+      // the winner is read before the run-off (which r refers to) is erased
+      const int bv = bk >= 0 ? r.cand[bk] : choice[bucket];
+      const float bt = bk >= 0 ? t[bk] : best;
       (void)hipGetLastError();
       for (hipEvent_t e : r.ev) (void)hipEventDestroy(e);
       runoff.erase(bucket);
       choice[bucket] = bv;
-      best_ms[bucket] = best;
+      best_ms[bucket] = bt;
       final_[bucket] = true;
     }
     return v;

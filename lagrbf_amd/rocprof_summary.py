@@ -140,6 +140,7 @@ def main():
     t.add_argument("--kernel", default="")
     t.add_argument("--json", default="")
     t.add_argument("--key", default="")
+    t.add_argument("--shape", type=int, default=-1, help="launch shape the counter runs used (stored per shape)")
     a = ap.parse_args()
     if a.cmd == "stats":
         stats(a.dir, a.top)
@@ -161,9 +162,10 @@ def main():
         ev = [(k, v) for k, v in res["kernels"].items() if "k_eval_" in k]
         if ev:
             name, big = max(ev, key=lambda kv: kv[1]["dispatches"])
-            data[a.key] = {"kernel": name, "fetch_bytes": big["fetch_bytes"], "write_bytes": big["write_bytes"],
-                           "fetch_scale": res["fetch_scale"], "write_scale": res["write_scale"],
-                           "copy_bytes": res["copy_bytes"]}
+            entry = {"kernel": name, "fetch_bytes": big["fetch_bytes"], "write_bytes": big["write_bytes"],
+                     "fetch_scale": res["fetch_scale"], "write_scale": res["write_scale"],
+                     "copy_bytes": res["copy_bytes"]}
+            data.setdefault(a.key, {}).setdefault("shapes", {})[str(a.shape)] = entry
             with open(a.json, "w") as fh:
                 json.dump(data, fh, indent=1)
 

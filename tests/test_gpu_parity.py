@@ -592,3 +592,32 @@ def test_six_field_evaluation(precision):
     assert err < (TOL_VAL if precision == "fp64" else 1e-5)
     for v in vals[1:]:
         assert np.array_equal(vals[0], v)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_tuner_runoff_keeps_a_valid_shape(precision):
+    """The autotuner's trial and run-off run to completion inside an RK loop (the
+    run-off's decision prefers, among shapes within 2% of the fastest, the one that
+    keeps neighbourhoods in fewer L2 caches): the shape it keeps is a valid one, the
+    later steps run it, and the positions equal a run with that shape forced, bit
+    for bit (every shape gives identical values)."""
+    opts, X, V, P = cases.fourier_3d(nlat=20, Np=20000, n=9, scheme=2)
+    opts.update(dt=1e-3, cfl_fac=1e6)
+    print(f"case: tuner run-off over RK4 steps, {precision}", flush=True)
+    out = {}
+    for forced in (None, "kept"):
+        o = dict(opts)
+        if forced == "kept":
+            o["lagrbf_eval_variant"] = kept
+        g = L.LagRBFCtx(petsc_options=o, precision=precision)
+        g.fill_data(X, V); g.fill_points(P); g.interpolate_c()
+        g.advect_steps_c(12, 1e-3)   # the trial, a run-off of up to 18 shapes, then the kept one
+        xs = np.zeros_like(P); g.output(coords=xs)
+        used = int(g.info("eval_variant"))
+        g.destroy()
+        if forced is None:
+            kept = used
+        out[forced] = xs
+        print(f"{precision} {forced}: shape {used}")
+        assert 0 <= used < 18
+    assert np.array_equal(out[None], out["kept"])
