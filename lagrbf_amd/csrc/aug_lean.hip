@@ -17,7 +17,8 @@
 //   - the m x NF dot products of a step (V^T r, P^T c, B1^T y, V^T u) take one
 //     block reduction (wave shuffles, one barrier pair) instead of one per term.
 // With an FP32 factor (the mixed path) the corrections are refined until they fall
-// below tol |x| (or stagnate at the FP64 floor of the residual), as in k_solve_aug.
+// below tol |x| (or stagnate at the FP64 floor of the residual), as in k_solve_aug;
+// their B22 solves run in FP32 arithmetic on an FP32 copy of the right-hand side.
 #include "kernels.hpp"
 #include "lean_solve.hpp"
 
@@ -96,6 +97,7 @@ __global__ __launch_bounds__(GNT) void k_solve_aug_lean(Grid g, const int32_t* _
   __shared__ double sv[MF], sb[MF], su1[MF], sh1[MF], sdb[MF], st2[MF];
   __shared__ double sR[MM * MM], sT[MM * MM], sB11[MM * MM];
   __shared__ double zb[16 * NF], lb[16 * 17];
+  __shared__ float zb32[16 * NF], lb32[16 * 17];
   __shared__ double smx[GNT / GW][2];
   const int p = list[blockIdx.x];
   const int n = pn[p];
@@ -118,6 +120,7 @@ __global__ __launch_bounds__(GNT) void k_solve_aug_lean(Grid g, const int32_t* _
   double* xs = gsm;                           // [3][S]
   double* c = xs + 3 * (size_t)S;             // [NF][S] solution
   double* y = c + (size_t)NF * S;             // [NF][S] residual, then h, then u
+  float* y32 = reinterpret_cast<float*>(y + (size_t)NF * S);   // FP32 factor: [NF][S] B22 correction
   for (int i = tid; i < n; i += GNT) {
 #pragma unroll
     for (int d = 0; d < 3; d++) xs[d * S + i] = xsg[(size_t)d * n + i];
@@ -272,8 +275,19 @@ __global__ __launch_bounds__(GNT) void k_solve_aug_lean(Grid g, const int32_t* _
       }
     }
     __syncthreads();
-    // u2 = B22^-1 (h2 - B21 u1)
-    llt_solve_lean<TL, double, NF>(L, nn, y + m, S, zb, lb);
+    // u2 = B22^-1 (h2 - B21 u1); with an FP32 factor in FP32 arithmetic, as the
+    // refinement's corrections need (the factor's rounding bounds their accuracy)
+    if constexpr (REF) {
+      for (int f = 0; f < NF; f++)
+        for (int i = tid; i < nn; i += GNT) y32[f * S + i] = (float)y[f * S + m + i];
+      __syncthreads();
+      llt_solve_lean<TL, float, NF>(L, nn, y32, S, zb32, lb32);
+      for (int f = 0; f < NF; f++)
+        for (int i = tid; i < nn; i += GNT) y[f * S + m + i] = (double)y32[f * S + i];
+      __syncthreads();
+    } else {
+      llt_solve_lean<TL, double, NF>(L, nn, y + m, S, zb, lb);
+    }
     // beta correction: R^-1 (h1 - B11 u1 - B12 u2)
     block_dots<NF, MM>(m, m, n, y, S, bw, red, sv);
     if (tid < NF) {
@@ -373,7 +387,11 @@ __global__ __launch_bounds__(GNT) void k_solve_aug_lean(Grid g, const int32_t* _
   }
 }
 
-static size_t aug_lean_smem(int nf_pad, int S) { return sizeof(double) * (size_t)S * (3 + 2 * (size_t)nf_pad); }
+// coordinates, solution and work vector in FP64; an FP32 factor adds the FP32
+// correction vector
+static size_t aug_lean_smem(bool ref, int nf_pad, int S) {
+  return sizeof(double) * (size_t)S * (3 + 2 * (size_t)nf_pad) + (ref ? sizeof(float) * (size_t)S * nf_pad : 0);
+}
 static int aug_lean_nf_pad(int nf) { return nf <= 4 ? nf : (nf <= 6 ? 6 : 8); }
 
 template <typename TL, int KT, int MM, int NF>
@@ -381,13 +399,14 @@ static void aug_lean_nf(const Grid& g, const int32_t* list, int nlist, const int
                         const int64_t* fac_off, const int32_t* mem_pool, const TL* fac, const double* values, int nf,
                         double* coef, int max_iter, double tol, int32_t* status, int32_t* iters, int S,
                         const AugView& a, hipStream_t s) {
+  constexpr bool REF = sizeof(TL) == sizeof(float);
   static bool attr = false;
   if (!attr) {
     (void)hipFuncSetAttribute((const void*)k_solve_aug_lean<TL, KT, MM, NF>,
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)aug_lean_smem(NF, GMAX));
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)aug_lean_smem(REF, NF, GMAX));
     attr = true;
   }
-  hipLaunchKernelGGL((k_solve_aug_lean<TL, KT, MM, NF>), dim3(nlist), dim3(GNT), aug_lean_smem(NF, S), s, g, list, pn,
+  hipLaunchKernelGGL((k_solve_aug_lean<TL, KT, MM, NF>), dim3(nlist), dim3(GNT), aug_lean_smem(REF, NF, S), s, g, list, pn,
                      mem_off, fac_off, mem_pool, fac, values, nf, coef, max_iter, tol, status, iters, S, a);
 }
 
