@@ -1,6 +1,6 @@
-"""The process-wide cache of large device blocks (devmem.cpp): a context set up on
-blocks released by an earlier context (cleared on reuse) gives bit-identical
-coefficients and values to the first one, for both precisions."""
+"""The process-wide cache of device memory (devmem.cpp): a context set up on physical
+chunks released by an earlier context (mapped again and cleared on reuse) gives
+bit-identical coefficients and values to the first one, for both precisions."""
 import numpy as np
 import pytest
 
@@ -34,3 +34,33 @@ def test_reused_blocks_give_identical_results(precision):
             assert cached >= 64 * 2**20
     assert np.array_equal(out[0], out[1])
     assert np.array_equal(coefs[0], coefs[1])
+
+
+def test_chunks_reused_by_a_context_of_another_size():
+    # a larger context after a smaller one maps the smaller one's chunks and new ones;
+    # its values match a second context of the same size set up on its own chunks
+    small = cases.fourier_3d(nlat=30, Np=5000, n=12)
+    large = cases.fourier_3d(nlat=40, Np=5000, n=16)
+    print("case: chunk reuse across context sizes, fp64", flush=True)
+
+    def run(case):
+        opts, X, V, P = case
+        opts = dict(opts, lagrbf_precision="fp64", advection=False, lagrbf_setup_all=True)
+        g = L.LagRBFCtx(petsc_options=opts)
+        g.fill_data(X, V); g.fill_points(P); g.interpolate_c()
+        v = np.zeros((P.shape[0], 3)); g.output(fvals=v)
+        held = g.info("device_cache_bytes")
+        g.destroy()
+        return v, held
+
+    run(small)
+    g0 = L.LagRBFCtx(petsc_options=dict(small[0]))
+    before = g0.info("device_cache_bytes")
+    g0.destroy()
+    v1, held1 = run(large)
+    v2, _ = run(large)
+    print(f"cached after the small context {before / 2**20:.0f} MiB, while the large one ran "
+          f"{held1 / 2**20:.0f} MiB", flush=True)
+    assert before >= 64 * 2**20
+    assert held1 < before   # the large context took the small one's chunks
+    assert np.array_equal(v1, v2)
