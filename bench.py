@@ -486,7 +486,8 @@ def main():
     ap.add_argument("--no-c1", action="store_true", help="skip the 2D rotation configuration")
     ap.add_argument("--c4-only", action="store_true", help="the adaptive DP45 configuration only")
     ap.add_argument("--no-c4", action="store_true", help="skip the adaptive DP45 measurement")
-    ap.add_argument("--c4-t-end", type=float, default=0.1, help="end time of the adaptive DP45 run")
+    ap.add_argument("--c4-t-end", type=float, default=1.0,
+                    help="end time of the adaptive DP45 run (1.0: the whole interval of the time blend)")
     ap.add_argument("--c4-scale", type=float, default=1.0, help="fraction of the C4 node and particle counts")
     ap.add_argument("--eval-variant", type=int, default=-1,
                     help="fix the evaluation launch shape (0..13, for profiling one kernel); -1: autotuned")
