@@ -173,103 +173,6 @@ void launch_build_blk(const Grid& g, bool fp32, const double* x, int N, const in
                        mem_off, coef, nft, cstride, f0, cscale, (double*)blk);
 }
 
-// Monomial q of the basis 1, x, y, (z), xx, xy, (xz), yy, (yz), (zz) is the product
-// of coordinates mono_factor(q, 0) and mono_factor(q, 1) (-1: factor 1).
-template <int DIM>
-__host__ __device__ constexpr int mono_factor(int q, int which) {
-  if (q == 0) return -1;
-  if (q <= DIM) return which == 0 ? q - 1 : -1;
-  int k = DIM + 1;
-  for (int i = 0; i < DIM; i++)
-    for (int j = i; j < DIM; j++) {
-      if (k == q) return which == 0 ? i : j;
-      k++;
-    }
-  return -1;
-}
-
-// Polynomial parts of the corner patches: out += sum_k w_k p_k(x) . beta_k for
-// every live particle.  One wave per cell (listed non-empty cells when the list
-// was built, every cell otherwise): the cell's corner patches are the same for all
-// its particles, so their beta rows are wave-uniform scalar loads, read once per
-// cell instead of once per particle.  A particle whose 3^d neighbourhood holds no
-// node has only empty corner patches, whose polynomial rows are zero: its stale
-// values stay as they are.
-template <int DIM, int NFT>
-__global__ __launch_bounds__(64) void k_add_poly(Grid g, const double* __restrict__ px, int64_t ld,
-                                                 const int32_t* __restrict__ perm, const int32_t* __restrict__ pstart,
-                                                 const int32_t* __restrict__ cell_list,
-                                                 const int32_t* __restrict__ list_count, const double* __restrict__ w,
-                                                 EvalExt ev, int nfields, double* __restrict__ out) {
-  int c = blockIdx.x;
-  if (cell_list) {
-    if (c >= *list_count) return;
-    c = cell_list[c];
-  } else if (c >= g.C) {
-    return;
-  }
-  const int pb = pstart[c], pe = pstart[c + 1];
-  if (pb >= pe) return;
-  const int ci = c % g.nc[0], cj = (c / g.nc[0]) % g.nc[1], ck = DIM == 3 ? c / (g.nc[0] * g.nc[1]) : 0;
-  for (int s = pb + (int)threadIdx.x; s < pe; s += 64) {
-    const int i = perm[s];
-    const double x[3] = {px[i], px[ld + i], DIM == 3 ? px[2 * ld + i] : 0.0};
-    double add[NFT];
-#pragma unroll
-    for (int f = 0; f < NFT; f++) add[f] = 0.0;
-#pragma unroll
-    for (int k = 0; k < (1 << DIM); k++) {
-      int a, b, e;
-      corner_offset(k, a, b, e);
-      const int pi = ci + a, pj = cj + b, pk = DIM == 3 ? ck + e : 0;
-      const int p = pi + g.n[0] * (pj + g.n[1] * pk);
-      const int idx[3] = {pi, pj, pk};
-      double xi[3] = {0.0, 0.0, 0.0};
-#pragma unroll
-      for (int d = 0; d < DIM; d++) xi[d] = (x[d] - (g.O[d] + idx[d] * g.h[d])) * ev.ps;
-      const double* pt = ev.ptab + (size_t)p * ev.mm * NFT;
-      const double wk = w[(size_t)k * ld + i];
-      double sp[NFT];
-#pragma unroll
-      for (int f = 0; f < NFT; f++) sp[f] = 0.0;
-      // monomial q = xi[fa] xi[fb] (factor 1 for -1), indices fixed at compile time so
-      // that nothing here is indexed at run time (that would put arrays in scratch)
-#pragma unroll
-      for (int q = 0; q < POLY_MAX; q++) {
-        if (q < ev.mm) {
-          const int fa = mono_factor<DIM>(q, 0), fb = mono_factor<DIM>(q, 1);
-          const double mq = (fa < 0 ? 1.0 : xi[fa]) * (fb < 0 ? 1.0 : xi[fb]);
-#pragma unroll
-          for (int f = 0; f < NFT; f++) sp[f] += mq * pt[q * NFT + f];
-        }
-      }
-#pragma unroll
-      for (int f = 0; f < NFT; f++) add[f] += wk * sp[f];
-    }
-#pragma unroll
-    for (int f = 0; f < NFT; f++)
-      if (f < nfields) out[(size_t)f * ld + i] += add[f];
-  }
-}
-
-template <int DIM>
-static void add_poly_dim(const Grid& g, const double* px, int64_t ld, const int32_t* perm, const int32_t* pstart,
-                         const int32_t* cell_list, const int32_t* list_count, int np, const double* w,
-                         const EvalExt& ev, int nft, int nfields, double* out, hipStream_t s) {
-  // listed cells: at most min(C, np) are non-empty
-  const int nb = cell_list ? std::max(1, std::min(g.C, np)) : g.C;
-  const dim3 grid(nb), blk(64);
-#define LRB_AP(NFT) hipLaunchKernelGGL((k_add_poly<DIM, NFT>), grid, blk, 0, s, g, px, ld, perm, pstart, cell_list, list_count, w, ev, nfields, out)
-  switch (nft) {
-    case 1: LRB_AP(1); break;
-    case 2: LRB_AP(2); break;
-    case 3: LRB_AP(3); break;
-    case 4: LRB_AP(4); break;
-    default: LRB_AP(6); break;
-  }
-#undef LRB_AP
-}
-
 void launch_eval_blk(const Grid& g, int ktype, bool fp32, const double* px, int64_t ld, const int32_t* perm,
                      const int32_t* pstart, const double* w, const void* blk, int nft, const int32_t* cs, int nfields,
                      double* out, int block, int np, const int32_t* cell_list, const int32_t* list_count,
@@ -288,10 +191,6 @@ void launch_eval_blk(const Grid& g, int ktype, bool fp32, const double* px, int6
     default: LRB_K(K_GAUSS); break;
   }
 #undef LRB_K
-  if (ev.ptab && ev.pcell && ev.palive) {
-    if (g.dim == 3) add_poly_dim<3>(g, px, ld, perm, pstart, cell_list, list_count, np, w, ev, nft, nfields, out, s);
-    else add_poly_dim<2>(g, px, ld, perm, pstart, cell_list, list_count, np, w, ev, nft, nfields, out, s);
-  }
 }
 
 }  // namespace lrb

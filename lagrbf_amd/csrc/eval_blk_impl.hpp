@@ -438,8 +438,7 @@ __device__ __forceinline__ void corner_eps2(const Grid& g, int ci, int cj, int c
 }
 
 // Shepard combination of the corner sums, val = sum_k w_k acc_k (the polynomial
-// parts of the corner patches, if any, are added by a separate pass: in this
-// epilogue they cost the node loops their registers).
+// parts of the corner patches, if any, come from poly_part).
 template <int DIM, int NFT, typename T>
 __device__ __forceinline__ void combine_corners(const double* __restrict__ w, int64_t ld, int i,
                                                 const T (&acc)[1 << DIM][NFT], double (&val)[NFT]) {
@@ -451,6 +450,104 @@ __device__ __forceinline__ void combine_corners(const double* __restrict__ w, in
 #pragma unroll
     for (int f = 0; f < NFT; f++) val[f] += (double)acc[k][f] * wk;
   }
+}
+
+// Monomial q of the basis 1, x, y, (z), xx, xy, (xz), yy, (yz), (zz) is the product
+// of coordinates mono_factor(q, 0) and mono_factor(q, 1) (-1: factor 1).
+template <int DIM>
+__host__ __device__ constexpr int mono_factor(int q, int which) {
+  if (q == 0) return -1;
+  if (q <= DIM) return which == 0 ? q - 1 : -1;
+  int k = DIM + 1;
+  for (int i = 0; i < DIM; i++)
+    for (int j = i; j < DIM; j++) {
+      if (k == q) return which == 0 ? i : j;
+      k++;
+    }
+  return -1;
+}
+
+// Polynomial parts of the corner patches of cell (ci, cj, ck) at particle i, in FP64:
+// add = sum_k w_k p_k(x) . beta_k, the monomials of the patch node's scaled local
+// coordinates (x - node_k) ps times the patch's polynomial coefficients (ptab row
+// [mm][NFT] of patch k).  The kernels compute it before their node loops and store it
+// in the output, then add the RBF part at the end: held in registers through the
+// node loops it would cost those loops their occupancy, computed after them it
+// raised the register peak of the epilogue.
+template <int DIM, int NFT>
+__device__ __forceinline__ void poly_part(const Grid& g, int ci, int cj, int ck, const double* __restrict__ px,
+                                          int64_t ld, int i, const double* __restrict__ w, const EvalExt& ev,
+                                          double (&add)[NFT]) {
+  const double x[3] = {px[i], px[ld + i], DIM == 3 ? px[2 * ld + i] : 0.0};
+#pragma unroll
+  for (int f = 0; f < NFT; f++) add[f] = 0.0;
+  // one corner at a time: unrolled, the 2^d corners' rows would be loaded together
+  // into more scalar registers than there are, and the spill would take vector
+  // registers from the whole kernel
+#pragma unroll 1
+  for (int k = 0; k < (1 << DIM); k++) {
+    int a, b, e;
+    corner_offset(k, a, b, e);
+    const int pi = ci + a, pj = cj + b, pk = DIM == 3 ? ck + e : 0;
+    const int p = pi + g.n[0] * (pj + g.n[1] * pk);
+    const int idx[3] = {pi, pj, pk};
+    double xi[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int d = 0; d < DIM; d++) xi[d] = (x[d] - fma((double)idx[d], g.h[d], g.O[d])) * ev.ps;
+    const double* pt = ev.ptab + (size_t)p * ev.mm * NFT;
+    const double wk = w[(size_t)k * ld + i];
+    double sp[NFT];
+#pragma unroll
+    for (int f = 0; f < NFT; f++) sp[f] = 0.0;
+    // monomial q = xi[fa] xi[fb] (factor 1 for -1), indices fixed at compile time so
+    // that nothing here is indexed at run time (that would put arrays in scratch)
+#pragma unroll
+    for (int q = 0; q < POLY_MAX; q++) {
+      if (q < ev.mm) {
+        const int fa = mono_factor<DIM>(q, 0), fb = mono_factor<DIM>(q, 1);
+        const double mq = (fa < 0 ? 1.0 : xi[fa]) * (fb < 0 ? 1.0 : xi[fb]);
+#pragma unroll
+        for (int f = 0; f < NFT; f++) sp[f] = fma(mq, pt[q * NFT + f], sp[f]);
+      }
+    }
+#pragma unroll
+    for (int f = 0; f < NFT; f++) add[f] = fma(wk, sp[f], add[f]);
+  }
+}
+
+// The polynomial parts of the particles perm[s], s = s0 + lane, s0 = sb, sb + step,
+// ... < pe, into out (fields < nfields): a loop of its own before the node loops, so
+// that none of its values (the corner patches' rows, their origins) stays live
+// through them; store_value then adds the RBF part.  Compiler barriers keep the rows'
+// loads inside the loop (hoisted, they would be live through the node loops) and
+// keep the stores from being forwarded to store_value's loads (which would hold the
+// values in registers through the node loops).
+template <int DIM, int NFT>
+__device__ __forceinline__ void store_poly(const Grid& g, int ci, int cj, int ck, const double* __restrict__ px,
+                                           int64_t ld, const int32_t* __restrict__ perm, int sb, int pe, int step,
+                                           int lane, const double* __restrict__ w, const EvalExt& ev, int nfields,
+                                           double* __restrict__ out) {
+#pragma unroll 1
+  for (int s0 = sb; s0 < pe; s0 += step) {
+    __asm__ volatile("" ::: "memory");
+    const int s = s0 + lane;
+    if (s >= pe) continue;
+    const int i = perm[s];
+    double add[NFT];
+    poly_part<DIM, NFT>(g, ci, cj, ck, px, ld, i, w, ev, add);
+#pragma unroll
+    for (int f = 0; f < NFT; f++)
+      if (f < nfields) out[(size_t)f * ld + i] = add[f];
+  }
+  __asm__ volatile("" ::: "memory");
+}
+
+template <int NFT>
+__device__ __forceinline__ void store_value(bool poly, int64_t ld, int i, const double (&val)[NFT], int nfields,
+                                            double* __restrict__ out) {
+#pragma unroll
+  for (int f = 0; f < NFT; f++)
+    if (f < nfields) out[(size_t)f * ld + i] = poly ? val[f] + out[(size_t)f * ld + i] : val[f];
 }
 
 // occupancy floor (waves per SIMD) of the evaluation kernels
@@ -526,6 +623,8 @@ void k_eval_blk(Grid g, const double* __restrict__ px, int64_t ld,
     nnb += cs[q + 1] - cs[q];
   }
   if (nnb == 0) return;
+  const bool poly = ev.ptab != nullptr;
+  if (poly) store_poly<DIM, NFT>(g, ci, cj, ck, px, ld, perm, pb + wv * 64, pe, nw * 64, lane, w, ev, nfields, out);
   // FP32: coordinates relative to the particle cell's origin
   const double ox = sizeof(T) == 4 ? g.O[0] + ci * g.h[0] : 0.0;
   const double oy = sizeof(T) == 4 ? g.O[1] + cj * g.h[1] : 0.0;
@@ -554,8 +653,7 @@ void k_eval_blk(Grid g, const double* __restrict__ px, int64_t ld,
         for (int f = 0; f < NFT; f++) acc[k][f] = acc4[k][f];
       double val[NFT];
       combine_corners<DIM, NFT, T>(w, ld, i, acc, val);
-#pragma unroll
-      for (int f = 0; f < NFT; f++) if (f < nfields) out[(size_t)f * ld + i] = val[f];
+      store_value<NFT>(poly, ld, i, val, nfields, out);
       continue;
     }
     T acc[NEN][NFT];
@@ -606,8 +704,7 @@ void k_eval_blk(Grid g, const double* __restrict__ px, int64_t ld,
     if (!live) continue;
     double val[NFT];
     combine_corners<DIM, NFT, T>(w, ld, i, acc, val);
-#pragma unroll
-    for (int f = 0; f < NFT; f++) if (f < nfields) out[(size_t)f * ld + i] = val[f];
+    store_value<NFT>(poly, ld, i, val, nfields, out);
   }
   }
 }
@@ -692,6 +789,8 @@ __global__ __launch_bounds__(256) void k_eval_sparse(Grid g, const double* __res
   const T eps = (T)g.eps_r;
   const double rc = 1.0 / g.eps_r;
   const T rcut2 = (T)(rc * rc * (sizeof(T) == 4 ? (1.0 + 1e-5) : (1.0 + 1e-12)));
+  const bool poly = ev.ptab != nullptr;
+  if (poly) store_poly<DIM, NFT>(g, ci, cj, ck, px, ld, perm, pb, pe, G, mlane, w, ev, nfields, out);
   const double ox = sizeof(T) == 4 ? g.O[0] + ci * g.h[0] : 0.0;
   const double oy = sizeof(T) == 4 ? g.O[1] + cj * g.h[1] : 0.0;
   const double oz = sizeof(T) == 4 && DIM == 3 ? g.O[2] + ck * g.h[2] : 0.0;
@@ -741,8 +840,7 @@ __global__ __launch_bounds__(256) void k_eval_sparse(Grid g, const double* __res
     if (!live) continue;
     double val[NFT];
     combine_corners<DIM, NFT, T>(w, ld, i, acc, val);
-#pragma unroll
-    for (int f = 0; f < NFT; f++) if (f < nfields) out[(size_t)f * ld + i] = val[f];
+    store_value<NFT>(poly, ld, i, val, nfields, out);
   }
 }
 
