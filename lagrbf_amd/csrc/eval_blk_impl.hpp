@@ -470,15 +470,10 @@ __host__ __device__ constexpr int mono_factor(int q, int which) {
 // Polynomial parts of the corner patches of cell (ci, cj, ck) at particle i, in FP64:
 // add = sum_k w_k p_k(x) . beta_k, the monomials of the patch node's scaled local
 // coordinates (x - node_k) ps times the patch's polynomial coefficients (ptab row
-// [mm][NFT] of patch k).  The kernels compute it before their node loops and store it
-// in the output, then add the RBF part at the end: held in registers through the
-// node loops it would cost those loops their occupancy, computed after them it
-// raised the register peak of the epilogue.
+// [mm][NFT] of patch k).
 template <int DIM, int NFT>
-__device__ __forceinline__ void poly_part(const Grid& g, int ci, int cj, int ck, const double* __restrict__ px,
-                                          int64_t ld, int i, const double* __restrict__ w, const EvalExt& ev,
-                                          double (&add)[NFT]) {
-  const double x[3] = {px[i], px[ld + i], DIM == 3 ? px[2 * ld + i] : 0.0};
+__device__ __forceinline__ void poly_part(const Grid& g, int ci, int cj, int ck, const double (&x)[3], int64_t ld,
+                                          int i, const double* __restrict__ w, const EvalExt& ev, double (&add)[NFT]) {
 #pragma unroll
   for (int f = 0; f < NFT; f++) add[f] = 0.0;
   // one corner at a time: unrolled, the 2^d corners' rows would be loaded together
@@ -515,39 +510,35 @@ __device__ __forceinline__ void poly_part(const Grid& g, int ci, int cj, int ck,
   }
 }
 
-// The polynomial parts of the particles perm[s], s = s0 + lane, s0 = sb, sb + step,
-// ... < pe, into out (fields < nfields): a loop of its own before the node loops, so
-// that none of its values (the corner patches' rows, their origins) stays live
-// through them; store_value then adds the RBF part.  Compiler barriers keep the rows'
-// loads inside the loop (hoisted, they would be live through the node loops) and
-// keep the stores from being forwarded to store_value's loads (which would hold the
-// values in registers through the node loops).
+// The value of particle i into out (fields < nfields): the RBF part val plus, with a
+// polynomial block, the polynomial part, computed here after the node loops.  The
+// compiler barrier keeps its loads (weights, coordinates, the corner patches' rows)
+// from being hoisted above the node loops, where they would stay live through them.
+// xd: the particle's FP64 coordinates when the caller holds them (FP64 evaluation),
+// else null and they are read from px.
 template <int DIM, int NFT>
-__device__ __forceinline__ void store_poly(const Grid& g, int ci, int cj, int ck, const double* __restrict__ px,
-                                           int64_t ld, const int32_t* __restrict__ perm, int sb, int pe, int step,
-                                           int lane, const double* __restrict__ w, const EvalExt& ev, int nfields,
-                                           double* __restrict__ out) {
-#pragma unroll 1
-  for (int s0 = sb; s0 < pe; s0 += step) {
+__device__ __forceinline__ void store_value(const Grid& g, int ci, int cj, int ck, const double* __restrict__ px,
+                                            const double* xd, int64_t ld, int i, const double* __restrict__ w,
+                                            const EvalExt& ev, const double (&val)[NFT], int nfields,
+                                            double* __restrict__ out) {
+  if (ev.ptab) {
     __asm__ volatile("" ::: "memory");
-    const int s = s0 + lane;
-    if (s >= pe) continue;
-    const int i = perm[s];
+    double x[3];
+    if (xd) {
+      x[0] = xd[0]; x[1] = xd[1]; x[2] = xd[2];
+    } else {
+      x[0] = px[i]; x[1] = px[ld + i]; x[2] = DIM == 3 ? px[2 * ld + i] : 0.0;
+    }
     double add[NFT];
-    poly_part<DIM, NFT>(g, ci, cj, ck, px, ld, i, w, ev, add);
+    poly_part<DIM, NFT>(g, ci, cj, ck, x, ld, i, w, ev, add);
 #pragma unroll
     for (int f = 0; f < NFT; f++)
-      if (f < nfields) out[(size_t)f * ld + i] = add[f];
+      if (f < nfields) out[(size_t)f * ld + i] = val[f] + add[f];
+    return;
   }
-  __asm__ volatile("" ::: "memory");
-}
-
-template <int NFT>
-__device__ __forceinline__ void store_value(bool poly, int64_t ld, int i, const double (&val)[NFT], int nfields,
-                                            double* __restrict__ out) {
 #pragma unroll
   for (int f = 0; f < NFT; f++)
-    if (f < nfields) out[(size_t)f * ld + i] = poly ? val[f] + out[(size_t)f * ld + i] : val[f];
+    if (f < nfields) out[(size_t)f * ld + i] = val[f];
 }
 
 // occupancy floor (waves per SIMD) of the evaluation kernels
@@ -623,8 +614,6 @@ void k_eval_blk(Grid g, const double* __restrict__ px, int64_t ld,
     nnb += cs[q + 1] - cs[q];
   }
   if (nnb == 0) return;
-  const bool poly = ev.ptab != nullptr;
-  if (poly) store_poly<DIM, NFT>(g, ci, cj, ck, px, ld, perm, pb + wv * 64, pe, nw * 64, lane, w, ev, nfields, out);
   // FP32: coordinates relative to the particle cell's origin
   const double ox = sizeof(T) == 4 ? g.O[0] + ci * g.h[0] : 0.0;
   const double oy = sizeof(T) == 4 ? g.O[1] + cj * g.h[1] : 0.0;
@@ -653,7 +642,10 @@ void k_eval_blk(Grid g, const double* __restrict__ px, int64_t ld,
         for (int f = 0; f < NFT; f++) acc[k][f] = acc4[k][f];
       double val[NFT];
       combine_corners<DIM, NFT, T>(w, ld, i, acc, val);
-      store_value<NFT>(poly, ld, i, val, nfields, out);
+      {
+        const double xd[3] = {(double)xp, (double)yp, (double)zp};
+        store_value<DIM, NFT>(g, ci, cj, ck, px, sizeof(T) == 8 ? xd : nullptr, ld, i, w, ev, val, nfields, out);
+      }
       continue;
     }
     T acc[NEN][NFT];
@@ -704,7 +696,10 @@ void k_eval_blk(Grid g, const double* __restrict__ px, int64_t ld,
     if (!live) continue;
     double val[NFT];
     combine_corners<DIM, NFT, T>(w, ld, i, acc, val);
-    store_value<NFT>(poly, ld, i, val, nfields, out);
+    {
+      const double xd[3] = {(double)xp, (double)yp, (double)zp};
+      store_value<DIM, NFT>(g, ci, cj, ck, px, sizeof(T) == 8 ? xd : nullptr, ld, i, w, ev, val, nfields, out);
+    }
   }
   }
 }
@@ -789,8 +784,6 @@ __global__ __launch_bounds__(256) void k_eval_sparse(Grid g, const double* __res
   const T eps = (T)g.eps_r;
   const double rc = 1.0 / g.eps_r;
   const T rcut2 = (T)(rc * rc * (sizeof(T) == 4 ? (1.0 + 1e-5) : (1.0 + 1e-12)));
-  const bool poly = ev.ptab != nullptr;
-  if (poly) store_poly<DIM, NFT>(g, ci, cj, ck, px, ld, perm, pb, pe, G, mlane, w, ev, nfields, out);
   const double ox = sizeof(T) == 4 ? g.O[0] + ci * g.h[0] : 0.0;
   const double oy = sizeof(T) == 4 ? g.O[1] + cj * g.h[1] : 0.0;
   const double oz = sizeof(T) == 4 && DIM == 3 ? g.O[2] + ck * g.h[2] : 0.0;
@@ -840,7 +833,10 @@ __global__ __launch_bounds__(256) void k_eval_sparse(Grid g, const double* __res
     if (!live) continue;
     double val[NFT];
     combine_corners<DIM, NFT, T>(w, ld, i, acc, val);
-    store_value<NFT>(poly, ld, i, val, nfields, out);
+    {
+      const double xd[3] = {(double)xp, (double)yp, (double)zp};
+      store_value<DIM, NFT>(g, ci, cj, ck, px, sizeof(T) == 8 ? xd : nullptr, ld, i, w, ev, val, nfields, out);
+    }
   }
 }
 
