@@ -880,6 +880,7 @@ __global__ void k_iota(int32_t* __restrict__ a, int n) {
 hipError_t launch_reorder(const ReorderSet& r, const int32_t* perm, int n, int64_t ld, hipStream_t s) {
   if (n <= 0) return hipSuccess;
   if (!perm) return hipErrorInvalidValue;
+  if (r.nd < 0 || r.nd > 6 || r.ni < 0 || r.ni > 4) return hipErrorInvalidValue;
   for (int a = 0; a < r.nd; a++)
     if (r.rows[a] > 0 && (!r.src[a] || !r.dst[a])) return hipErrorInvalidValue;
   for (int a = 0; a < r.ni; a++)
@@ -930,14 +931,29 @@ __global__ void k_stage(Grid g, const double* __restrict__ pux, const double* __
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= Np || !alive[i]) return;
   const int dim = g.dim;
-  double x[3] = {0.0, 0.0, 0.0};
-  for (int d = 0; d < dim; d++) x[d] = px[d * ld + i];
+  // every operand is loaded before anything is stored, so the loads are in flight
+  // together (the targets may alias the sources: x0 is px for one-stage updates, and
+  // vel one of the stage velocities; each element is read before it is written)
+  double x[3] = {0.0, 0.0, 0.0}, xs[3] = {0.0, 0.0, 0.0}, vs[7][3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    if (d < dim) {
+      x[d] = px[d * ld + i];
+      xs[d] = x0[d * ld + i];
+    }
+#pragma unroll
+    for (int j = 0; j < 7; j++) vs[j][d] = (d < dim && j < a.nv) ? a.v[j][d * ld + i] : 0.0;
+  }
   const double scale = a.dt_req ? (a.dt_req[a.req[i]] * a.mul) / a.div : a.scale;
-  for (int d = 0; d < dim; d++) {
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    if (d >= dim) break;
     double v = 0.0;
-    for (int j = 0; j < a.nv; j++) v += a.a[j] * a.v[j][d * ld + i];
+#pragma unroll
+    for (int j = 0; j < 7; j++)
+      if (j < a.nv) v += a.a[j] * vs[j][d];
     if (a.write_vel) vel[d * ld + i] = v;
-    x[d] = x0[d * ld + i] + scale * v;
+    x[d] = xs[d] + scale * v;
     if (a.proj_in_loop) project_point(x, dim, a.rmin, a.rmax);
   }
   if (a.project && !a.proj_in_loop) project_point(x, dim, a.rmin, a.rmax);
