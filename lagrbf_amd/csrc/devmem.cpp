@@ -12,9 +12,13 @@
 // that overshoots it by a bounded amount, then new chunks of at most kChunkMax bytes
 // for the rest.  Reused chunks are cleared (the driver clears new ones) before the
 // range is handed out.  When a new chunk cannot be created, the cached chunks not
-// taken go back to the driver and the creation is tried once more.  A device
-// without virtual memory management gets plain hipMalloc / hipFree, and so does every
+// taken go back to the driver and the creation is tried once more.  A device without
+// virtual memory management gets plain hipMalloc / hipFree, and so does every
 // reservation when LAGRBF_DEVICE_CACHE=0 is set in the environment.
+//
+// Kernels and the runtime's linear copies and fills work across the chunks of a
+// range; its pitched copies (hipMemcpy2D*) refuse a region that spans more than one
+// chunk ("invalid argument"), so the library makes none on these ranges.
 #include "ctx.hpp"
 
 #include <algorithm>

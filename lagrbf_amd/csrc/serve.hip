@@ -100,13 +100,26 @@ size_t request_bytes(int dim, int nen, int scheme, int n) {
 template <typename T>
 int grow_rows(DBuf<T>& b, int rows, int64_t old_ld, int64_t new_ld, int n, hipStream_t s) {
   if (rows <= 0) return 0;
+  const char* fn = "batch_reserve";
   T* q = nullptr;
-  if (hipMalloc(&q, sizeof(T) * (size_t)new_ld * rows) != hipSuccess) return E_MEM;
-  if (b.p && n > 0)
-    if (hipMemcpy2DAsync(q, sizeof(T) * new_ld, b.p, sizeof(T) * old_ld, sizeof(T) * n, rows, hipMemcpyDeviceToDevice, s) !=
-        hipSuccess)
-      return E_LIB;
-  if (hipStreamSynchronize(s) != hipSuccess) return E_LIB;
+  hipError_t e = dev_malloc((void**)&q, sizeof(T) * (size_t)new_ld * rows);
+  if (e != hipSuccess)
+    return set_err(E_MEM, fn, "%d rows of %lld elements: %s", rows, (long long)new_ld, hipGetErrorString(e));
+  // one copy per row: a pitched copy is refused when its rows span several of the
+  // device cache's physical chunks (the range is contiguous only virtually)
+  for (int r = 0; b.p && n > 0 && r < rows; r++) {
+    e = hipMemcpyAsync(q + (size_t)r * new_ld, b.p + (size_t)r * old_ld, sizeof(T) * n, hipMemcpyDeviceToDevice, s);
+    if (e != hipSuccess) {
+      dev_free(q);
+      return set_err(E_LIB, fn, "copying row %d of %d (%d elements, leading dimension %lld -> %lld): %s", r, rows, n,
+                     (long long)old_ld, (long long)new_ld, hipGetErrorString(e));
+    }
+  }
+  e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    dev_free(q);
+    return set_err(E_LIB, fn, "stream synchronisation before the row copy's release: %s", hipGetErrorString(e));
+  }
   b.release();
   b.p = q;
   b.cap = (size_t)new_ld * rows;
@@ -196,7 +209,12 @@ int admit(Server& sv, Batch& b, ServeReq& r) {
   return 0;
 }
 
-static constexpr int SMALL_BATCH_VARIANT = 1;   // listed cells, contiguous XCD runs
+// untuned batches: FP64 listed cells in contiguous XCD runs; FP32 listed cells in
+// runs of 4 with the accumulation on the matrix cores (the fastest FP32 shape at
+// serving densities: 1.46e8 against 1.16e8 particle-steps/s for the FP32 form of the
+// FP64 choice, and 1.30e8 with every batch density tuned)
+static constexpr int SMALL_BATCH_VARIANT = 1;
+static constexpr int SMALL_BATCH_VARIANT_F32 = 16;
 static constexpr int SERVE_TUNE_MIN_DEFAULT = 1 << 22;
 
 // Evaluation launch shape chosen by the dataset's autotuner for this batch's
@@ -219,17 +237,18 @@ void eval_batch(Batch& b, double* out) {
   };
   // the batch is stored in cell order at every tick
   const int bucket = density_bucket(b.Np, d.g.C);
-  // A batch below serve_tune_min particles runs the listed shape in contiguous
-  // XCD runs untimed: batch sizes wander over many density buckets as requests
-  // come and go, and a trial of every shape there costs more than the fastest
-  // shape saves on so few particles.
+  // A batch below serve_tune_min particles runs a fixed listed-cell shape untimed:
+  // batch sizes wander over many density buckets as requests come and go, and a
+  // trial of every shape there costs more than the fastest shape saves on so few
+  // particles.
   const bool small = d.tuner.forced < 0 && b.Np < d.serve_tune_min;
-  const int planned = small ? SMALL_BATCH_VARIANT : d.tuner.planned(bucket);
+  const int small_v = f32 ? SMALL_BATCH_VARIANT_F32 : SMALL_BATCH_VARIANT;
+  const int planned = small ? small_v : d.tuner.planned(bucket);
   // the non-empty cell list, for the shapes that use it (and the tuner's trials)
   if (planned < 0 || eval_variant_listed(planned))
     launch_nonempty_cells(b.pstart.p, d.g.C, b.cell_list.p, b.list_count.p, b.s);
   if (small)
-    launch(SMALL_BATCH_VARIANT);
+    launch(small_v);
   else
     d.tuner.run(bucket, d.nf == d.g.dim, f32, true, true, b.s, launch);
 }

@@ -148,3 +148,26 @@ def test_budget_admission_order():
     assert t2[1] >= t1[2]                        # admitted after the first finished
     assert np.array_equal(res[q1][0], base[r1][0]) and np.array_equal(res[q2][0], base[r2][0])
     srv.close()
+
+
+def test_batch_growth_on_reused_device_chunks():
+    """A batch's arrays grow (rows copied to a larger leading dimension) while the
+    process's device cache holds the chunks of an earlier, smaller server: the grown
+    arrays are mapped from several chunks, and every growth copies rows out of such
+    ranges.  The first request's result equals its run alone."""
+    print("case: serve batch growth on reused device chunks", flush=True)
+    specs = [(1, "rk4", 3, 2e-3, 1_100_000, 31), (1, "rk4", 2, 2e-3, 1_300_000, 32), (1, "rk4", 2, 2e-3, 2_700_000, 33)]
+    srv = make_server()
+    r = submit(srv, specs[0])
+    alone = drain(srv, [r])[r][0]
+    srv.close()
+    srv = make_server()
+    rids = [submit(srv, specs[0])]
+    srv.step()
+    rids.append(submit(srv, specs[1]))   # the batch grows while the first request runs
+    srv.step()
+    rids.append(submit(srv, specs[2]))   # and grows again
+    res = drain(srv, rids)
+    assert np.array_equal(res[rids[0]][0], alone)
+    assert all(np.all(np.isfinite(res[q][0])) for q in rids)
+    srv.close()
