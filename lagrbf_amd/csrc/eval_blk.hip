@@ -5,6 +5,8 @@
 // per-cell node lists), BS = 4 + 2^d * nft elements:
 //   [x, y, (z), pad | coefficients of the node in the patches of the 2^d corners
 //    of its cell, corner order Q1, nft fields each]
+// (FP32 rows: [-2 x, -2 y, (-2 z), x^2 + y^2 (+ z^2) | ...], x, y, z relative to
+// the centre of the node's cell.)
 // Each (patch, member) coefficient appears exactly once: a member of patch p lies
 // in one cell of p, and p is the patch of one corner of that cell.  An inactive
 // patch (not set up) holds zeros.
@@ -21,8 +23,9 @@
 // loads at constant offsets, used as SGPR operands.
 //
 // T = double: absolute FP64 coordinates.  T = float (mixed precision): particle
-// and node coordinates relative to their own cell origin (shifted in FP64 before
-// rounding), FP32 coefficients, FP64 Shepard combination.
+// and node coordinates relative to their own cell's centre (shifted in FP64 before
+// rounding), the row holding -2 n and |n|^2 for the node n so that r^2 takes four
+// operations (row_r2), FP32 coefficients, FP64 Shepard combination.
 #include "eval_blk_impl.hpp"
 
 #include <hip/hip_ext.h>
@@ -59,10 +62,19 @@ __global__ void k_build_blk(Grid g, const double* __restrict__ x, int N, const i
   const int qi = q % g.nc[0], qj = (q / g.nc[0]) % g.nc[1], qk = dim == 3 ? q / (g.nc[0] * g.nc[1]) : 0;
   const int qq[3] = {qi, qj, qk};
   T* row = blk + (size_t)s * BS;
-  for (int d = 0; d < 4; d++) {
-    double v = 0.0;
-    if (d < dim) v = sizeof(T) == 4 ? x[(size_t)j * dim + d] - (g.O[d] + qq[d] * g.h[d]) : x[(size_t)j * dim + d];
-    row[d] = (T)v;
+  if (sizeof(T) == 4) {
+    // -2 n and |n|^2 of the node n relative to its cell's centre, |n|^2 from the
+    // rounded coordinates
+    double n2 = 0.0;
+    for (int d = 0; d < 3; d++) {
+      float v = 0.f;
+      if (d < dim) v = (float)(x[(size_t)j * dim + d] - (g.O[d] + (qq[d] + 0.5) * g.h[d]));
+      row[d] = (T)(-2.f * v);
+      n2 += (double)v * (double)v;
+    }
+    row[3] = (T)n2;
+  } else {
+    for (int d = 0; d < 4; d++) row[d] = (T)(d < dim ? x[(size_t)j * dim + d] : 0.0);
   }
   for (int kp = 0; kp < g.nen; kp++) {
     int a, b, c;

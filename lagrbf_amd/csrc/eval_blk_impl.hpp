@@ -21,8 +21,9 @@
 // loads at constant offsets, used as SGPR operands.
 //
 // T = double: absolute FP64 coordinates.  T = float (mixed precision): particle
-// and node coordinates relative to their own cell origin (shifted in FP64 before
-// rounding), FP32 coefficients, FP64 Shepard combination.
+// and node coordinates relative to their own cell's centre (shifted in FP64 before
+// rounding), the row holding -2 n and |n|^2 for the node n so that r^2 takes four
+// operations (row_r2), FP32 coefficients, FP64 Shepard combination.
 #pragma once
 #include "kernels.hpp"
 
@@ -96,22 +97,54 @@ __host__ __device__ constexpr int blk_stride() { return 4 + (1 << DIM) * NFT; }
 template <typename T>
 __device__ __forceinline__ T dist2(T d0, T d1, T d2) { return fma(d2, d2, fma(d1, d1, d0 * d0)); }
 
+// r^2 of a particle and the node of a table row (n0 .. n3: the row's first four
+// entries).  FP64 rows hold the node's coordinates, and the difference is squared.
+// FP32 rows hold a = -2 n and |n|^2, n the node relative to the centre of its cell,
+// and the particle comes relative to the same centre, with cp = |p|^2 (cell_r2):
+// r^2 = cp + |n|^2 + a . p, four operations instead of six.  Its rounding error is a
+// few ulp of cp + |n|^2 (cell-centred coordinates keep both within a few h^2, the
+// same order as the subtracting form's error at the distances that dominate phi),
+// and a sum rounded below zero, where particle and node coincide, is taken by
+// magnitude (a free source modifier).
+template <int DIM, typename T>
+__device__ __forceinline__ T row_r2(T xp, T yp, T zp, T cp, T n0, T n1, T n2, T n3) {
+  if constexpr (sizeof(T) == 8) {
+    const T d0 = xp - n0, d1 = yp - n1, d2 = DIM == 3 ? zp - n2 : T(0);
+    return dist2(d0, d1, d2);
+  } else {
+    T s = cp + n3;
+    if constexpr (DIM == 3) s = fma(n2, zp, s);
+    s = fma(n1, yp, s);
+    s = fma(n0, xp, s);
+    return fabsf(s);
+  }
+}
+
+// cp of row_r2: |p|^2 of the particle in the neighbour cell's centred frame (FP32;
+// unused in FP64)
+template <int DIM, typename T>
+__device__ __forceinline__ T cell_r2(T xp, T yp, T zp) {
+  if constexpr (sizeof(T) == 8) return T(0);
+  return dist2(xp, yp, DIM == 3 ? zp : T(0));
+}
+
 // xp, yp, zp: particle coordinates in the frame of the neighbour cell's node rows
-// (FP64: absolute; FP32: relative to the neighbour cell's origin).
+// (FP64: absolute; FP32: relative to the neighbour cell's centre).
 template <int DIM, int KT, int NFT, typename T, int NBI>
 __device__ __forceinline__ void accum_blk(T xp, T yp, T zp, const T* __restrict__ B, int m,
                                           T (&acc)[1 << DIM][NFT], T eps, T rcut2, const T (&e2)[1 << DIM]) {
   constexpr int NEN = 1 << DIM, BS = blk_stride<DIM, NFT>();
-  // the next row's coordinates are loaded one node ahead (the table ends with a
+  const T cp = cell_r2<DIM, T>(xp, yp, zp);
+  // the next row's leading entries are loaded one node ahead (the table ends with a
   // padding row, so row m is always readable)
-  T n0 = B[0], n1 = B[1], n2 = DIM == 3 ? B[2] : T(0);
+  T n0 = B[0], n1 = B[1], n2 = DIM == 3 ? B[2] : T(0), n3 = B[3];
   for (int r = 0; r < m; r++) {
     const T* Br = B + (size_t)r * BS;
-    const T c0 = n0, c1 = n1, c2 = n2;
+    const T c0 = n0, c1 = n1, c2 = n2, c3 = n3;
     n0 = Br[BS]; n1 = Br[BS + 1];
     if constexpr (DIM == 3) n2 = Br[BS + 2];
-    const T d0 = xp - c0, d1 = yp - c1, d2 = DIM == 3 ? zp - c2 : T(0);
-    const T r2 = dist2(d0, d1, d2);
+    n3 = Br[BS + 3];
+    const T r2 = row_r2<DIM, T>(xp, yp, zp, cp, c0, c1, c2, c3);
     if constexpr (compact_support_blk<KT>()) {
       // every lane beyond the support: phi = 0 exactly, nothing to add
       if (!__any(r2 < rcut2)) continue;
@@ -160,11 +193,10 @@ __device__ __forceinline__ void load_row(BlkRow<DIM, NFT, T>& row, const T* __re
 }
 
 template <int DIM, int KT, int NFT, typename T, int NBI>
-__device__ __forceinline__ void node_blk(const BlkRow<DIM, NFT, T>& row, T xp, T yp, T zp, T (&acc)[1 << DIM][NFT],
-                                         T eps, T rcut2, const T (&e2)[1 << DIM]) {
+__device__ __forceinline__ void node_blk(const BlkRow<DIM, NFT, T>& row, T xp, T yp, T zp, T cp,
+                                         T (&acc)[1 << DIM][NFT], T eps, T rcut2, const T (&e2)[1 << DIM]) {
   constexpr int NEN = 1 << DIM;
-  const T d0 = xp - row.v[0], d1 = yp - row.v[1], d2 = DIM == 3 ? zp - row.v[2] : T(0);
-  const T r2 = dist2(d0, d1, d2);
+  const T r2 = row_r2<DIM, T>(xp, yp, zp, cp, row.v[0], row.v[1], row.v[2], row.v[3]);
   // every lane beyond a compact support: phi = 0 exactly, nothing to add
   if (compact_support_blk<KT>() && !__any(r2 < rcut2)) return;
   if constexpr (KT == K_GAUSS_LOCAL) {
@@ -200,6 +232,7 @@ template <int DIM, int KT, int NFT, typename T, int NBI>
 __device__ __forceinline__ void accum_blk_pipe(T xp, T yp, T zp, const T* __restrict__ B, int m,
                                                T (&acc)[1 << DIM][NFT], T eps, T rcut2, const T (&e2)[1 << DIM]) {
   constexpr int BS = blk_stride<DIM, NFT>();
+  const T cp = cell_r2<DIM, T>(xp, yp, zp);
   BlkRow<DIM, NFT, T> ra, rb;
   load_row<DIM, NFT, T, NBI>(ra, B);
   wait_scalar_loads();
@@ -213,7 +246,7 @@ __device__ __forceinline__ void accum_blk_pipe(T xp, T yp, T zp, const T* __rest
     q += BS;
     asm volatile("" ::: "memory");
     __builtin_amdgcn_sched_barrier(0);
-    node_blk<DIM, KT, NFT, T, NBI>(ra, xp, yp, zp, acc, eps, rcut2, e2);
+    node_blk<DIM, KT, NFT, T, NBI>(ra, xp, yp, zp, cp, acc, eps, rcut2, e2);
     __builtin_amdgcn_sched_barrier(0);
     wait_scalar_loads();
     asm volatile("" ::"s"(rb.v[2]), "s"(rb.v[3]));   // keeps the 16-byte load whole
@@ -221,13 +254,13 @@ __device__ __forceinline__ void accum_blk_pipe(T xp, T yp, T zp, const T* __rest
     q += BS;
     asm volatile("" ::: "memory");
     __builtin_amdgcn_sched_barrier(0);
-    node_blk<DIM, KT, NFT, T, NBI>(rb, xp, yp, zp, acc, eps, rcut2, e2);
+    node_blk<DIM, KT, NFT, T, NBI>(rb, xp, yp, zp, cp, acc, eps, rcut2, e2);
     __builtin_amdgcn_sched_barrier(0);
     wait_scalar_loads();
     asm volatile("" ::"s"(ra.v[2]), "s"(ra.v[3]));
     r -= 2;
   }
-  if (r == 1) node_blk<DIM, KT, NFT, T, NBI>(ra, xp, yp, zp, acc, eps, rcut2, e2);
+  if (r == 1) node_blk<DIM, KT, NFT, T, NBI>(ra, xp, yp, zp, cp, acc, eps, rcut2, e2);
 }
 
 // Two rows per wait, for rows of at most 16 entries: rows j, j + 1 are computed
@@ -239,6 +272,7 @@ template <int DIM, int KT, int NFT, typename T, int NBI>
 __device__ __forceinline__ void accum_blk_pipe2(T xp, T yp, T zp, const T* __restrict__ B, int m,
                                                 T (&acc)[1 << DIM][NFT], T eps, T rcut2, const T (&e2)[1 << DIM]) {
   constexpr int BS = blk_stride<DIM, NFT>();
+  const T cp = cell_r2<DIM, T>(xp, yp, zp);
   BlkRow<DIM, NFT, T> a0, a1, b0, b1;
   load_row<DIM, NFT, T, NBI>(a0, B);
   load_row<DIM, NFT, T, NBI>(a1, B + BS);
@@ -251,8 +285,8 @@ __device__ __forceinline__ void accum_blk_pipe2(T xp, T yp, T zp, const T* __res
     q += 2 * BS;
     asm volatile("" ::: "memory");
     __builtin_amdgcn_sched_barrier(0);
-    node_blk<DIM, KT, NFT, T, NBI>(a0, xp, yp, zp, acc, eps, rcut2, e2);
-    node_blk<DIM, KT, NFT, T, NBI>(a1, xp, yp, zp, acc, eps, rcut2, e2);
+    node_blk<DIM, KT, NFT, T, NBI>(a0, xp, yp, zp, cp, acc, eps, rcut2, e2);
+    node_blk<DIM, KT, NFT, T, NBI>(a1, xp, yp, zp, cp, acc, eps, rcut2, e2);
     __builtin_amdgcn_sched_barrier(0);
     wait_scalar_loads();
     asm volatile("" ::"s"(b0.v[2]), "s"(b0.v[3]), "s"(b1.v[2]), "s"(b1.v[3]));
@@ -261,20 +295,20 @@ __device__ __forceinline__ void accum_blk_pipe2(T xp, T yp, T zp, const T* __res
     q += 2 * BS;
     asm volatile("" ::: "memory");
     __builtin_amdgcn_sched_barrier(0);
-    node_blk<DIM, KT, NFT, T, NBI>(b0, xp, yp, zp, acc, eps, rcut2, e2);
-    node_blk<DIM, KT, NFT, T, NBI>(b1, xp, yp, zp, acc, eps, rcut2, e2);
+    node_blk<DIM, KT, NFT, T, NBI>(b0, xp, yp, zp, cp, acc, eps, rcut2, e2);
+    node_blk<DIM, KT, NFT, T, NBI>(b1, xp, yp, zp, cp, acc, eps, rcut2, e2);
     __builtin_amdgcn_sched_barrier(0);
     wait_scalar_loads();
     asm volatile("" ::"s"(a0.v[2]), "s"(a0.v[3]), "s"(a1.v[2]), "s"(a1.v[3]));
     r -= 4;
   }
   // 0 <= r < 4 rows left; a0, a1 hold the first two of them
-  if (r >= 1) node_blk<DIM, KT, NFT, T, NBI>(a0, xp, yp, zp, acc, eps, rcut2, e2);
-  if (r >= 2) node_blk<DIM, KT, NFT, T, NBI>(a1, xp, yp, zp, acc, eps, rcut2, e2);
+  if (r >= 1) node_blk<DIM, KT, NFT, T, NBI>(a0, xp, yp, zp, cp, acc, eps, rcut2, e2);
+  if (r >= 2) node_blk<DIM, KT, NFT, T, NBI>(a1, xp, yp, zp, cp, acc, eps, rcut2, e2);
   if (r == 3) {
     load_row<DIM, NFT, T, NBI>(b0, q);
     wait_scalar_loads();
-    node_blk<DIM, KT, NFT, T, NBI>(b0, xp, yp, zp, acc, eps, rcut2, e2);
+    node_blk<DIM, KT, NFT, T, NBI>(b0, xp, yp, zp, cp, acc, eps, rcut2, e2);
   }
 }
 
@@ -303,12 +337,13 @@ __device__ __forceinline__ void mfma_rows(float xp, float yp, float zp, const fl
                                           ev_f4 (&acc)[1 << DIM], float eps, float rcut2,
                                           const float (&e2)[1 << DIM], int lq) {
   constexpr int NEN = 1 << DIM, BS = blk_stride<DIM, NFT>();
+  const float cp = cell_r2<DIM, float>(xp, yp, zp);
   // every load of the R rows is issued before any of their arithmetic, so the
   // rows share one memory latency
-  float c[R][3], cf[R][NEN];
+  float c[R][4], cf[R][NEN];
 #pragma unroll
   for (int r = 0; r < R; r++) {
-    c[r][0] = B[r * BS]; c[r][1] = B[r * BS + 1]; c[r][2] = DIM == 3 ? B[r * BS + 2] : 0.f;
+    c[r][0] = B[r * BS]; c[r][1] = B[r * BS + 1]; c[r][2] = DIM == 3 ? B[r * BS + 2] : 0.f; c[r][3] = B[r * BS + 3];
 #pragma unroll
     for (int k = 0; k < NEN; k++) {
       const int kp = corner_in_nbr<DIM>(NBI, k);
@@ -317,8 +352,7 @@ __device__ __forceinline__ void mfma_rows(float xp, float yp, float zp, const fl
   }
 #pragma unroll
   for (int r = 0; r < R; r++) {
-    const float d0 = xp - c[r][0], d1 = yp - c[r][1], d2 = DIM == 3 ? zp - c[r][2] : 0.f;
-    const float r2 = dist2(d0, d1, d2);
+    const float r2 = row_r2<DIM, float>(xp, yp, zp, cp, c[r][0], c[r][1], c[r][2], c[r][3]);
     bool use = true;
     if constexpr (compact_support_blk<KT>()) use = __any(r2 < rcut2);
     if (use) {
@@ -401,8 +435,8 @@ __device__ __forceinline__ void accum_nbrs(const Grid& g, int ci, int cj, int ck
       const int q = a + g.nc[0] * (bb + g.nc[1] * e);
       const int nb = cs[q], ne = cs[q + 1];
       if (nb < ne) {
-        // FP32: particle relative to the neighbour cell's origin (node rows are
-        // relative to their own cell's origin); FP64: absolute coordinates
+        // FP32: particle relative to the neighbour cell's centre (node rows are
+        // relative to their own cell's centre); FP64: absolute coordinates
         T qx = xp, qy = yp, qz = zp;
         if constexpr (sizeof(T) == 4) {
           qx = xp - (T)(di * g.h[0]); qy = yp - (T)(dj * g.h[1]);
@@ -614,10 +648,10 @@ void k_eval_blk(Grid g, const double* __restrict__ px, int64_t ld,
     nnb += cs[q + 1] - cs[q];
   }
   if (nnb == 0) return;
-  // FP32: coordinates relative to the particle cell's origin
-  const double ox = sizeof(T) == 4 ? g.O[0] + ci * g.h[0] : 0.0;
-  const double oy = sizeof(T) == 4 ? g.O[1] + cj * g.h[1] : 0.0;
-  const double oz = sizeof(T) == 4 && DIM == 3 ? g.O[2] + ck * g.h[2] : 0.0;
+  // FP32: coordinates relative to the particle cell's centre
+  const double ox = sizeof(T) == 4 ? g.O[0] + (ci + 0.5) * g.h[0] : 0.0;
+  const double oy = sizeof(T) == 4 ? g.O[1] + (cj + 0.5) * g.h[1] : 0.0;
+  const double oz = sizeof(T) == 4 && DIM == 3 ? g.O[2] + (ck + 0.5) * g.h[2] : 0.0;
   for (int s0 = pb + wv * 64; s0 < pe; s0 += nw * 64) {
     const int s = s0 + lane;
     const bool live = s < pe;
@@ -715,14 +749,12 @@ template <int DIM, int KT, int NFT, typename T, int NBI>
 __device__ __forceinline__ void accum_sparse(T xp, T yp, T zp, const T* __restrict__ B, int m, int mmax,
                                              T (&acc)[1 << DIM][NFT], T eps, T rcut2, const T (&e2)[1 << DIM]) {
   constexpr int NEN = 1 << DIM, BS = blk_stride<DIM, NFT>();
+  const T cp = cell_r2<DIM, T>(xp, yp, zp);
   for (int r = 0; r < mmax; r++) {
     const bool act = r < m;
     T r2 = rcut2 * T(4);
     const T* Br = B + (size_t)r * BS;
-    if (act) {
-      const T d0 = xp - Br[0], d1 = yp - Br[1], d2 = DIM == 3 ? zp - Br[2] : T(0);
-      r2 = dist2(d0, d1, d2);
-    }
+    if (act) r2 = row_r2<DIM, T>(xp, yp, zp, cp, Br[0], Br[1], DIM == 3 ? Br[2] : T(0), Br[3]);
     if constexpr (compact_support_blk<KT>()) {
       if (!__any(r2 < rcut2)) continue;
     }
@@ -784,9 +816,9 @@ __global__ __launch_bounds__(256) void k_eval_sparse(Grid g, const double* __res
   const T eps = (T)g.eps_r;
   const double rc = 1.0 / g.eps_r;
   const T rcut2 = (T)(rc * rc * (sizeof(T) == 4 ? (1.0 + 1e-5) : (1.0 + 1e-12)));
-  const double ox = sizeof(T) == 4 ? g.O[0] + ci * g.h[0] : 0.0;
-  const double oy = sizeof(T) == 4 ? g.O[1] + cj * g.h[1] : 0.0;
-  const double oz = sizeof(T) == 4 && DIM == 3 ? g.O[2] + ck * g.h[2] : 0.0;
+  const double ox = sizeof(T) == 4 ? g.O[0] + (ci + 0.5) * g.h[0] : 0.0;
+  const double oy = sizeof(T) == 4 ? g.O[1] + (cj + 0.5) * g.h[1] : 0.0;
+  const double oz = sizeof(T) == 4 && DIM == 3 ? g.O[2] + (ck + 0.5) * g.h[2] : 0.0;
   for (int k = 0; k < nchunk; k++) {
     const int s = pb + k * G + mlane;
     const bool live = s < pe;
