@@ -168,7 +168,7 @@ def test_linear_reproduction_gpu():
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
 def test_phs3_launch_shapes_identical(precision):
     """Every evaluation launch shape (the matrix-core accumulation included) gives
-    the same values bit for bit with the polynomial pass."""
+    the same values bit for bit with the polynomial part."""
     opts, X, V, P = small_c2(Np=20000, precision=precision)
     opts["advection"] = False
     vals = []
@@ -180,6 +180,25 @@ def test_phs3_launch_shapes_identical(precision):
         vals.append(out)
     for v in range(1, 18):
         assert np.array_equal(vals[0], vals[v]), f"variant {v} differs"
+
+
+@pytest.mark.parametrize("kernel", [7, 2])
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_particles_on_data_nodes(kernel, precision):
+    """Particles placed exactly on data nodes, where r = 0: the FP32 evaluation forms
+    r^2 as |p|^2 + |n|^2 - 2 p.n, which may round below zero there (taken by
+    magnitude), so every value must stay finite and agree with the oracle to the
+    path's bar (FP64 1e-12, FP32 evaluation 1e-4)."""
+    opts, X, V, _ = small_c2(Np=10, kernel=kernel, precision=precision, deg=1 if kernel == 7 else -1)
+    opts["advection"] = False
+    P = np.ascontiguousarray(X[::3])
+    print(f"case: {P.shape[0]} particles on data nodes, kernel {kernel}, {precision}", flush=True)
+    g, o, out = run_both(opts, X, V, P)
+    vg, vo = out[0][0], out[1][0]
+    assert np.all(np.isfinite(vg))
+    err = rel_err(vg, vo)
+    print(f"on-node values: rel err {err:.3e}")
+    assert err < (TOL_VAL if precision == "fp64" else 1e-4)
 
 
 def test_phs3_rk4_positions_fp64():
