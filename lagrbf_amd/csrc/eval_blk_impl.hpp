@@ -82,7 +82,19 @@ __host__ __device__ constexpr bool two_rows_fit(int nbi) {
 template <int KT, typename T>
 __device__ __forceinline__ T phi_r2(T r2, T eps) {
   if constexpr (KT == K_PHS3) return r2 * sqrt_fast(r2);
+  else if constexpr (KT == K_GAUSS && sizeof(T) == 4) return __builtin_amdgcn_exp2f(-(eps * eps * 1.44269504f) * r2);
   else return rbf<KT, T>(sqrt_fast(r2), eps);
+}
+
+// The Gaussian of a corner patch with its own shape parameter, e2k = eps_k^2 (FP64)
+// or eps_k^2 log2(e) (FP32, corner_eps2).  In FP32 the hardware exp2 (v_exp_f32,
+// ~1 ulp) takes the place of expf's range-reduced sequence (a dozen instructions per
+// exponential, 2.4 exponentials per node): the argument's own rounding, a few ulp of
+// eps_k^2 r^2, already bounds the result's accuracy at that level.
+template <typename T>
+__device__ __forceinline__ T gauss_local(T e2k, T r2) {
+  if constexpr (sizeof(T) == 4) return __builtin_amdgcn_exp2f(-e2k * r2);
+  else return exp(-e2k * r2);
 }
 
 template <int KT>
@@ -155,7 +167,7 @@ __device__ __forceinline__ void accum_blk(T xp, T yp, T zp, const T* __restrict_
     for (int k = 0; k < NEN; k++) {
       const int kp = corner_in_nbr<DIM>(NBI, k);
       if (kp >= 0) {
-        if constexpr (KT == K_GAUSS_LOCAL) phi = exp(-e2[k] * r2);
+        if constexpr (KT == K_GAUSS_LOCAL) phi = gauss_local<T>(e2[k], r2);
 #pragma unroll
         for (int f = 0; f < NFT; f++) acc[k][f] += phi * Br[4 + kp * NFT + f];
       }
@@ -205,7 +217,7 @@ __device__ __forceinline__ void node_blk(const BlkRow<DIM, NFT, T>& row, T xp, T
     for (int k = 0; k < NEN; k++) {
       const int kp = corner_in_nbr<DIM>(NBI, k);
       if (kp >= 0) {
-        const T phi = exp(-e2[k] * r2);
+        const T phi = gauss_local<T>(e2[k], r2);
 #pragma unroll
         for (int f = 0; f < NFT; f++) acc[k][f] += phi * row.v[4 + kp * NFT + f];
       }
@@ -362,7 +374,7 @@ __device__ __forceinline__ void mfma_rows(float xp, float yp, float zp, const fl
       for (int k = 0; k < NEN; k++) {
         const int kp = corner_in_nbr<DIM>(NBI, k);
         if (kp >= 0) {
-          if constexpr (KT == K_GAUSS_LOCAL) phi = exp(-e2[k] * r2);
+          if constexpr (KT == K_GAUSS_LOCAL) phi = gauss_local<float>(e2[k], r2);
           acc[k] = __builtin_amdgcn_mfma_f32_4x4x1f32(cf[r][k], phi, acc[k], 0, 0, 0);
         }
       }
@@ -466,7 +478,7 @@ __device__ __forceinline__ void corner_eps2(const Grid& g, int ci, int cj, int c
       int a, b, c;
       corner_offset(k, a, b, c);
       const double e = ev.peps[(ci + a) + g.n[0] * ((cj + b) + g.n[1] * (DIM == 3 ? ck + c : 0))];
-      e2[k] = (T)(e * e);
+      e2[k] = (T)(sizeof(T) == 4 ? e * e * 1.4426950408889634 : e * e);   // FP32: log2(e) folded in
     }
   }
 }
@@ -765,7 +777,7 @@ __device__ __forceinline__ void accum_sparse(T xp, T yp, T zp, const T* __restri
       for (int k = 0; k < NEN; k++) {
         const int kp = corner_in_nbr<DIM>(NBI, k);
         if (kp >= 0) {
-          if constexpr (KT == K_GAUSS_LOCAL) phi = exp(-e2[k] * r2);
+          if constexpr (KT == K_GAUSS_LOCAL) phi = gauss_local<T>(e2[k], r2);
 #pragma unroll
           for (int f = 0; f < NFT; f++) acc[k][f] += phi * Br[4 + kp * NFT + f];
         }
