@@ -64,3 +64,35 @@ def test_chunks_reused_by_a_context_of_another_size():
     assert before >= 64 * 2**20
     assert held1 < before   # the large context took the small one's chunks
     assert np.array_equal(v1, v2)
+
+
+def test_cache_off_gives_identical_values(tmp_path):
+    """LAGRBF_DEVICE_CACHE=0 (plain hipMalloc / hipFree for every pool) in a child
+    process gives the same values bit for bit as the cached ranges here."""
+    import os
+    import subprocess
+    import sys
+    opts, X, V, P = cases.fourier_3d(nlat=40, Np=20000, n=16)
+    opts.update(lagrbf_precision="mixed", advection=False, lagrbf_setup_all=True)
+    g = L.LagRBFCtx(petsc_options=dict(opts))
+    g.fill_data(X, V); g.fill_points(P); g.interpolate_c()
+    here = np.zeros((P.shape[0], 3)); g.output(fvals=here)
+    g.destroy()
+    out = tmp_path / "off.npy"
+    code = (
+        "import numpy as np, lagrbf_amd as L\n"
+        "from tests import cases\n"
+        "opts, X, V, P = cases.fourier_3d(nlat=40, Np=20000, n=16)\n"
+        "opts.update(lagrbf_precision='mixed', advection=False, lagrbf_setup_all=True)\n"
+        "g = L.LagRBFCtx(petsc_options=dict(opts))\n"
+        "g.fill_data(X, V); g.fill_points(P); g.interpolate_c()\n"
+        "v = np.zeros((P.shape[0], 3)); g.output(fvals=v)\n"
+        "assert g.info('device_cache_bytes') == 0\n"
+        "g.destroy()\n"
+        f"np.save({str(out)!r}, v)\n")
+    env = dict(os.environ, LAGRBF_DEVICE_CACHE="0")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    print("case: device cache off in a child process", flush=True)
+    r = subprocess.run([sys.executable, "-c", code], env=env, cwd=root, capture_output=True, text=True, timeout=240)
+    assert r.returncode == 0, r.stderr[-2000:]
+    assert np.array_equal(np.load(out), here)
