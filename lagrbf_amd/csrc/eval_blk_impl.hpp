@@ -83,6 +83,9 @@ template <int KT, typename T>
 __device__ __forceinline__ T phi_r2(T r2, T eps) {
   if constexpr (KT == K_PHS3) return r2 * sqrt_fast(r2);
   else if constexpr (KT == K_GAUSS && sizeof(T) == 4) return __builtin_amdgcn_exp2f(-(eps * eps * 1.44269504f) * r2);
+  // FP32 inverse multiquadric 1 / sqrt(1 + eps^2 r^2): the hardware reciprocal square
+  // root (~1 ulp) instead of a square root, a square and a correctly rounded divide
+  else if constexpr (KT == K_IMQ && sizeof(T) == 4) return __builtin_amdgcn_rsqf(fma(eps * eps, r2, 1.0f));
   else return rbf<KT, T>(sqrt_fast(r2), eps);
 }
 

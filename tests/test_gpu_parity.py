@@ -72,6 +72,25 @@ def test_interp_2d_all_kernels(kernel, shape, tol):
     assert err < tol
 
 
+@pytest.mark.parametrize("kernel,shape", [(1, 1.0), (2, 1.0), (5, 10.0), (6, 10.0)])
+def test_interp_2d_fp32_evaluation_kernels(kernel, shape):
+    """The FP32 evaluation (mixed path) of every kernel type it covers besides r^3 and
+    the per-patch Gaussian (tested with their polynomial blocks): Wendland C0 and C2,
+    the inverse multiquadric (hardware reciprocal square root) and the Gaussian
+    (hardware exp2), against the FP64 oracle.  The FP32 values carry the rounding of
+    the FP32 coefficients and sums: 1e-4 relative bounds it where the patch sums cancel
+    (the global kernels at shape factor 10)."""
+    opts, X, V, P = cases.rotation_2d(N=2048, Np=3000, n=9, kernel=kernel, advection=False)
+    opts["pu_mesh_shape_factor"] = shape
+    opts["lagrbf_precision"] = "mixed"
+    print(f"case: rotation2d FP32 evaluation kernel={kernel} shape={shape}", flush=True)
+    g, o, res = interp_both(opts, X, V, P)
+    assert np.all(np.isfinite(res[0][0]))
+    err = rel_err(res[0][0], res[1][0])
+    print(f"kernel {kernel} FP32 evaluation: max rel err {err:.3e}")
+    assert err < 1e-4
+
+
 @pytest.mark.parametrize("solve_method", [0, 1, 2])
 def test_interp_3d(solve_method):
     opts, X, V, P = cases.fourier_3d(nlat=20, Np=2000, n=5, solve_method=solve_method)
